@@ -1,0 +1,249 @@
+"""MI355X-native Schwarzschild null-geodesic tracer (host side).
+
+The compute runs in hand-written HIP kernels (libbhs_hip.so, gfx950) reached through the C-ABI of
+include/bhs_trace.h; this module is the Python view of that ABI. PyTorch is used only to allocate
+device memory and to hand over streams.
+
+Reference interfaces mirrored:
+  trace(...)        bhs::physics::trace (src/physics/Geodesic.h:107-140), same defaults and
+                    TraceResult fields (status Escaped/Captured/MaxSteps, dir, steps, minR,
+                    equatorCrossings); FP64 on the GPU.
+  Tracer.render     the thin-disk trace pass of shaders/blackhole.frag:67-253 driven by
+                    Renderer::tracePass/setTraceUniforms (src/Renderer.cpp:242-387), returning per-ray
+                    geometry (class, steps, escape direction, disk crossings r/phi/g) instead of colour.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+from ._capi import (CLASS_DISK, CLASS_ESCAPE, CLASS_HORIZON, DISK_NONE, DISK_THIN, FLAG_BAND64,
+                    FLAG_ESCAPED, FLAG_FALLBACK, FLAG_MAXSTEPS, FLAG_ROOT64, FLAG_SURROGATE,
+                    PREC_F32, PREC_F32_BAND64, PREC_F64, TRACE_CAPTURED, TRACE_ESCAPED,
+                    TRACE_MAXSTEPS)
+from .camera import PRESETS, Camera, look_basis, orbit_camera
+
+__all__ = ["Tracer", "BhsError", "TraceResult", "RayBuffers", "march_params", "trace",
+           "Camera", "orbit_camera", "look_basis", "PRESETS", "critical_impact_parameter",
+           "CLASS_HORIZON", "CLASS_DISK", "CLASS_ESCAPE", "FLAG_MAXSTEPS", "FLAG_ESCAPED",
+           "FLAG_BAND64", "FLAG_ROOT64", "FLAG_SURROGATE", "FLAG_FALLBACK", "PREC_F32",
+           "PREC_F32_BAND64", "PREC_F64", "DISK_NONE", "DISK_THIN", "TRACE_ESCAPED",
+           "TRACE_CAPTURED", "TRACE_MAXSTEPS"]
+
+kHorizon = 2.0
+kPhotonSphere = 3.0
+kIsco = 6.0
+
+
+def critical_impact_parameter() -> float:
+    """b_crit = 3 sqrt(3) (src/physics/Geodesic.h:19)."""
+    return 3.0 * np.sqrt(3.0)
+
+
+class BhsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_capi.STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+@dataclass
+class TraceResult:
+    """TraceResult of src/physics/Geodesic.h:97-103."""
+    status: int
+    dir: np.ndarray
+    steps: int
+    minR: float
+    equatorCrossings: int
+
+    Escaped, Captured, MaxSteps = TRACE_ESCAPED, TRACE_CAPTURED, TRACE_MAXSTEPS
+
+
+def march_params(max_steps=450, step_scale=None, disk_step=None, disk_inner=6.0, disk_outer=20.0,
+                 disk_thickness=0.035, disk_mode=DISK_THIN, precision=PREC_F32_BAND64, root64=0,
+                 band_rel=1e-3) -> _capi.MarchParams:
+    """bhs_march_params; step_scale/disk_step default to the quality preset of max_steps
+    (src/Settings.cpp:15-30), else to the High preset."""
+    ss, dstep = PRESETS.get(max_steps, PRESETS[450])
+    return _capi.MarchParams(max_steps, disk_mode, precision, root64,
+                             ss if step_scale is None else step_scale,
+                             dstep if disk_step is None else disk_step, disk_inner, disk_outer,
+                             disk_thickness, band_rel)
+
+
+def _camera_struct(cam: Camera) -> _capi.Camera:
+    c = _capi.Camera()
+    c.pos[:] = [float(v) for v in np.asarray(cam.pos, np.float32)]
+    c.basis[:] = [float(v) for v in np.asarray(cam.basis, np.float32)]
+    c.tan_half_fov = float(cam.tan_half_fov)
+    c.width, c.height = int(cam.width), int(cam.height)
+    c.jitter[:] = [float(cam.jitter[0]), float(cam.jitter[1])]
+    c.jitter_hashed = int(cam.jitter_hashed)
+    c.camera_id = int(cam.camera_id) & 0xFFFFFFFF
+    c.sample = int(cam.sample) & 0xFFFFFFFF
+    c.seed = int(cam.seed) & 0xFFFFFFFFFFFFFFFF
+    return c
+
+
+class RayBuffers:
+    """Device structure-of-arrays output (bhs_ray_soa) held as torch tensors."""
+
+    def __init__(self, n: int, device):
+        import torch
+        self.n = n
+        self.cls = torch.empty(n, dtype=torch.uint8, device=device)
+        self.flags = torch.empty(n, dtype=torch.uint8, device=device)
+        self.n_cross = torch.empty(n, dtype=torch.uint8, device=device)
+        self.steps = torch.empty(n, dtype=torch.int16, device=device)   # uint16 bit pattern
+        self.esc = torch.empty((3, n), dtype=torch.float32, device=device)
+        self.disk = torch.empty((6, n), dtype=torch.float32, device=device)
+        self.b = torch.empty(n, dtype=torch.float32, device=device)
+
+    def soa(self) -> _capi.RaySoA:
+        return _capi.RaySoA(self.cls.data_ptr(), self.flags.data_ptr(), self.n_cross.data_ptr(),
+                            self.steps.data_ptr(), self.esc.data_ptr(), self.disk.data_ptr(),
+                            self.b.data_ptr())
+
+    def steps_i32(self):
+        import torch
+        return self.steps.to(torch.int32) & 0xFFFF
+
+    def numpy(self) -> dict:
+        """Host copies: cls, flags, n_cross, steps (int32), esc (n,3), disk_r/phi/g (n,2), b."""
+        d = self.disk.cpu().numpy()
+        return dict(cls=self.cls.cpu().numpy(), flags=self.flags.cpu().numpy(),
+                    n_cross=self.n_cross.cpu().numpy(),
+                    steps=self.steps.cpu().numpy().view(np.uint16).astype(np.int32),
+                    esc=self.esc.cpu().numpy().T.copy(), disk_r=d[0:2].T.copy(),
+                    disk_phi=d[2:4].T.copy(), disk_g=d[4:6].T.copy(), b=self.b.cpu().numpy())
+
+
+class Tracer:
+    """One bhs_ctx on one HIP device."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _capi.load()
+        st = C.c_int(0)
+        self._ctx = self._lib.bhs_create(device, C.byref(st))
+        if not self._ctx:
+            raise BhsError(st.value, f"bhs_create(device={device}) failed")
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.bhs_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str):
+        if rc != _capi.BHS_OK:
+            msg = self._lib.bhs_last_error(self._ctx)
+            raise BhsError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    @staticmethod
+    def _stream_ptr(stream):
+        if stream is None:
+            import torch
+            return torch.cuda.current_stream().cuda_stream
+        return getattr(stream, "cuda_stream", stream)
+
+    # ---- law T: bhs::physics::trace --------------------------------------------------------
+    def trace_batch(self, pos3, dir3, max_steps=200000, step_scale=0.005, esc_tol=1e-4,
+                    r_esc_min=30.0) -> np.ndarray:
+        pos3 = np.ascontiguousarray(pos3, dtype=np.float64).reshape(-1, 3)
+        dir3 = np.ascontiguousarray(dir3, dtype=np.float64).reshape(-1, 3)
+        if pos3.shape != dir3.shape:
+            raise ValueError("pos3 and dir3 must have the same shape")
+        n = pos3.shape[0]
+        out = (_capi.TraceResult * max(n, 1))()
+        rc = self._lib.bhs_trace_rays(self._ctx, pos3.ctypes.data, dir3.ctypes.data, n, max_steps,
+                                      step_scale, esc_tol, r_esc_min, C.addressof(out))
+        self._check(rc, "bhs_trace_rays")
+        dt = np.dtype([("status", "<i4"), ("steps", "<i4"), ("equator_crossings", "<i4"),
+                       ("pad_", "<i4"), ("dir", "<f8", 3), ("min_r", "<f8")])
+        return np.frombuffer(bytes(out), dtype=dt, count=n).copy()
+
+    def trace(self, cam_pos, d, max_steps=200000, step_scale=0.005, esc_tol=1e-4,
+              r_esc_min=30.0) -> TraceResult:
+        r = self.trace_batch([cam_pos], [d], max_steps, step_scale, esc_tol, r_esc_min)[0]
+        return TraceResult(int(r["status"]), np.array(r["dir"]), int(r["steps"]), float(r["min_r"]),
+                           int(r["equator_crossings"]))
+
+    # ---- law S ---------------------------------------------------------------------------------
+    def render(self, cam: Camera, params: _capi.MarchParams | None = None,
+               out: RayBuffers | None = None, stream=None) -> RayBuffers:
+        """Law-S render of one camera into device buffers (stream-ordered, asynchronous)."""
+        if params is None:
+            params = march_params()
+        if out is None:
+            out = RayBuffers(cam.width * cam.height, f"cuda:{self.device}")
+        cs = _camera_struct(cam)
+        soa = out.soa()
+        rc = self._lib.bhs_render(self._ctx, C.byref(cs), C.byref(params), C.byref(soa),
+                                  C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_render")
+        return out
+
+    def march_rays(self, pos3, dir3, params: _capi.MarchParams | None = None,
+                   out: RayBuffers | None = None, stream=None) -> RayBuffers:
+        """Law S on explicit rays; pos3/dir3 are (n,3) float32 CUDA tensors."""
+        import torch
+        if params is None:
+            params = march_params()
+        pos3 = pos3.contiguous()
+        dir3 = dir3.contiguous()
+        if pos3.dtype != torch.float32 or dir3.dtype != torch.float32 or pos3.shape != dir3.shape:
+            raise ValueError("pos3/dir3 must be float32 tensors of equal shape (n,3)")
+        n = pos3.shape[0]
+        if out is None:
+            out = RayBuffers(n, pos3.device)
+        soa = out.soa()
+        rc = self._lib.bhs_march_rays(self._ctx, pos3.data_ptr(), dir3.data_ptr(), n,
+                                      C.byref(params), C.byref(soa),
+                                      C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_march_rays")
+        return out
+
+    def last_stats(self) -> dict:
+        s = _capi.Stats()
+        self._check(self._lib.bhs_last_stats(self._ctx, C.byref(s)), "bhs_last_stats")
+        return dict(rays=s.rays, steps=s.steps, crossings=s.crossings, escapes=s.escapes,
+                    band64=s.band64, fallback=s.fallback)
+
+    def set_timing(self, on: bool):
+        self._check(self._lib.bhs_set_timing(self._ctx, 1 if on else 0), "bhs_set_timing")
+
+    def timing(self) -> dict:
+        """Average GPU ms of the main march kernel and of the whole call since the last call."""
+        km, tm, n = C.c_double(), C.c_double(), C.c_int32()
+        self._check(self._lib.bhs_timing(self._ctx, C.byref(km), C.byref(tm), C.byref(n)),
+                    "bhs_timing")
+        return dict(march_ms=km.value, total_ms=tm.value, launches=n.value)
+
+    def sync(self):
+        self._check(self._lib.bhs_sync(self._ctx), "bhs_sync")
+
+
+_tls = threading.local()
+
+
+def default_tracer() -> Tracer:
+    t = getattr(_tls, "tracer", None)
+    if t is None:
+        t = Tracer(0)
+        _tls.tracer = t
+    return t
+
+
+def trace(cam_pos, d, max_steps=200000, step_scale=0.005, esc_tol=1e-4,
+          r_esc_min=30.0) -> TraceResult:
+    """bhs::physics::trace (src/physics/Geodesic.h:107-140) on the GPU, same defaults."""
+    return default_tracer().trace(cam_pos, d, max_steps, step_scale, esc_tol, r_esc_min)

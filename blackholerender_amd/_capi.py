@@ -1,0 +1,96 @@
+"""ctypes view of the C-ABI in include/bhs_trace.h (libbhs_hip.so).
+
+Loading the library needs no GPU; every compute entry point does. The library must be built
+(`python -m blackholerender_amd.build` or __graft_entry__.build()); there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbhs_hip.so")
+
+BHS_OK = 0
+STATUS_NAMES = {0: "BHS_OK", -1: "BHS_E_INVALID", -2: "BHS_E_DEVICE", -3: "BHS_E_ALLOC",
+                -4: "BHS_E_LAUNCH", -5: "BHS_E_IO", -6: "BHS_E_STATE"}
+
+TRACE_ESCAPED, TRACE_CAPTURED, TRACE_MAXSTEPS = 0, 1, 2
+CLASS_HORIZON, CLASS_DISK, CLASS_ESCAPE = 0, 1, 2
+FLAG_MAXSTEPS, FLAG_ESCAPED, FLAG_BAND64, FLAG_ROOT64, FLAG_SURROGATE, FLAG_FALLBACK = 1, 2, 4, 8, 16, 32
+PREC_F32, PREC_F32_BAND64, PREC_F64 = 0, 1, 2
+DISK_NONE, DISK_THIN = 0, 1
+
+
+class TraceResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("steps", C.c_int32), ("equator_crossings", C.c_int32),
+                ("pad_", C.c_int32), ("dir", C.c_double * 3), ("min_r", C.c_double)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("basis", C.c_float * 9), ("tan_half_fov", C.c_float),
+                ("width", C.c_int32), ("height", C.c_int32), ("jitter", C.c_float * 2),
+                ("jitter_hashed", C.c_int32), ("camera_id", C.c_uint32), ("sample", C.c_uint32),
+                ("pad_", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class MarchParams(C.Structure):
+    _fields_ = [("max_steps", C.c_int32), ("disk_mode", C.c_int32), ("precision", C.c_int32),
+                ("root64", C.c_int32), ("step_scale", C.c_float), ("disk_step", C.c_float),
+                ("disk_inner", C.c_float), ("disk_outer", C.c_float),
+                ("disk_thickness", C.c_float), ("band_rel", C.c_float)]
+
+
+class RaySoA(C.Structure):
+    _fields_ = [("cls_dev", C.c_void_p), ("flags_dev", C.c_void_p), ("n_cross_dev", C.c_void_p),
+                ("steps_dev", C.c_void_p), ("esc_dev", C.c_void_p), ("disk_dev", C.c_void_p),
+                ("b_dev", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("steps", C.c_uint64), ("crossings", C.c_uint64),
+                ("escapes", C.c_uint64), ("band64", C.c_uint64), ("fallback", C.c_uint64),
+                ("pad_", C.c_uint64 * 2)]
+
+
+_lib = None
+
+
+def load():
+    """Load libbhs_hip.so and declare the signatures; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m blackholerender_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    L.bhs_version.restype = C.c_int
+    L.bhs_create.argtypes = [C.c_int, C.POINTER(C.c_int)]
+    L.bhs_create.restype = vp
+    L.bhs_destroy.argtypes = [vp]
+    L.bhs_destroy.restype = None
+    L.bhs_last_error.argtypes = [vp]
+    L.bhs_last_error.restype = C.c_char_p
+    L.bhs_sync.argtypes = [vp]
+    L.bhs_trace_rays.argtypes = [vp, vp, vp, i64, i32, f64, f64, f64, vp]
+    L.bhs_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams), C.POINTER(RaySoA), vp]
+    L.bhs_march_rays.argtypes = [vp, vp, vp, i64, C.POINTER(MarchParams), C.POINTER(RaySoA), vp]
+    L.bhs_last_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.bhs_set_timing.argtypes = [vp, C.c_int]
+    L.bhs_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(i32)]
+    for name in ("bhs_sync", "bhs_trace_rays", "bhs_render", "bhs_march_rays", "bhs_last_stats",
+                 "bhs_set_timing", "bhs_timing"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def header_symbols(header_path: str | None = None) -> list[str]:
+    """Function names declared in include/bhs_trace.h."""
+    import re
+    if header_path is None:
+        header_path = os.path.join(os.path.dirname(HERE), "include", "bhs_trace.h")
+    text = open(header_path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bhs_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,406 @@
+// capi.hip — the C-ABI of include/bhs_trace.h: context, argument checking, scratch management and
+// stream ordering around the kernels of kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/bhs_trace.h"
+#include "kernels.hpp"
+
+struct bhs_ctx {
+    int device = 0;
+    std::string err;
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t last_stream = nullptr;
+    // Render scratch: [Stats | band_count (pad 256) | band_ids | roots]
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // Law-T staging (pos3, dir3, results).
+    void* tstage = nullptr;
+    size_t tstage_bytes = 0;
+    // GPU timers: per launch, events before the call, around the main kernel, after the call.
+    struct TimingRec { hipEvent_t call0, k0, k1, call1; };
+    bool timing = false;
+    std::vector<TimingRec> timers;
+    size_t timers_used = 0;
+};
+
+namespace {
+
+int fail(bhs_ctx* ctx, int code, const char* fmt, const char* detail = nullptr) {
+    if (ctx) {
+        char buf[512];
+        std::snprintf(buf, sizeof buf, fmt, detail ? detail : "");
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int hip_fail(bhs_ctx* ctx, hipError_t e, const char* where) {
+    if (ctx) ctx->err = std::string(where) + ": " + hipGetErrorString(e);
+    return (e == hipErrorOutOfMemory) ? BHS_E_ALLOC : BHS_E_DEVICE;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// Grows a device buffer, after draining work that may still use the old one.
+int ensure(bhs_ctx* ctx, void** buf, size_t* cap, size_t need, const char* what) {
+    if (*cap >= need) return BHS_OK;
+    if (*buf) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(*buf);
+        *buf = nullptr;
+        *cap = 0;
+    }
+    size_t want = need + need / 4 + 4096;
+    hipError_t e = hipMalloc(buf, want);
+    if (e != hipSuccess) {
+        *buf = nullptr;
+        (void)hipGetLastError();
+        return fail(ctx, BHS_E_ALLOC, "device allocation of %s failed", what);
+    }
+    *cap = want;
+    return BHS_OK;
+}
+
+bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+int check_params(bhs_ctx* ctx, const bhs_march_params* p) {
+    if (!p) return fail(ctx, BHS_E_INVALID, "params is NULL%s");
+    if (p->max_steps < 0) return fail(ctx, BHS_E_INVALID, "max_steps < 0%s");
+    if (p->disk_mode != BHS_DISK_NONE && p->disk_mode != BHS_DISK_THIN)
+        return fail(ctx, BHS_E_INVALID, "disk_mode must be BHS_DISK_NONE or BHS_DISK_THIN%s");
+    if (p->precision < BHS_PREC_F32 || p->precision > BHS_PREC_F64)
+        return fail(ctx, BHS_E_INVALID, "unknown precision%s");
+    if (!(p->step_scale > 0.0f) || !std::isfinite(p->step_scale))
+        return fail(ctx, BHS_E_INVALID, "step_scale must be finite and > 0%s");
+    if (!std::isfinite(p->disk_step) || !std::isfinite(p->disk_inner) ||
+        !std::isfinite(p->disk_outer) || !std::isfinite(p->disk_thickness) ||
+        !std::isfinite(p->band_rel) || p->band_rel < 0.0f)
+        return fail(ctx, BHS_E_INVALID, "disk/band parameters must be finite (band_rel >= 0)%s");
+    return BHS_OK;
+}
+
+bhs::MarchConst make_const(const bhs_march_params* p) {
+    bhs::MarchConst c;
+    std::memset(&c, 0, sizeof c);
+    c.max_steps = p->max_steps;
+    c.thin = p->disk_mode == BHS_DISK_THIN;
+    c.step_scale = p->step_scale;
+    c.disk_step = p->disk_step;
+    c.disk_inner = p->disk_inner;
+    c.disk_outer = p->disk_outer;
+    c.disk_thickness = p->disk_thickness;
+    c.band_abs = p->band_rel * bhs::kBcritF;
+    c.precision = p->precision;
+    c.root64 = p->root64 ? 1 : 0;
+    return c;
+}
+
+uint32_t pcg_host(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+// Seed of the hashed per-pixel jitter: pcg chain over (sample, camera, seed high, seed low).
+uint32_t jitter_base(uint64_t seed, uint32_t camera_id, uint32_t sample) {
+    uint32_t h = pcg_host(sample);
+    h = pcg_host(h ^ camera_id);
+    h = pcg_host(h ^ (uint32_t)(seed >> 32));
+    h = pcg_host(h ^ (uint32_t)seed);
+    return h;
+}
+
+int check_camera(bhs_ctx* ctx, const bhs_camera* cam) {
+    if (!cam) return fail(ctx, BHS_E_INVALID, "camera is NULL%s");
+    if (cam->width <= 0 || cam->height <= 0)
+        return fail(ctx, BHS_E_INVALID, "camera width and height must be > 0%s");
+    if ((int64_t)cam->width * cam->height > (int64_t)0x7fffffff)
+        return fail(ctx, BHS_E_INVALID, "camera has more than 2^31-1 pixels%s");
+    if (!finite3(cam->pos) || !std::isfinite(cam->tan_half_fov) || !(cam->tan_half_fov > 0.0f))
+        return fail(ctx, BHS_E_INVALID, "camera position / tan_half_fov must be finite (tan > 0)%s");
+    for (int k = 0; k < 9; ++k)
+        if (!std::isfinite(cam->basis[k])) return fail(ctx, BHS_E_INVALID, "camera basis not finite%s");
+    const float r2 = cam->pos[0] * cam->pos[0] + cam->pos[1] * cam->pos[1] + cam->pos[2] * cam->pos[2];
+    if (!(r2 > 0.0f)) return fail(ctx, BHS_E_INVALID, "camera at the origin%s");
+    if (!std::isfinite(cam->jitter[0]) || !std::isfinite(cam->jitter[1]))
+        return fail(ctx, BHS_E_INVALID, "camera jitter not finite%s");
+    return BHS_OK;
+}
+
+void fill_camera(bhs::MarchConst& c, const bhs_camera* cam) {
+    for (int k = 0; k < 3; ++k) c.pos[k] = cam->pos[k];
+    for (int k = 0; k < 9; ++k) c.basis[k] = cam->basis[k];
+    c.tan_half_fov = cam->tan_half_fov;
+    c.width = cam->width;
+    c.height = cam->height;
+    c.jitter[0] = cam->jitter[0];
+    c.jitter[1] = cam->jitter[1];
+    c.jitter_hashed = cam->jitter_hashed ? 1 : 0;
+    c.jitter_base = jitter_base(cam->seed, cam->camera_id, cam->sample);
+}
+
+bhs::SoA make_soa(const bhs_ray_soa* o, int64_t n) {
+    bhs::SoA s;
+    s.cls = o->cls_dev;
+    s.flags = o->flags_dev;
+    s.n_cross = o->n_cross_dev;
+    s.steps = o->steps_dev;
+    s.esc = o->esc_dev;
+    s.disk = o->disk_dev;
+    s.b = o->b_dev;
+    s.n = n;
+    return s;
+}
+
+constexpr size_t kStatsBytes = 256;   // Stats + band counter, padded
+
+// Next timer record of the context (null when timing is off or events cannot be created).
+bhs_ctx::TimingRec* next_timer(bhs_ctx* ctx) {
+    if (!ctx->timing) return nullptr;
+    if (ctx->timers_used == ctx->timers.size()) {
+        bhs_ctx::TimingRec r{};
+        if (hipEventCreate(&r.call0) != hipSuccess || hipEventCreate(&r.k0) != hipSuccess ||
+            hipEventCreate(&r.k1) != hipSuccess || hipEventCreate(&r.call1) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        ctx->timers.push_back(r);
+    }
+    return &ctx->timers[ctx->timers_used++];
+}
+
+// Lays out the render scratch for n rays; returns the Scratch view and the bytes to clear.
+int prepare_scratch(bhs_ctx* ctx, int64_t n, bool roots, bhs::Scratch* sc) {
+    const size_t band_bytes = ((size_t)n * 4 + 255) & ~(size_t)255;
+    const size_t root_bytes = roots ? (size_t)n * 2 * sizeof(bhs::RootState<float>) : 0;
+    const size_t need = kStatsBytes + band_bytes + root_bytes;
+    int st = ensure(ctx, &ctx->scratch, &ctx->scratch_bytes, need, "render scratch");
+    if (st != BHS_OK) return st;
+    char* base = (char*)ctx->scratch;
+    sc->stats = (bhs::Stats*)base;
+    sc->band_count = (uint32_t*)(base + sizeof(bhs::Stats));
+    sc->band_ids = (uint32_t*)(base + kStatsBytes);
+    sc->band_cap = (uint32_t)n;
+    sc->roots = roots ? (bhs::RootState<float>*)(base + kStatsBytes + band_bytes) : nullptr;
+    return BHS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bhs_version(void) { return BHS_ABI_VERSION; }
+
+bhs_ctx* bhs_create(int device, int* status) {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        if (status) *status = BHS_E_DEVICE;
+        return nullptr;
+    }
+    bhs_ctx* ctx = new (std::nothrow) bhs_ctx();
+    if (!ctx) {
+        if (status) *status = BHS_E_ALLOC;
+        return nullptr;
+    }
+    ctx->device = device;
+    DeviceGuard g(device);
+    if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        bhs_destroy(ctx);
+        if (status) *status = BHS_E_DEVICE;
+        return nullptr;
+    }
+    if (status) *status = BHS_OK;
+    return ctx;
+}
+
+void bhs_destroy(bhs_ctx* ctx) {
+    if (!ctx) return;
+    {
+        DeviceGuard g(ctx->device);
+        (void)hipDeviceSynchronize();
+        if (ctx->scratch) (void)hipFree(ctx->scratch);
+        if (ctx->tstage) (void)hipFree(ctx->tstage);
+        for (auto& r : ctx->timers) {
+            (void)hipEventDestroy(r.call0);
+            (void)hipEventDestroy(r.k0);
+            (void)hipEventDestroy(r.k1);
+            (void)hipEventDestroy(r.call1);
+        }
+        if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+        if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+        if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+    }
+    delete ctx;
+}
+
+const char* bhs_last_error(const bhs_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int bhs_sync(bhs_ctx* ctx) {
+    if (!ctx) return BHS_E_INVALID;
+    DeviceGuard g(ctx->device);
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(ctx, e, "bhs_sync");
+    return BHS_OK;
+}
+
+int bhs_trace_rays(bhs_ctx* ctx, const double* pos3_host, const double* dir3_host, int64_t n,
+                   int32_t max_steps, double step_scale, double esc_tol, double r_esc_min,
+                   bhs_trace_result* out_host) {
+    if (!ctx) return BHS_E_INVALID;
+    if (n < 0 || (n > 0 && (!pos3_host || !dir3_host || !out_host)))
+        return fail(ctx, BHS_E_INVALID, "bhs_trace_rays: NULL buffer or n < 0%s");
+    if (n == 0) return BHS_OK;
+    if (max_steps < 0) return fail(ctx, BHS_E_INVALID, "bhs_trace_rays: max_steps < 0%s");
+    DeviceGuard g(ctx->device);
+    const size_t in_bytes = (size_t)n * 3 * sizeof(double);
+    const size_t out_bytes = (size_t)n * sizeof(bhs_trace_result);
+    int st = ensure(ctx, &ctx->tstage, &ctx->tstage_bytes, 2 * in_bytes + out_bytes, "trace staging");
+    if (st != BHS_OK) return st;
+    char* base = (char*)ctx->tstage;
+    double* dpos = (double*)base;
+    double* ddir = (double*)(base + in_bytes);
+    void* dout = base + 2 * in_bytes;
+    hipStream_t s = ctx->aux;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(dpos, pos3_host, in_bytes, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(ddir, dir3_host, in_bytes, hipMemcpyHostToDevice, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_trace_rays upload");
+    if ((e = bhs::launch_trace_T(dpos, ddir, n, max_steps, step_scale, esc_tol, r_esc_min, dout, s)) !=
+        hipSuccess)
+        return hip_fail(ctx, e, "bhs_trace_rays launch");
+    if ((e = hipMemcpyAsync(out_host, dout, out_bytes, hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_trace_rays download");
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(ctx, e, "bhs_trace_rays");
+    return BHS_OK;
+}
+
+int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
+               const bhs_ray_soa* out, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    int st;
+    if ((st = check_camera(ctx, cam)) != BHS_OK) return st;
+    if ((st = check_params(ctx, params)) != BHS_OK) return st;
+    if (!out) return fail(ctx, BHS_E_INVALID, "bhs_render: out is NULL%s");
+    if (params->root64 && (!out->n_cross_dev || !out->flags_dev || !out->disk_dev))
+        return fail(ctx, BHS_E_INVALID, "bhs_render: root64 needs n_cross, flags and disk outputs%s");
+    DeviceGuard g(ctx->device);
+    const int64_t n = (int64_t)cam->width * cam->height;
+    bhs::MarchConst c = make_const(params);
+    fill_camera(c, cam);
+    c.ray_offset = 0;
+    const bool roots = params->root64 && params->precision != BHS_PREC_F64;
+    bhs::Scratch sc;
+    if ((st = prepare_scratch(ctx, n, roots, &sc)) != BHS_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    bhs_ctx::TimingRec* tr = next_timer(ctx);
+    if (tr) (void)hipEventRecord(tr->call0, s);
+    if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_render clear");
+    if ((e = bhs::launch_render(c, make_soa(out, n), sc, s, tr ? tr->k0 : nullptr,
+                                tr ? tr->k1 : nullptr)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_render launch");
+    if (tr) (void)hipEventRecord(tr->call1, s);
+    ctx->last_stream = s;
+    return BHS_OK;
+}
+
+int bhs_march_rays(bhs_ctx* ctx, const float* pos3_dev, const float* dir3_dev, int64_t n,
+                   const bhs_march_params* params, const bhs_ray_soa* out, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    int st;
+    if ((st = check_params(ctx, params)) != BHS_OK) return st;
+    if (n < 0 || (n > 0 && (!pos3_dev || !dir3_dev || !out)))
+        return fail(ctx, BHS_E_INVALID, "bhs_march_rays: NULL buffer or n < 0%s");
+    DeviceGuard g(ctx->device);
+    bhs::Scratch sc;
+    if ((st = prepare_scratch(ctx, 0, false, &sc)) != BHS_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_march_rays clear");
+    ctx->last_stream = s;
+    if (n == 0) return BHS_OK;
+    bhs::MarchConst c = make_const(params);
+    bhs_ctx::TimingRec* tr = next_timer(ctx);
+    if (tr) (void)hipEventRecord(tr->call0, s);
+    if ((e = bhs::launch_march_rays(pos3_dev, dir3_dev, n, c, make_soa(out, n), sc, s,
+                                    tr ? tr->k0 : nullptr, tr ? tr->k1 : nullptr)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_march_rays launch");
+    if (tr) (void)hipEventRecord(tr->call1, s);
+    return BHS_OK;
+}
+
+int bhs_set_timing(bhs_ctx* ctx, int on) {
+    if (!ctx) return BHS_E_INVALID;
+    ctx->timing = on != 0;
+    ctx->timers_used = 0;
+    return BHS_OK;
+}
+
+int bhs_timing(bhs_ctx* ctx, double* march_ms_avg, double* total_ms_avg, int32_t* launches) {
+    if (!ctx) return BHS_E_INVALID;
+    DeviceGuard g(ctx->device);
+    double km = 0.0, tm = 0.0;
+    const size_t n = ctx->timers_used;
+    for (size_t i = 0; i < n; ++i) {
+        const bhs_ctx::TimingRec& r = ctx->timers[i];
+        float a = 0.0f, b = 0.0f;
+        hipError_t e;
+        if ((e = hipEventSynchronize(r.call1)) != hipSuccess) return hip_fail(ctx, e, "bhs_timing");
+        if ((e = hipEventElapsedTime(&a, r.k0, r.k1)) != hipSuccess ||
+            (e = hipEventElapsedTime(&b, r.call0, r.call1)) != hipSuccess)
+            return hip_fail(ctx, e, "bhs_timing elapsed");
+        km += a;
+        tm += b;
+    }
+    if (march_ms_avg) *march_ms_avg = n ? km / (double)n : 0.0;
+    if (total_ms_avg) *total_ms_avg = n ? tm / (double)n : 0.0;
+    if (launches) *launches = (int32_t)n;
+    ctx->timers_used = 0;
+    return BHS_OK;
+}
+
+int bhs_last_stats(bhs_ctx* ctx, bhs_stats* out_host) {
+    if (!ctx || !out_host) return BHS_E_INVALID;
+    std::memset(out_host, 0, sizeof *out_host);
+    if (!ctx->scratch) return BHS_OK;
+    DeviceGuard g(ctx->device);
+    hipError_t e = hipStreamSynchronize(ctx->last_stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "bhs_last_stats sync");
+    bhs::Stats s;
+    if ((e = hipMemcpy(&s, ctx->scratch, sizeof s, hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_last_stats copy");
+    out_host->rays = s.rays;
+    out_host->steps = s.steps;
+    out_host->crossings = s.crossings;
+    out_host->escapes = s.escapes;
+    out_host->band64 = s.band64;
+    out_host->fallback = s.fallback;
+    return BHS_OK;
+}
+
+}  // extern "C"

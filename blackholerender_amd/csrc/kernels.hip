@@ -1,0 +1,347 @@
+// kernels.hip — law-S render/march kernels (FP32 with FP64 band, FP64), FP64 crossing-root
+// refinement, and the law-T (bhs::physics::trace) kernel.
+#include "kernels.hpp"
+
+#include "../../include/bhs_trace.h"
+
+namespace bhs {
+
+namespace {
+
+constexpr int kTile = 16;       // 16x16 pixel tile per 256-thread block, 8x8 per wave
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ void store_ray(const SoA& o, int64_t i, const RayOut& r) {
+    const int64_t n = o.n;
+    if (o.cls) o.cls[i] = (uint8_t)r.cls;
+    if (o.flags) o.flags[i] = (uint8_t)r.flags;
+    if (o.n_cross) o.n_cross[i] = (uint8_t)r.n_cross;
+    if (o.steps) o.steps[i] = (uint16_t)(r.steps > 65535 ? 65535 : r.steps);
+    if (o.esc) {
+        o.esc[i] = r.esc[0];
+        o.esc[i + n] = r.esc[1];
+        o.esc[i + 2 * n] = r.esc[2];
+    }
+    if (o.disk) {
+        o.disk[i] = r.dr[0];
+        o.disk[i + n] = r.dr[1];
+        o.disk[i + 2 * n] = r.dphi[0];
+        o.disk[i + 3 * n] = r.dphi[1];
+        o.disk[i + 4 * n] = r.dg[0];
+        o.disk[i + 5 * n] = r.dg[1];
+    }
+    if (o.b) o.b[i] = r.b;
+}
+
+// Adds one lane's contribution to the launch counters: one atomic per wave and counter.
+__device__ __forceinline__ void add_stats(Stats* st, uint32_t ray, uint32_t steps, uint32_t cross,
+                                          uint32_t esc, uint32_t band) {
+    ray = wave_sum(ray);
+    steps = wave_sum(steps);
+    cross = wave_sum(cross);
+    esc = wave_sum(esc);
+    band = wave_sum(band);
+    if ((threadIdx.x & 63) == 0) {
+        if (ray) atomicAdd(&st->rays, (unsigned long long)ray);
+        if (steps) atomicAdd(&st->steps, (unsigned long long)steps);
+        if (cross) atomicAdd(&st->crossings, (unsigned long long)cross);
+        if (esc) atomicAdd(&st->escapes, (unsigned long long)esc);
+        if (band) atomicAdd(&st->band64, (unsigned long long)band);
+    }
+}
+
+// Pixel of thread t in block (bx, by): 16x16 tile, each wave an 8x8 sub-tile.
+__device__ __forceinline__ void tile_pixel(int& col, int& row) {
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    col = blockIdx.x * kTile + (w & 1) * 8 + (l & 7);
+    row = blockIdx.y * kTile + (w >> 1) * 8 + (l >> 3);
+}
+
+__device__ __forceinline__ bool in_band(const MarchConst& c, float h) {
+    return __builtin_fabsf(h - kBcritF) < c.band_abs;
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP32 law-S render of one camera. In BAND64 precision, rays in the FP64 band are appended to the
+// band list and left to k_band64.
+__global__ __launch_bounds__(kBlock) void k_render_f32(const MarchConst c, const SoA out, Scratch sc) {
+    int col, row;
+    tile_pixel(col, row);
+    const bool valid = col < c.width && row < c.height;
+    uint32_t steps = 0, cross = 0, esc = 0;
+    if (valid) {
+        const int64_t pix = (int64_t)row * c.width + col;
+        const V3<float> d = ray_dir(c, pix);
+        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+        const Setup<float> s = lawS_setup(P, d);
+        if (c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.h)) {
+            const uint32_t k = atomicAdd(sc.band_count, 1u);
+            if (k < sc.band_cap) sc.band_ids[k] = (uint32_t)pix;
+        } else {
+            RayOut o;
+            lawS_march(P, s, c, o, sc.roots ? sc.roots + 2 * pix : (RootState<float>*)nullptr);
+            store_ray(out, c.ray_offset + pix, o);
+            steps = (uint32_t)o.steps;
+            cross = o.n_cross;
+            esc = (o.flags & 2u) ? 1u : 0u;
+        }
+    }
+    add_stats(sc.stats, valid ? 1u : 0u, steps, cross, esc, 0u);
+}
+
+// FP64 law-S march of one ray, refined crossings included (as the oracle's double march).
+__device__ __forceinline__ RayOut march_f64(const MarchConst& c, const V3<float>& Pf,
+                                            const V3<float>& df) {
+    const V3<double> P{(double)Pf.x, (double)Pf.y, (double)Pf.z};
+    const V3<double> d{(double)df.x, (double)df.y, (double)df.z};
+    const Setup<double> s = lawS_setup(P, d);
+    RayOut o;
+    RootState<double> rs[2];
+    lawS_march(P, s, c, o, c.root64 ? rs : (RootState<double>*)nullptr);
+    if (c.root64 && o.n_cross > 0) {
+        const uint32_t m = o.n_cross < 2 ? o.n_cross : 2;
+        for (uint32_t k = 0; k < m; ++k) {
+            double hx, hz;
+            if (!root64(rs[k].x0, rs[k].v0, rs[k].dt, s.c3, hx, hz)) continue;
+            const double rh = __builtin_sqrt(__builtin_fma(hz, hz, hx * hx));
+            o.dr[k] = (float)rh;
+            o.dphi[k] = atan2_ieee((float)hz, (float)hx);
+            o.dg[k] = (float)redshift(rh, s.b_axis, s.lapse_g);
+        }
+        o.flags |= BHS_FLAG_ROOT64;
+    }
+    o.flags |= BHS_FLAG_BAND64;
+    return o;
+}
+
+// FP64 law-S render of every ray of one camera.
+__global__ __launch_bounds__(kBlock) void k_render_f64(const MarchConst c, const SoA out, Scratch sc) {
+    int col, row;
+    tile_pixel(col, row);
+    const bool valid = col < c.width && row < c.height;
+    uint32_t steps = 0, cross = 0, esc = 0;
+    if (valid) {
+        const int64_t pix = (int64_t)row * c.width + col;
+        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+        const RayOut o = march_f64(c, P, ray_dir(c, pix));
+        store_ray(out, c.ray_offset + pix, o);
+        steps = (uint32_t)o.steps;
+        cross = o.n_cross;
+        esc = (o.flags & 2u) ? 1u : 0u;
+    }
+    add_stats(sc.stats, valid ? 1u : 0u, steps, cross, esc, valid ? 1u : 0u);
+}
+
+// FP64 march of the band list (grid-stride; the count is read on the device).
+__global__ __launch_bounds__(64) void k_band64(const MarchConst c, const SoA out, Scratch sc) {
+    const uint32_t cnt = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
+    for (uint32_t base = blockIdx.x * 64; base < cnt; base += gridDim.x * 64) {
+        const uint32_t k = base + threadIdx.x;
+        uint32_t steps = 0, cross = 0, esc = 0, one = 0;
+        if (k < cnt) {
+            const int64_t pix = sc.band_ids[k];
+            const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+            const RayOut o = march_f64(c, P, ray_dir(c, pix));
+            store_ray(out, c.ray_offset + pix, o);
+            steps = (uint32_t)o.steps;
+            cross = o.n_cross;
+            esc = (o.flags & 2u) ? 1u : 0u;
+            one = 1;
+        }
+        add_stats(sc.stats, 0u, steps, cross, esc, one);
+    }
+}
+
+// FP64 root refinement of the crossings recorded by the FP32 march (rays [0, n) of one camera).
+__global__ __launch_bounds__(kBlock) void k_root64(const MarchConst c, const SoA out, Scratch sc,
+                                                   int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t g = c.ray_offset + i;
+    const uint32_t nc = out.n_cross[g];
+    const uint32_t fl = out.flags[g];
+    if (nc == 0 || (fl & BHS_FLAG_BAND64)) return;
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    // c3, b_axis and the redshift lapse of this ray, from its float setup.
+    const Setup<float> s = lawS_setup(P, ray_dir(c, i));
+    const uint32_t m = nc < 2 ? nc : 2;
+    for (uint32_t k = 0; k < m; ++k) {
+        const RootState<float>& q = sc.roots[2 * i + k];
+        const double X0[3] = {q.x0[0], q.x0[1], q.x0[2]};
+        const double V0[3] = {q.v0[0], q.v0[1], q.v0[2]};
+        double hx, hz;
+        if (!root64(X0, V0, (double)q.dt, (double)s.c3, hx, hz)) continue;
+        const double rh = __builtin_sqrt(__builtin_fma(hz, hz, hx * hx));
+        out.disk[g + k * out.n] = (float)rh;
+        out.disk[g + (2 + k) * out.n] = atan2_ieee((float)hz, (float)hx);
+        out.disk[g + (4 + k) * out.n] =
+            (float)redshift((double)rh, (double)s.b_axis, (double)s.lapse_g);
+    }
+    out.flags[g] = (uint8_t)(fl | BHS_FLAG_ROOT64);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Law S on explicit rays (float position and direction per ray).
+__global__ __launch_bounds__(kBlock) void k_march_rays(const float* __restrict__ pos3,
+                                                       const float* __restrict__ dir3, int64_t n,
+                                                       const MarchConst c, const SoA out, Scratch sc) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t steps = 0, cross = 0, esc = 0, band = 0;
+    if (valid) {
+        const V3<float> P{pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2]};
+        const V3<float> d{dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2]};
+        const Setup<float> s = lawS_setup(P, d);
+        RayOut o;
+        if (c.precision == BHS_PREC_F64 ||
+            (c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.h))) {
+            o = march_f64(c, P, d);
+            band = 1;
+        } else {
+            RootState<float> rs[2];
+            lawS_march(P, s, c, o, c.root64 ? rs : (RootState<float>*)nullptr);
+            if (c.root64 && o.n_cross > 0) {
+                const uint32_t m = o.n_cross < 2 ? o.n_cross : 2;
+                for (uint32_t k = 0; k < m; ++k) {
+                    const double X0[3] = {rs[k].x0[0], rs[k].x0[1], rs[k].x0[2]};
+                    const double V0[3] = {rs[k].v0[0], rs[k].v0[1], rs[k].v0[2]};
+                    double hx, hz;
+                    if (!root64(X0, V0, (double)rs[k].dt, (double)s.c3, hx, hz)) continue;
+                    const double rh = __builtin_sqrt(__builtin_fma(hz, hz, hx * hx));
+                    o.dr[k] = (float)rh;
+                    o.dphi[k] = atan2_ieee((float)hz, (float)hx);
+                    o.dg[k] = (float)redshift((double)rh, (double)s.b_axis, (double)s.lapse_g);
+                }
+                o.flags |= BHS_FLAG_ROOT64;
+            }
+        }
+        store_ray(out, i, o);
+        steps = (uint32_t)o.steps;
+        cross = o.n_cross;
+        esc = (o.flags & 2u) ? 1u : 0u;
+    }
+    add_stats(sc.stats, valid ? 1u : 0u, steps, cross, esc, band);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Law T: bhs::physics::trace (src/physics/Geodesic.h:107-140), FP64, one lane per ray.
+__global__ __launch_bounds__(64) void k_trace_T(const double* __restrict__ pos3,
+                                                const double* __restrict__ dir3, int64_t n,
+                                                int32_t max_steps, double step_scale, double esc_tol,
+                                                double r_esc_min, bhs_trace_result* __restrict__ out) {
+    using namespace lawT;
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const D3 cam{pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2]};
+    const D3 dir{dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2]};
+    bhs_trace_result res;
+    res.status = BHS_TRACE_MAXSTEPS;
+    res.steps = 0;
+    res.equator_crossings = 0;
+    res.pad_ = 0;
+    res.dir[0] = res.dir[1] = res.dir[2] = 0.0;
+    res.min_r = 1e30;
+    D3 x = cam;
+    D3 v = initial_velocity(cam, dir);
+    const D3 hv = gcross(x, v);
+    const double h2 = gdot(hv, hv);
+    const double h = __builtin_sqrt(h2);
+    double prev_y = x.y;
+    bool done = false;
+    for (int32_t k = 0; k < max_steps; ++k) {
+        const double r = glen(x);
+        if (r < res.min_r) res.min_r = r;
+        if (r < 2.0 + 0.01) {
+            res.status = BHS_TRACE_CAPTURED;
+            res.steps = k;
+            done = true;
+            break;
+        }
+        const double vlen = glen(v);
+        const double b = h / vlen;
+        if (gdot(x, v) > 0.0 && r > r_esc_min && 0.75 * b * b * b / (r * r * r * r) < esc_tol) {
+            res.status = BHS_TRACE_ESCAPED;
+            const D3 e = far_field(x, v, h2);
+            res.dir[0] = e.x; res.dir[1] = e.y; res.dir[2] = e.z;
+            res.steps = k;
+            done = true;
+            break;
+        }
+        const double ds0 = step_scale * r;
+        const double ds = ds0 > 1e-4 ? ds0 : 1e-4;
+        rk4(x, v, h2, ds / vlen);
+        if ((x.y > 0.0) != (prev_y > 0.0)) ++res.equator_crossings;
+        prev_y = x.y;
+        res.steps = k + 1;
+    }
+    if (!done) res.status = BHS_TRACE_MAXSTEPS;
+    out[i] = res;
+}
+
+__global__ void k_zero(uint32_t* p, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+}  // namespace
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+    const size_t words = bytes / 4;
+    const unsigned blocks = (unsigned)((words + 255) / 256 < 1024 ? (words + 255) / 256 : 1024);
+    hipLaunchKernelGGL(k_zero, dim3(blocks ? blocks : 1), dim3(256), 0, s, (uint32_t*)p, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_render(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s,
+                         hipEvent_t ev_begin, hipEvent_t ev_end) {
+    const dim3 grid((c.width + kTile - 1) / kTile, (c.height + kTile - 1) / kTile);
+    const int64_t n = (int64_t)c.width * c.height;
+    hipError_t e;
+    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
+    if (c.precision == BHS_PREC_F64)
+        hipLaunchKernelGGL(k_render_f64, grid, dim3(kBlock), 0, s, c, out, sc);
+    else
+        hipLaunchKernelGGL(k_render_f32, grid, dim3(kBlock), 0, s, c, out, sc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
+    if (c.precision == BHS_PREC_F64) return hipSuccess;
+    if (c.precision == BHS_PREC_F32_BAND64) {
+        hipLaunchKernelGGL(k_band64, dim3(256), dim3(64), 0, s, c, out, sc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (c.root64 && sc.roots) {
+        hipLaunchKernelGGL(k_root64, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           c, out, sc, n);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, const MarchConst& c,
+                             const SoA& out, const Scratch& sc, hipStream_t s, hipEvent_t ev_begin,
+                             hipEvent_t ev_end) {
+    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    hipError_t e;
+    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_march_rays, dim3(blocks), dim3(kBlock), 0, s, pos3, dir3, n, c, out, sc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+hipError_t launch_trace_T(const double* pos3, const double* dir3, int64_t n, int32_t max_steps,
+                          double step_scale, double esc_tol, double r_esc_min, void* out,
+                          hipStream_t s) {
+    const unsigned blocks = (unsigned)((n + 63) / 64);
+    hipLaunchKernelGGL(k_trace_T, dim3(blocks), dim3(64), 0, s, pos3, dir3, n, max_steps, step_scale,
+                       esc_tol, r_esc_min, (bhs_trace_result*)out);
+    return hipGetLastError();
+}
+
+}  // namespace bhs

@@ -1,0 +1,48 @@
+// kernels.hpp — kernel entry points of the tracer library and their launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lawS_device.hpp"
+
+namespace bhs {
+
+// Output planes (device pointers, any may be null) — bhs_ray_soa of include/bhs_trace.h.
+struct SoA {
+    uint8_t* cls;
+    uint8_t* flags;
+    uint8_t* n_cross;
+    uint16_t* steps;
+    float* esc;    // 3 planes of n
+    float* disk;   // 6 planes of n: r0 r1 phi0 phi1 g0 g1
+    float* b;
+    int64_t n;     // plane stride
+};
+
+// Device counters of one launch (layout of bhs_stats).
+struct Stats {
+    unsigned long long rays, steps, crossings, escapes, band64, fallback, pad0, pad1;
+};
+
+// Scratch used by one render launch.
+struct Scratch {
+    Stats* stats;
+    uint32_t* band_count;        // number of rays in the FP64 band list
+    uint32_t* band_ids;          // ray index (camera-local pixel for single-camera launches)
+    uint32_t band_cap;
+    RootState<float>* roots;     // 2 per ray when root refinement is on, else null
+};
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+// ev_begin/ev_end (may be null) are recorded on `s` around the main march kernel.
+hipError_t launch_render(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s,
+                         hipEvent_t ev_begin, hipEvent_t ev_end);
+hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, const MarchConst& c,
+                             const SoA& out, const Scratch& sc, hipStream_t s, hipEvent_t ev_begin,
+                             hipEvent_t ev_end);
+hipError_t launch_trace_T(const double* pos3, const double* dir3, int64_t n, int32_t max_steps,
+                          double step_scale, double esc_tol, double r_esc_min, void* out,
+                          hipStream_t s);
+
+}  // namespace bhs

@@ -1,0 +1,436 @@
+// lawS_device.hpp — device implementation of the law-S march (shaders/blackhole.frag:67-253,
+// thin-disk variant, lensing on, far mode off, no bodies/gas, disk opacity 0), of the ray generation
+// (blackhole.frag:68-72) and of law T (src/physics/Geodesic.h:107-140).
+//
+// Arithmetic contract (the whole library is compiled with -ffp-contract=off and correctly rounded
+// FP32 division/square root): every multiply-add below that is fused is written as an explicit
+// fma, division and square root are IEEE, nothing is reassociated. The operation order is fixed:
+//   dot(a,b) = fma(a.z,b.z, fma(a.y,b.y, a.x*b.x)),  normalize(a) = a * (1/sqrt(dot(a,a))),
+//   cross(a,b).x = fma(a.y,b.z, -(a.z*b.y)) (cyclic),
+// so one ray's result does not depend on the GPU, the launch shape or the order rays are processed
+// in, and a CPU implementation of the same order reproduces it bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bhs {
+
+template <typename R> struct Ar;
+template <> struct Ar<float> {
+    static __device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+    static __device__ __forceinline__ float sqrt(float a) { return __builtin_sqrtf(a); }
+    static __device__ __forceinline__ float abs(float a) { return __builtin_fabsf(a); }
+};
+template <> struct Ar<double> {
+    static __device__ __forceinline__ double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+    static __device__ __forceinline__ double sqrt(double a) { return __builtin_sqrt(a); }
+    static __device__ __forceinline__ double abs(double a) { return __builtin_fabs(a); }
+};
+
+template <typename R> struct V3 { R x, y, z; };
+
+template <typename R>
+__device__ __forceinline__ R dot3(const V3<R>& a, const V3<R>& b) {
+    return Ar<R>::fma(a.z, b.z, Ar<R>::fma(a.y, b.y, a.x * b.x));
+}
+
+constexpr float kBcritF = 5.19615242270663f;
+
+// Per-camera march constants, derived on the host from bhs_camera + bhs_march_params.
+struct MarchConst {
+    float pos[3];
+    float basis[9];
+    float tan_half_fov;
+    int32_t width, height;
+    float jitter[2];
+    int32_t jitter_hashed;
+    uint32_t jitter_base;
+    int32_t max_steps;
+    int32_t thin;
+    float step_scale, disk_step, disk_inner, disk_outer, disk_thickness;
+    float band_abs;          // |h - b_crit| < band_abs -> FP64 march (0 disables)
+    int32_t precision;       // BHS_PREC_*
+    int32_t root64;
+    int64_t ray_offset;      // first output index of this camera (batched launches)
+};
+
+// Law-S outputs of one ray, in registers.
+struct RayOut {
+    float esc[3];
+    float dr[2], dphi[2], dg[2];
+    float b;
+    int32_t steps;
+    uint32_t cls, flags, n_cross;
+};
+
+// ---------------------------------------------------------------------------------- hashing
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {  // shaders/common.glsl:28-32
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+// ---------------------------------------------------------------------------------- atan2
+// IEEE-only atan2: t = min/max; t > tan(pi/8) -> pi/4 + atan((t-1)/(t+1)); atan u = u + u^3 P(u^2)
+// with a degree-4 least-squares P (error 1.8e-9 before rounding); quadrant fix-up of atan2.
+__device__ __forceinline__ float atan2_ieee(float y, float x) {
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    const float mx = ax > ay ? ax : ay;
+    const float mn = ax > ay ? ay : ax;
+    if (mx == 0.0f) return 0.0f;
+    float t = mn / mx;
+    float base = 0.0f;
+    if (t > 0.41421356f) {
+        t = (t - 1.0f) / (t + 1.0f);
+        base = 0.78539816f;
+    }
+    const float s = t * t;
+    float p = -0.065067887f;
+    p = __builtin_fmaf(p, s, 0.10763717f);
+    p = __builtin_fmaf(p, s, -0.14266309f);
+    p = __builtin_fmaf(p, s, 0.19999637f);
+    p = __builtin_fmaf(p, s, -0.33333331f);
+    float a = base + __builtin_fmaf(t * s, p, t);
+    if (ay > ax) a = 1.5707964f - a;
+    if (x < 0.0f) a = 3.1415927f - a;
+    if (y < 0.0f) a = -a;
+    return a;
+}
+
+// ---------------------------------------------------------------------------------- ray generation
+// blackhole.frag:68-72: px = fragCoord + jitter; ndc = px/res*2-1; dirCam = normalize(ndc.x tanH
+// aspect, ndc.y tanH, -1); dir = normalize(B dirCam). Row 0 is the bottom row. In hashed mode the
+// per-pixel u in [0,1)^2 (pcg keyed by seed, camera, sample, pixel) replaces the pixel centre.
+__device__ __forceinline__ V3<float> ray_dir(const MarchConst& c, int64_t pixel) {
+    const int32_t col = (int32_t)(pixel % c.width), row = (int32_t)(pixel / c.width);
+    float px, py;
+    if (c.jitter_hashed) {
+        const uint32_t pi2 = 2u * (uint32_t)pixel;
+        px = (float)col + (float)(pcg(c.jitter_base + pi2) >> 8) * 0x1p-24f;
+        py = (float)row + (float)(pcg(c.jitter_base + pi2 + 1u) >> 8) * 0x1p-24f;
+    } else {
+        px = ((float)col + 0.5f) + c.jitter[0];
+        py = ((float)row + 0.5f) + c.jitter[1];
+    }
+    const float wf = (float)c.width, hf = (float)c.height;
+    const float ndx = __builtin_fmaf(px / wf, 2.0f, -1.0f);
+    const float ndy = __builtin_fmaf(py / hf, 2.0f, -1.0f);
+    const float aspect = wf / hf;
+    V3<float> d0{(ndx * c.tan_half_fov) * aspect, ndy * c.tan_half_fov, -1.0f};
+    const float i0 = 1.0f / __builtin_sqrtf(dot3(d0, d0));
+    d0.x *= i0; d0.y *= i0; d0.z *= i0;
+    const float* B = c.basis;
+    V3<float> d{__builtin_fmaf(B[6], d0.z, __builtin_fmaf(B[3], d0.y, B[0] * d0.x)),
+                __builtin_fmaf(B[7], d0.z, __builtin_fmaf(B[4], d0.y, B[1] * d0.x)),
+                __builtin_fmaf(B[8], d0.z, __builtin_fmaf(B[5], d0.y, B[2] * d0.x))};
+    const float i1 = 1.0f / __builtin_sqrtf(dot3(d, d));
+    d.x *= i1; d.y *= i1; d.z *= i1;
+    return d;
+}
+
+// ---------------------------------------------------------------------------------- law S setup
+template <typename R> struct Setup {
+    V3<R> v;
+    R h2, h, b_axis, lapse_g, far_mul, c3;
+};
+
+// blackhole.frag:77-93 with initialVelocity (geodesic.glsl:27-34) and impactParameterAboutAxis
+// (geodesic.glsl:37-40, axis +y).
+template <typename R>
+__device__ __forceinline__ Setup<R> lawS_setup(const V3<R>& P, const V3<R>& d) {
+    using A = Ar<R>;
+    Setup<R> s;
+    const R rcam = A::sqrt(dot3(P, P));
+    const R inv_r = R(1) / rcam;
+    const V3<R> rh{P.x * inv_r, P.y * inv_r, P.z * inv_r};
+    const R nr = dot3(d, rh);
+    const V3<R> np{A::fma(-nr, rh.x, d.x), A::fma(-nr, rh.y, d.y), A::fma(-nr, rh.z, d.z)};
+    const R lapse2 = R(1) - R(2) / rcam;
+    const R f = A::sqrt(lapse2 > R(0) ? lapse2 : R(0));
+    const R nrf = nr * f;
+    V3<R> v{A::fma(nrf, rh.x, np.x), A::fma(nrf, rh.y, np.y), A::fma(nrf, rh.z, np.z)};
+    const R iv = R(1) / A::sqrt(dot3(v, v));
+    v.x *= iv; v.y *= iv; v.z *= iv;
+    s.v = v;
+    const V3<R> c{A::fma(P.y, v.z, -(P.z * v.y)), A::fma(P.z, v.x, -(P.x * v.z)),
+                  A::fma(P.x, v.y, -(P.y * v.x))};
+    s.h2 = dot3(c, c);
+    s.h = A::sqrt(s.h2);
+    const R cdy = A::fma(P.z, d.x, -(P.x * d.z));
+    const R e6 = R(1e-6), e4 = R(1e-4);
+    s.b_axis = -cdy / A::sqrt(lapse2 > e6 ? lapse2 : e6);
+    s.lapse_g = A::sqrt(lapse2 > e4 ? lapse2 : e4);
+    const R fm = (s.h - R(6)) / R(3);
+    s.far_mul = fm < R(1) ? R(1) : (fm > R(3) ? R(3) : fm);
+    s.c3 = R(-3) * s.h2;
+    return s;
+}
+
+// ---------------------------------------------------------------------------------- RK4
+// rk4Step (geodesic.glsl:16-23) with geoAccel (:10-14): a = x * (c3 / (r2 r2 r)), c3 = -3 h2.
+// r2, r of x come from the top of the march iteration and serve the first stage.
+template <typename R>
+__device__ __forceinline__ void rk4(V3<R>& x, V3<R>& v, R c3, R dt, R r2, R r) {
+    using A = Ar<R>;
+    const R hdt = R(0.5) * dt;
+    const R s1 = c3 / ((r2 * r2) * r);
+    const V3<R> a1{x.x * s1, x.y * s1, x.z * s1};
+    const V3<R> x2{A::fma(hdt, v.x, x.x), A::fma(hdt, v.y, x.y), A::fma(hdt, v.z, x.z)};
+    const V3<R> v2{A::fma(hdt, a1.x, v.x), A::fma(hdt, a1.y, v.y), A::fma(hdt, a1.z, v.z)};
+    const R q2 = dot3(x2, x2);
+    const R s2 = c3 / ((q2 * q2) * A::sqrt(q2));
+    const V3<R> a2{x2.x * s2, x2.y * s2, x2.z * s2};
+    const V3<R> x3{A::fma(hdt, v2.x, x.x), A::fma(hdt, v2.y, x.y), A::fma(hdt, v2.z, x.z)};
+    const V3<R> v3{A::fma(hdt, a2.x, v.x), A::fma(hdt, a2.y, v.y), A::fma(hdt, a2.z, v.z)};
+    const R q3 = dot3(x3, x3);
+    const R s3 = c3 / ((q3 * q3) * A::sqrt(q3));
+    const V3<R> a3{x3.x * s3, x3.y * s3, x3.z * s3};
+    const V3<R> x4{A::fma(dt, v3.x, x.x), A::fma(dt, v3.y, x.y), A::fma(dt, v3.z, x.z)};
+    const V3<R> v4{A::fma(dt, a3.x, v.x), A::fma(dt, a3.y, v.y), A::fma(dt, a3.z, v.z)};
+    const R q4 = dot3(x4, x4);
+    const R s4 = c3 / ((q4 * q4) * A::sqrt(q4));
+    const V3<R> a4{x4.x * s4, x4.y * s4, x4.z * s4};
+    const R dt6 = dt * R(0.16666666666666666);
+    x.x = A::fma(dt6, A::fma(R(2), v3.x, A::fma(R(2), v2.x, v.x)) + v4.x, x.x);
+    x.y = A::fma(dt6, A::fma(R(2), v3.y, A::fma(R(2), v2.y, v.y)) + v4.y, x.y);
+    x.z = A::fma(dt6, A::fma(R(2), v3.z, A::fma(R(2), v2.z, v.z)) + v4.z, x.z);
+    v.x = A::fma(dt6, A::fma(R(2), a3.x, A::fma(R(2), a2.x, a1.x)) + a4.x, v.x);
+    v.y = A::fma(dt6, A::fma(R(2), a3.y, A::fma(R(2), a2.y, a1.y)) + a4.y, v.y);
+    v.z = A::fma(dt6, A::fma(R(2), a3.z, A::fma(R(2), a2.z, a1.z)) + a4.z, v.z);
+}
+
+// farFieldDirection (geodesic.glsl:43-53); r = |x|, iv = 1/|v|.
+template <typename R>
+__device__ __forceinline__ V3<R> far_field(const V3<R>& x, const V3<R>& v, R r, R iv, R h2) {
+    using A = Ar<R>;
+    const V3<R> vh{v.x * iv, v.y * iv, v.z * iv};
+    const R p = dot3(x, vh);
+    const V3<R> bv{A::fma(-p, vh.x, x.x), A::fma(-p, vh.y, x.y), A::fma(-p, vh.z, x.z)};
+    R s = p / r;
+    s = s < R(0) ? R(0) : (s > R(1) ? R(1) : s);
+    const R r3 = (r * r) * r;
+    const R a = h2 * iv;
+    const R ir3 = R(1) / r3;
+    const R op = R(1) + s;
+    const R kk = (R(2) + s) / (((r3 * r) * op) * op);
+    V3<R> e{v.x + (-(a * A::fma(bv.x, kk, vh.x * ir3))), v.y + (-(a * A::fma(bv.y, kk, vh.y * ir3))),
+            v.z + (-(a * A::fma(bv.z, kk, vh.z * ir3)))};
+    const R ie = R(1) / A::sqrt(dot3(e, e));
+    e.x *= ie; e.y *= ie; e.z *= ie;
+    return e;
+}
+
+// redshiftFactor (disk.glsl:37-42).
+template <typename R>
+__device__ __forceinline__ R redshift(R r, R b_axis, R lapse) {
+    using A = Ar<R>;
+    const R omega = R(1) / (r * A::sqrt(r));
+    R t = R(1) - R(3) / r;
+    if (t < R(0.02)) t = R(0.02);
+    const R ut = R(1) / A::sqrt(t);
+    R m = A::fma(-omega, b_axis, R(1));
+    if (m < R(0.05)) m = R(0.05);
+    return R(1) / ((lapse * ut) * m);
+}
+
+// Crossing state saved for FP64 root refinement: x0, v0 and dt of the crossing step, in the
+// precision of the march that produced it.
+template <typename R> struct RootState { R x0[3], v0[3], dt; };
+
+// ---------------------------------------------------------------------------------- law S march
+// blackhole.frag:142-253 (thin disk). `rs` (may be null) receives the state of the steps that
+// produced crossing slots 0 and 1.
+template <typename R>
+__device__ __forceinline__ void lawS_march(const V3<R>& P, const Setup<R>& s, const MarchConst& c,
+                                           RayOut& o, RootState<R>* rs) {
+    using A = Ar<R>;
+    const R ss = (R)c.step_scale;
+    const R dstep = (R)c.disk_step;
+    const R inner = (R)c.disk_inner, outer = (R)c.disk_outer;
+    const R slab_k = R(2.5) * (R)c.disk_thickness;
+    const R lo = inner - R(1), hi = outer + R(1);
+    const R lo2 = lo > R(0) ? lo * lo : R(-1);
+    const R hi2 = hi * hi;
+    const R er = outer * R(1.25);
+    const R r_esc_min = er > R(12) ? er : R(12);
+    const bool thin = c.thin != 0;
+
+    V3<R> x = P, v = s.v;
+    int32_t status = 0, steps = 0;
+    uint32_t n_cross = 0;
+    o.esc[0] = o.esc[1] = o.esc[2] = 0.0f;
+    o.dr[0] = o.dr[1] = o.dphi[0] = o.dphi[1] = o.dg[0] = o.dg[1] = 0.0f;
+
+    for (int32_t i = 0; i < c.max_steps; ++i) {
+        const R r2 = dot3(x, x);
+        const R r = A::sqrt(r2);
+        if (r < R(2.02)) { status = 2; break; }                              // :144
+        const R iv = R(1) / A::sqrt(dot3(v, v));
+        if (dot3(x, v) > R(0) && r > r_esc_min) {                            // :151-155
+            const R b = s.h * iv;
+            const R q = (((R(0.75) * b) * b) * b) / (r2 * r2);
+            if (q < R(0.01)) {
+                const V3<R> e = far_field(x, v, r, iv, s.h2);
+                o.esc[0] = (float)e.x; o.esc[1] = (float)e.y; o.esc[2] = (float)e.z;
+                status = 1;
+                break;
+            }
+        }
+        R ds = (ss * r) * s.far_mul;                                          // :169
+        if (ds < R(0.02)) ds = R(0.02);
+        if (thin) {                                                           // :170-181
+            const R rc2 = A::fma(x.z, x.z, x.x * x.x);
+            if (rc2 > lo2 && rc2 < hi2) {
+                const R rcyl = A::sqrt(rc2);
+                const R outside = A::fma(-slab_k, rcyl, A::abs(x.y));
+                if (outside > R(0)) {
+                    const R den = A::fma(A::abs(v.y), iv, slab_k);
+                    R cl = outside / den;
+                    if (cl < dstep) cl = dstep;
+                    if (cl < ds) ds = cl;
+                }
+            }
+        }
+        const R dt = ds * iv;                                                 // :192
+        const V3<R> x0 = x;
+        const V3<R> v0 = v;
+        rk4(x, v, s.c3, dt, r2, r);                                           // :194
+        steps = i + 1;
+        if (thin && ((x.y > R(0)) != (x0.y > R(0)))) {                        // :209-216
+            const R f = x0.y / (x0.y - x.y);
+            const R hx = A::fma(f, x.x - x0.x, x0.x);
+            const R hz = A::fma(f, x.z - x0.z, x0.z);
+            const R rh = A::sqrt(A::fma(hz, hz, hx * hx));
+            if (rh >= inner && rh <= outer) {                                 // disk.glsl:114-115
+                if (n_cross < 2) {
+                    o.dr[n_cross] = (float)rh;
+                    o.dphi[n_cross] = atan2_ieee((float)hz, (float)hx);
+                    o.dg[n_cross] = (float)redshift(rh, s.b_axis, s.lapse_g);
+                    if (rs) {
+                        RootState<R>& q = rs[n_cross];
+                        q.x0[0] = x0.x; q.x0[1] = x0.y; q.x0[2] = x0.z;
+                        q.v0[0] = v0.x; q.v0[1] = v0.y; q.v0[2] = v0.z;
+                        q.dt = dt;
+                    }
+                }
+                if (n_cross < 255u) ++n_cross;
+            }
+        }
+    }
+    uint32_t flags = 0;
+    if (status == 0) { status = 2; flags |= 1u; }                            // :252
+    if (status == 1) flags |= 2u;
+    o.flags = flags;
+    o.steps = steps;
+    o.n_cross = n_cross;
+    o.cls = n_cross > 0 ? 1u : (status == 1 ? 2u : 0u);
+    o.b = (float)s.h;
+}
+
+// ---------------------------------------------------------------------------------- FP64 root
+// Root bracketing of y(tau) = rk4(x0, v0, tau).y on [0, dt] (Illinois variant of regula falsi,
+// stop when |b - a| <= 1e-14 dt, at most 60 evaluations). Returns false when the double step does
+// not bracket a sign change (the chord result then stands).
+__device__ __forceinline__ bool root64(const double X0[3], const double V0[3], double dt, double c3,
+                                       double& hx, double& hz) {
+    double a = 0.0, fa = X0[1];
+    double b = dt, fb;
+    {
+        V3<double> x{X0[0], X0[1], X0[2]}, v{V0[0], V0[1], V0[2]};
+        const double q = dot3(x, x);
+        rk4(x, v, c3, b, q, __builtin_sqrt(q));
+        fb = x.y;
+    }
+    if (fa == 0.0) { hx = X0[0]; hz = X0[2]; return true; }
+    if ((fa > 0.0) == (fb > 0.0)) return false;
+    double cx = 0.0, cz = 0.0;
+    int side = 0;
+    for (int it = 0; it < 60; ++it) {
+        const double c = (a * fb - b * fa) / (fb - fa);
+        V3<double> x{X0[0], X0[1], X0[2]}, v{V0[0], V0[1], V0[2]};
+        const double q = dot3(x, x);
+        rk4(x, v, c3, c, q, __builtin_sqrt(q));
+        const double fc = x.y;
+        cx = x.x; cz = x.z;
+        if (fc == 0.0) break;
+        if ((fc > 0.0) == (fb > 0.0)) {
+            b = c; fb = fc;
+            if (side == -1) fa = 0.5 * fa;
+            side = -1;
+        } else {
+            a = c; fa = fc;
+            if (side == 1) fb = 0.5 * fb;
+            side = 1;
+        }
+        if (__builtin_fabs(b - a) <= 1e-14 * dt) break;
+    }
+    hx = cx;
+    hz = cz;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------- law T
+// src/physics/Geodesic.h in GLM's operation order (no fused multiply-add): dot = (ax bx + ay by) +
+// az bz, normalize = v * (1/sqrt(dot)), vec/scalar divides each component.
+namespace lawT {
+struct D3 { double x, y, z; };
+__device__ __forceinline__ double gdot(const D3& a, const D3& b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ double glen(const D3& a) { return __builtin_sqrt(gdot(a, a)); }
+__device__ __forceinline__ D3 gcross(const D3& x, const D3& y) {
+    return D3{x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
+}
+__device__ __forceinline__ D3 gnorm(const D3& v) {
+    const double is = 1.0 / __builtin_sqrt(gdot(v, v));
+    return D3{v.x * is, v.y * is, v.z * is};
+}
+__device__ __forceinline__ D3 accel(const D3& x, double h2) {   // Geodesic.h:21-25
+    const double r2 = gdot(x, x);
+    const double r = __builtin_sqrt(r2);
+    const double s = -3.0 * h2 / (r2 * r2 * r);
+    return D3{x.x * s, x.y * s, x.z * s};
+}
+__device__ __forceinline__ void rk4(D3& x, D3& v, double h2, double dt) {   // Geodesic.h:27-34
+    const double hd = 0.5 * dt;
+    const D3 k1v = accel(x, h2), k1x = v;
+    const D3 k2v = accel(D3{x.x + hd * k1x.x, x.y + hd * k1x.y, x.z + hd * k1x.z}, h2);
+    const D3 k2x{v.x + hd * k1v.x, v.y + hd * k1v.y, v.z + hd * k1v.z};
+    const D3 k3v = accel(D3{x.x + hd * k2x.x, x.y + hd * k2x.y, x.z + hd * k2x.z}, h2);
+    const D3 k3x{v.x + hd * k2v.x, v.y + hd * k2v.y, v.z + hd * k2v.z};
+    const D3 k4v = accel(D3{x.x + dt * k3x.x, x.y + dt * k3x.y, x.z + dt * k3x.z}, h2);
+    const D3 k4x{v.x + dt * k3v.x, v.y + dt * k3v.y, v.z + dt * k3v.z};
+    const double d6 = dt / 6.0;
+    x.x += d6 * (((k1x.x + 2.0 * k2x.x) + 2.0 * k3x.x) + k4x.x);
+    x.y += d6 * (((k1x.y + 2.0 * k2x.y) + 2.0 * k3x.y) + k4x.y);
+    x.z += d6 * (((k1x.z + 2.0 * k2x.z) + 2.0 * k3x.z) + k4x.z);
+    v.x += d6 * (((k1v.x + 2.0 * k2v.x) + 2.0 * k3v.x) + k4v.x);
+    v.y += d6 * (((k1v.y + 2.0 * k2v.y) + 2.0 * k3v.y) + k4v.y);
+    v.z += d6 * (((k1v.z + 2.0 * k2v.z) + 2.0 * k3v.z) + k4v.z);
+}
+__device__ __forceinline__ D3 initial_velocity(const D3& cam, const D3& dir) {   // :40-47
+    const double r = glen(cam);
+    const D3 rh{cam.x / r, cam.y / r, cam.z / r};
+    const double nr = gdot(dir, rh);
+    const D3 np{dir.x - nr * rh.x, dir.y - nr * rh.y, dir.z - nr * rh.z};
+    const double t = 1.0 - 2.0 / r;
+    const double f = __builtin_sqrt(t > 0.0 ? t : 0.0);
+    const double nf = nr * f;
+    return gnorm(D3{nf * rh.x + np.x, nf * rh.y + np.y, nf * rh.z + np.z});
+}
+__device__ __forceinline__ D3 far_field(const D3& x, const D3& v, double h2) {   // :65-75
+    const double vlen = glen(v);
+    const D3 vh{v.x / vlen, v.y / vlen, v.z / vlen};
+    const double r = glen(x);
+    const double p = gdot(x, vh);
+    const D3 bv{x.x - p * vh.x, x.y - p * vh.y, x.z - p * vh.z};
+    double s = p / r;
+    s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
+    const double r3 = r * r * r;
+    const double a = -(h2 / vlen);
+    const double kk = (2.0 + s) / (r3 * r * (1.0 + s) * (1.0 + s));
+    return gnorm(D3{v.x + a * (vh.x / r3 + bv.x * kk), v.y + a * (vh.y / r3 + bv.y * kk),
+                    v.z + a * (vh.z / r3 + bv.z * kk)});
+}
+}  // namespace lawT
+
+}  // namespace bhs

@@ -1,0 +1,66 @@
+"""Named workloads and the algorithmic work model of the law-S march.
+
+Work model (source-level count of lawS_device.hpp; fused multiply-add = 2 flops, add/sub/mul/div/
+sqrt = 1, comparisons/abs/min/max/select = 0; only always-executed work is counted, so the figure is
+a lower bound):
+  per ray            113  ray generation (47) + setup (65) + band test (1)
+  per iteration      145  |x| 6, 1/|v| 7, x.v 5, ds 2, rcyl^2 3, dt 1, RK4 121
+                          (RK4: 4 accelerations 4 x (dot 5 + sqrt 1 + 2 mul + div 1 + 3 mul),
+                          minus the first dot/sqrt reused from the iteration head, 12 fma-pairs for
+                          the stages, dt/6, 2 x 3 x 7 for the x and v updates)
+  per escape          50  far-field direction (geodesic.glsl:43-53)
+  per crossing        12  chord + |hit.xz|; +29 for each of the first two (atan2 17, redshift 12)
+Branch-dependent extras (slab clamp 3-6, escape test 6 per outgoing iteration past rEscMin) are not
+counted.
+"""
+from __future__ import annotations
+
+import math
+
+FLOPS_PER_RAY = 113
+FLOPS_PER_STEP = 145
+FLOPS_PER_ESCAPE = 50
+FLOPS_PER_CROSSING = 12
+FLOPS_PER_RECORDED_CROSSING = 29
+
+# MI355X published peaks.
+PEAK_FP32_TFLOPS = 157.3       # vector FP32 (the non-packed VALU rate is half of it)
+PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0      # dense MFMA
+PEAK_HBM_GBS = 8000.0
+
+
+def march_flops(rays: int, steps: int, escapes: int, crossings: int, recorded_crossings: int) -> int:
+    return (FLOPS_PER_RAY * rays + FLOPS_PER_STEP * steps + FLOPS_PER_ESCAPE * escapes +
+            FLOPS_PER_CROSSING * crossings + FLOPS_PER_RECORDED_CROSSING * recorded_crossings)
+
+
+# The reference's CPU-runnable camera and the 1080p benchmark camera: r_obs = 30 M, inclination
+# 85 deg, vertical FOV 60 deg, High preset (450 steps, step scale 0.05, disk step 0.18), thin disk
+# [6, 20] M with thickness 0.035.
+C1 = dict(name="c1_256x256", r_obs=30.0, inclination_deg=85.0, fov_deg=60.0, width=256, height=256,
+          max_steps=450, disk_inner=6.0, disk_outer=20.0, jitter_hashed=0, seed=0)
+C2 = dict(name="c2_1920x1080", r_obs=30.0, inclination_deg=85.0, fov_deg=60.0, width=1920,
+          height=1080, max_steps=450, disk_inner=6.0, disk_outer=20.0, jitter_hashed=1,
+          seed=20260601)
+
+
+def camera_for(cfg, **over):
+    from .camera import orbit_camera
+    c = dict(cfg)
+    c.update(over)
+    return orbit_camera(c["r_obs"], c["inclination_deg"], c["fov_deg"], c["width"], c["height"],
+                        jitter_hashed=c["jitter_hashed"], seed=c["seed"],
+                        camera_id=c.get("camera_id", 0), sample=c.get("sample", 0))
+
+
+def params_for(cfg, precision=None, root64=0):
+    from . import PREC_F32_BAND64, march_params
+    return march_params(max_steps=cfg["max_steps"], disk_inner=cfg["disk_inner"],
+                        disk_outer=cfg["disk_outer"],
+                        precision=PREC_F32_BAND64 if precision is None else precision,
+                        root64=root64)
+
+
+def b_crit() -> float:
+    return 3.0 * math.sqrt(3.0)

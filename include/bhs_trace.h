@@ -1,0 +1,174 @@
+/*
+ * bhs_trace.h — C-ABI of the MI355X Schwarzschild null-geodesic tracer.
+ *
+ * Plain C, plain pointers and sizes: callable from C, C++ (include/bhs/physics/Geodesic.h
+ * builds the reference's bhs::physics API on it), ctypes, or any other FFI.
+ *
+ * Units: G = c = M = 1 (horizon r = 2, photon sphere r = 3, b_crit = 3*sqrt(3)).
+ *
+ * Conventions
+ *  - Every entry point returns an int status: BHS_OK (0) or a negative bhs_status. No exception
+ *    or abort crosses the ABI; bhs_last_error(ctx) holds the text of the last failure.
+ *  - Device buffers are caller-owned and named *_dev. Host buffers are named *_host or are plain
+ *    struct arguments. The context owns its streams, scratch buffers, graphs and MLP weights.
+ *  - bhs_trace_rays is synchronous (like the reference's trace()). Everything taking a `stream`
+ *    argument is stream-ordered and asynchronous; `stream` is a hipStream_t (NULL = the null stream).
+ *  - A context may be used by one host thread at a time.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *  - bhs_trace_rays           : bhs::physics::trace, src/physics/Geodesic.h:107-140 (law T),
+ *                               batched; TraceResult is src/physics/Geodesic.h:97-103.
+ *  - bhs_render / bhs_render_batch :
+ *                               the trace pass Renderer::tracePass -> setTraceUniforms ->
+ *                               drawFullscreen (src/Renderer.cpp:369-387, 242-356, 154-157) running
+ *                               shaders/blackhole.frag:67-253 with DISK_THIN, lensing on, far mode off,
+ *                               no bodies, no gas, disk opacity 0 (law S). The GLSL colour output is
+ *                               replaced by per-ray geometry in structure-of-arrays form.
+ *  - bhs_march_rays           : the same law-S march (blackhole.frag:77-253) on explicit rays.
+ *  - bhs_mlp_*, bhs_render_surrogate : no counterpart in the reference (learned surrogate of the
+ *                               march, falling back to it near b_crit).
+ */
+#ifndef BHS_TRACE_H
+#define BHS_TRACE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BHS_ABI_VERSION 1
+
+typedef enum bhs_status {
+    BHS_OK = 0,
+    BHS_E_INVALID = -1,   /* bad argument (NULL, size, range) */
+    BHS_E_DEVICE = -2,    /* no such device / HIP runtime error */
+    BHS_E_ALLOC = -3,     /* device or host allocation failed */
+    BHS_E_LAUNCH = -4,    /* kernel launch or execution failed */
+    BHS_E_IO = -5,        /* file read/write failed */
+    BHS_E_STATE = -6      /* call not valid in the current state (e.g. no MLP weights) */
+} bhs_status;
+
+/* TraceResult::Status (src/physics/Geodesic.h:98), same numeric order. */
+enum { BHS_TRACE_ESCAPED = 0, BHS_TRACE_CAPTURED = 1, BHS_TRACE_MAXSTEPS = 2 };
+
+/* Per-ray class written by the law-S entry points. A ray whose march runs out of steps is
+ * BHS_CLASS_HORIZON with BHS_FLAG_MAXSTEPS set (blackhole.frag:252). A ray is BHS_CLASS_DISK when at
+ * least one equatorial crossing lands in [disk_inner, disk_outer] (opaque-disk geometry). */
+enum { BHS_CLASS_HORIZON = 0, BHS_CLASS_DISK = 1, BHS_CLASS_ESCAPE = 2 };
+
+enum {
+    BHS_FLAG_MAXSTEPS = 1,    /* march hit max_steps */
+    BHS_FLAG_ESCAPED = 2,     /* terminal status was escape (also set for disk rays that escaped) */
+    BHS_FLAG_BAND64 = 4,      /* |b - b_crit| < band: the ray was marched in FP64 */
+    BHS_FLAG_ROOT64 = 8,      /* disk crossings refined by FP64 root bracketing of the RK4 step */
+    BHS_FLAG_SURROGATE = 16,  /* outputs come from the MLP surrogate */
+    BHS_FLAG_FALLBACK = 32    /* surrogate path: ray sent back to the integrator */
+};
+
+enum { BHS_PREC_F32 = 0, BHS_PREC_F32_BAND64 = 1, BHS_PREC_F64 = 2 };
+enum { BHS_DISK_NONE = 0, BHS_DISK_THIN = 1 };
+
+/* TraceResult (src/physics/Geodesic.h:97-103). */
+typedef struct bhs_trace_result {
+    int32_t status;           /* BHS_TRACE_* */
+    int32_t steps;
+    int32_t equator_crossings;
+    int32_t pad_;
+    double dir[3];            /* asymptotic direction if escaped, else 0 */
+    double min_r;
+} bhs_trace_result;
+
+/* One pinhole camera (uniforms uResolution, uJitter, uCamPos, uCamBasis, uTanHalfFov of
+ * blackhole.frag:27-31, set by Renderer::setTraceUniforms, src/Renderer.cpp:246-252). */
+typedef struct bhs_camera {
+    float pos[3];
+    float basis[9];           /* column-major 3x3, columns right, up, back (camera looks down -back) */
+    float tan_half_fov;       /* tan of half the vertical field of view */
+    int32_t width, height;
+    float jitter[2];          /* offset added to the pixel centre (uJitter); unused when hashed */
+    int32_t jitter_hashed;    /* 1: per-pixel u in [0,1)^2 from (seed, camera_id, pixel, sample)
+                                 replaces the pixel centre */
+    uint32_t camera_id;
+    uint32_t sample;
+    uint32_t pad_;
+    uint64_t seed;
+} bhs_camera;
+
+/* March settings (Settings fields that setTraceUniforms forwards: src/Settings.h:19-37). */
+typedef struct bhs_march_params {
+    int32_t max_steps;        /* uMaxSteps */
+    int32_t disk_mode;        /* BHS_DISK_NONE / BHS_DISK_THIN (uDiskMode) */
+    int32_t precision;        /* BHS_PREC_* */
+    int32_t root64;           /* 1: refine each in-annulus crossing by FP64 root bracketing */
+    float step_scale;         /* uStepScale */
+    float disk_step;          /* uDiskStep */
+    float disk_inner, disk_outer, disk_thickness;
+    float band_rel;           /* FP64 band half-width as a fraction of b_crit (default 1e-3) */
+} bhs_march_params;
+
+/* Structure-of-arrays ray output in device memory; plane k of a multi-plane array starts at
+ * base + k * n. Any pointer may be NULL to skip that output. */
+typedef struct bhs_ray_soa {
+    uint8_t* cls_dev;         /* n   : BHS_CLASS_* */
+    uint8_t* flags_dev;       /* n   : BHS_FLAG_* */
+    uint8_t* n_cross_dev;     /* n   : in-annulus crossings (saturates at 255) */
+    uint16_t* steps_dev;      /* n   : march iterations */
+    float* esc_dev;           /* 3n  : escape direction x,y,z planes (0 unless escaped) */
+    float* disk_dev;          /* 6n  : r0, r1, phi0, phi1, g0, g1 of the first two crossings */
+    float* b_dev;             /* n   : impact parameter h = |x0 x v0| */
+} bhs_ray_soa;
+
+/* Aggregate counters of one launch (device-side, read after sync). */
+typedef struct bhs_stats {
+    uint64_t rays;
+    uint64_t steps;           /* sum of march iterations */
+    uint64_t crossings;       /* sum of in-annulus crossings */
+    uint64_t escapes;         /* rays that escaped (far-field correction applied) */
+    uint64_t band64;          /* rays marched in FP64 */
+    uint64_t fallback;        /* surrogate rays sent back to the integrator */
+    uint64_t pad_[2];
+} bhs_stats;
+
+int bhs_version(void);
+
+/* Context on HIP device `device`. Returns NULL on failure with *status set. */
+typedef struct bhs_ctx bhs_ctx;
+bhs_ctx* bhs_create(int device, int* status);
+void bhs_destroy(bhs_ctx* ctx);
+const char* bhs_last_error(const bhs_ctx* ctx);
+int bhs_sync(bhs_ctx* ctx);
+
+/* Batched bhs::physics::trace (src/physics/Geodesic.h:107-140), FP64, law T. Host arrays, n rays,
+ * pos3/dir3 interleaved xyz. Synchronous. */
+int bhs_trace_rays(bhs_ctx* ctx, const double* pos3_host, const double* dir3_host, int64_t n,
+                   int32_t max_steps, double step_scale, double esc_tol, double r_esc_min,
+                   bhs_trace_result* out_host);
+
+/* Law-S render of one camera: width*height rays, pixel p = row*width + col with row 0 at the
+ * bottom (gl_FragCoord convention). Stream-ordered. */
+int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
+               const bhs_ray_soa* out, void* stream);
+
+/* Law-S march of explicit rays: pos3_dev/dir3_dev are device float arrays (xyz interleaved).
+ * Stream-ordered. */
+int bhs_march_rays(bhs_ctx* ctx, const float* pos3_dev, const float* dir3_dev, int64_t n,
+                   const bhs_march_params* params, const bhs_ray_soa* out, void* stream);
+
+/* Aggregate counters of the most recent bhs_render / bhs_march_rays / bhs_render_batch call
+ * (synchronises the context's last stream). */
+int bhs_last_stats(bhs_ctx* ctx, bhs_stats* out_host);
+
+/* GPU timers (the trace-pass GL_TIME_ELAPSED timer of Renderer::render, src/Renderer.cpp:159-178).
+ * While timing is on, every launch records HIP events on its own stream around its main march
+ * kernel and around the whole call. bhs_timing waits for them and returns the averages (ms) over
+ * the launches recorded since the previous bhs_timing call, then starts a new window. */
+int bhs_set_timing(bhs_ctx* ctx, int on);
+int bhs_timing(bhs_ctx* ctx, double* march_ms_avg, double* total_ms_avg, int32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BHS_TRACE_H */

@@ -1,0 +1,241 @@
+"""CPU oracle of the Schwarzschild null-geodesic tracer (ctypes view of libbhs_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, always as the checker (or the timed CPU baseline), never by the product package.
+
+Law T restates bhs::physics (src/physics/Geodesic.h:15-140) in FP64; law S restates the thin-disk
+march of shaders/blackhole.frag:67-253 in float and double with one pinned operation order
+(oracle/lawS_body.inc). Build: ``make -C oracle``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libbhs_oracle.so")
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle library with its Makefile (gcc); returns the library path."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-s", "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+class TraceResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("steps", C.c_int32), ("equator_crossings", C.c_int32),
+                ("pad_", C.c_int32), ("dir", C.c_double * 3), ("min_r", C.c_double)]
+
+
+class LawSParams(C.Structure):
+    _fields_ = [("max_steps", C.c_int32), ("disk_mode", C.c_int32), ("root64", C.c_int32),
+                ("pad_", C.c_int32), ("step_scale", C.c_float), ("disk_step", C.c_float),
+                ("disk_inner", C.c_float), ("disk_outer", C.c_float),
+                ("disk_thickness", C.c_float), ("band_rel", C.c_float)]
+
+
+class LawSRay(C.Structure):
+    _fields_ = [("cls", C.c_int32), ("flags", C.c_int32), ("steps", C.c_int32),
+                ("n_cross", C.c_int32), ("esc", C.c_float * 3), ("disk_r", C.c_float * 2),
+                ("disk_phi", C.c_float * 2), ("disk_g", C.c_float * 2), ("b", C.c_float)]
+
+
+LAWS_RAY_DTYPE = np.dtype([("cls", "<i4"), ("flags", "<i4"), ("steps", "<i4"), ("n_cross", "<i4"),
+                           ("esc", "<f4", 3), ("disk_r", "<f4", 2), ("disk_phi", "<f4", 2),
+                           ("disk_g", "<f4", 2), ("b", "<f4")])
+assert LAWS_RAY_DTYPE.itemsize == C.sizeof(LawSRay)
+TRACE_DTYPE = np.dtype([("status", "<i4"), ("steps", "<i4"), ("equator_crossings", "<i4"),
+                        ("pad_", "<i4"), ("dir", "<f8", 3), ("min_r", "<f8")])
+assert TRACE_DTYPE.itemsize == C.sizeof(TraceResult)
+
+_lib = None
+_D3 = C.c_double * 3
+_F3 = C.c_float * 3
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB_PATH)
+        dp, fp = C.POINTER(C.c_double), C.POINTER(C.c_float)
+        L.oracle_critical_impact_parameter.restype = C.c_double
+        L.oracle_accel.argtypes = [dp, C.c_double, dp]
+        L.oracle_rk4_step.argtypes = [dp, dp, C.c_double, C.c_double]
+        L.oracle_initial_velocity.argtypes = [dp, dp, dp]
+        L.oracle_impact_parameter.argtypes = [dp, dp]
+        L.oracle_impact_parameter.restype = C.c_double
+        L.oracle_impact_parameter_about_axis.argtypes = [dp, dp, dp]
+        L.oracle_impact_parameter_about_axis.restype = C.c_double
+        L.oracle_far_field_direction.argtypes = [dp, dp, C.c_double, dp]
+        L.oracle_shadow_angular_radius.argtypes = [C.c_double]
+        L.oracle_shadow_angular_radius.restype = C.c_double
+        L.oracle_kepler_omega.argtypes = [C.c_double]
+        L.oracle_kepler_omega.restype = C.c_double
+        L.oracle_redshift_factor.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.oracle_redshift_factor.restype = C.c_double
+        L.oracle_trace_T.argtypes = [dp, dp, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                     C.POINTER(TraceResult)]
+        L.oracle_trace_T_batch.argtypes = [dp, dp, C.c_int64, C.c_int32, C.c_double, C.c_double,
+                                           C.c_double, C.c_void_p, C.c_int32]
+        L.oracle_ray_dir.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                     C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, fp]
+        L.oracle_pcg.argtypes = [C.c_uint32]
+        L.oracle_pcg.restype = C.c_uint32
+        for name in ("oracle_lawS_f32", "oracle_lawS_f64", "oracle_lawS_band"):
+            getattr(L, name).argtypes = [fp, fp, C.POINTER(LawSParams), C.POINTER(LawSRay)]
+        L.oracle_lawS_h_f32.argtypes = [fp, fp]
+        L.oracle_lawS_h_f32.restype = C.c_float
+        L.oracle_render.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                    C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(LawSParams),
+                                    C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_void_p,
+                                    C.c_int32]
+        L.oracle_atan2f.argtypes = [C.c_float, C.c_float]
+        L.oracle_atan2f.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------------------------- law T helpers
+def _d3(v):
+    return _D3(*[float(t) for t in v])
+
+
+def critical_impact_parameter() -> float:
+    return lib().oracle_critical_impact_parameter()
+
+
+def accel(x, h2):
+    o = _D3()
+    lib().oracle_accel(_d3(x), h2, o)
+    return np.array(o[:])
+
+
+def rk4_step(x, v, h2, dt):
+    xx, vv = _d3(x), _d3(v)
+    lib().oracle_rk4_step(xx, vv, h2, dt)
+    return np.array(xx[:]), np.array(vv[:])
+
+
+def initial_velocity(cam, d):
+    o = _D3()
+    lib().oracle_initial_velocity(_d3(cam), _d3(d), o)
+    return np.array(o[:])
+
+
+def impact_parameter(cam, d) -> float:
+    return lib().oracle_impact_parameter(_d3(cam), _d3(d))
+
+
+def impact_parameter_about_axis(cam, d, axis) -> float:
+    return lib().oracle_impact_parameter_about_axis(_d3(cam), _d3(d), _d3(axis))
+
+
+def far_field_direction(x, v, h2):
+    o = _D3()
+    lib().oracle_far_field_direction(_d3(x), _d3(v), h2, o)
+    return np.array(o[:])
+
+
+def shadow_angular_radius(r) -> float:
+    return lib().oracle_shadow_angular_radius(r)
+
+
+def kepler_omega(r) -> float:
+    return lib().oracle_kepler_omega(r)
+
+
+def redshift_factor(r_emit, b_axis, r_cam) -> float:
+    return lib().oracle_redshift_factor(r_emit, b_axis, r_cam)
+
+
+def trace(cam, d, max_steps=200000, step_scale=0.005, esc_tol=1e-4, r_esc_min=30.0) -> TraceResult:
+    """bhs::physics::trace (src/physics/Geodesic.h:107-140) with its default arguments."""
+    res = TraceResult()
+    lib().oracle_trace_T(_d3(cam), _d3(d), max_steps, step_scale, esc_tol, r_esc_min, C.byref(res))
+    return res
+
+
+def trace_batch(pos3, dir3, max_steps=200000, step_scale=0.005, esc_tol=1e-4, r_esc_min=30.0,
+                threads=8) -> np.ndarray:
+    pos3 = np.ascontiguousarray(pos3, dtype=np.float64).reshape(-1, 3)
+    dir3 = np.ascontiguousarray(dir3, dtype=np.float64).reshape(-1, 3)
+    n = pos3.shape[0]
+    out = np.zeros(n, dtype=TRACE_DTYPE)
+    dp = C.POINTER(C.c_double)
+    lib().oracle_trace_T_batch(pos3.ctypes.data_as(dp), dir3.ctypes.data_as(dp), n, max_steps,
+                               step_scale, esc_tol, r_esc_min, out.ctypes.data, threads)
+    return out
+
+
+# ---------------------------------------------------------------------------------- law S
+def lawS_params(max_steps=450, step_scale=0.05, disk_step=0.18, disk_inner=6.0, disk_outer=20.0,
+                disk_thickness=0.035, disk_mode=1, root64=0, band_rel=1e-3) -> LawSParams:
+    return LawSParams(max_steps, disk_mode, root64, 0, step_scale, disk_step, disk_inner,
+                      disk_outer, disk_thickness, band_rel)
+
+
+_MODES = {"f32": 0, "f64": 1, "band": 2}
+
+
+def _f(a, n):
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(n)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def march(pos, d, params: LawSParams, mode="f32") -> LawSRay:
+    """Law S on one ray (float position and direction)."""
+    out = LawSRay()
+    p, pp = _f(pos, 3)
+    dd, dp_ = _f(d, 3)
+    fn = {"f32": lib().oracle_lawS_f32, "f64": lib().oracle_lawS_f64,
+          "band": lib().oracle_lawS_band}[mode]
+    fn(pp, dp_, C.byref(params), C.byref(out))
+    return out
+
+
+def lawS_h_f32(pos, d) -> float:
+    p, pp = _f(pos, 3)
+    dd, dp_ = _f(d, 3)
+    return lib().oracle_lawS_h_f32(pp, dp_)
+
+
+def atan2f(y, x) -> float:
+    return lib().oracle_atan2f(y, x)
+
+
+def pcg(v: int) -> int:
+    return lib().oracle_pcg(v & 0xFFFFFFFF)
+
+
+def ray_dir(cam, pixel: int):
+    """Float view direction of `pixel` for a camera dict (see camera_dict)."""
+    p, pp = _f(cam["pos"], 3)
+    b, bp = _f(cam["basis"], 9)
+    j, jp = _f(cam.get("jitter", (0.0, 0.0)), 2)
+    o = _F3()
+    lib().oracle_ray_dir(pp, bp, cam["tan_half_fov"], cam["width"], cam["height"], jp,
+                         int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
+                         int(cam.get("camera_id", 0)), int(cam.get("sample", 0)), pixel, o)
+    return np.array(o[:], dtype=np.float32)
+
+
+def render(cam, params: LawSParams, mode="f32", first=0, count=None, stride=1,
+           threads=8) -> np.ndarray:
+    """Law S over pixels first, first+stride, ... of a camera dict; returns a LAWS_RAY_DTYPE array."""
+    n = cam["width"] * cam["height"]
+    if count is None:
+        count = (n - first + stride - 1) // stride
+    out = np.zeros(count, dtype=LAWS_RAY_DTYPE)
+    p, pp = _f(cam["pos"], 3)
+    b, bp = _f(cam["basis"], 9)
+    j, jp = _f(cam.get("jitter", (0.0, 0.0)), 2)
+    lib().oracle_render(pp, bp, cam["tan_half_fov"], cam["width"], cam["height"], jp,
+                        int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
+                        int(cam.get("camera_id", 0)), int(cam.get("sample", 0)), C.byref(params),
+                        _MODES[mode], first, count, stride, out.ctypes.data, threads)
+    return out

@@ -1,0 +1,469 @@
+/*
+ * bhs_oracle.c — CPU oracle (test infrastructure; see bhs_oracle.h for what may use it).
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no fast-math: every float operation is the
+ * IEEE operation written here).
+ */
+#include "bhs_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ================================================================================================
+ * law T — src/physics/Geodesic.h, FP64 in GLM's operation order:
+ *   glm::dot(a,b) = (a.x*b.x + a.y*b.y) + a.z*b.z; glm::length = sqrt(dot); glm::normalize(v) =
+ *   v * (1/sqrt(dot(v,v))); glm::cross as written in GLM; vec/scalar divides each component.
+ * ============================================================================================== */
+static inline double gdot(const double a[3], const double b[3]) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline double glen(const double a[3]) { return sqrt(gdot(a, a)); }
+static inline void gcross(const double x[3], const double y[3], double o[3]) {
+    const double o0 = x[1] * y[2] - y[1] * x[2];
+    const double o1 = x[2] * y[0] - y[2] * x[0];
+    const double o2 = x[0] * y[1] - y[0] * x[1];
+    o[0] = o0; o[1] = o1; o[2] = o2;
+}
+static inline void gnormalize(const double v[3], double o[3]) {
+    const double is = 1.0 / sqrt(gdot(v, v));
+    o[0] = v[0] * is; o[1] = v[1] * is; o[2] = v[2] * is;
+}
+
+double oracle_critical_impact_parameter(void) { return 3.0 * sqrt(3.0); } /* Geodesic.h:19 */
+
+void oracle_accel(const double x[3], double h2, double out[3]) { /* Geodesic.h:21-25 */
+    const double r2 = gdot(x, x);
+    const double r = sqrt(r2);
+    const double s = -3.0 * h2 / (r2 * r2 * r);
+    out[0] = x[0] * s; out[1] = x[1] * s; out[2] = x[2] * s;
+}
+
+void oracle_rk4_step(double x[3], double v[3], double h2, double dt) { /* Geodesic.h:27-34 */
+    double k1v[3], k2v[3], k3v[3], k4v[3], k1x[3], k2x[3], k3x[3], k4x[3], t[3];
+    const double hd = 0.5 * dt;
+    oracle_accel(x, h2, k1v);
+    for (int k = 0; k < 3; ++k) k1x[k] = v[k];
+    for (int k = 0; k < 3; ++k) t[k] = x[k] + hd * k1x[k];
+    oracle_accel(t, h2, k2v);
+    for (int k = 0; k < 3; ++k) k2x[k] = v[k] + hd * k1v[k];
+    for (int k = 0; k < 3; ++k) t[k] = x[k] + hd * k2x[k];
+    oracle_accel(t, h2, k3v);
+    for (int k = 0; k < 3; ++k) k3x[k] = v[k] + hd * k2v[k];
+    for (int k = 0; k < 3; ++k) t[k] = x[k] + dt * k3x[k];
+    oracle_accel(t, h2, k4v);
+    for (int k = 0; k < 3; ++k) k4x[k] = v[k] + dt * k3v[k];
+    const double d6 = dt / 6.0;
+    for (int k = 0; k < 3; ++k) {
+        x[k] += d6 * (((k1x[k] + 2.0 * k2x[k]) + 2.0 * k3x[k]) + k4x[k]);
+        v[k] += d6 * (((k1v[k] + 2.0 * k2v[k]) + 2.0 * k3v[k]) + k4v[k]);
+    }
+}
+
+void oracle_initial_velocity(const double cam[3], const double dir[3], double out[3]) { /* :40-47 */
+    const double r = glen(cam);
+    double rh[3], np[3], w[3];
+    for (int k = 0; k < 3; ++k) rh[k] = cam[k] / r;
+    const double nr = gdot(dir, rh);
+    for (int k = 0; k < 3; ++k) np[k] = dir[k] - nr * rh[k];
+    const double t = 1.0 - 2.0 / r;
+    const double f = sqrt(t > 0.0 ? t : 0.0);
+    const double nf = nr * f;
+    for (int k = 0; k < 3; ++k) w[k] = nf * rh[k] + np[k];
+    gnormalize(w, out);
+}
+
+double oracle_impact_parameter(const double cam[3], const double dir[3]) { /* :50-54 */
+    const double r = glen(cam);
+    double rh[3], c[3];
+    for (int k = 0; k < 3; ++k) rh[k] = cam[k] / r;
+    gcross(rh, dir, c);
+    const double sin_a = glen(c);
+    const double t = 1.0 - 2.0 / r;
+    return r * sin_a / sqrt(t > 1e-12 ? t : 1e-12);
+}
+
+double oracle_impact_parameter_about_axis(const double cam[3], const double dir[3],
+                                          const double axis[3]) { /* :58-61 */
+    const double r = glen(cam);
+    double c[3];
+    gcross(cam, dir, c);
+    const double t = 1.0 - 2.0 / r;
+    return -gdot(c, axis) / sqrt(t > 1e-12 ? t : 1e-12);
+}
+
+void oracle_far_field_direction(const double x[3], const double v[3], double h2,
+                                double out[3]) { /* :65-75 */
+    const double vlen = glen(v);
+    double vh[3], bv[3], w[3];
+    for (int k = 0; k < 3; ++k) vh[k] = v[k] / vlen;
+    const double r = glen(x);
+    const double p = gdot(x, vh);
+    for (int k = 0; k < 3; ++k) bv[k] = x[k] - p * vh[k];
+    double s = p / r;
+    s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
+    const double r3 = r * r * r;
+    const double a = -(h2 / vlen);
+    const double kk = (2.0 + s) / (r3 * r * (1.0 + s) * (1.0 + s));
+    for (int k = 0; k < 3; ++k) w[k] = v[k] + a * (vh[k] / r3 + bv[k] * kk);
+    gnormalize(w, out);
+}
+
+double oracle_shadow_angular_radius(double r) { /* :78-82 */
+    const double t = 1.0 - 2.0 / r;
+    double s = oracle_critical_impact_parameter() / r * sqrt(t > 0.0 ? t : 0.0);
+    s = s < 1.0 ? s : 1.0;
+    const double a = asin(s);
+    return (r >= 3.0) ? a : (M_PI - a);
+}
+
+double oracle_kepler_omega(double r) { return pow(r, -1.5); } /* :85 */
+
+double oracle_redshift_factor(double r_emit, double b_axis, double r_cam) { /* :90-95 */
+    const double omega = oracle_kepler_omega(r_emit);
+    const double t1 = 1.0 - 3.0 / r_emit;
+    const double ut = 1.0 / sqrt(t1 > 1e-6 ? t1 : 1e-6);
+    const double t2 = 1.0 - 2.0 / r_cam;
+    const double lapse = sqrt(t2 > 1e-6 ? t2 : 1e-6);
+    return 1.0 / (lapse * ut * (1.0 - omega * b_axis));
+}
+
+void oracle_trace_T(const double cam[3], const double dir[3], int32_t max_steps, double step_scale,
+                    double esc_tol, double r_esc_min, oracle_trace_result* res) { /* :107-140 */
+    memset(res, 0, sizeof *res);
+    res->status = 2;
+    res->min_r = 1e30;
+    double x[3] = {cam[0], cam[1], cam[2]}, v[3], hv[3];
+    oracle_initial_velocity(cam, dir, v);
+    gcross(x, v, hv);
+    const double h2 = gdot(hv, hv);
+    const double h = sqrt(h2);
+    double prev_y = x[1];
+    for (int32_t i = 0; i < max_steps; ++i) {
+        const double r = glen(x);
+        if (r < res->min_r) res->min_r = r;
+        if (r < 2.0 + 0.01) {
+            res->status = 1;
+            res->steps = i;
+            return;
+        }
+        const double vlen = glen(v);
+        const double b = h / vlen;
+        if (gdot(x, v) > 0.0 && r > r_esc_min && 0.75 * b * b * b / (r * r * r * r) < esc_tol) {
+            res->status = 0;
+            oracle_far_field_direction(x, v, h2, res->dir);
+            res->steps = i;
+            return;
+        }
+        const double ds0 = step_scale * r;
+        const double ds = ds0 > 1e-4 ? ds0 : 1e-4;
+        oracle_rk4_step(x, v, h2, ds / vlen);
+        if ((x[1] > 0.0) != (prev_y > 0.0)) ++res->equator_crossings;
+        prev_y = x[1];
+        res->steps = i + 1;
+    }
+    res->status = 2;
+}
+
+/* ================================================================================================
+ * Atan2 from IEEE operations only. Reduction: t = min/max in [0,1]; for t > tan(pi/8),
+ * atan t = pi/4 + atan((t-1)/(t+1)); then atan u = u + u^3 P(u^2) with P a degree-4 least-squares
+ * fit on [0, tan^2(pi/8)] (max error 1.8e-9 before float rounding). Quadrant fix-up as atan2.
+ * ============================================================================================== */
+float oracle_atan2f(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = ax > ay ? ax : ay;
+    const float mn = ax > ay ? ay : ax;
+    if (mx == 0.0f) return 0.0f;
+    float t = mn / mx;
+    float base = 0.0f;
+    if (t > 0.41421356f) {
+        t = (t - 1.0f) / (t + 1.0f);
+        base = 0.78539816f;
+    }
+    const float s = t * t;
+    float pl = -0.065067887f;
+    pl = fmaf(pl, s, 0.10763717f);
+    pl = fmaf(pl, s, -0.14266309f);
+    pl = fmaf(pl, s, 0.19999637f);
+    pl = fmaf(pl, s, -0.33333331f);
+    float a = base + fmaf(t * s, pl, t);
+    if (ay > ax) a = 1.5707964f - a;
+    if (x < 0.0f) a = 3.1415927f - a;
+    if (y < 0.0f) a = -a;
+    return a;
+}
+
+/* ================================================================================================
+ * law S in float and in double
+ * ============================================================================================== */
+#define REAL float
+#define FMA fmaf
+#define SQRT sqrtf
+#define FABS fabsf
+#define L(x) ((float)(x))
+#define FN(n) n##_f32
+#include "lawS_body.inc"
+#undef REAL
+#undef FMA
+#undef SQRT
+#undef FABS
+#undef L
+#undef FN
+
+#define REAL double
+#define FMA fma
+#define SQRT sqrt
+#define FABS fabs
+#define L(x) ((double)(x))
+#define FN(n) n##_f64
+#include "lawS_body.inc"
+#undef REAL
+#undef FMA
+#undef SQRT
+#undef FABS
+#undef L
+#undef FN
+
+static const float kBcritF = 5.19615242270663f;
+
+/* FP64 root bracketing of the disk crossing inside one RK4 step: y(tau) = (rk4(x0, v0, tau)).y has
+ * opposite signs at tau = 0 and tau = dt; the Illinois variant of regula falsi narrows the bracket
+ * until |b - a| <= 1e-14 dt (at most 60 evaluations). Returns 1 and the crossing (x, z) on success,
+ * 0 if the double step does not bracket a sign change. */
+static int root64(const double st[8], double c3, double* hx, double* hz) {
+    const double dt = st[6];
+    double xa[3], va[3];
+    double a = 0.0, fa = st[1];
+    double b = dt, fb;
+    for (int k = 0; k < 3; ++k) { xa[k] = st[k]; va[k] = st[3 + k]; }
+    {
+        const double q = dot3_f64(xa, xa);
+        rk4_f64(xa, va, c3, b, q, sqrt(q));
+        fb = xa[1];
+    }
+    if (fa == 0.0) { *hx = st[0]; *hz = st[2]; return 1; }
+    if ((fa > 0.0) == (fb > 0.0)) return 0;
+    double c = b, xc[3] = {xa[0], xa[1], xa[2]};
+    int side = 0;
+    for (int it = 0; it < 60; ++it) {
+        c = (a * fb - b * fa) / (fb - fa);
+        double xs[3], vs[3];
+        for (int k = 0; k < 3; ++k) { xs[k] = st[k]; vs[k] = st[3 + k]; }
+        const double q = dot3_f64(xs, xs);
+        rk4_f64(xs, vs, c3, c, q, sqrt(q));
+        const double fc = xs[1];
+        for (int k = 0; k < 3; ++k) xc[k] = xs[k];
+        if (fc == 0.0) break;
+        if ((fc > 0.0) == (fb > 0.0)) {
+            b = c; fb = fc;
+            if (side == -1) fa = 0.5 * fa;
+            side = -1;
+        } else {
+            a = c; fa = fc;
+            if (side == 1) fb = 0.5 * fb;
+            side = 1;
+        }
+        if (fabs(b - a) <= 1e-14 * dt) break;
+    }
+    *hx = xc[0];
+    *hz = xc[2];
+    return 1;
+}
+
+static void apply_root64(const oracle_lawS_params* p, const double st[16], double c3, double b_axis,
+                         double lapse_g, oracle_lawS_ray* out) {
+    const int m = out->n_cross < 2 ? out->n_cross : 2;
+    for (int k = 0; k < m; ++k) {
+        const double* s = st + 8 * k;
+        if (s[7] != 1.0) continue;
+        double hx, hz;
+        if (!root64(s, c3, &hx, &hz)) continue;
+        const double rh = sqrt(fma(hz, hz, hx * hx));
+        out->disk_r[k] = (float)rh;
+        out->disk_phi[k] = oracle_atan2f((float)hz, (float)hx);
+        out->disk_g[k] = (float)redshift_f64(rh, b_axis, lapse_g);
+    }
+    (void)p;
+    out->flags |= 8;
+}
+
+float oracle_lawS_h_f32(const float pos[3], const float dir[3]) {
+    setup_f32 s;
+    lawS_setup_f32(pos, dir, &s);
+    return s.h;
+}
+
+void oracle_lawS_f32(const float pos[3], const float dir[3], const oracle_lawS_params* p,
+                     oracle_lawS_ray* out) {
+    setup_f32 s;
+    double st[16];
+    memset(st, 0, sizeof st);
+    lawS_setup_f32(pos, dir, &s);
+    lawS_march_f32(pos, &s, p, out, p->root64 ? st : NULL);
+    if (p->root64 && out->n_cross > 0)
+        apply_root64(p, st, (double)s.c3, (double)s.b_axis, (double)s.lapse_g, out);
+}
+
+void oracle_lawS_f64(const float pos[3], const float dir[3], const oracle_lawS_params* p,
+                     oracle_lawS_ray* out) {
+    const double P[3] = {pos[0], pos[1], pos[2]};
+    const double d[3] = {dir[0], dir[1], dir[2]};
+    setup_f64 s;
+    double st[16];
+    memset(st, 0, sizeof st);
+    lawS_setup_f64(P, d, &s);
+    lawS_march_f64(P, &s, p, out, p->root64 ? st : NULL);
+    if (p->root64 && out->n_cross > 0) apply_root64(p, st, s.c3, s.b_axis, s.lapse_g, out);
+    out->flags |= 4;
+}
+
+void oracle_lawS_band(const float pos[3], const float dir[3], const oracle_lawS_params* p,
+                      oracle_lawS_ray* out) {
+    const float h = oracle_lawS_h_f32(pos, dir);
+    if (fabsf(h - kBcritF) < p->band_rel * kBcritF)
+        oracle_lawS_f64(pos, dir, p, out);
+    else
+        oracle_lawS_f32(pos, dir, p, out);
+}
+
+/* ================================================================================================
+ * Ray generation
+ * ============================================================================================== */
+uint32_t oracle_pcg(uint32_t v) { /* common.glsl:28-32 */
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+static uint32_t jitter_base(uint64_t seed, uint32_t camera_id, uint32_t sample) {
+    uint32_t h = oracle_pcg(sample);
+    h = oracle_pcg(h ^ camera_id);
+    h = oracle_pcg(h ^ (uint32_t)(seed >> 32));
+    h = oracle_pcg(h ^ (uint32_t)seed);
+    return h;
+}
+
+void oracle_ray_dir(const float pos[3], const float B[9], float tan_half_fov, int32_t width,
+                    int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                    uint32_t camera_id, uint32_t sample, int64_t pixel, float dir_out[3]) {
+    (void)pos;
+    const int64_t col = pixel % width, row = pixel / width;
+    float px, py;
+    if (jitter_hashed) {
+        const uint32_t base = jitter_base(seed, camera_id, sample);
+        const uint32_t pi2 = 2u * (uint32_t)pixel;
+        px = (float)col + (float)(oracle_pcg(base + pi2) >> 8) * 0x1p-24f;
+        py = (float)row + (float)(oracle_pcg(base + pi2 + 1u) >> 8) * 0x1p-24f;
+    } else {
+        px = ((float)col + 0.5f) + jitter[0];
+        py = ((float)row + 0.5f) + jitter[1];
+    }
+    const float wf = (float)width, hf = (float)height;
+    const float ndx = fmaf(px / wf, 2.0f, -1.0f);
+    const float ndy = fmaf(py / hf, 2.0f, -1.0f);
+    const float aspect = wf / hf;
+    float c[3] = {(ndx * tan_half_fov) * aspect, ndy * tan_half_fov, -1.0f};
+    const float ic = 1.0f / sqrtf(fmaf(c[2], c[2], fmaf(c[1], c[1], c[0] * c[0])));
+    for (int k = 0; k < 3; ++k) c[k] *= ic;
+    float d[3];
+    for (int k = 0; k < 3; ++k) d[k] = fmaf(B[6 + k], c[2], fmaf(B[3 + k], c[1], B[k] * c[0]));
+    const float id = 1.0f / sqrtf(fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0])));
+    for (int k = 0; k < 3; ++k) dir_out[k] = d[k] * id;
+}
+
+/* ================================================================================================
+ * Threaded batch drivers (CPU baseline and test reference)
+ * ============================================================================================== */
+typedef struct render_job {
+    const float* pos;
+    const float* basis;
+    float tan_half_fov;
+    int32_t width, height;
+    const float* jitter;
+    int32_t jitter_hashed;
+    uint64_t seed;
+    uint32_t camera_id, sample;
+    const oracle_lawS_params* p;
+    int32_t mode;
+    int64_t first, count, stride;
+    oracle_lawS_ray* out;
+    int32_t tid, nthreads;
+} render_job;
+
+static void* render_worker(void* arg) {
+    render_job* j = (render_job*)arg;
+    for (int64_t k = j->tid; k < j->count; k += j->nthreads) {
+        const int64_t pix = j->first + k * j->stride;
+        float d[3];
+        oracle_ray_dir(j->pos, j->basis, j->tan_half_fov, j->width, j->height, j->jitter,
+                       j->jitter_hashed, j->seed, j->camera_id, j->sample, pix, d);
+        if (j->mode == 0) oracle_lawS_f32(j->pos, d, j->p, &j->out[k]);
+        else if (j->mode == 1) oracle_lawS_f64(j->pos, d, j->p, &j->out[k]);
+        else oracle_lawS_band(j->pos, d, j->p, &j->out[k]);
+    }
+    return NULL;
+}
+
+void oracle_render(const float pos[3], const float basis[9], float tan_half_fov, int32_t width,
+                   int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                   uint32_t camera_id, uint32_t sample, const oracle_lawS_params* p, int32_t mode,
+                   int64_t first, int64_t count, int64_t stride, oracle_lawS_ray* out,
+                   int32_t threads) {
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if (stride < 1) stride = 1;
+    static const float zero2[2] = {0.0f, 0.0f};
+    render_job jobs[256];
+    pthread_t th[256];
+    for (int t = 0; t < threads; ++t) {
+        render_job j = {pos, basis, tan_half_fov, width, height, jitter ? jitter : zero2,
+                        jitter_hashed, seed, camera_id, sample, p, mode, first, count, stride, out,
+                        t, threads};
+        jobs[t] = j;
+    }
+    if (threads == 1) {
+        render_worker(&jobs[0]);
+        return;
+    }
+    for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, render_worker, &jobs[t]);
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+}
+
+typedef struct trace_job {
+    const double* pos3;
+    const double* dir3;
+    int64_t n;
+    int32_t max_steps;
+    double step_scale, esc_tol, r_esc_min;
+    oracle_trace_result* out;
+    int32_t tid, nthreads;
+} trace_job;
+
+static void* trace_worker(void* arg) {
+    trace_job* j = (trace_job*)arg;
+    for (int64_t k = j->tid; k < j->n; k += j->nthreads)
+        oracle_trace_T(j->pos3 + 3 * k, j->dir3 + 3 * k, j->max_steps, j->step_scale, j->esc_tol,
+                       j->r_esc_min, &j->out[k]);
+    return NULL;
+}
+
+void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int32_t max_steps,
+                          double step_scale, double esc_tol, double r_esc_min,
+                          oracle_trace_result* out, int32_t threads) {
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    trace_job jobs[256];
+    pthread_t th[256];
+    for (int t = 0; t < threads; ++t) {
+        trace_job j = {pos3, dir3, n, max_steps, step_scale, esc_tol, r_esc_min, out, t, threads};
+        jobs[t] = j;
+    }
+    if (threads == 1) {
+        trace_worker(&jobs[0]);
+        return;
+    }
+    for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, trace_worker, &jobs[t]);
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+}

@@ -1,0 +1,116 @@
+/*
+ * bhs_oracle.h — CPU oracle of the Schwarzschild null-geodesic tracer.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker of the HIP product path: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. Nothing in the product
+ * (blackholerender_amd/, include/) links or calls it.
+ *
+ * It restates, in plain C, two control laws of the reference:
+ *   law T  bhs::physics::trace and its helpers, src/physics/Geodesic.h:15-140 (FP64, GLM operation
+ *          order, no fused multiply-add: the reference's CPU build has no FMA contraction);
+ *   law S  the thin-disk march of shaders/blackhole.frag:67-253 (DISK_THIN, lensing on, far mode off,
+ *          no bodies, no gas, disk opacity 0), with geodesic.glsl and disk.glsl:24,33-42,113-117.
+ *          Law S is written once (lawS_body.inc) and compiled twice, in float and in double, with one
+ *          pinned operation order (explicit fma, IEEE division and square root, no contraction) so a
+ *          device implementation with the same order can be compared bit for bit.
+ *
+ * Pins (tests/test_oracle_*.py): the 15 checks of tests/geodesic_tests.cpp restated; closed forms
+ * (impact parameter, shadow radius, redshift); exact Schwarzschild deflection by quadrature; RK4
+ * step-halving order; orbit-plane invariance; float vs double law S agreement; brute-force
+ * convergence of class on tiny images.
+ */
+#ifndef BHS_ORACLE_H
+#define BHS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- law T: src/physics/Geodesic.h ------------------------------------------------------------ */
+typedef struct oracle_trace_result {
+    int32_t status; /* 0 escaped, 1 captured, 2 max steps (Geodesic.h:98) */
+    int32_t steps;
+    int32_t equator_crossings;
+    int32_t pad_;
+    double dir[3];
+    double min_r;
+} oracle_trace_result;
+
+double oracle_critical_impact_parameter(void);                                     /* :19     */
+void oracle_accel(const double x[3], double h2, double out[3]);                    /* :21-25  */
+void oracle_rk4_step(double x[3], double v[3], double h2, double dt);               /* :27-34  */
+void oracle_initial_velocity(const double cam[3], const double dir[3], double out[3]); /* :40-47 */
+double oracle_impact_parameter(const double cam[3], const double dir[3]);          /* :50-54  */
+double oracle_impact_parameter_about_axis(const double cam[3], const double dir[3],
+                                          const double axis[3]);                   /* :58-61  */
+void oracle_far_field_direction(const double x[3], const double v[3], double h2,
+                                double out[3]);                                    /* :65-75  */
+double oracle_shadow_angular_radius(double r);                                     /* :78-82  */
+double oracle_kepler_omega(double r);                                              /* :85     */
+double oracle_redshift_factor(double r_emit, double b_axis, double r_cam);         /* :90-95  */
+void oracle_trace_T(const double cam[3], const double dir[3], int32_t max_steps, double step_scale,
+                    double esc_tol, double r_esc_min, oracle_trace_result* out);   /* :107-140 */
+/* n rays, pos3/dir3 interleaved xyz; `threads` worker threads (<=0: 1). */
+void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int32_t max_steps,
+                          double step_scale, double esc_tol, double r_esc_min,
+                          oracle_trace_result* out, int32_t threads);
+
+/* ---- law S: shaders/blackhole.frag ------------------------------------------------------------ */
+typedef struct oracle_lawS_params {
+    int32_t max_steps, disk_mode; /* disk_mode 0 none, 1 thin */
+    int32_t root64;               /* 1: FP64 root bracketing of each in-annulus crossing */
+    int32_t pad_;
+    float step_scale, disk_step, disk_inner, disk_outer, disk_thickness;
+    float band_rel;               /* band half-width / b_crit used by oracle_lawS_band */
+} oracle_lawS_params;
+
+/* One ray's law-S result; float fields hold the value the march produced (float march: exact;
+ * double march: rounded to float once, at the end). */
+typedef struct oracle_lawS_ray {
+    int32_t cls;      /* 0 horizon, 1 disk, 2 escape */
+    int32_t flags;    /* 1 max steps, 2 escaped, 4 marched in FP64, 8 root-refined crossings */
+    int32_t steps;
+    int32_t n_cross;
+    float esc[3];
+    float disk_r[2], disk_phi[2], disk_g[2];
+    float b;          /* h = |x0 x v0| of the march's own precision */
+} oracle_lawS_ray;
+
+/* Ray generation (blackhole.frag:68-72 with Renderer.cpp:246-252 and common.glsl:28-32 pcg for the
+ * hashed jitter). basis is column-major (right, up, back). Writes the float view direction. */
+void oracle_ray_dir(const float pos[3], const float basis[9], float tan_half_fov, int32_t width,
+                    int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                    uint32_t camera_id, uint32_t sample, int64_t pixel, float dir_out[3]);
+uint32_t oracle_pcg(uint32_t v);
+
+/* Law S in float and in double (same operation order). Inputs are the float camera position and
+ * float direction; the double march promotes them exactly. */
+void oracle_lawS_f32(const float pos[3], const float dir[3], const oracle_lawS_params* p,
+                     oracle_lawS_ray* out);
+void oracle_lawS_f64(const float pos[3], const float dir[3], const oracle_lawS_params* p,
+                     oracle_lawS_ray* out);
+/* Float march with the FP64 band: rays whose float h satisfies |h - b_crit| < band_rel * b_crit
+ * are marched in double instead (flag 4). */
+void oracle_lawS_band(const float pos[3], const float dir[3], const oracle_lawS_params* p,
+                      oracle_lawS_ray* out);
+/* The float setup's h (impact parameter) for a ray. */
+float oracle_lawS_h_f32(const float pos[3], const float dir[3]);
+
+/* Image of one camera: rays [first, first+count) of width*height pixels, `mode` 0 float, 1 double,
+ * 2 float with FP64 band; out[count]. */
+void oracle_render(const float pos[3], const float basis[9], float tan_half_fov, int32_t width,
+                   int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                   uint32_t camera_id, uint32_t sample, const oracle_lawS_params* p, int32_t mode,
+                   int64_t first, int64_t count, int64_t stride, oracle_lawS_ray* out,
+                   int32_t threads);
+
+/* Atan2 built from IEEE operations only (range reduction + degree-4 polynomial in t^2); identical
+ * in any IEEE-754 float implementation that honours fmaf. */
+float oracle_atan2f(float y, float x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
