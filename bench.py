@@ -76,7 +76,20 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision", default="band", choices=["f32", "band", "f64"])
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 256 MiB device-to-device copy (calibrates the HBM counters "
+                         "FETCH_SIZE / WRITE_SIZE against a known byte count)")
     args = ap.parse_args(argv)
+    if args.copy_probe:
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 4, "dispatches": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every ray is "
               "independent; see DESIGN.md for how the frame would shard). Run with --gpus 1.",
@@ -95,6 +108,7 @@ def main(argv=None) -> int:
     n = cam.n_rays
     out = bhs.RayBuffers(n, "cuda:0")
     stream = torch.cuda.current_stream()
+    tuned = tracer.autotune(cam, params, out, reps=3) if not args.no_tune else tracer.get_tuning()
     for _ in range(args.warmup):
         tracer.render(cam, params, out, stream)
     torch.cuda.synchronize()
@@ -140,7 +154,8 @@ def main(argv=None) -> int:
         "config": {"workload": "plain law-S march, 1920x1080 (2,073,600 rays), 450 steps, "
                                "r_obs=30M, incl 85deg, FOV 60deg, thin disk [6,20]M, "
                                f"precision={args.precision}",
-                   "path": "plain", "rays_per_step": n, "parallelism": "single GPU"},
+                   "path": "plain", "rays_per_step": n, "parallelism": "single GPU",
+                   "launch": tuned},
         "roofline": {"bound": "valu", "achieved": round(tflops, 3),
                      "peak": W.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(tflops / W.PEAK_FP32_TFLOPS, 4), "traffic": None,

@@ -64,7 +64,7 @@ class TraceResult:
 
 def march_params(max_steps=450, step_scale=None, disk_step=None, disk_inner=6.0, disk_outer=20.0,
                  disk_thickness=0.035, disk_mode=DISK_THIN, precision=PREC_F32_BAND64, root64=0,
-                 band_rel=1e-3) -> _capi.MarchParams:
+                 band_rel=0.02) -> _capi.MarchParams:
     """bhs_march_params; step_scale/disk_step default to the quality preset of max_steps
     (src/Settings.cpp:15-30), else to the High preset."""
     ss, dstep = PRESETS.get(max_steps, PRESETS[450])
@@ -217,6 +217,34 @@ class Tracer:
         self._check(self._lib.bhs_last_stats(self._ctx, C.byref(s)), "bhs_last_stats")
         return dict(rays=s.rays, steps=s.steps, crossings=s.crossings, escapes=s.escapes,
                     band64=s.band64, fallback=s.fallback)
+
+    def set_tuning(self, mode: int, block: int = 256, waves_per_simd: int = 8,
+                   refill_min: int = 32):
+        t = _capi.Tuning(mode, block, waves_per_simd, refill_min)
+        self._check(self._lib.bhs_set_tuning(self._ctx, C.byref(t)), "bhs_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = _capi.Tuning()
+        self._check(self._lib.bhs_get_tuning(self._ctx, C.byref(t)), "bhs_get_tuning")
+        return dict(mode=t.mode, block=t.block, waves_per_simd=t.waves_per_simd,
+                    refill_min=t.refill_min)
+
+    def autotune(self, cam: Camera, params: _capi.MarchParams | None = None,
+                 out: RayBuffers | None = None, reps: int = 5) -> dict:
+        """Runs the library's autotuner (bhs_autotune) on this camera; keeps and returns the best
+        launch shape with its median main-kernel ms."""
+        if params is None:
+            params = march_params()
+        if out is None:
+            out = RayBuffers(cam.width * cam.height, f"cuda:{self.device}")
+        cs = _camera_struct(cam)
+        soa = out.soa()
+        best = _capi.Tuning()
+        ms = C.c_double()
+        self._check(self._lib.bhs_autotune(self._ctx, C.byref(cs), C.byref(params), C.byref(soa),
+                                           reps, C.byref(best), C.byref(ms)), "bhs_autotune")
+        return dict(mode=best.mode, block=best.block, waves_per_simd=best.waves_per_simd,
+                    refill_min=best.refill_min, ms=ms.value)
 
     def set_timing(self, on: bool):
         self._check(self._lib.bhs_set_timing(self._ctx, 1 if on else 0), "bhs_set_timing")

@@ -47,6 +47,11 @@ class RaySoA(C.Structure):
                 ("b_dev", C.c_void_p)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("waves_per_simd", C.c_int32),
+                ("refill_min", C.c_int32)]
+
+
 class Stats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("steps", C.c_uint64), ("crossings", C.c_uint64),
                 ("escapes", C.c_uint64), ("band64", C.c_uint64), ("fallback", C.c_uint64),
@@ -79,8 +84,13 @@ def load():
     L.bhs_last_stats.argtypes = [vp, C.POINTER(Stats)]
     L.bhs_set_timing.argtypes = [vp, C.c_int]
     L.bhs_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(i32)]
+    L.bhs_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.bhs_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.bhs_autotune.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams), C.POINTER(RaySoA), i32,
+                               C.POINTER(Tuning), C.POINTER(C.c_double)]
     for name in ("bhs_sync", "bhs_trace_rays", "bhs_render", "bhs_march_rays", "bhs_last_stats",
-                 "bhs_set_timing", "bhs_timing"):
+                 "bhs_set_timing", "bhs_timing", "bhs_set_tuning", "bhs_get_tuning",
+                 "bhs_autotune"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -2,6 +2,7 @@
 // stream ordering around the kernels of kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -29,6 +30,8 @@ struct bhs_ctx {
     bool timing = false;
     std::vector<TimingRec> timers;
     size_t timers_used = 0;
+    int num_cus = 256;
+    bhs::Tuning tuning{1, 256, 7, 32};
 };
 
 namespace {
@@ -170,7 +173,7 @@ bhs::SoA make_soa(const bhs_ray_soa* o, int64_t n) {
     return s;
 }
 
-constexpr size_t kStatsBytes = 256;   // Stats + band counter, padded
+constexpr size_t kStatsBytes = 256;   // Stats + band counter + work counter, padded
 
 // Next timer record of the context (null when timing is off or events cannot be created).
 bhs_ctx::TimingRec* next_timer(bhs_ctx* ctx) {
@@ -197,6 +200,7 @@ int prepare_scratch(bhs_ctx* ctx, int64_t n, bool roots, bhs::Scratch* sc) {
     char* base = (char*)ctx->scratch;
     sc->stats = (bhs::Stats*)base;
     sc->band_count = (uint32_t*)(base + sizeof(bhs::Stats));
+    sc->work = (uint32_t*)(base + sizeof(bhs::Stats) + 4);
     sc->band_ids = (uint32_t*)(base + kStatsBytes);
     sc->band_cap = (uint32_t)n;
     sc->roots = roots ? (bhs::RootState<float>*)(base + kStatsBytes + band_bytes) : nullptr;
@@ -224,6 +228,10 @@ bhs_ctx* bhs_create(int device, int* status) {
     }
     ctx->device = device;
     DeviceGuard g(device);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+        ctx->num_cus = cus;
     if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
@@ -320,7 +328,8 @@ int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* para
     if (tr) (void)hipEventRecord(tr->call0, s);
     if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_render clear");
-    if ((e = bhs::launch_render(c, make_soa(out, n), sc, s, tr ? tr->k0 : nullptr,
+    if ((e = bhs::launch_render(c, make_soa(out, n), sc, ctx->tuning, ctx->num_cus, s, ctx->aux,
+                                ctx->ev_fork, ctx->ev_join, tr ? tr->k0 : nullptr,
                                 tr ? tr->k1 : nullptr)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_render launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
@@ -351,6 +360,88 @@ int bhs_march_rays(bhs_ctx* ctx, const float* pos3_dev, const float* dir3_dev, i
                                     tr ? tr->k0 : nullptr, tr ? tr->k1 : nullptr)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_march_rays launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
+    return BHS_OK;
+}
+
+int bhs_set_tuning(bhs_ctx* ctx, const bhs_tuning* t) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!t) return fail(ctx, BHS_E_INVALID, "tuning is NULL%s");
+    if ((t->mode != 0 && t->mode != 1) || (t->block != 64 && t->block != 256) ||
+        t->waves_per_simd < 1 || t->waves_per_simd > 8 || t->refill_min < 1 || t->refill_min > 64)
+        return fail(ctx, BHS_E_INVALID,
+                    "tuning out of range (mode 0|1, block 64|256, waves_per_simd 1..8, refill_min 1..64)%s");
+    ctx->tuning = bhs::Tuning{t->mode, t->block, t->waves_per_simd, t->refill_min};
+    return BHS_OK;
+}
+
+int bhs_get_tuning(bhs_ctx* ctx, bhs_tuning* t) {
+    if (!ctx || !t) return BHS_E_INVALID;
+    t->mode = ctx->tuning.mode;
+    t->block = ctx->tuning.block;
+    t->waves_per_simd = ctx->tuning.waves_per_simd;
+    t->refill_min = ctx->tuning.refill_min;
+    return BHS_OK;
+}
+
+int bhs_autotune(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
+                 const bhs_ray_soa* out, int32_t reps, bhs_tuning* best, double* best_ms) {
+    if (!ctx) return BHS_E_INVALID;
+    if (reps < 1 || reps > 1000) return fail(ctx, BHS_E_INVALID, "reps must be in 1..1000%s");
+    static const bhs::Tuning cands[] = {
+        {0, 256, 8, 64}, {0, 64, 8, 64},
+        {1, 256, 8, 64}, {1, 256, 8, 32}, {1, 256, 8, 16},
+        {1, 256, 7, 64}, {1, 256, 7, 32}, {1, 256, 7, 16},
+        {1, 256, 6, 64}, {1, 256, 6, 32}, {1, 256, 6, 16},
+        {1, 64, 8, 32}, {1, 64, 7, 32}, {1, 256, 4, 32},
+    };
+    DeviceGuard g(ctx->device);
+    const bhs::Tuning saved = ctx->tuning;
+    const bool saved_timing = ctx->timing;
+    double best_t = 1e30;
+    bhs::Tuning best_c = saved;
+    std::vector<double> ts;
+    for (const bhs::Tuning& cand : cands) {
+        ctx->tuning = cand;
+        ctx->timing = true;
+        ctx->timers_used = 0;
+        ts.clear();
+        for (int r = 0; r <= reps; ++r) {
+            int st = bhs_render(ctx, cam, params, out, nullptr);
+            if (st != BHS_OK) {
+                ctx->tuning = saved;
+                ctx->timing = saved_timing;
+                ctx->timers_used = 0;
+                return st;
+            }
+        }
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            ctx->tuning = saved;
+            ctx->timing = saved_timing;
+            return hip_fail(ctx, e, "bhs_autotune");
+        }
+        for (size_t i = 1; i < ctx->timers_used; ++i) {   // first launch is the warm-up
+            float a = 0.0f;
+            if (hipEventElapsedTime(&a, ctx->timers[i].k0, ctx->timers[i].k1) == hipSuccess) ts.push_back(a);
+        }
+        ctx->timers_used = 0;
+        if (ts.empty()) continue;
+        std::sort(ts.begin(), ts.end());
+        const double med = ts[ts.size() / 2];
+        if (med < best_t) {
+            best_t = med;
+            best_c = cand;
+        }
+    }
+    ctx->tuning = best_c;
+    ctx->timing = saved_timing;
+    if (best) {
+        best->mode = best_c.mode;
+        best->block = best_c.block;
+        best->waves_per_simd = best_c.waves_per_simd;
+        best->refill_min = best_c.refill_min;
+    }
+    if (best_ms) *best_ms = best_t;
     return BHS_OK;
 }
 

@@ -62,36 +62,180 @@ __device__ __forceinline__ void tile_pixel(int& col, int& row) {
     row = blockIdx.y * kTile + (w >> 1) * 8 + (l >> 3);
 }
 
-__device__ __forceinline__ bool in_band(const MarchConst& c, float h) {
-    return __builtin_fabsf(h - kBcritF) < c.band_abs;
+__device__ __forceinline__ bool in_band(const MarchConst& c, float b) {
+    return __builtin_fabsf(b - kBcritF) < c.band_abs;
 }
 
 // ------------------------------------------------------------------------------------------------
-// FP32 law-S render of one camera. In BAND64 precision, rays in the FP64 band are appended to the
-// band list and left to k_band64.
-__global__ __launch_bounds__(kBlock) void k_render_f32(const MarchConst c, const SoA out, Scratch sc) {
+// Lists the rays of the FP64 band (float setup, |h - b_crit| < band_abs) of one camera.
+__global__ __launch_bounds__(kBlock) void k_band_scan(const MarchConst c, Scratch sc) {
     int col, row;
     tile_pixel(col, row);
+    if (col >= c.width || row >= c.height) return;
+    const int64_t pix = (int64_t)row * c.width + col;
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    const Setup<float> s = lawS_setup(P, ray_dir(c, pix));
+    if (in_band(c, s.b)) {
+        const uint32_t k = atomicAdd(sc.band_count, 1u);
+        if (k < sc.band_cap) sc.band_ids[k] = (uint32_t)pix;
+    }
+}
+
+// FP32 law-S render of one camera, one lane per pixel on a static grid of 8x8-pixel wave tiles
+// (BLOCK = 64: one tile per block; 256: 16x16 pixels per block). In BAND64 precision the rays of
+// the FP64 band are skipped here (k_band_scan lists them, k_band64 marches them).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
+void k_render_f32(const MarchConst c, const SoA out, Scratch sc) {
+    int col, row;
+    if (BLOCK == 256) {
+        tile_pixel(col, row);
+    } else {
+        col = blockIdx.x * 8 + (threadIdx.x & 7);
+        row = blockIdx.y * 8 + (threadIdx.x >> 3);
+    }
     const bool valid = col < c.width && row < c.height;
-    uint32_t steps = 0, cross = 0, esc = 0;
+    uint32_t steps = 0, cross = 0, esc = 0, rays = 0;
     if (valid) {
         const int64_t pix = (int64_t)row * c.width + col;
         const V3<float> d = ray_dir(c, pix);
         const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
         const Setup<float> s = lawS_setup(P, d);
-        if (c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.h)) {
-            const uint32_t k = atomicAdd(sc.band_count, 1u);
-            if (k < sc.band_cap) sc.band_ids[k] = (uint32_t)pix;
-        } else {
+        if (!(c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.b))) {
             RayOut o;
             lawS_march(P, s, c, o, sc.roots ? sc.roots + 2 * pix : (RootState<float>*)nullptr);
             store_ray(out, c.ray_offset + pix, o);
             steps = (uint32_t)o.steps;
             cross = o.n_cross;
             esc = (o.flags & 2u) ? 1u : 0u;
+            rays = 1;
         }
     }
-    add_stats(sc.stats, valid ? 1u : 0u, steps, cross, esc, 0u);
+    add_stats(sc.stats, rays, steps, cross, esc, 0u);
+}
+
+// Work item q of the persistent kernels: ray index in 8x8-tile-major order over the image
+// (tile t = q / 64 row-major over the tile grid, lane q % 64 row-major inside the tile). Items of the
+// padding beyond the image map to -1.
+__device__ __forceinline__ int64_t item_pixel(const MarchConst& c, uint32_t q, uint32_t tiles_x) {
+    const uint32_t t = q >> 6, l = q & 63u;
+    const uint32_t col = (t % tiles_x) * 8u + (l & 7u), row = (t / tiles_x) * 8u + (l >> 3);
+    return (col < (uint32_t)c.width && row < (uint32_t)c.height) ? (int64_t)row * c.width + col : -1;
+}
+
+// Sink writing a ray's records straight to the SoA output (and the crossing states to the
+// root-refinement scratch).
+struct GlobalSink {
+    const SoA& o;
+    int64_t i;
+    RootState<float>* rs;
+    __device__ __forceinline__ void escape(float ex, float ey, float ez) {
+        if (o.esc) {
+            o.esc[i] = ex;
+            o.esc[i + o.n] = ey;
+            o.esc[i + 2 * o.n] = ez;
+        }
+    }
+    __device__ __forceinline__ void cross(uint32_t slot, float r, float phi, float g,
+                                          const RootState<float>& q) {
+        if (o.disk) {
+            const int64_t j = i + (int64_t)slot * o.n;
+            o.disk[j] = r;
+            o.disk[j + 2 * o.n] = phi;
+            o.disk[j + 4 * o.n] = g;
+        }
+        if (rs) rs[slot] = q;
+    }
+};
+
+// Scalar outputs of a finished ray, and zeros for the records it did not produce.
+__device__ __forceinline__ void store_finish(const SoA& o, int64_t i, const MarchState<float>& st,
+                                             float b) {
+    uint32_t cls, flags;
+    march_class(st, cls, flags);
+    const int64_t n = o.n;
+    if (o.cls) o.cls[i] = (uint8_t)cls;
+    if (o.flags) o.flags[i] = (uint8_t)flags;
+    if (o.n_cross) o.n_cross[i] = (uint8_t)st.n_cross;
+    if (o.steps) o.steps[i] = (uint16_t)(st.i > 65535 ? 65535 : st.i);
+    if (o.b) o.b[i] = b;
+    if (o.esc && st.status != 1) {
+        o.esc[i] = 0.0f;
+        o.esc[i + n] = 0.0f;
+        o.esc[i + 2 * n] = 0.0f;
+    }
+    if (o.disk) {
+        for (uint32_t k = st.n_cross; k < 2; ++k) {
+            o.disk[i + k * n] = 0.0f;
+            o.disk[i + (2 + k) * n] = 0.0f;
+            o.disk[i + (4 + k) * n] = 0.0f;
+        }
+    }
+}
+
+// FP32 law-S render of one camera by resident waves that pull rays from a global work queue.
+// Each wave keeps 64 rays in flight; when at least `refill_min` lanes have finished, it takes that
+// many new rays with one atomic (wave ballot + per-lane prefix count of the idle lanes) and
+// continues, so finished lanes are refilled instead of idling until the slowest lane ends.
+// Exit: a wave leaves when the queue is exhausted and all its lanes are finished; every ray ends
+// within max_steps iterations and the counter only grows, so every wave reaches it.
+template <int BLOCK, bool ROOTS, int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+void k_render_f32_persist(const MarchConst c, const SoA out, Scratch sc, uint32_t n_items,
+                          uint32_t tiles_x, uint32_t refill_min) {
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    const MarchK<float> k = march_consts<float>(c);
+    const bool band = c.precision == BHS_PREC_F32_BAND64;
+    const uint32_t lane = threadIdx.x & 63u;
+    MarchState<float> st;
+    Setup<float> s;
+    uint32_t pix = 0;
+    bool active = false;
+    bool more = true;   // wave-uniform: the queue may still hold items
+    uint32_t n_rays = 0, n_steps = 0, n_cross = 0, n_esc = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (more && n_idle >= refill_min) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(sc.work, n_idle);
+            base = (uint32_t)__shfl((int)base, 0);
+            if (base + n_idle >= n_items) more = false;
+            if (!active) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const uint32_t q = base + rank;
+                if (q < n_items) {
+                    const int64_t p = item_pixel(c, q, tiles_x);
+                    if (p >= 0) {
+                        pix = (uint32_t)p;
+                        s = lawS_setup(P, ray_dir(c, p));
+                        if (!(band && in_band(c, s.b))) {
+                            march_init(st, P, s);
+                            active = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (!more) break;
+            continue;
+        }
+        if (active) {
+            const int64_t gi = c.ray_offset + pix;
+            GlobalSink sink{out, gi, ROOTS ? sc.roots + 2 * (int64_t)pix : (RootState<float>*)nullptr};
+            if (march_step(st, s, k, sink)) {
+                store_finish(out, gi, st, s.b);
+                n_rays += 1;
+                n_steps += (uint32_t)st.i;
+                n_cross += st.n_cross;
+                n_esc += st.status == 1 ? 1u : 0u;
+                active = false;
+            }
+        }
+    }
+    add_stats(sc.stats, n_rays, n_steps, n_cross, n_esc, 0u);
 }
 
 // FP64 law-S march of one ray, refined crossings included (as the oracle's double march).
@@ -139,6 +283,8 @@ __global__ __launch_bounds__(kBlock) void k_render_f64(const MarchConst c, const
 
 // FP64 march of the band list (grid-stride; the count is read on the device).
 __global__ __launch_bounds__(64) void k_band64(const MarchConst c, const SoA out, Scratch sc) {
+    // Few waves with long serial chains run beside the FP32 march: issue them first on their SIMD.
+    __builtin_amdgcn_s_setprio(3);
     const uint32_t cnt = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
     for (uint32_t base = blockIdx.x * 64; base < cnt; base += gridDim.x * 64) {
         const uint32_t k = base + threadIdx.x;
@@ -153,7 +299,7 @@ __global__ __launch_bounds__(64) void k_band64(const MarchConst c, const SoA out
             esc = (o.flags & 2u) ? 1u : 0u;
             one = 1;
         }
-        add_stats(sc.stats, 0u, steps, cross, esc, one);
+        add_stats(sc.stats, one, steps, cross, esc, one);
     }
 }
 
@@ -199,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_march_rays(const float* __restrict__
         const Setup<float> s = lawS_setup(P, d);
         RayOut o;
         if (c.precision == BHS_PREC_F64 ||
-            (c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.h))) {
+            (c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.b))) {
             o = march_f64(c, P, d);
             band = 1;
         } else {
@@ -298,23 +444,76 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_render(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s,
-                         hipEvent_t ev_begin, hipEvent_t ev_end) {
+// Persistent-kernel instantiation for a block size, a root-state flag and an occupancy target
+// (waves per SIMD 8, 7, 6 or 4: the VGPR budget is 512 / waves).
+template <int BLOCK, bool ROOTS>
+hipError_t launch_persist_w(dim3 g, int wps, hipStream_t s, const MarchConst& c, const SoA& out,
+                            const Scratch& sc, uint32_t n_items, uint32_t tiles_x, uint32_t rmin) {
+    if (wps >= 8)
+        hipLaunchKernelGGL((k_render_f32_persist<BLOCK, ROOTS, 8>), g, dim3(BLOCK), 0, s, c, out, sc, n_items, tiles_x, rmin);
+    else if (wps == 7)
+        hipLaunchKernelGGL((k_render_f32_persist<BLOCK, ROOTS, 7>), g, dim3(BLOCK), 0, s, c, out, sc, n_items, tiles_x, rmin);
+    else if (wps >= 5)
+        hipLaunchKernelGGL((k_render_f32_persist<BLOCK, ROOTS, 6>), g, dim3(BLOCK), 0, s, c, out, sc, n_items, tiles_x, rmin);
+    else
+        hipLaunchKernelGGL((k_render_f32_persist<BLOCK, ROOTS, 4>), g, dim3(BLOCK), 0, s, c, out, sc, n_items, tiles_x, rmin);
+    return hipGetLastError();
+}
+
+hipError_t launch_persist(dim3 g, int block, int wps, bool roots, hipStream_t s, const MarchConst& c,
+                          const SoA& out, const Scratch& sc, uint32_t n_items, uint32_t tiles_x,
+                          uint32_t rmin) {
+    if (block == 64)
+        return roots ? launch_persist_w<64, true>(g, wps, s, c, out, sc, n_items, tiles_x, rmin)
+                     : launch_persist_w<64, false>(g, wps, s, c, out, sc, n_items, tiles_x, rmin);
+    return roots ? launch_persist_w<256, true>(g, wps, s, c, out, sc, n_items, tiles_x, rmin)
+                 : launch_persist_w<256, false>(g, wps, s, c, out, sc, n_items, tiles_x, rmin);
+}
+
+hipError_t launch_render(const MarchConst& c, const SoA& out, const Scratch& sc, const Tuning& t,
+                         int num_cus, hipStream_t s, hipStream_t aux, hipEvent_t ev_fork,
+                         hipEvent_t ev_join, hipEvent_t ev_begin, hipEvent_t ev_end) {
     const dim3 grid((c.width + kTile - 1) / kTile, (c.height + kTile - 1) / kTile);
     const int64_t n = (int64_t)c.width * c.height;
     hipError_t e;
+    const bool band = c.precision == BHS_PREC_F32_BAND64;
+    if (band) {
+        // Fork: the FP64 band rays (few, but up to max_steps long) run on the auxiliary stream
+        // while the FP32 march runs on `s`, instead of trailing it.
+        hipLaunchKernelGGL(k_band_scan, grid, dim3(kBlock), 0, s, c, sc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_band64, dim3(256), dim3(64), 0, aux, c, out, sc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev_join, aux)) != hipSuccess) return e;
+    }
     if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
-    if (c.precision == BHS_PREC_F64)
+    if (c.precision == BHS_PREC_F64) {
         hipLaunchKernelGGL(k_render_f64, grid, dim3(kBlock), 0, s, c, out, sc);
-    else
-        hipLaunchKernelGGL(k_render_f32, grid, dim3(kBlock), 0, s, c, out, sc);
+    } else if (t.mode == 1) {
+        const uint32_t tiles_x = (uint32_t)((c.width + 7) / 8), tiles_y = (uint32_t)((c.height + 7) / 8);
+        const uint32_t n_items = tiles_x * tiles_y * 64u;
+        const int block = t.block == 64 ? 64 : 256;
+        const int wps = t.waves_per_simd < 1 ? 1 : (t.waves_per_simd > 8 ? 8 : t.waves_per_simd);
+        const uint32_t waves = (uint32_t)num_cus * 4u * (uint32_t)wps;
+        const uint32_t need = (n_items + 63u) / 64u;
+        const uint32_t blocks = (waves < need ? waves : need) * 64u / (uint32_t)block;
+        const uint32_t rmin = t.refill_min < 1 ? 1u : (t.refill_min > 64 ? 64u : (uint32_t)t.refill_min);
+        const dim3 g(blocks ? blocks : 1);
+        if ((e = launch_persist(g, block, wps, sc.roots != nullptr, s, c, out, sc, n_items, tiles_x,
+                                rmin)) != hipSuccess)
+            return e;
+    } else if (t.block == 64) {
+        const dim3 g8((c.width + 7) / 8, (c.height + 7) / 8);
+        hipLaunchKernelGGL(k_render_f32<64>, g8, dim3(64), 0, s, c, out, sc);
+    } else {
+        hipLaunchKernelGGL(k_render_f32<256>, grid, dim3(256), 0, s, c, out, sc);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
+    if (band && (e = hipStreamWaitEvent(s, ev_join, 0)) != hipSuccess) return e;
     if (c.precision == BHS_PREC_F64) return hipSuccess;
-    if (c.precision == BHS_PREC_F32_BAND64) {
-        hipLaunchKernelGGL(k_band64, dim3(256), dim3(64), 0, s, c, out, sc);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
     if (c.root64 && sc.roots) {
         hipLaunchKernelGGL(k_root64, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                            c, out, sc, n);

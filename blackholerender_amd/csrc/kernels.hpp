@@ -28,16 +28,28 @@ struct Stats {
 // Scratch used by one render launch.
 struct Scratch {
     Stats* stats;
+    uint32_t* work;              // work-queue counter of the persistent kernels
     uint32_t* band_count;        // number of rays in the FP64 band list
     uint32_t* band_ids;          // ray index (camera-local pixel for single-camera launches)
     uint32_t band_cap;
     RootState<float>* roots;     // 2 per ray when root refinement is on, else null
 };
 
+// Launch-shape choices of the FP32 march (see bhs_tuning in include/bhs_trace.h).
+struct Tuning {
+    int32_t mode;            // 0 static 8x8-tile grid, 1 persistent work queue with lane refill
+    int32_t block;           // threads per block: 64 or 256
+    int32_t waves_per_simd;  // persistent: resident waves per SIMD (1..8)
+    int32_t refill_min;      // persistent: refill when at least this many lanes are idle (1..64)
+};
+
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
-// ev_begin/ev_end (may be null) are recorded on `s` around the main march kernel.
-hipError_t launch_render(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s,
-                         hipEvent_t ev_begin, hipEvent_t ev_end);
+// ev_begin/ev_end (may be null) are recorded on `s` around the main march kernel. In BAND64
+// precision the FP64 band kernel runs on `aux` between ev_fork (recorded on s) and ev_join (waited
+// for on s).
+hipError_t launch_render(const MarchConst& c, const SoA& out, const Scratch& sc, const Tuning& t,
+                         int num_cus, hipStream_t s, hipStream_t aux, hipEvent_t ev_fork,
+                         hipEvent_t ev_join, hipEvent_t ev_begin, hipEvent_t ev_end);
 hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, const MarchConst& c,
                              const SoA& out, const Scratch& sc, hipStream_t s, hipEvent_t ev_begin,
                              hipEvent_t ev_end);

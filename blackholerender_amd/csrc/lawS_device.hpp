@@ -132,11 +132,13 @@ __device__ __forceinline__ V3<float> ray_dir(const MarchConst& c, int64_t pixel)
 // ---------------------------------------------------------------------------------- law S setup
 template <typename R> struct Setup {
     V3<R> v;
-    R h2, h, b_axis, lapse_g, far_mul, c3;
+    R h2, h, b, b_axis, lapse_g, far_mul, c3;
 };
 
 // blackhole.frag:77-93 with initialVelocity (geodesic.glsl:27-34) and impactParameterAboutAxis
-// (geodesic.glsl:37-40, axis +y).
+// (geodesic.glsl:37-40, axis +y), plus the impact parameter b = r sin(alpha)/sqrt(1-2/r)
+// (Geodesic.h:50-54) that selects the FP64 band (h = |x0 x v0| is not b: they differ by the factor
+// sqrt(1 - 2 cos^2(alpha)/r) / sqrt(1 - 2/r)).
 template <typename R>
 __device__ __forceinline__ Setup<R> lawS_setup(const V3<R>& P, const V3<R>& d) {
     using A = Ar<R>;
@@ -157,6 +159,13 @@ __device__ __forceinline__ Setup<R> lawS_setup(const V3<R>& P, const V3<R>& d) {
                   A::fma(P.x, v.y, -(P.y * v.x))};
     s.h2 = dot3(c, c);
     s.h = A::sqrt(s.h2);
+    {
+        const V3<R> cr{A::fma(rh.y, d.z, -(rh.z * d.y)), A::fma(rh.z, d.x, -(rh.x * d.z)),
+                       A::fma(rh.x, d.y, -(rh.y * d.x))};
+        const R sin_a = A::sqrt(dot3(cr, cr));
+        const R e12 = R(1e-12);
+        s.b = (rcam * sin_a) / A::sqrt(lapse2 > e12 ? lapse2 : e12);
+    }
     const R cdy = A::fma(P.z, d.x, -(P.x * d.z));
     const R e6 = R(1e-6), e4 = R(1e-4);
     s.b_axis = -cdy / A::sqrt(lapse2 > e6 ? lapse2 : e6);
@@ -239,93 +248,156 @@ __device__ __forceinline__ R redshift(R r, R b_axis, R lapse) {
 template <typename R> struct RootState { R x0[3], v0[3], dt; };
 
 // ---------------------------------------------------------------------------------- law S march
-// blackhole.frag:142-253 (thin disk). `rs` (may be null) receives the state of the steps that
-// produced crossing slots 0 and 1.
+// Per-camera constants of the march in the march's precision (blackhole.frag:89 rEscMin,
+// disk.glsl:24 slab half-height factor, the slab-clamp annulus of :172 as squared radii).
+template <typename R> struct MarchK {
+    R ss, dstep, inner, outer, slab_k, lo2, hi2, r_esc_min;
+    int32_t max_steps;
+    bool thin;
+};
+
 template <typename R>
-__device__ __forceinline__ void lawS_march(const V3<R>& P, const Setup<R>& s, const MarchConst& c,
-                                           RayOut& o, RootState<R>* rs) {
+__device__ __forceinline__ MarchK<R> march_consts(const MarchConst& c) {
+    MarchK<R> k;
+    k.ss = (R)c.step_scale;
+    k.dstep = (R)c.disk_step;
+    k.inner = (R)c.disk_inner;
+    k.outer = (R)c.disk_outer;
+    k.slab_k = R(2.5) * (R)c.disk_thickness;
+    const R lo = k.inner - R(1), hi = k.outer + R(1);
+    k.lo2 = lo > R(0) ? lo * lo : R(-1);
+    k.hi2 = hi * hi;
+    const R er = k.outer * R(1.25);
+    k.r_esc_min = er > R(12) ? er : R(12);
+    k.max_steps = c.max_steps;
+    k.thin = c.thin != 0;
+    return k;
+}
+
+// March state of one ray: position/velocity, completed iterations, terminal status (0 marching,
+// 1 escaped, 2 captured) and the number of in-annulus crossings. Crossing records and the escape
+// direction leave through a sink as soon as they are produced, so they occupy no registers.
+template <typename R> struct MarchState {
+    V3<R> x, v;
+    int32_t i;
+    int32_t status;
+    uint32_t n_cross;
+};
+
+template <typename R>
+__device__ __forceinline__ void march_init(MarchState<R>& st, const V3<R>& P, const Setup<R>& s) {
+    st.x = P;
+    st.v = s.v;
+    st.i = 0;
+    st.status = 0;
+    st.n_cross = 0;
+}
+
+// Sink that collects a ray's records in registers (RayOut) and optionally its crossing states.
+template <typename R> struct RegSink {
+    RayOut& o;
+    RootState<R>* rs;
+    __device__ __forceinline__ void escape(float ex, float ey, float ez) {
+        o.esc[0] = ex; o.esc[1] = ey; o.esc[2] = ez;
+    }
+    __device__ __forceinline__ void cross(uint32_t slot, float r, float phi, float g,
+                                          const RootState<R>& q) {
+        if (slot == 0) {
+            o.dr[0] = r; o.dphi[0] = phi; o.dg[0] = g;
+            if (rs) rs[0] = q;
+        } else {
+            o.dr[1] = r; o.dphi[1] = phi; o.dg[1] = g;
+            if (rs) rs[1] = q;
+        }
+    }
+};
+
+// One iteration of the march loop (blackhole.frag:142-216, thin disk). Returns true when the ray
+// is finished: horizon (:144), escape (:151-155) or out of steps (loop bound, :252).
+template <typename R, typename Sink>
+__device__ __forceinline__ bool march_step(MarchState<R>& st, const Setup<R>& s, const MarchK<R>& k,
+                                           Sink& sink) {
     using A = Ar<R>;
-    const R ss = (R)c.step_scale;
-    const R dstep = (R)c.disk_step;
-    const R inner = (R)c.disk_inner, outer = (R)c.disk_outer;
-    const R slab_k = R(2.5) * (R)c.disk_thickness;
-    const R lo = inner - R(1), hi = outer + R(1);
-    const R lo2 = lo > R(0) ? lo * lo : R(-1);
-    const R hi2 = hi * hi;
-    const R er = outer * R(1.25);
-    const R r_esc_min = er > R(12) ? er : R(12);
-    const bool thin = c.thin != 0;
-
-    V3<R> x = P, v = s.v;
-    int32_t status = 0, steps = 0;
-    uint32_t n_cross = 0;
-    o.esc[0] = o.esc[1] = o.esc[2] = 0.0f;
-    o.dr[0] = o.dr[1] = o.dphi[0] = o.dphi[1] = o.dg[0] = o.dg[1] = 0.0f;
-
-    for (int32_t i = 0; i < c.max_steps; ++i) {
-        const R r2 = dot3(x, x);
-        const R r = A::sqrt(r2);
-        if (r < R(2.02)) { status = 2; break; }                              // :144
-        const R iv = R(1) / A::sqrt(dot3(v, v));
-        if (dot3(x, v) > R(0) && r > r_esc_min) {                            // :151-155
-            const R b = s.h * iv;
-            const R q = (((R(0.75) * b) * b) * b) / (r2 * r2);
-            if (q < R(0.01)) {
-                const V3<R> e = far_field(x, v, r, iv, s.h2);
-                o.esc[0] = (float)e.x; o.esc[1] = (float)e.y; o.esc[2] = (float)e.z;
-                status = 1;
-                break;
-            }
+    if (st.i >= k.max_steps) return true;
+    V3<R>& x = st.x;
+    V3<R>& v = st.v;
+    const R r2 = dot3(x, x);
+    const R r = A::sqrt(r2);
+    if (r < R(2.02)) { st.status = 2; return true; }                     // :144
+    const R iv = R(1) / A::sqrt(dot3(v, v));
+    if (dot3(x, v) > R(0) && r > k.r_esc_min) {                          // :151-155
+        const R b = s.h * iv;
+        const R q = (((R(0.75) * b) * b) * b) / (r2 * r2);
+        if (q < R(0.01)) {
+            const V3<R> e = far_field(x, v, r, iv, s.h2);
+            sink.escape((float)e.x, (float)e.y, (float)e.z);
+            st.status = 1;
+            return true;
         }
-        R ds = (ss * r) * s.far_mul;                                          // :169
-        if (ds < R(0.02)) ds = R(0.02);
-        if (thin) {                                                           // :170-181
-            const R rc2 = A::fma(x.z, x.z, x.x * x.x);
-            if (rc2 > lo2 && rc2 < hi2) {
-                const R rcyl = A::sqrt(rc2);
-                const R outside = A::fma(-slab_k, rcyl, A::abs(x.y));
-                if (outside > R(0)) {
-                    const R den = A::fma(A::abs(v.y), iv, slab_k);
-                    R cl = outside / den;
-                    if (cl < dstep) cl = dstep;
-                    if (cl < ds) ds = cl;
-                }
-            }
-        }
-        const R dt = ds * iv;                                                 // :192
-        const V3<R> x0 = x;
-        const V3<R> v0 = v;
-        rk4(x, v, s.c3, dt, r2, r);                                           // :194
-        steps = i + 1;
-        if (thin && ((x.y > R(0)) != (x0.y > R(0)))) {                        // :209-216
-            const R f = x0.y / (x0.y - x.y);
-            const R hx = A::fma(f, x.x - x0.x, x0.x);
-            const R hz = A::fma(f, x.z - x0.z, x0.z);
-            const R rh = A::sqrt(A::fma(hz, hz, hx * hx));
-            if (rh >= inner && rh <= outer) {                                 // disk.glsl:114-115
-                if (n_cross < 2) {
-                    o.dr[n_cross] = (float)rh;
-                    o.dphi[n_cross] = atan2_ieee((float)hz, (float)hx);
-                    o.dg[n_cross] = (float)redshift(rh, s.b_axis, s.lapse_g);
-                    if (rs) {
-                        RootState<R>& q = rs[n_cross];
-                        q.x0[0] = x0.x; q.x0[1] = x0.y; q.x0[2] = x0.z;
-                        q.v0[0] = v0.x; q.v0[1] = v0.y; q.v0[2] = v0.z;
-                        q.dt = dt;
-                    }
-                }
-                if (n_cross < 255u) ++n_cross;
+    }
+    R ds = (k.ss * r) * s.far_mul;                                        // :169
+    if (ds < R(0.02)) ds = R(0.02);
+    if (k.thin) {                                                         // :170-181
+        const R rc2 = A::fma(x.z, x.z, x.x * x.x);
+        if (rc2 > k.lo2 && rc2 < k.hi2) {
+            const R rcyl = A::sqrt(rc2);
+            const R outside = A::fma(-k.slab_k, rcyl, A::abs(x.y));
+            if (outside > R(0)) {
+                const R den = A::fma(A::abs(v.y), iv, k.slab_k);
+                R cl = outside / den;
+                if (cl < k.dstep) cl = k.dstep;
+                if (cl < ds) ds = cl;
             }
         }
     }
-    uint32_t flags = 0;
-    if (status == 0) { status = 2; flags |= 1u; }                            // :252
+    const R dt = ds * iv;                                                 // :192
+    const V3<R> x0 = x;
+    const V3<R> v0 = v;
+    rk4(x, v, s.c3, dt, r2, r);                                           // :194
+    st.i = st.i + 1;                                                      // :196
+    if (k.thin && ((x.y > R(0)) != (x0.y > R(0)))) {                      // :209-216
+        const R f = x0.y / (x0.y - x.y);
+        const R hx = A::fma(f, x.x - x0.x, x0.x);
+        const R hz = A::fma(f, x.z - x0.z, x0.z);
+        const R rh = A::sqrt(A::fma(hz, hz, hx * hx));
+        if (rh >= k.inner && rh <= k.outer) {                             // disk.glsl:114-115
+            if (st.n_cross < 2) {
+                const RootState<R> q{{x0.x, x0.y, x0.z}, {v0.x, v0.y, v0.z}, dt};
+                sink.cross(st.n_cross, (float)rh, atan2_ieee((float)hz, (float)hx),
+                           (float)redshift(rh, s.b_axis, s.lapse_g), q);
+            }
+            if (st.n_cross < 255u) st.n_cross = st.n_cross + 1;
+        }
+    }
+    return false;
+}
+
+// Terminal class and flags (blackhole.frag:252: out of steps counts as captured).
+template <typename R>
+__device__ __forceinline__ void march_class(const MarchState<R>& st, uint32_t& cls, uint32_t& flags) {
+    int32_t status = st.status;
+    flags = 0;
+    if (status == 0) { status = 2; flags |= 1u; }
     if (status == 1) flags |= 2u;
-    o.flags = flags;
-    o.steps = steps;
-    o.n_cross = n_cross;
-    o.cls = n_cross > 0 ? 1u : (status == 1 ? 2u : 0u);
-    o.b = (float)s.h;
+    cls = st.n_cross > 0 ? 1u : (status == 1 ? 2u : 0u);
+}
+
+// The whole march of one ray (blackhole.frag:142-253) into a RayOut.
+template <typename R>
+__device__ __forceinline__ void lawS_march(const V3<R>& P, const Setup<R>& s, const MarchConst& c,
+                                           RayOut& o, RootState<R>* rs) {
+    const MarchK<R> k = march_consts<R>(c);
+    MarchState<R> st;
+    march_init(st, P, s);
+    o.esc[0] = o.esc[1] = o.esc[2] = 0.0f;
+    o.dr[0] = o.dr[1] = o.dphi[0] = o.dphi[1] = o.dg[0] = o.dg[1] = 0.0f;
+    RegSink<R> sink{o, rs};
+    while (!march_step(st, s, k, sink)) {
+    }
+    march_class(st, o.cls, o.flags);
+    o.steps = st.i;
+    o.n_cross = st.n_cross;
+    o.b = (float)s.b;
 }
 
 // ---------------------------------------------------------------------------------- FP64 root

@@ -105,7 +105,8 @@ typedef struct bhs_march_params {
     float step_scale;         /* uStepScale */
     float disk_step;          /* uDiskStep */
     float disk_inner, disk_outer, disk_thickness;
-    float band_rel;           /* FP64 band half-width as a fraction of b_crit (default 1e-3) */
+    float band_rel;           /* FP64 band half-width as a fraction of b_crit (0.02 keeps the FP32
+                                 escape direction within 1e-4 rad of FP64 at 1080p) */
 } bhs_march_params;
 
 /* Structure-of-arrays ray output in device memory; plane k of a multi-plane array starts at
@@ -159,6 +160,27 @@ int bhs_march_rays(bhs_ctx* ctx, const float* pos3_dev, const float* dir3_dev, i
 /* Aggregate counters of the most recent bhs_render / bhs_march_rays / bhs_render_batch call
  * (synchronises the context's last stream). */
 int bhs_last_stats(bhs_ctx* ctx, bhs_stats* out_host);
+
+/* Launch shape of the FP32 march. mode 0: one lane per pixel on a static grid of 8x8-pixel wave
+ * tiles; mode 1: resident waves pull rays from a global work queue and refill finished lanes
+ * (wave ballot + prefix count, one atomic per refill). Chosen per context by bhs_autotune or set by
+ * hand; bit-identical results in every shape. */
+typedef struct bhs_tuning {
+    int32_t mode;             /* 0 static tiles, 1 persistent work queue */
+    int32_t block;            /* threads per block: 64 or 256 */
+    int32_t waves_per_simd;   /* mode 1: resident waves per SIMD, 1..8 */
+    int32_t refill_min;       /* mode 1: refill when at least this many lanes are idle, 1..64 */
+} bhs_tuning;
+
+int bhs_set_tuning(bhs_ctx* ctx, const bhs_tuning* t);
+int bhs_get_tuning(bhs_ctx* ctx, bhs_tuning* t);
+
+/* Project autotuner: times bhs_render of `cam` over a fixed candidate set of launch shapes (each
+ * `reps` times after one warm-up, median of the main-kernel GPU time), keeps the fastest in the
+ * context and returns it in *best (and its median ms in *best_ms if non-NULL). `out` receives the
+ * renders. Synchronous. */
+int bhs_autotune(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
+                 const bhs_ray_soa* out, int32_t reps, bhs_tuning* best, double* best_ms);
 
 /* GPU timers (the trace-pass GL_TIME_ELAPSED timer of Renderer::render, src/Renderer.cpp:159-178).
  * While timing is on, every launch records HIP events on its own stream around its main march

@@ -90,6 +90,8 @@ def lib():
             getattr(L, name).argtypes = [fp, fp, C.POINTER(LawSParams), C.POINTER(LawSRay)]
         L.oracle_lawS_h_f32.argtypes = [fp, fp]
         L.oracle_lawS_h_f32.restype = C.c_float
+        L.oracle_lawS_b_f32.argtypes = [fp, fp]
+        L.oracle_lawS_b_f32.restype = C.c_float
         L.oracle_render.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
                                     C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(LawSParams),
                                     C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_void_p,
@@ -174,7 +176,7 @@ def trace_batch(pos3, dir3, max_steps=200000, step_scale=0.005, esc_tol=1e-4, r_
 
 # ---------------------------------------------------------------------------------- law S
 def lawS_params(max_steps=450, step_scale=0.05, disk_step=0.18, disk_inner=6.0, disk_outer=20.0,
-                disk_thickness=0.035, disk_mode=1, root64=0, band_rel=1e-3) -> LawSParams:
+                disk_thickness=0.035, disk_mode=1, root64=0, band_rel=0.02) -> LawSParams:
     return LawSParams(max_steps, disk_mode, root64, 0, step_scale, disk_step, disk_inner,
                       disk_outer, disk_thickness, band_rel)
 
@@ -202,6 +204,12 @@ def lawS_h_f32(pos, d) -> float:
     p, pp = _f(pos, 3)
     dd, dp_ = _f(d, 3)
     return lib().oracle_lawS_h_f32(pp, dp_)
+
+
+def lawS_b_f32(pos, d) -> float:
+    p, pp = _f(pos, 3)
+    dd, dp_ = _f(d, 3)
+    return lib().oracle_lawS_b_f32(pp, dp_)
 
 
 def atan2f(y, x) -> float:

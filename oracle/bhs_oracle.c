@@ -295,6 +295,12 @@ float oracle_lawS_h_f32(const float pos[3], const float dir[3]) {
     return s.h;
 }
 
+float oracle_lawS_b_f32(const float pos[3], const float dir[3]) {
+    setup_f32 s;
+    lawS_setup_f32(pos, dir, &s);
+    return s.b;
+}
+
 void oracle_lawS_f32(const float pos[3], const float dir[3], const oracle_lawS_params* p,
                      oracle_lawS_ray* out) {
     setup_f32 s;
@@ -321,8 +327,8 @@ void oracle_lawS_f64(const float pos[3], const float dir[3], const oracle_lawS_p
 
 void oracle_lawS_band(const float pos[3], const float dir[3], const oracle_lawS_params* p,
                       oracle_lawS_ray* out) {
-    const float h = oracle_lawS_h_f32(pos, dir);
-    if (fabsf(h - kBcritF) < p->band_rel * kBcritF)
+    const float b = oracle_lawS_b_f32(pos, dir);
+    if (fabsf(b - kBcritF) < p->band_rel * kBcritF)
         oracle_lawS_f64(pos, dir, p, out);
     else
         oracle_lawS_f32(pos, dir, p, out);

@@ -75,7 +75,7 @@ typedef struct oracle_lawS_ray {
     int32_t n_cross;
     float esc[3];
     float disk_r[2], disk_phi[2], disk_g[2];
-    float b;          /* h = |x0 x v0| of the march's own precision */
+    float b;          /* impact parameter r sin(alpha)/sqrt(1-2/r) of the march's own precision */
 } oracle_lawS_ray;
 
 /* Ray generation (blackhole.frag:68-72 with Renderer.cpp:246-252 and common.glsl:28-32 pcg for the
@@ -91,12 +91,13 @@ void oracle_lawS_f32(const float pos[3], const float dir[3], const oracle_lawS_p
                      oracle_lawS_ray* out);
 void oracle_lawS_f64(const float pos[3], const float dir[3], const oracle_lawS_params* p,
                      oracle_lawS_ray* out);
-/* Float march with the FP64 band: rays whose float h satisfies |h - b_crit| < band_rel * b_crit
- * are marched in double instead (flag 4). */
+/* Float march with the FP64 band: rays whose float impact parameter b satisfies
+ * |b - b_crit| < band_rel * b_crit are marched in double instead (flag 4). */
 void oracle_lawS_band(const float pos[3], const float dir[3], const oracle_lawS_params* p,
                       oracle_lawS_ray* out);
-/* The float setup's h (impact parameter) for a ray. */
+/* The float setup's h = |x0 x v0| and impact parameter b for a ray. */
 float oracle_lawS_h_f32(const float pos[3], const float dir[3]);
+float oracle_lawS_b_f32(const float pos[3], const float dir[3]);
 
 /* Image of one camera: rays [first, first+count) of width*height pixels, `mode` 0 float, 1 double,
  * 2 float with FP64 band; out[count]. */

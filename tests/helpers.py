@@ -54,7 +54,7 @@ def escape_angle_error(esc_a, esc_b):
     return np.arctan2(cr, dt)
 
 
-def oracle_params_for(cfg, root64=0, band_rel=1e-3):
+def oracle_params_for(cfg, root64=0, band_rel=0.02):
     """oracle.LawSParams with the settings of a workload dict (blackholerender_amd.workload)."""
     import oracle
     from blackholerender_amd.camera import PRESETS

@@ -34,3 +34,17 @@ def test_create_without_device_fails_cleanly():
     st = C.c_int(0)
     ctx = _capi.load().bhs_create(0, C.byref(st))
     assert not ctx and st.value == -2
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_reference_checks():
+    """The reference's geodesic checks in C++ through include/bhs/physics/Geodesic.h (trace on the
+    GPU via the C-ABI); the binary is built by __graft_entry__.build()."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "geodesic_dropin_tests")
+    assert os.path.exists(exe), "build tests/cpp first (make -C tests/cpp)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failures" in r.stdout
