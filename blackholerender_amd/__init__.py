@@ -192,6 +192,42 @@ class Tracer:
         self._check(rc, "bhs_render")
         return out
 
+    @staticmethod
+    def _batch_structs(cams, params):
+        n = len(cams)
+        if isinstance(params, _capi.MarchParams):
+            params = [params] * n
+        if len(params) != n:
+            raise ValueError("one MarchParams per camera")
+        ca = (_capi.Camera * n)(*[_camera_struct(c) for c in cams])
+        pa = (_capi.MarchParams * n)(*params)
+        return ca, pa
+
+    @staticmethod
+    def batch_offsets(cams) -> list:
+        """First output index of each camera of a batch (prefix sums of width*height)."""
+        offs, o = [], 0
+        for c in cams:
+            offs.append(o)
+            o += c.width * c.height
+        return offs + [o]
+
+    def render_batch(self, cams, params, out: RayBuffers | None = None, stream=None) -> RayBuffers:
+        """Law-S render of up to 64 cameras in one persistent launch (bhs_render_batch); camera k's
+        rays start at batch_offsets(cams)[k]."""
+        ca, pa = self._batch_structs(cams, params)
+        if out is None:
+            out = RayBuffers(self.batch_offsets(cams)[-1], f"cuda:{self.device}")
+        soa = out.soa()
+        rc = self._lib.bhs_render_batch(self._ctx, ca, pa, len(cams), C.byref(soa),
+                                        C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_render_batch")
+        return out
+
+    def graph(self, cams, params, out: RayBuffers | None = None) -> "BatchGraph":
+        """Captures the batch launch into a HIP graph (bhs_graph_create) for replay."""
+        return BatchGraph(self, cams, params, out)
+
     def march_rays(self, pos3, dir3, params: _capi.MarchParams | None = None,
                    out: RayBuffers | None = None, stream=None) -> RayBuffers:
         """Law S on explicit rays; pos3/dir3 are (n,3) float32 CUDA tensors."""
@@ -258,6 +294,52 @@ class Tracer:
 
     def sync(self):
         self._check(self._lib.bhs_sync(self._ctx), "bhs_sync")
+
+
+class BatchGraph:
+    """A captured batch launch (bhs_graph_*): replay with launch(), new poses with update()."""
+
+    def __init__(self, tracer: Tracer, cams, params, out: RayBuffers | None = None):
+        self.tracer = tracer
+        self.cams = list(cams)
+        ca, pa = tracer._batch_structs(self.cams, params)
+        self.out = out if out is not None else RayBuffers(tracer.batch_offsets(self.cams)[-1],
+                                                          f"cuda:{tracer.device}")
+        self._soa = self.out.soa()
+        h = C.c_void_p()
+        rc = tracer._lib.bhs_graph_create(tracer._ctx, ca, pa, len(self.cams), C.byref(self._soa),
+                                          C.byref(h))
+        tracer._check(rc, "bhs_graph_create")
+        self._g = h
+
+    def launch(self, stream=None):
+        rc = self.tracer._lib.bhs_graph_launch(self.tracer._ctx, self._g,
+                                               C.c_void_p(self.tracer._stream_ptr(stream)))
+        self.tracer._check(rc, "bhs_graph_launch")
+        return self.out
+
+    def update(self, cams, params):
+        ca, pa = self.tracer._batch_structs(list(cams), params)
+        self.tracer._check(self.tracer._lib.bhs_graph_update(self.tracer._ctx, self._g, ca, pa),
+                           "bhs_graph_update")
+
+    def stats(self) -> dict:
+        s = _capi.Stats()
+        self.tracer._check(self.tracer._lib.bhs_graph_stats(self.tracer._ctx, self._g, C.byref(s)),
+                           "bhs_graph_stats")
+        return dict(rays=s.rays, steps=s.steps, crossings=s.crossings, escapes=s.escapes,
+                    band64=s.band64, fallback=s.fallback)
+
+    def close(self):
+        if getattr(self, "_g", None):
+            self.tracer._lib.bhs_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _tls = threading.local()

@@ -84,13 +84,21 @@ def load():
     L.bhs_last_stats.argtypes = [vp, C.POINTER(Stats)]
     L.bhs_set_timing.argtypes = [vp, C.c_int]
     L.bhs_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(i32)]
+    L.bhs_render_batch.argtypes = [vp, vp, vp, i32, C.POINTER(RaySoA), vp]
+    L.bhs_graph_create.argtypes = [vp, vp, vp, i32, C.POINTER(RaySoA), C.POINTER(vp)]
+    L.bhs_graph_update.argtypes = [vp, vp, vp, vp]
+    L.bhs_graph_launch.argtypes = [vp, vp, vp]
+    L.bhs_graph_stats.argtypes = [vp, vp, C.POINTER(Stats)]
+    L.bhs_graph_destroy.argtypes = [vp]
+    L.bhs_graph_destroy.restype = None
     L.bhs_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.bhs_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.bhs_autotune.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams), C.POINTER(RaySoA), i32,
                                C.POINTER(Tuning), C.POINTER(C.c_double)]
     for name in ("bhs_sync", "bhs_trace_rays", "bhs_render", "bhs_march_rays", "bhs_last_stats",
                  "bhs_set_timing", "bhs_timing", "bhs_set_tuning", "bhs_get_tuning",
-                 "bhs_autotune"):
+                 "bhs_autotune", "bhs_render_batch", "bhs_graph_create", "bhs_graph_update",
+                 "bhs_graph_launch", "bhs_graph_stats"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

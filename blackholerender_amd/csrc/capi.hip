@@ -32,6 +32,30 @@ struct bhs_ctx {
     size_t timers_used = 0;
     int num_cus = 256;
     bhs::Tuning tuning{1, 256, 7, 32};
+    // Batch camera tables: device copy, pinned staging buffer and the event of its last upload.
+    void* table_dev = nullptr;
+    size_t table_dev_bytes = 0;
+    void* table_host = nullptr;
+    size_t table_host_bytes = 0;
+    hipEvent_t ev_upload = nullptr;
+    bool upload_pending = false;
+};
+
+// A captured batch launch (bhs_graph_*): owns its camera table and scratch so later calls on the
+// context cannot move them, and its own capture/fork streams and events.
+struct bhs_graph {
+    int device = 0;
+    void* mem = nullptr;          // [table | scratch]
+    size_t bytes = 0;
+    int32_t ncam = 0;
+    int64_t n_rays = 0;
+    uint32_t n_items = 0;
+    std::vector<int32_t> dims;    // width, height per camera (updates must keep them)
+    bool any_band = false, any_root = false;
+    hipStream_t cap = nullptr, aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -190,21 +214,87 @@ bhs_ctx::TimingRec* next_timer(bhs_ctx* ctx) {
     return &ctx->timers[ctx->timers_used++];
 }
 
-// Lays out the render scratch for n rays; returns the Scratch view and the bytes to clear.
-int prepare_scratch(bhs_ctx* ctx, int64_t n, bool roots, bhs::Scratch* sc) {
-    const size_t band_bytes = ((size_t)n * 4 + 255) & ~(size_t)255;
-    const size_t root_bytes = roots ? (size_t)n * 2 * sizeof(bhs::RootState<float>) : 0;
-    const size_t need = kStatsBytes + band_bytes + root_bytes;
-    int st = ensure(ctx, &ctx->scratch, &ctx->scratch_bytes, need, "render scratch");
-    if (st != BHS_OK) return st;
-    char* base = (char*)ctx->scratch;
+// Render scratch for n_rays rays and n_items work items: [Stats | band count | work counter]
+// (kStatsBytes, cleared per launch) [band ids] [root states, 2 per ray].
+size_t scratch_bytes(int64_t n_rays, int64_t n_items, bool roots) {
+    const size_t band_bytes = ((size_t)n_items * 4 + 255) & ~(size_t)255;
+    const size_t root_bytes = roots ? (size_t)n_rays * 2 * sizeof(bhs::RootState<float>) : 0;
+    return kStatsBytes + band_bytes + root_bytes;
+}
+
+void layout_scratch(void* mem, int64_t n_items, bool roots, bhs::Scratch* sc) {
+    const size_t band_bytes = ((size_t)n_items * 4 + 255) & ~(size_t)255;
+    char* base = (char*)mem;
     sc->stats = (bhs::Stats*)base;
     sc->band_count = (uint32_t*)(base + sizeof(bhs::Stats));
     sc->work = (uint32_t*)(base + sizeof(bhs::Stats) + 4);
     sc->band_ids = (uint32_t*)(base + kStatsBytes);
-    sc->band_cap = (uint32_t)n;
+    sc->band_cap = (uint32_t)n_items;
     sc->roots = roots ? (bhs::RootState<float>*)(base + kStatsBytes + band_bytes) : nullptr;
+}
+
+int prepare_scratch(bhs_ctx* ctx, int64_t n_rays, int64_t n_items, bool roots, bhs::Scratch* sc) {
+    int st = ensure(ctx, &ctx->scratch, &ctx->scratch_bytes, scratch_bytes(n_rays, n_items, roots),
+                    "render scratch");
+    if (st != BHS_OK) return st;
+    layout_scratch(ctx->scratch, n_items, roots, sc);
     return BHS_OK;
+}
+
+// Host-side camera table of a batch: constants, output offsets (prefix sums of width*height) and
+// item prefix sums. Checks every camera and its params.
+struct BatchTable {
+    std::vector<bhs::MarchConst> cams;
+    std::vector<uint32_t> item_base;
+    int64_t n_rays = 0;
+    uint32_t n_items = 0;
+    bool any_band = false, any_root = false;
+};
+
+int build_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* params, int32_t ncam,
+                BatchTable* t) {
+    if (!cams || !params) return fail(ctx, BHS_E_INVALID, "batch: cameras or params is NULL%s");
+    if (ncam < 1 || (uint32_t)ncam > bhs::kMaxBatchCams)
+        return fail(ctx, BHS_E_INVALID, "batch: ncam must be in 1..64%s");
+    t->cams.resize((size_t)ncam);
+    t->item_base.assign((size_t)ncam + 1, 0u);
+    uint64_t items = 0;
+    int64_t rays = 0;
+    for (int32_t k = 0; k < ncam; ++k) {
+        int st;
+        if ((st = check_camera(ctx, &cams[k])) != BHS_OK) return st;
+        if ((st = check_params(ctx, &params[k])) != BHS_OK) return st;
+        if (params[k].precision == BHS_PREC_F64)
+            return fail(ctx, BHS_E_INVALID, "batch: BHS_PREC_F64 is not supported in batches%s");
+        bhs::MarchConst c = make_const(&params[k]);
+        fill_camera(c, &cams[k]);
+        c.ray_offset = rays;
+        t->cams[(size_t)k] = c;
+        t->item_base[(size_t)k] = (uint32_t)items;
+        items += bhs::camera_items(cams[k].width, cams[k].height);
+        rays += (int64_t)cams[k].width * cams[k].height;
+        if (items > 0xFFFF0000ull) return fail(ctx, BHS_E_INVALID, "batch: more than 2^32 rays%s");
+        t->any_band = t->any_band || params[k].precision == BHS_PREC_F32_BAND64;
+        t->any_root = t->any_root || params[k].root64 != 0;
+    }
+    t->item_base[(size_t)ncam] = (uint32_t)items;
+    t->n_items = (uint32_t)items;
+    t->n_rays = rays;
+    return BHS_OK;
+}
+
+size_t table_cams_bytes(int32_t ncam) {
+    return ((size_t)ncam * sizeof(bhs::MarchConst) + 255) & ~(size_t)255;
+}
+size_t table_bytes(int32_t ncam) {
+    return table_cams_bytes(ncam) + ((((size_t)ncam + 1) * sizeof(uint32_t) + 255) & ~(size_t)255);
+}
+
+// Copies a batch table into one host buffer laid out as on the device.
+void pack_table(const BatchTable& t, char* dst) {
+    const int32_t ncam = (int32_t)t.cams.size();
+    std::memcpy(dst, t.cams.data(), t.cams.size() * sizeof(bhs::MarchConst));
+    std::memcpy(dst + table_cams_bytes(ncam), t.item_base.data(), t.item_base.size() * sizeof(uint32_t));
 }
 
 }  // namespace
@@ -234,6 +324,7 @@ bhs_ctx* bhs_create(int device, int* status) {
         ctx->num_cus = cus;
     if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_upload, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
         bhs_destroy(ctx);
@@ -257,6 +348,9 @@ void bhs_destroy(bhs_ctx* ctx) {
             (void)hipEventDestroy(r.k1);
             (void)hipEventDestroy(r.call1);
         }
+        if (ctx->table_dev) (void)hipFree(ctx->table_dev);
+        if (ctx->table_host) (void)hipHostFree(ctx->table_host);
+        if (ctx->ev_upload) (void)hipEventDestroy(ctx->ev_upload);
         if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
         if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
         if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
@@ -321,7 +415,9 @@ int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* para
     c.ray_offset = 0;
     const bool roots = params->root64 && params->precision != BHS_PREC_F64;
     bhs::Scratch sc;
-    if ((st = prepare_scratch(ctx, n, roots, &sc)) != BHS_OK) return st;
+    if ((st = prepare_scratch(ctx, n, bhs::camera_items(cam->width, cam->height), roots, &sc)) !=
+        BHS_OK)
+        return st;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e;
     bhs_ctx::TimingRec* tr = next_timer(ctx);
@@ -346,7 +442,7 @@ int bhs_march_rays(bhs_ctx* ctx, const float* pos3_dev, const float* dir3_dev, i
         return fail(ctx, BHS_E_INVALID, "bhs_march_rays: NULL buffer or n < 0%s");
     DeviceGuard g(ctx->device);
     bhs::Scratch sc;
-    if ((st = prepare_scratch(ctx, 0, false, &sc)) != BHS_OK) return st;
+    if ((st = prepare_scratch(ctx, 0, 0, false, &sc)) != BHS_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e;
     if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess)
@@ -360,6 +456,203 @@ int bhs_march_rays(bhs_ctx* ctx, const float* pos3_dev, const float* dir3_dev, i
                                     tr ? tr->k0 : nullptr, tr ? tr->k1 : nullptr)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_march_rays launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
+    return BHS_OK;
+}
+
+int bhs_render_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* params,
+                     int32_t ncam, const bhs_ray_soa* out, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!out) return fail(ctx, BHS_E_INVALID, "bhs_render_batch: out is NULL%s");
+    BatchTable t;
+    int st;
+    if ((st = build_batch(ctx, cams, params, ncam, &t)) != BHS_OK) return st;
+    if (t.any_root && (!out->n_cross_dev || !out->flags_dev || !out->disk_dev))
+        return fail(ctx, BHS_E_INVALID, "bhs_render_batch: root64 needs n_cross, flags and disk outputs%s");
+    DeviceGuard g(ctx->device);
+    const size_t tb = table_bytes(ncam);
+    if ((st = ensure(ctx, &ctx->table_dev, &ctx->table_dev_bytes, tb, "camera table")) != BHS_OK) return st;
+    if (ctx->table_host_bytes < tb) {
+        if (ctx->upload_pending) (void)hipEventSynchronize(ctx->ev_upload);
+        if (ctx->table_host) (void)hipHostFree(ctx->table_host);
+        ctx->table_host = nullptr;
+        ctx->table_host_bytes = 0;
+        if (hipHostMalloc(&ctx->table_host, tb, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->table_host = nullptr;
+            return fail(ctx, BHS_E_ALLOC, "pinned allocation of the camera table failed%s");
+        }
+        ctx->table_host_bytes = tb;
+    }
+    bhs::Scratch sc;
+    if ((st = prepare_scratch(ctx, t.n_rays, t.n_items, t.any_root, &sc)) != BHS_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if (ctx->upload_pending) (void)hipEventSynchronize(ctx->ev_upload);   // staging buffer is free
+    pack_table(t, (char*)ctx->table_host);
+    bhs_ctx::TimingRec* tr = next_timer(ctx);
+    if (tr) (void)hipEventRecord(tr->call0, s);
+    if ((e = hipMemcpyAsync(ctx->table_dev, ctx->table_host, tb, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipEventRecord(ctx->ev_upload, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_render_batch upload");
+    ctx->upload_pending = true;
+    if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_render_batch clear");
+    const bhs::MarchConst* table = (const bhs::MarchConst*)ctx->table_dev;
+    const uint32_t* base = (const uint32_t*)((char*)ctx->table_dev + table_cams_bytes(ncam));
+    if ((e = bhs::launch_batch(table, base, (uint32_t)ncam, t.n_items, t.any_band, t.any_root,
+                               make_soa(out, t.n_rays), sc, ctx->tuning, ctx->num_cus, s, ctx->aux,
+                               ctx->ev_fork, ctx->ev_join, tr ? tr->k0 : nullptr,
+                               tr ? tr->k1 : nullptr)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_render_batch launch");
+    if (tr) (void)hipEventRecord(tr->call1, s);
+    ctx->last_stream = s;
+    return BHS_OK;
+}
+
+void bhs_graph_destroy(bhs_graph* g) {
+    if (!g) return;
+    {
+        DeviceGuard dg(g->device);
+        if (g->cap) (void)hipStreamSynchronize(g->cap);
+        (void)hipDeviceSynchronize();
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        if (g->ev_fork) (void)hipEventDestroy(g->ev_fork);
+        if (g->ev_join) (void)hipEventDestroy(g->ev_join);
+        if (g->aux) (void)hipStreamDestroy(g->aux);
+        if (g->cap) (void)hipStreamDestroy(g->cap);
+        if (g->mem) (void)hipFree(g->mem);
+    }
+    delete g;
+}
+
+int bhs_graph_create(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* params,
+                     int32_t ncam, const bhs_ray_soa* out, bhs_graph** graph) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!out || !graph) return fail(ctx, BHS_E_INVALID, "bhs_graph_create: out or graph is NULL%s");
+    *graph = nullptr;
+    BatchTable t;
+    int st;
+    if ((st = build_batch(ctx, cams, params, ncam, &t)) != BHS_OK) return st;
+    if (t.any_root && (!out->n_cross_dev || !out->flags_dev || !out->disk_dev))
+        return fail(ctx, BHS_E_INVALID, "bhs_graph_create: root64 needs n_cross, flags and disk outputs%s");
+    DeviceGuard dg(ctx->device);
+    bhs_graph* g = new (std::nothrow) bhs_graph();
+    if (!g) return fail(ctx, BHS_E_ALLOC, "bhs_graph_create: out of host memory%s");
+    g->device = ctx->device;
+    g->ncam = ncam;
+    g->n_rays = t.n_rays;
+    g->n_items = t.n_items;
+    g->any_band = t.any_band;
+    g->any_root = t.any_root;
+    for (int32_t k = 0; k < ncam; ++k) {
+        g->dims.push_back(cams[k].width);
+        g->dims.push_back(cams[k].height);
+    }
+    const size_t tb = table_bytes(ncam);
+    g->bytes = tb + scratch_bytes(t.n_rays, t.n_items, t.any_root);
+    hipError_t e;
+    if ((e = hipMalloc(&g->mem, g->bytes)) != hipSuccess) {
+        g->mem = nullptr;
+        bhs_graph_destroy(g);
+        return hip_fail(ctx, e, "bhs_graph_create alloc");
+    }
+    std::vector<char> host(tb);
+    pack_table(t, host.data());
+    if ((e = hipMemcpy(g->mem, host.data(), tb, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&g->aux, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming)) != hipSuccess) {
+        bhs_graph_destroy(g);
+        return hip_fail(ctx, e, "bhs_graph_create setup");
+    }
+    bhs::Scratch sc;
+    void* smem = (char*)g->mem + tb;
+    layout_scratch(smem, t.n_items, t.any_root, &sc);
+    const bhs::MarchConst* table = (const bhs::MarchConst*)g->mem;
+    const uint32_t* base = (const uint32_t*)((char*)g->mem + table_cams_bytes(ncam));
+    if ((e = hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+        bhs_graph_destroy(g);
+        return hip_fail(ctx, e, "bhs_graph_create begin capture");
+    }
+    hipError_t le = bhs::launch_zero(smem, kStatsBytes, g->cap);
+    if (le == hipSuccess)
+        le = bhs::launch_batch(table, base, (uint32_t)ncam, t.n_items, t.any_band, t.any_root,
+                               make_soa(out, t.n_rays), sc, ctx->tuning, ctx->num_cus, g->cap,
+                               g->aux, g->ev_fork, g->ev_join, nullptr, nullptr);
+    e = hipStreamEndCapture(g->cap, &g->graph);
+    if (le != hipSuccess || e != hipSuccess) {
+        bhs_graph_destroy(g);
+        return hip_fail(ctx, le != hipSuccess ? le : e, "bhs_graph_create capture");
+    }
+    if ((e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess) {
+        bhs_graph_destroy(g);
+        return hip_fail(ctx, e, "bhs_graph_create instantiate");
+    }
+    *graph = g;
+    return BHS_OK;
+}
+
+int bhs_graph_update(bhs_ctx* ctx, bhs_graph* g, const bhs_camera* cams,
+                     const bhs_march_params* params) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!g) return fail(ctx, BHS_E_INVALID, "bhs_graph_update: graph is NULL%s");
+    BatchTable t;
+    int st;
+    if ((st = build_batch(ctx, cams, params, g->ncam, &t)) != BHS_OK) return st;
+    for (int32_t k = 0; k < g->ncam; ++k)
+        if (cams[k].width != g->dims[2 * k] || cams[k].height != g->dims[2 * k + 1])
+            return fail(ctx, BHS_E_INVALID, "bhs_graph_update: camera sizes must not change%s");
+    if (t.any_band != g->any_band || t.any_root != g->any_root)
+        return fail(ctx, BHS_E_INVALID, "bhs_graph_update: band/root settings must not change%s");
+    DeviceGuard dg(g->device);
+    hipError_t e;
+    if ((e = hipStreamSynchronize(g->cap)) != hipSuccess) return hip_fail(ctx, e, "bhs_graph_update");
+    (void)hipDeviceSynchronize();
+    std::vector<char> host(table_bytes(g->ncam));
+    pack_table(t, host.data());
+    if ((e = hipMemcpy(g->mem, host.data(), host.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_graph_update copy");
+    return BHS_OK;
+}
+
+int bhs_graph_launch(bhs_ctx* ctx, bhs_graph* g, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!g) return fail(ctx, BHS_E_INVALID, "bhs_graph_launch: graph is NULL%s");
+    DeviceGuard dg(g->device);
+    hipStream_t s = (hipStream_t)stream;
+    bhs_ctx::TimingRec* tr = next_timer(ctx);
+    if (tr) {
+        (void)hipEventRecord(tr->call0, s);
+        (void)hipEventRecord(tr->k0, s);
+    }
+    hipError_t e = hipGraphLaunch(g->exec, s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "bhs_graph_launch");
+    if (tr) {
+        (void)hipEventRecord(tr->k1, s);
+        (void)hipEventRecord(tr->call1, s);
+    }
+    return BHS_OK;
+}
+
+int bhs_graph_stats(bhs_ctx* ctx, bhs_graph* g, bhs_stats* out_host) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!g || !out_host) return fail(ctx, BHS_E_INVALID, "bhs_graph_stats: NULL argument%s");
+    DeviceGuard dg(g->device);
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(ctx, e, "bhs_graph_stats sync");
+    bhs::Stats st;
+    if ((e = hipMemcpy(&st, (char*)g->mem + table_bytes(g->ncam), sizeof st, hipMemcpyDeviceToHost)) !=
+        hipSuccess)
+        return hip_fail(ctx, e, "bhs_graph_stats copy");
+    std::memset(out_host, 0, sizeof *out_host);
+    out_host->rays = st.rays;
+    out_host->steps = st.steps;
+    out_host->crossings = st.crossings;
+    out_host->escapes = st.escapes;
+    out_host->band64 = st.band64;
+    out_host->fallback = st.fallback;
     return BHS_OK;
 }
 

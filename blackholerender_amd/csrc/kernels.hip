@@ -1,5 +1,6 @@
-// kernels.hip — law-S render/march kernels (FP32 with FP64 band, FP64), FP64 crossing-root
-// refinement, and the law-T (bhs::physics::trace) kernel.
+// kernels.hip — law-S render/march kernels (FP32 with FP64 band, FP64), the persistent
+// work-queue kernel for one camera or a batch of cameras, FP64 crossing-root refinement, and the
+// law-T (bhs::physics::trace) kernel.
 #include "kernels.hpp"
 
 #include "../../include/bhs_trace.h"
@@ -38,7 +39,8 @@ __device__ __forceinline__ void store_ray(const SoA& o, int64_t i, const RayOut&
     if (o.b) o.b[i] = r.b;
 }
 
-// Adds one lane's contribution to the launch counters: one atomic per wave and counter.
+// Adds one lane's contribution to the launch counters: one atomic per wave and counter. Every lane
+// of the wave must call it.
 __device__ __forceinline__ void add_stats(Stats* st, uint32_t ray, uint32_t steps, uint32_t cross,
                                           uint32_t esc, uint32_t band) {
     ray = wave_sum(ray);
@@ -66,62 +68,44 @@ __device__ __forceinline__ bool in_band(const MarchConst& c, float b) {
     return __builtin_fabsf(b - kBcritF) < c.band_abs;
 }
 
-// ------------------------------------------------------------------------------------------------
-// Lists the rays of the FP64 band (float setup, |h - b_crit| < band_abs) of one camera.
-__global__ __launch_bounds__(kBlock) void k_band_scan(const MarchConst c, Scratch sc) {
-    int col, row;
-    tile_pixel(col, row);
-    if (col >= c.width || row >= c.height) return;
-    const int64_t pix = (int64_t)row * c.width + col;
-    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-    const Setup<float> s = lawS_setup(P, ray_dir(c, pix));
-    if (in_band(c, s.b)) {
-        const uint32_t k = atomicAdd(sc.band_count, 1u);
-        if (k < sc.band_cap) sc.band_ids[k] = (uint32_t)pix;
-    }
-}
-
-// FP32 law-S render of one camera, one lane per pixel on a static grid of 8x8-pixel wave tiles
-// (BLOCK = 64: one tile per block; 256: 16x16 pixels per block). In BAND64 precision the rays of
-// the FP64 band are skipped here (k_band_scan lists them, k_band64 marches them).
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
-void k_render_f32(const MarchConst c, const SoA out, Scratch sc) {
-    int col, row;
-    if (BLOCK == 256) {
-        tile_pixel(col, row);
-    } else {
-        col = blockIdx.x * 8 + (threadIdx.x & 7);
-        row = blockIdx.y * 8 + (threadIdx.x >> 3);
-    }
-    const bool valid = col < c.width && row < c.height;
-    uint32_t steps = 0, cross = 0, esc = 0, rays = 0;
-    if (valid) {
-        const int64_t pix = (int64_t)row * c.width + col;
-        const V3<float> d = ray_dir(c, pix);
-        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-        const Setup<float> s = lawS_setup(P, d);
-        if (!(c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.b))) {
-            RayOut o;
-            lawS_march(P, s, c, o, sc.roots ? sc.roots + 2 * pix : (RootState<float>*)nullptr);
-            store_ray(out, c.ray_offset + pix, o);
-            steps = (uint32_t)o.steps;
-            cross = o.n_cross;
-            esc = (o.flags & 2u) ? 1u : 0u;
-            rays = 1;
-        }
-    }
-    add_stats(sc.stats, rays, steps, cross, esc, 0u);
-}
-
-// Work item q of the persistent kernels: ray index in 8x8-tile-major order over the image
-// (tile t = q / 64 row-major over the tile grid, lane q % 64 row-major inside the tile). Items of the
-// padding beyond the image map to -1.
-__device__ __forceinline__ int64_t item_pixel(const MarchConst& c, uint32_t q, uint32_t tiles_x) {
+// Work item q of one camera: ray index in 8x8-tile-major order over its image (tile t = q / 64
+// row-major over the tile grid, lane q % 64 row-major inside the tile). Items of the padding beyond
+// the image map to -1.
+__device__ __forceinline__ int64_t item_pixel(const MarchConst& c, uint32_t q) {
+    const uint32_t tiles_x = ((uint32_t)c.width + 7u) >> 3;
     const uint32_t t = q >> 6, l = q & 63u;
     const uint32_t col = (t % tiles_x) * 8u + (l & 7u), row = (t / tiles_x) * 8u + (l >> 3);
     return (col < (uint32_t)c.width && row < (uint32_t)c.height) ? (int64_t)row * c.width + col : -1;
 }
+
+// Largest k < ncam with base[k] <= q (base[0] = 0, base[ncam] = total items).
+__device__ __forceinline__ uint32_t find_cam(const uint32_t* base, uint32_t ncam, uint32_t q) {
+    uint32_t lo = 0, hi = ncam;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= q) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Camera sources of the item-indexed kernels: one camera passed by value, or a device table with
+// the prefix sums of the per-camera item counts.
+struct OneCam {
+    MarchConst c;
+    uint32_t n_items;
+    __device__ __forceinline__ uint32_t find(uint32_t) const { return 0u; }
+    __device__ __forceinline__ const MarchConst& cam(uint32_t) const { return c; }
+    __device__ __forceinline__ uint32_t base(uint32_t) const { return 0u; }
+};
+struct CamTable {
+    const MarchConst* cams;
+    const uint32_t* item_base;
+    uint32_t ncam;
+    uint32_t n_items;
+    __device__ __forceinline__ uint32_t find(uint32_t q) const { return find_cam(item_base, ncam, q); }
+    __device__ __forceinline__ const MarchConst& cam(uint32_t k) const { return cams[k]; }
+    __device__ __forceinline__ uint32_t base(uint32_t k) const { return item_base[k]; }
+};
 
 // Sink writing a ray's records straight to the SoA output (and the crossing states to the
 // root-refinement scratch).
@@ -173,71 +157,6 @@ __device__ __forceinline__ void store_finish(const SoA& o, int64_t i, const Marc
     }
 }
 
-// FP32 law-S render of one camera by resident waves that pull rays from a global work queue.
-// Each wave keeps 64 rays in flight; when at least `refill_min` lanes have finished, it takes that
-// many new rays with one atomic (wave ballot + per-lane prefix count of the idle lanes) and
-// continues, so finished lanes are refilled instead of idling until the slowest lane ends.
-// Exit: a wave leaves when the queue is exhausted and all its lanes are finished; every ray ends
-// within max_steps iterations and the counter only grows, so every wave reaches it.
-template <int BLOCK, bool ROOTS, int WPE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
-void k_render_f32_persist(const MarchConst c, const SoA out, Scratch sc, uint32_t n_items,
-                          uint32_t tiles_x, uint32_t refill_min) {
-    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-    const MarchK<float> k = march_consts<float>(c);
-    const bool band = c.precision == BHS_PREC_F32_BAND64;
-    const uint32_t lane = threadIdx.x & 63u;
-    MarchState<float> st;
-    Setup<float> s;
-    uint32_t pix = 0;
-    bool active = false;
-    bool more = true;   // wave-uniform: the queue may still hold items
-    uint32_t n_rays = 0, n_steps = 0, n_cross = 0, n_esc = 0;
-    while (true) {
-        const uint64_t idle = __ballot(!active);
-        const uint32_t n_idle = (uint32_t)__popcll(idle);
-        if (more && n_idle >= refill_min) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(sc.work, n_idle);
-            base = (uint32_t)__shfl((int)base, 0);
-            if (base + n_idle >= n_items) more = false;
-            if (!active) {
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-                    (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                const uint32_t q = base + rank;
-                if (q < n_items) {
-                    const int64_t p = item_pixel(c, q, tiles_x);
-                    if (p >= 0) {
-                        pix = (uint32_t)p;
-                        s = lawS_setup(P, ray_dir(c, p));
-                        if (!(band && in_band(c, s.b))) {
-                            march_init(st, P, s);
-                            active = true;
-                        }
-                    }
-                }
-            }
-        }
-        if (!__any(active)) {
-            if (!more) break;
-            continue;
-        }
-        if (active) {
-            const int64_t gi = c.ray_offset + pix;
-            GlobalSink sink{out, gi, ROOTS ? sc.roots + 2 * (int64_t)pix : (RootState<float>*)nullptr};
-            if (march_step(st, s, k, sink)) {
-                store_finish(out, gi, st, s.b);
-                n_rays += 1;
-                n_steps += (uint32_t)st.i;
-                n_cross += st.n_cross;
-                n_esc += st.status == 1 ? 1u : 0u;
-                active = false;
-            }
-        }
-    }
-    add_stats(sc.stats, n_rays, n_steps, n_cross, n_esc, 0u);
-}
-
 // FP64 law-S march of one ray, refined crossings included (as the oracle's double march).
 __device__ __forceinline__ RayOut march_f64(const MarchConst& c, const V3<float>& Pf,
                                             const V3<float>& df) {
@@ -263,6 +182,120 @@ __device__ __forceinline__ RayOut march_f64(const MarchConst& c, const V3<float>
     return o;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Lists the work items of the FP64 band (float setup, |b - b_crit| < band_abs).
+template <typename Cams>
+__global__ __launch_bounds__(kBlock) void k_band_scan(const Cams cams, Scratch sc) {
+    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < cams.n_items;
+         q += gridDim.x * kBlock) {
+        const uint32_t ci = cams.find(q);
+        const MarchConst& c = cams.cam(ci);
+        if (c.precision != BHS_PREC_F32_BAND64) continue;
+        const int64_t pix = item_pixel(c, q - cams.base(ci));
+        if (pix < 0) continue;
+        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+        const Setup<float> s = lawS_setup(P, ray_dir(c, pix));
+        if (in_band(c, s.b)) {
+            const uint32_t k = atomicAdd(sc.band_count, 1u);
+            if (k < sc.band_cap) sc.band_ids[k] = q;
+        }
+    }
+}
+
+// FP64 march of the band list (grid-stride; the count is read on the device).
+template <typename Cams>
+__global__ __launch_bounds__(64) void k_band64(const Cams cams, const SoA out, Scratch sc) {
+    // Few waves with long serial chains run beside the FP32 march: issue them first on their SIMD.
+    __builtin_amdgcn_s_setprio(3);
+    const uint32_t cnt = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
+    for (uint32_t base = blockIdx.x * 64; base < cnt; base += gridDim.x * 64) {
+        const uint32_t k = base + threadIdx.x;
+        uint32_t steps = 0, cross = 0, esc = 0, one = 0;
+        if (k < cnt) {
+            const uint32_t q = sc.band_ids[k];
+            const uint32_t ci = cams.find(q);
+            const MarchConst& c = cams.cam(ci);
+            const int64_t pix = item_pixel(c, q - cams.base(ci));
+            const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+            const RayOut o = march_f64(c, P, ray_dir(c, pix));
+            store_ray(out, c.ray_offset + pix, o);
+            steps = (uint32_t)o.steps;
+            cross = o.n_cross;
+            esc = (o.flags & 2u) ? 1u : 0u;
+            one = 1;
+        }
+        add_stats(sc.stats, one, steps, cross, esc, one);
+    }
+}
+
+// FP64 root refinement of the crossings recorded by the FP32 march (all items).
+template <typename Cams>
+__global__ __launch_bounds__(kBlock) void k_root64(const Cams cams, const SoA out, Scratch sc) {
+    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < cams.n_items;
+         q += gridDim.x * kBlock) {
+        const uint32_t ci = cams.find(q);
+        const MarchConst& c = cams.cam(ci);
+        if (!c.root64) continue;
+        const int64_t pix = item_pixel(c, q - cams.base(ci));
+        if (pix < 0) continue;
+        const int64_t g = c.ray_offset + pix;
+        const uint32_t nc = out.n_cross[g];
+        const uint32_t fl = out.flags[g];
+        if (nc == 0 || (fl & BHS_FLAG_BAND64)) continue;
+        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+        // c3, b_axis and the redshift lapse of this ray, from its float setup.
+        const Setup<float> s = lawS_setup(P, ray_dir(c, pix));
+        const uint32_t m = nc < 2 ? nc : 2;
+        for (uint32_t k = 0; k < m; ++k) {
+            const RootState<float>& r = sc.roots[2 * g + k];
+            const double X0[3] = {r.x0[0], r.x0[1], r.x0[2]};
+            const double V0[3] = {r.v0[0], r.v0[1], r.v0[2]};
+            double hx, hz;
+            if (!root64(X0, V0, (double)r.dt, (double)s.c3, hx, hz)) continue;
+            const double rh = __builtin_sqrt(__builtin_fma(hz, hz, hx * hx));
+            out.disk[g + k * out.n] = (float)rh;
+            out.disk[g + (2 + k) * out.n] = atan2_ieee((float)hz, (float)hx);
+            out.disk[g + (4 + k) * out.n] =
+                (float)redshift((double)rh, (double)s.b_axis, (double)s.lapse_g);
+        }
+        out.flags[g] = (uint8_t)(fl | BHS_FLAG_ROOT64);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP32 law-S render of one camera, one lane per pixel on a static grid of 8x8-pixel wave tiles
+// (BLOCK = 64: one tile per block; 256: 16x16 pixels per block). In BAND64 precision the rays of
+// the FP64 band are skipped here (k_band_scan lists them, k_band64 marches them).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
+void k_render_f32(const MarchConst c, const SoA out, Scratch sc) {
+    int col, row;
+    if (BLOCK == 256) {
+        tile_pixel(col, row);
+    } else {
+        col = blockIdx.x * 8 + (threadIdx.x & 7);
+        row = blockIdx.y * 8 + (threadIdx.x >> 3);
+    }
+    const bool valid = col < c.width && row < c.height;
+    uint32_t steps = 0, cross = 0, esc = 0, rays = 0;
+    if (valid) {
+        const int64_t pix = (int64_t)row * c.width + col;
+        const V3<float> d = ray_dir(c, pix);
+        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+        const Setup<float> s = lawS_setup(P, d);
+        if (!(c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.b))) {
+            RayOut o;
+            lawS_march(P, s, c, o, sc.roots ? sc.roots + 2 * pix : (RootState<float>*)nullptr);
+            store_ray(out, c.ray_offset + pix, o);
+            steps = (uint32_t)o.steps;
+            cross = o.n_cross;
+            esc = (o.flags & 2u) ? 1u : 0u;
+            rays = 1;
+        }
+    }
+    add_stats(sc.stats, rays, steps, cross, esc, 0u);
+}
+
 // FP64 law-S render of every ray of one camera.
 __global__ __launch_bounds__(kBlock) void k_render_f64(const MarchConst c, const SoA out, Scratch sc) {
     int col, row;
@@ -281,54 +314,89 @@ __global__ __launch_bounds__(kBlock) void k_render_f64(const MarchConst c, const
     add_stats(sc.stats, valid ? 1u : 0u, steps, cross, esc, valid ? 1u : 0u);
 }
 
-// FP64 march of the band list (grid-stride; the count is read on the device).
-__global__ __launch_bounds__(64) void k_band64(const MarchConst c, const SoA out, Scratch sc) {
-    // Few waves with long serial chains run beside the FP32 march: issue them first on their SIMD.
-    __builtin_amdgcn_s_setprio(3);
-    const uint32_t cnt = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
-    for (uint32_t base = blockIdx.x * 64; base < cnt; base += gridDim.x * 64) {
-        const uint32_t k = base + threadIdx.x;
-        uint32_t steps = 0, cross = 0, esc = 0, one = 0;
-        if (k < cnt) {
-            const int64_t pix = sc.band_ids[k];
-            const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-            const RayOut o = march_f64(c, P, ray_dir(c, pix));
-            store_ray(out, c.ray_offset + pix, o);
-            steps = (uint32_t)o.steps;
-            cross = o.n_cross;
-            esc = (o.flags & 2u) ? 1u : 0u;
-            one = 1;
+// ------------------------------------------------------------------------------------------------
+// FP32 law-S march by resident waves that pull rays from a global work queue: one camera (BATCH =
+// false, constants in kernel arguments) or up to kMaxBatchCams cameras of different sizes and step
+// budgets (BATCH = true, camera table and item prefix sums staged in LDS; each lane looks up its
+// ray's camera there). Each wave keeps 64 rays in flight; when at least `refill_min` lanes have
+// finished, it takes that many new rays with one atomic (wave ballot + per-lane prefix count of the
+// idle lanes), so finished lanes are refilled instead of idling until the slowest lane ends.
+// Exit: a wave leaves when the queue is exhausted and all its lanes are finished; every ray ends
+// within its max_steps iterations and the counter only grows, so every wave reaches it.
+template <int BLOCK, bool ROOTS, int WPE, bool BATCH>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
+               const uint32_t* __restrict__ item_base, uint32_t ncam, uint32_t n_items,
+               const SoA out, Scratch sc, uint32_t refill_min) {
+    __shared__ MarchConst s_cam[BATCH ? kMaxBatchCams : 1];
+    __shared__ MarchK<float> s_k[BATCH ? kMaxBatchCams : 1];
+    __shared__ uint32_t s_base[BATCH ? kMaxBatchCams + 1 : 1];
+    if (BATCH) {
+        for (uint32_t i = threadIdx.x; i < ncam; i += BLOCK) {
+            s_cam[i] = table[i];
+            s_k[i] = march_consts<float>(table[i]);
         }
-        add_stats(sc.stats, one, steps, cross, esc, one);
+        for (uint32_t i = threadIdx.x; i <= ncam; i += BLOCK) s_base[i] = item_base[i];
+        __syncthreads();
     }
-}
-
-// FP64 root refinement of the crossings recorded by the FP32 march (rays [0, n) of one camera).
-__global__ __launch_bounds__(kBlock) void k_root64(const MarchConst c, const SoA out, Scratch sc,
-                                                   int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const int64_t g = c.ray_offset + i;
-    const uint32_t nc = out.n_cross[g];
-    const uint32_t fl = out.flags[g];
-    if (nc == 0 || (fl & BHS_FLAG_BAND64)) return;
-    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-    // c3, b_axis and the redshift lapse of this ray, from its float setup.
-    const Setup<float> s = lawS_setup(P, ray_dir(c, i));
-    const uint32_t m = nc < 2 ? nc : 2;
-    for (uint32_t k = 0; k < m; ++k) {
-        const RootState<float>& q = sc.roots[2 * i + k];
-        const double X0[3] = {q.x0[0], q.x0[1], q.x0[2]};
-        const double V0[3] = {q.v0[0], q.v0[1], q.v0[2]};
-        double hx, hz;
-        if (!root64(X0, V0, (double)q.dt, (double)s.c3, hx, hz)) continue;
-        const double rh = __builtin_sqrt(__builtin_fma(hz, hz, hx * hx));
-        out.disk[g + k * out.n] = (float)rh;
-        out.disk[g + (2 + k) * out.n] = atan2_ieee((float)hz, (float)hx);
-        out.disk[g + (4 + k) * out.n] =
-            (float)redshift((double)rh, (double)s.b_axis, (double)s.lapse_g);
+    const MarchK<float> k1 = march_consts<float>(c1);
+    const uint32_t lane = threadIdx.x & 63u;
+    MarchState<float> st;
+    Setup<float> s;
+    uint32_t pix = 0, ci = 0;
+    bool active = false;
+    bool more = true;   // wave-uniform: the queue may still hold items
+    uint32_t n_rays = 0, n_steps = 0, n_cross = 0, n_esc = 0;
+    while (true) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (more && n_idle >= refill_min) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(sc.work, n_idle);
+            base = (uint32_t)__shfl((int)base, 0);
+            if (base + n_idle >= n_items) more = false;
+            if (!active) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                const uint32_t q = base + rank;
+                if (q < n_items) {
+                    uint32_t local = q;
+                    if (BATCH) {
+                        ci = find_cam(s_base, ncam, q);
+                        local = q - s_base[ci];
+                    }
+                    const MarchConst& c = BATCH ? s_cam[ci] : c1;
+                    const int64_t p = item_pixel(c, local);
+                    if (p >= 0) {
+                        pix = (uint32_t)p;
+                        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+                        s = lawS_setup(P, ray_dir(c, p));
+                        if (!(c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.b))) {
+                            march_init(st, P, s);
+                            active = true;
+                        }
+                    }
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (!more) break;
+            continue;
+        }
+        if (active) {
+            const int64_t gi = (BATCH ? s_cam[ci].ray_offset : c1.ray_offset) + pix;
+            GlobalSink sink{out, gi, ROOTS ? sc.roots + 2 * gi : (RootState<float>*)nullptr};
+            if (march_step(st, s, BATCH ? s_k[ci] : k1, sink)) {
+                store_finish(out, gi, st, s.b);
+                n_rays += 1;
+                n_steps += (uint32_t)st.i;
+                n_cross += st.n_cross;
+                n_esc += st.status == 1 ? 1u : 0u;
+                active = false;
+            }
+        }
     }
-    out.flags[g] = (uint8_t)(fl | BHS_FLAG_ROOT64);
+    add_stats(sc.stats, n_rays, n_steps, n_cross, n_esc, 0u);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -435,6 +503,69 @@ __global__ void k_zero(uint32_t* p, size_t words) {
         p[i] = 0u;
 }
 
+// Grid of the item-indexed helper kernels (grid-stride loops).
+dim3 item_grid(uint32_t n_items) {
+    uint32_t b = (n_items + kBlock - 1) / kBlock;
+    if (b > 8192u) b = 8192u;
+    return dim3(b ? b : 1u);
+}
+
+// Persistent-kernel instantiation for a block size, a root-state flag, an occupancy target
+// (waves per SIMD 8, 7, 6 or 4: the VGPR budget is 512 / waves) and single/batched cameras.
+template <int BLOCK, bool ROOTS, bool BATCH>
+void launch_persist_w(dim3 g, int wps, hipStream_t s, const MarchConst& c1, const MarchConst* table,
+                      const uint32_t* item_base, uint32_t ncam, uint32_t n_items, const SoA& out,
+                      const Scratch& sc, uint32_t rmin) {
+    if (wps >= 8)
+        hipLaunchKernelGGL((k_persist<BLOCK, ROOTS, 8, BATCH>), g, dim3(BLOCK), 0, s, c1, table,
+                           item_base, ncam, n_items, out, sc, rmin);
+    else if (wps == 7)
+        hipLaunchKernelGGL((k_persist<BLOCK, ROOTS, 7, BATCH>), g, dim3(BLOCK), 0, s, c1, table,
+                           item_base, ncam, n_items, out, sc, rmin);
+    else if (wps >= 5)
+        hipLaunchKernelGGL((k_persist<BLOCK, ROOTS, 6, BATCH>), g, dim3(BLOCK), 0, s, c1, table,
+                           item_base, ncam, n_items, out, sc, rmin);
+    else
+        hipLaunchKernelGGL((k_persist<BLOCK, ROOTS, 4, BATCH>), g, dim3(BLOCK), 0, s, c1, table,
+                           item_base, ncam, n_items, out, sc, rmin);
+}
+
+template <bool BATCH>
+hipError_t launch_persist(const Tuning& t, int num_cus, hipStream_t s, const MarchConst& c1,
+                          const MarchConst* table, const uint32_t* item_base, uint32_t ncam,
+                          uint32_t n_items, const SoA& out, const Scratch& sc) {
+    const int block = t.block == 64 ? 64 : 256;
+    const int wps = t.waves_per_simd < 1 ? 1 : (t.waves_per_simd > 8 ? 8 : t.waves_per_simd);
+    const uint32_t waves = (uint32_t)num_cus * 4u * (uint32_t)wps;
+    const uint32_t need = (n_items + 63u) / 64u;
+    const uint32_t blocks = (waves < need ? waves : need) * 64u / (uint32_t)block;
+    const uint32_t rmin = t.refill_min < 1 ? 1u : (t.refill_min > 64 ? 64u : (uint32_t)t.refill_min);
+    const dim3 g(blocks ? blocks : 1u);
+    const bool roots = sc.roots != nullptr;
+    if (block == 64) {
+        if (roots) launch_persist_w<64, true, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin);
+        else launch_persist_w<64, false, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin);
+    } else {
+        if (roots) launch_persist_w<256, true, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin);
+        else launch_persist_w<256, false, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin);
+    }
+    return hipGetLastError();
+}
+
+// The band fork: scan on s, FP64 march on aux between ev_fork and ev_join.
+template <typename Cams>
+hipError_t fork_band(const Cams& cams, const SoA& out, const Scratch& sc, hipStream_t s,
+                     hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join) {
+    hipError_t e;
+    hipLaunchKernelGGL(k_band_scan<Cams>, item_grid(cams.n_items), dim3(kBlock), 0, s, cams, sc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_band64<Cams>, dim3(256), dim3(64), 0, aux, cams, out, sc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipEventRecord(ev_join, aux);
+}
+
 }  // namespace
 
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
@@ -444,79 +575,63 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Persistent-kernel instantiation for a block size, a root-state flag and an occupancy target
-// (waves per SIMD 8, 7, 6 or 4: the VGPR budget is 512 / waves).
-template <int BLOCK, bool ROOTS>
-hipError_t launch_persist_w(dim3 g, int wps, hipStream_t s, const MarchConst& c, const SoA& out,
-                            const Scratch& sc, uint32_t n_items, uint32_t tiles_x, uint32_t rmin) {
-    if (wps >= 8)
-        hipLaunchKernelGGL((k_render_f32_persist<BLOCK, ROOTS, 8>), g, dim3(BLOCK), 0, s, c, out, sc, n_items, tiles_x, rmin);
-    else if (wps == 7)
-        hipLaunchKernelGGL((k_render_f32_persist<BLOCK, ROOTS, 7>), g, dim3(BLOCK), 0, s, c, out, sc, n_items, tiles_x, rmin);
-    else if (wps >= 5)
-        hipLaunchKernelGGL((k_render_f32_persist<BLOCK, ROOTS, 6>), g, dim3(BLOCK), 0, s, c, out, sc, n_items, tiles_x, rmin);
-    else
-        hipLaunchKernelGGL((k_render_f32_persist<BLOCK, ROOTS, 4>), g, dim3(BLOCK), 0, s, c, out, sc, n_items, tiles_x, rmin);
-    return hipGetLastError();
-}
-
-hipError_t launch_persist(dim3 g, int block, int wps, bool roots, hipStream_t s, const MarchConst& c,
-                          const SoA& out, const Scratch& sc, uint32_t n_items, uint32_t tiles_x,
-                          uint32_t rmin) {
-    if (block == 64)
-        return roots ? launch_persist_w<64, true>(g, wps, s, c, out, sc, n_items, tiles_x, rmin)
-                     : launch_persist_w<64, false>(g, wps, s, c, out, sc, n_items, tiles_x, rmin);
-    return roots ? launch_persist_w<256, true>(g, wps, s, c, out, sc, n_items, tiles_x, rmin)
-                 : launch_persist_w<256, false>(g, wps, s, c, out, sc, n_items, tiles_x, rmin);
+uint32_t camera_items(int32_t width, int32_t height) {
+    return (uint32_t)((width + 7) / 8) * (uint32_t)((height + 7) / 8) * 64u;
 }
 
 hipError_t launch_render(const MarchConst& c, const SoA& out, const Scratch& sc, const Tuning& t,
                          int num_cus, hipStream_t s, hipStream_t aux, hipEvent_t ev_fork,
                          hipEvent_t ev_join, hipEvent_t ev_begin, hipEvent_t ev_end) {
     const dim3 grid((c.width + kTile - 1) / kTile, (c.height + kTile - 1) / kTile);
-    const int64_t n = (int64_t)c.width * c.height;
+    const OneCam one{c, camera_items(c.width, c.height)};
     hipError_t e;
     const bool band = c.precision == BHS_PREC_F32_BAND64;
-    if (band) {
-        // Fork: the FP64 band rays (few, but up to max_steps long) run on the auxiliary stream
-        // while the FP32 march runs on `s`, instead of trailing it.
-        hipLaunchKernelGGL(k_band_scan, grid, dim3(kBlock), 0, s, c, sc);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_band64, dim3(256), dim3(64), 0, aux, c, out, sc);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = hipEventRecord(ev_join, aux)) != hipSuccess) return e;
-    }
+    // Fork: the FP64 band rays (few, but up to max_steps long) run on the auxiliary stream while
+    // the FP32 march runs on `s`, instead of trailing it.
+    if (band && (e = fork_band(one, out, sc, s, aux, ev_fork, ev_join)) != hipSuccess) return e;
     if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
     if (c.precision == BHS_PREC_F64) {
         hipLaunchKernelGGL(k_render_f64, grid, dim3(kBlock), 0, s, c, out, sc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     } else if (t.mode == 1) {
-        const uint32_t tiles_x = (uint32_t)((c.width + 7) / 8), tiles_y = (uint32_t)((c.height + 7) / 8);
-        const uint32_t n_items = tiles_x * tiles_y * 64u;
-        const int block = t.block == 64 ? 64 : 256;
-        const int wps = t.waves_per_simd < 1 ? 1 : (t.waves_per_simd > 8 ? 8 : t.waves_per_simd);
-        const uint32_t waves = (uint32_t)num_cus * 4u * (uint32_t)wps;
-        const uint32_t need = (n_items + 63u) / 64u;
-        const uint32_t blocks = (waves < need ? waves : need) * 64u / (uint32_t)block;
-        const uint32_t rmin = t.refill_min < 1 ? 1u : (t.refill_min > 64 ? 64u : (uint32_t)t.refill_min);
-        const dim3 g(blocks ? blocks : 1);
-        if ((e = launch_persist(g, block, wps, sc.roots != nullptr, s, c, out, sc, n_items, tiles_x,
-                                rmin)) != hipSuccess)
+        if ((e = launch_persist<false>(t, num_cus, s, c, nullptr, nullptr, 1u, one.n_items, out,
+                                       sc)) != hipSuccess)
             return e;
     } else if (t.block == 64) {
         const dim3 g8((c.width + 7) / 8, (c.height + 7) / 8);
         hipLaunchKernelGGL(k_render_f32<64>, g8, dim3(64), 0, s, c, out, sc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     } else {
         hipLaunchKernelGGL(k_render_f32<256>, grid, dim3(256), 0, s, c, out, sc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
     if (band && (e = hipStreamWaitEvent(s, ev_join, 0)) != hipSuccess) return e;
-    if (c.precision == BHS_PREC_F64) return hipSuccess;
-    if (c.root64 && sc.roots) {
-        hipLaunchKernelGGL(k_root64, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                           c, out, sc, n);
+    if (c.precision != BHS_PREC_F64 && c.root64 && sc.roots) {
+        hipLaunchKernelGGL(k_root64<OneCam>, item_grid(one.n_items), dim3(kBlock), 0, s, one, out, sc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_batch(const MarchConst* table, const uint32_t* item_base, uint32_t ncam,
+                        uint32_t n_items, bool any_band, bool any_root, const SoA& out,
+                        const Scratch& sc, const Tuning& t, int num_cus, hipStream_t s,
+                        hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join,
+                        hipEvent_t ev_begin, hipEvent_t ev_end) {
+    const CamTable cams{table, item_base, ncam, n_items};
+    hipError_t e;
+    if (any_band && (e = fork_band(cams, out, sc, s, aux, ev_fork, ev_join)) != hipSuccess) return e;
+    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
+    MarchConst none;
+    __builtin_memset(&none, 0, sizeof none);
+    if ((e = launch_persist<true>(t, num_cus, s, none, table, item_base, ncam, n_items, out, sc)) !=
+        hipSuccess)
+        return e;
+    if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
+    if (any_band && (e = hipStreamWaitEvent(s, ev_join, 0)) != hipSuccess) return e;
+    if (any_root && sc.roots) {
+        hipLaunchKernelGGL(k_root64<CamTable>, item_grid(n_items), dim3(kBlock), 0, s, cams, out, sc);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;

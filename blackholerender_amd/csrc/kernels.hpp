@@ -30,10 +30,16 @@ struct Scratch {
     Stats* stats;
     uint32_t* work;              // work-queue counter of the persistent kernels
     uint32_t* band_count;        // number of rays in the FP64 band list
-    uint32_t* band_ids;          // ray index (camera-local pixel for single-camera launches)
-    uint32_t band_cap;
-    RootState<float>* roots;     // 2 per ray when root refinement is on, else null
+    uint32_t* band_ids;          // work items (camera tile order) of the FP64 band rays
+    uint32_t band_cap;           // capacity of band_ids (>= the launch's work items)
+    RootState<float>* roots;     // 2 per ray (global ray index) when root refinement is on
 };
+
+// Cameras of one batched launch: the camera table and item prefix sums are staged in LDS.
+constexpr uint32_t kMaxBatchCams = 64;
+
+// Work items of one camera: its image padded to whole 8x8 tiles.
+uint32_t camera_items(int32_t width, int32_t height);
 
 // Launch-shape choices of the FP32 march (see bhs_tuning in include/bhs_trace.h).
 struct Tuning {
@@ -50,6 +56,12 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_render(const MarchConst& c, const SoA& out, const Scratch& sc, const Tuning& t,
                          int num_cus, hipStream_t s, hipStream_t aux, hipEvent_t ev_fork,
                          hipEvent_t ev_join, hipEvent_t ev_begin, hipEvent_t ev_end);
+// Batched launch over a device camera table (item_base: ncam + 1 prefix sums of camera_items).
+hipError_t launch_batch(const MarchConst* table, const uint32_t* item_base, uint32_t ncam,
+                        uint32_t n_items, bool any_band, bool any_root, const SoA& out,
+                        const Scratch& sc, const Tuning& t, int num_cus, hipStream_t s,
+                        hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join,
+                        hipEvent_t ev_begin, hipEvent_t ev_end);
 hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, const MarchConst& c,
                              const SoA& out, const Scratch& sc, hipStream_t s, hipEvent_t ev_begin,
                              hipEvent_t ev_end);

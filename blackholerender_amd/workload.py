@@ -62,5 +62,27 @@ def params_for(cfg, precision=None, root64=0):
                         root64=root64)
 
 
+def batch_c3(seed: int = 20260603, n: int = 64):
+    """The 64-camera batch: resolutions uniform in {64, 128, ..., 1024} per axis, r_obs
+    log-uniform in [8, 1000] M, inclination uniform in [0, 90] deg, vertical FOV uniform in
+    [20, 90] deg, max steps uniform in {200, 300, 450, 700} with the matching quality preset, thin
+    disk [6, 20] M, seeded per-pixel jitter. Returns (cameras, march params)."""
+    import numpy as np
+    from . import PREC_F32_BAND64, march_params
+    from .camera import orbit_camera
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cams, params = [], []
+    for k in range(n):
+        w = int(rng.integers(1, 17)) * 64
+        h = int(rng.integers(1, 17)) * 64
+        r_obs = float(np.exp(rng.uniform(np.log(8.0), np.log(1000.0))))
+        inc = float(rng.uniform(0.0, 90.0))
+        fov = float(rng.uniform(20.0, 90.0))
+        steps = int(rng.choice([200, 300, 450, 700]))
+        cams.append(orbit_camera(r_obs, inc, fov, w, h, jitter_hashed=1, seed=seed, camera_id=k))
+        params.append(march_params(max_steps=steps, precision=PREC_F32_BAND64))
+    return cams, params
+
+
 def b_crit() -> float:
     return 3.0 * math.sqrt(3.0)

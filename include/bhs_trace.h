@@ -152,6 +152,28 @@ int bhs_trace_rays(bhs_ctx* ctx, const double* pos3_host, const double* dir3_hos
 int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
                const bhs_ray_soa* out, void* stream);
 
+/* Law-S render of up to 64 cameras (each with its own size and march settings, FP32 or FP32 with
+ * the FP64 band) in one persistent launch: camera k's rays occupy out[offset_k, offset_k + w_k h_k)
+ * with offset_k = sum of w_j h_j over j < k, each camera in the pixel order of bhs_render. The
+ * camera table is uploaded with the launch (stream-ordered). */
+int bhs_render_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* params,
+                     int32_t ncam, const bhs_ray_soa* out, void* stream);
+
+/* The same batch captured once into a HIP graph and replayed: bhs_graph_create captures (and
+ * instantiates) the whole launch sequence for these cameras, settings and output buffers with the
+ * context's current tuning; bhs_graph_launch replays it on `stream`; bhs_graph_update replaces the
+ * camera poses/settings (sizes and band/root choices must stay) before the next replay;
+ * bhs_graph_stats reads the counters of the last replay (synchronises). The graph owns its camera
+ * table and scratch; `out` must stay allocated while the graph is used. */
+typedef struct bhs_graph bhs_graph;
+int bhs_graph_create(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* params,
+                     int32_t ncam, const bhs_ray_soa* out, bhs_graph** graph);
+int bhs_graph_update(bhs_ctx* ctx, bhs_graph* graph, const bhs_camera* cams,
+                     const bhs_march_params* params);
+int bhs_graph_launch(bhs_ctx* ctx, bhs_graph* graph, void* stream);
+int bhs_graph_stats(bhs_ctx* ctx, bhs_graph* graph, bhs_stats* out_host);
+void bhs_graph_destroy(bhs_graph* graph);
+
 /* Law-S march of explicit rays: pos3_dev/dir3_dev are device float arrays (xyz interleaved).
  * Stream-ordered. */
 int bhs_march_rays(bhs_ctx* ctx, const float* pos3_dev, const float* dir3_dev, int64_t n,
