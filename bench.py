@@ -77,6 +77,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--precision", default="band", choices=["f32", "band", "f64"])
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default)")
+    ap.add_argument("--tuning", default=None,
+                    help="fixed launch shape 'mode,block,waves_per_simd,refill_min' (skips the "
+                         "autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates the HBM counters "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count)")
@@ -108,7 +111,13 @@ def main(argv=None) -> int:
     n = cam.n_rays
     out = bhs.RayBuffers(n, "cuda:0")
     stream = torch.cuda.current_stream()
-    tuned = tracer.autotune(cam, params, out, reps=3) if not args.no_tune else tracer.get_tuning()
+    if args.tuning:
+        tracer.set_tuning(*[int(v) for v in args.tuning.split(",")])
+        tuned = tracer.get_tuning()
+    elif args.no_tune:
+        tuned = tracer.get_tuning()
+    else:
+        tuned = tracer.autotune(cam, params, out, reps=3)
     for _ in range(args.warmup):
         tracer.render(cam, params, out, stream)
     torch.cuda.synchronize()

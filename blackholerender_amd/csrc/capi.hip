@@ -682,10 +682,11 @@ int bhs_autotune(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* pa
     if (reps < 1 || reps > 1000) return fail(ctx, BHS_E_INVALID, "reps must be in 1..1000%s");
     static const bhs::Tuning cands[] = {
         {0, 256, 8, 64}, {0, 64, 8, 64},
-        {1, 256, 8, 64}, {1, 256, 8, 32}, {1, 256, 8, 16},
-        {1, 256, 7, 64}, {1, 256, 7, 32}, {1, 256, 7, 16},
-        {1, 256, 6, 64}, {1, 256, 6, 32}, {1, 256, 6, 16},
-        {1, 64, 8, 32}, {1, 64, 7, 32}, {1, 256, 4, 32},
+        {1, 256, 8, 64}, {1, 256, 8, 32},
+        {1, 256, 7, 64}, {1, 256, 7, 48}, {1, 256, 7, 32},
+        {1, 256, 6, 64}, {1, 256, 6, 32},
+        {1, 256, 4, 64}, {1, 256, 4, 48}, {1, 256, 4, 32},
+        {1, 64, 7, 64}, {1, 64, 4, 64}, {1, 64, 4, 32},
     };
     DeviceGuard g(ctx->device);
     const bhs::Tuning saved = ctx->tuning;

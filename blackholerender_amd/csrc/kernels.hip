@@ -383,17 +383,18 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
             if (!more) break;
             continue;
         }
-        if (active) {
-            const int64_t gi = (BATCH ? s_cam[ci].ray_offset : c1.ray_offset) + pix;
-            GlobalSink sink{out, gi, ROOTS ? sc.roots + 2 * gi : (RootState<float>*)nullptr};
-            if (march_step(st, s, BATCH ? s_k[ci] : k1, sink)) {
-                store_finish(out, gi, st, s.b);
-                n_rays += 1;
-                n_steps += (uint32_t)st.i;
-                n_cross += st.n_cross;
-                n_esc += st.status == 1 ? 1u : 0u;
-                active = false;
-            }
+        // Every lane steps (idle lanes on stale state, writing nothing), so the ray state stays in
+        // place across the loop instead of being copied at a divergent merge.
+        const int64_t gi = (BATCH ? s_cam[ci].ray_offset : c1.ray_offset) + pix;
+        GlobalSink sink{out, gi, ROOTS ? sc.roots + 2 * gi : (RootState<float>*)nullptr};
+        const bool fin = march_step(st, s, BATCH ? s_k[ci] : k1, sink, active);
+        if (active && fin) {
+            store_finish(out, gi, st, s.b);
+            n_rays += 1;
+            n_steps += (uint32_t)st.i;
+            n_cross += st.n_cross;
+            n_esc += st.status == 1 ? 1u : 0u;
+            active = false;
         }
     }
     add_stats(sc.stats, n_rays, n_steps, n_cross, n_esc, 0u);
@@ -625,7 +626,11 @@ hipError_t launch_batch(const MarchConst* table, const uint32_t* item_base, uint
     if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
     MarchConst none;
     __builtin_memset(&none, 0, sizeof none);
-    if ((e = launch_persist<true>(t, num_cus, s, none, table, item_base, ncam, n_items, out, sc)) !=
+    // 256-thread blocks: the LDS camera table (~11 KB per block) would cap 64-thread blocks at
+    // 3-4 waves per SIMD.
+    Tuning tb = t;
+    tb.block = 256;
+    if ((e = launch_persist<true>(tb, num_cus, s, none, table, item_base, ncam, n_items, out, sc)) !=
         hipSuccess)
         return e;
     if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;

@@ -16,15 +16,38 @@
 
 namespace bhs {
 
+// Correctly rounded FP32 square root of x >= 0 (finite or +inf). The hardware estimate s (within
+// 1 ulp) is corrected by the residuals of its two float neighbours: sd = s - 1 ulp is the root if
+// fma(-sd, s, x) <= 0, su = s + 1 ulp if fma(-su, s, x) > 0, otherwise s. Inputs below 2^-96 are
+// scaled by 2^32 first and the root by 2^-16 after (both exact), so the residuals never lose bits to
+// denormals. x = 0 and +inf come out exact. This is the compiler's own IEEE sqrt expansion without
+// its branch on special values, which non-negative operands never need; written branch-free it
+// keeps the march loop free of divergent control flow.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const bool tiny = x < 0x1p-96f;
+    const float xs = tiny ? x * 0x1p+32f : x;
+    const float s = __builtin_amdgcn_sqrtf(xs);
+    const float sd = __int_as_float(__float_as_int(s) - 1);
+    const float su = __int_as_float(__float_as_int(s) + 1);
+    const float rd = __builtin_fmaf(-sd, s, xs);
+    const float ru = __builtin_fmaf(-su, s, xs);
+    float r = rd <= 0.0f ? sd : s;
+    r = ru > 0.0f ? su : r;
+    return tiny ? r * 0x1p-16f : r;
+}
+
 template <typename R> struct Ar;
 template <> struct Ar<float> {
     static __device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
     static __device__ __forceinline__ float sqrt(float a) { return __builtin_sqrtf(a); }
+    // Square root of the march loop's operands (see sqrt_rn): same result as sqrt().
+    static __device__ __forceinline__ float sqrt_loop(float a) { return sqrt_rn(a); }
     static __device__ __forceinline__ float abs(float a) { return __builtin_fabsf(a); }
 };
 template <> struct Ar<double> {
     static __device__ __forceinline__ double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
     static __device__ __forceinline__ double sqrt(double a) { return __builtin_sqrt(a); }
+    static __device__ __forceinline__ double sqrt_loop(double a) { return __builtin_sqrt(a); }
     static __device__ __forceinline__ double abs(double a) { return __builtin_fabs(a); }
 };
 
@@ -188,17 +211,17 @@ __device__ __forceinline__ void rk4(V3<R>& x, V3<R>& v, R c3, R dt, R r2, R r) {
     const V3<R> x2{A::fma(hdt, v.x, x.x), A::fma(hdt, v.y, x.y), A::fma(hdt, v.z, x.z)};
     const V3<R> v2{A::fma(hdt, a1.x, v.x), A::fma(hdt, a1.y, v.y), A::fma(hdt, a1.z, v.z)};
     const R q2 = dot3(x2, x2);
-    const R s2 = c3 / ((q2 * q2) * A::sqrt(q2));
+    const R s2 = c3 / ((q2 * q2) * A::sqrt_loop(q2));
     const V3<R> a2{x2.x * s2, x2.y * s2, x2.z * s2};
     const V3<R> x3{A::fma(hdt, v2.x, x.x), A::fma(hdt, v2.y, x.y), A::fma(hdt, v2.z, x.z)};
     const V3<R> v3{A::fma(hdt, a2.x, v.x), A::fma(hdt, a2.y, v.y), A::fma(hdt, a2.z, v.z)};
     const R q3 = dot3(x3, x3);
-    const R s3 = c3 / ((q3 * q3) * A::sqrt(q3));
+    const R s3 = c3 / ((q3 * q3) * A::sqrt_loop(q3));
     const V3<R> a3{x3.x * s3, x3.y * s3, x3.z * s3};
     const V3<R> x4{A::fma(dt, v3.x, x.x), A::fma(dt, v3.y, x.y), A::fma(dt, v3.z, x.z)};
     const V3<R> v4{A::fma(dt, a3.x, v.x), A::fma(dt, a3.y, v.y), A::fma(dt, a3.z, v.z)};
     const R q4 = dot3(x4, x4);
-    const R s4 = c3 / ((q4 * q4) * A::sqrt(q4));
+    const R s4 = c3 / ((q4 * q4) * A::sqrt_loop(q4));
     const V3<R> a4{x4.x * s4, x4.y * s4, x4.z * s4};
     const R dt6 = dt * R(0.16666666666666666);
     x.x = A::fma(dt6, A::fma(R(2), v3.x, A::fma(R(2), v2.x, v.x)) + v4.x, x.x);
@@ -313,34 +336,41 @@ template <typename R> struct RegSink {
 };
 
 // One iteration of the march loop (blackhole.frag:142-216, thin disk). Returns true when the ray
-// is finished: horizon (:144), escape (:151-155) or out of steps (loop bound, :252).
+// is finished: out of steps (loop bound, :252), horizon (:144) or escape (:151-155), checked in
+// that order. An inactive lane (live = false) or a finished ray changes nothing but x and v.
+//
+// x and v are advanced unconditionally: for a ray that finishes in this iteration they are never
+// read again, and a branch around the update would make every divergent merge copy the ray state.
+// Only the counters, the status and the records are predicated.
 template <typename R, typename Sink>
 __device__ __forceinline__ bool march_step(MarchState<R>& st, const Setup<R>& s, const MarchK<R>& k,
-                                           Sink& sink) {
+                                           Sink& sink, bool live = true) {
     using A = Ar<R>;
-    if (st.i >= k.max_steps) return true;
     V3<R>& x = st.x;
     V3<R>& v = st.v;
+    const bool over = !live || st.i >= k.max_steps;
     const R r2 = dot3(x, x);
-    const R r = A::sqrt(r2);
-    if (r < R(2.02)) { st.status = 2; return true; }                     // :144
-    const R iv = R(1) / A::sqrt(dot3(v, v));
-    if (dot3(x, v) > R(0) && r > k.r_esc_min) {                          // :151-155
+    const R r = A::sqrt_loop(r2);
+    const bool horizon = r < R(2.02);                                    // :144
+    const R iv = R(1) / A::sqrt_loop(dot3(v, v));
+    bool escape = false;
+    if (!over && !horizon && dot3(x, v) > R(0) && r > k.r_esc_min) {     // :151-155
         const R b = s.h * iv;
         const R q = (((R(0.75) * b) * b) * b) / (r2 * r2);
         if (q < R(0.01)) {
             const V3<R> e = far_field(x, v, r, iv, s.h2);
             sink.escape((float)e.x, (float)e.y, (float)e.z);
-            st.status = 1;
-            return true;
+            escape = true;
         }
     }
+    if (!over) st.status = horizon ? 2 : (escape ? 1 : 0);
+    const bool done = over || horizon || escape;
     R ds = (k.ss * r) * s.far_mul;                                        // :169
     if (ds < R(0.02)) ds = R(0.02);
     if (k.thin) {                                                         // :170-181
         const R rc2 = A::fma(x.z, x.z, x.x * x.x);
         if (rc2 > k.lo2 && rc2 < k.hi2) {
-            const R rcyl = A::sqrt(rc2);
+            const R rcyl = A::sqrt_loop(rc2);
             const R outside = A::fma(-k.slab_k, rcyl, A::abs(x.y));
             if (outside > R(0)) {
                 const R den = A::fma(A::abs(v.y), iv, k.slab_k);
@@ -354,6 +384,7 @@ __device__ __forceinline__ bool march_step(MarchState<R>& st, const Setup<R>& s,
     const V3<R> x0 = x;
     const V3<R> v0 = v;
     rk4(x, v, s.c3, dt, r2, r);                                           // :194
+    if (done) return true;
     st.i = st.i + 1;                                                      // :196
     if (k.thin && ((x.y > R(0)) != (x0.y > R(0)))) {                      // :209-216
         const R f = x0.y / (x0.y - x.y);
