@@ -64,14 +64,14 @@ class TraceResult:
 
 def march_params(max_steps=450, step_scale=None, disk_step=None, disk_inner=6.0, disk_outer=20.0,
                  disk_thickness=0.035, disk_mode=DISK_THIN, precision=PREC_F32_BAND64, root64=0,
-                 band_rel=0.02) -> _capi.MarchParams:
+                 band_rel=0.02, cross_all=0) -> _capi.MarchParams:
     """bhs_march_params; step_scale/disk_step default to the quality preset of max_steps
     (src/Settings.cpp:15-30), else to the High preset."""
     ss, dstep = PRESETS.get(max_steps, PRESETS[450])
     return _capi.MarchParams(max_steps, disk_mode, precision, root64,
                              ss if step_scale is None else step_scale,
                              dstep if disk_step is None else disk_step, disk_inner, disk_outer,
-                             disk_thickness, band_rel)
+                             disk_thickness, band_rel, cross_all)
 
 
 def _camera_struct(cam: Camera) -> _capi.Camera:

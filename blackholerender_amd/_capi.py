@@ -38,7 +38,8 @@ class MarchParams(C.Structure):
     _fields_ = [("max_steps", C.c_int32), ("disk_mode", C.c_int32), ("precision", C.c_int32),
                 ("root64", C.c_int32), ("step_scale", C.c_float), ("disk_step", C.c_float),
                 ("disk_inner", C.c_float), ("disk_outer", C.c_float),
-                ("disk_thickness", C.c_float), ("band_rel", C.c_float)]
+                ("disk_thickness", C.c_float), ("band_rel", C.c_float),
+                ("cross_all", C.c_int32)]
 
 
 class RaySoA(C.Structure):

@@ -137,6 +137,7 @@ bhs::MarchConst make_const(const bhs_march_params* p) {
     c.band_abs = p->band_rel * bhs::kBcritF;
     c.precision = p->precision;
     c.root64 = p->root64 ? 1 : 0;
+    c.cross_all = p->cross_all ? 1 : 0;
     return c;
 }
 

@@ -75,6 +75,8 @@ struct MarchConst {
     float band_abs;          // |h - b_crit| < band_abs -> FP64 march (0 disables)
     int32_t precision;       // BHS_PREC_*
     int32_t root64;
+    int32_t cross_all;       // record crossings at any radius
+    int32_t pad_;
     int64_t ray_offset;      // first output index of this camera (batched launches)
 };
 
@@ -277,6 +279,7 @@ template <typename R> struct MarchK {
     R ss, dstep, inner, outer, slab_k, lo2, hi2, r_esc_min;
     int32_t max_steps;
     bool thin;
+    bool cross_all;
 };
 
 template <typename R>
@@ -294,6 +297,7 @@ __device__ __forceinline__ MarchK<R> march_consts(const MarchConst& c) {
     k.r_esc_min = er > R(12) ? er : R(12);
     k.max_steps = c.max_steps;
     k.thin = c.thin != 0;
+    k.cross_all = c.cross_all != 0;
     return k;
 }
 
@@ -391,7 +395,7 @@ __device__ __forceinline__ bool march_step(MarchState<R>& st, const Setup<R>& s,
         const R hx = A::fma(f, x.x - x0.x, x0.x);
         const R hz = A::fma(f, x.z - x0.z, x0.z);
         const R rh = A::sqrt(A::fma(hz, hz, hx * hx));
-        if (rh >= k.inner && rh <= k.outer) {                             // disk.glsl:114-115
+        if (k.cross_all || (rh >= k.inner && rh <= k.outer)) {            // disk.glsl:114-115
             if (st.n_cross < 2) {
                 const RootState<R> q{{x0.x, x0.y, x0.z}, {v0.x, v0.y, v0.z}, dt};
                 sink.cross(st.n_cross, (float)rh, atan2_ieee((float)hz, (float)hx),

@@ -107,6 +107,9 @@ typedef struct bhs_march_params {
     float disk_inner, disk_outer, disk_thickness;
     float band_rel;           /* FP64 band half-width as a fraction of b_crit (0.02 keeps the FP32
                                  escape direction within 1e-4 rad of FP64 at 1080p) */
+    int32_t cross_all;        /* 1: record the first two equatorial crossings at any radius (class
+                                 disk then means "crossed the plane"); step control still uses
+                                 the disk annulus. Used to generate surrogate training data. */
 } bhs_march_params;
 
 /* Structure-of-arrays ray output in device memory; plane k of a multi-plane array starts at

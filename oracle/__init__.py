@@ -33,7 +33,7 @@ class TraceResult(C.Structure):
 
 class LawSParams(C.Structure):
     _fields_ = [("max_steps", C.c_int32), ("disk_mode", C.c_int32), ("root64", C.c_int32),
-                ("pad_", C.c_int32), ("step_scale", C.c_float), ("disk_step", C.c_float),
+                ("cross_all", C.c_int32), ("step_scale", C.c_float), ("disk_step", C.c_float),
                 ("disk_inner", C.c_float), ("disk_outer", C.c_float),
                 ("disk_thickness", C.c_float), ("band_rel", C.c_float)]
 
@@ -96,6 +96,8 @@ def lib():
                                     C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(LawSParams),
                                     C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_void_p,
                                     C.c_int32]
+        L.oracle_march_batch.argtypes = [fp, fp, C.c_int64, C.POINTER(LawSParams), C.c_int32,
+                                         C.c_void_p, C.c_int32]
         L.oracle_atan2f.argtypes = [C.c_float, C.c_float]
         L.oracle_atan2f.restype = C.c_float
         _lib = L
@@ -176,8 +178,9 @@ def trace_batch(pos3, dir3, max_steps=200000, step_scale=0.005, esc_tol=1e-4, r_
 
 # ---------------------------------------------------------------------------------- law S
 def lawS_params(max_steps=450, step_scale=0.05, disk_step=0.18, disk_inner=6.0, disk_outer=20.0,
-                disk_thickness=0.035, disk_mode=1, root64=0, band_rel=0.02) -> LawSParams:
-    return LawSParams(max_steps, disk_mode, root64, 0, step_scale, disk_step, disk_inner,
+                disk_thickness=0.035, disk_mode=1, root64=0, band_rel=0.02,
+                cross_all=0) -> LawSParams:
+    return LawSParams(max_steps, disk_mode, root64, cross_all, step_scale, disk_step, disk_inner,
                       disk_outer, disk_thickness, band_rel)
 
 
@@ -197,6 +200,18 @@ def march(pos, d, params: LawSParams, mode="f32") -> LawSRay:
     fn = {"f32": lib().oracle_lawS_f32, "f64": lib().oracle_lawS_f64,
           "band": lib().oracle_lawS_band}[mode]
     fn(pp, dp_, C.byref(params), C.byref(out))
+    return out
+
+
+def march_batch(pos3, dir3, params: LawSParams, mode="f32", threads=8) -> np.ndarray:
+    """Law S on n explicit rays ((n,3) float32 positions and directions)."""
+    pos3 = np.ascontiguousarray(pos3, dtype=np.float32).reshape(-1, 3)
+    dir3 = np.ascontiguousarray(dir3, dtype=np.float32).reshape(-1, 3)
+    n = pos3.shape[0]
+    out = np.zeros(n, dtype=LAWS_RAY_DTYPE)
+    fp = C.POINTER(C.c_float)
+    lib().oracle_march_batch(pos3.ctypes.data_as(fp), dir3.ctypes.data_as(fp), n, C.byref(params),
+                             _MODES[mode], out.ctypes.data, threads)
     return out
 
 

@@ -437,6 +437,46 @@ void oracle_render(const float pos[3], const float basis[9], float tan_half_fov,
     for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
 }
 
+typedef struct march_job {
+    const float* pos3;
+    const float* dir3;
+    int64_t n;
+    const oracle_lawS_params* p;
+    int32_t mode;
+    oracle_lawS_ray* out;
+    int32_t tid, nthreads;
+} march_job;
+
+static void* march_worker(void* arg) {
+    march_job* j = (march_job*)arg;
+    for (int64_t k = j->tid; k < j->n; k += j->nthreads) {
+        const float* P = j->pos3 + 3 * k;
+        const float* d = j->dir3 + 3 * k;
+        if (j->mode == 0) oracle_lawS_f32(P, d, j->p, &j->out[k]);
+        else if (j->mode == 1) oracle_lawS_f64(P, d, j->p, &j->out[k]);
+        else oracle_lawS_band(P, d, j->p, &j->out[k]);
+    }
+    return NULL;
+}
+
+void oracle_march_batch(const float* pos3, const float* dir3, int64_t n, const oracle_lawS_params* p,
+                        int32_t mode, oracle_lawS_ray* out, int32_t threads) {
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    march_job jobs[256];
+    pthread_t th[256];
+    for (int t = 0; t < threads; ++t) {
+        march_job j = {pos3, dir3, n, p, mode, out, t, threads};
+        jobs[t] = j;
+    }
+    if (threads == 1) {
+        march_worker(&jobs[0]);
+        return;
+    }
+    for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, march_worker, &jobs[t]);
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+}
+
 typedef struct trace_job {
     const double* pos3;
     const double* dir3;

@@ -61,7 +61,8 @@ void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int
 typedef struct oracle_lawS_params {
     int32_t max_steps, disk_mode; /* disk_mode 0 none, 1 thin */
     int32_t root64;               /* 1: FP64 root bracketing of each in-annulus crossing */
-    int32_t pad_;
+    int32_t cross_all;            /* 1: record equatorial crossings at any radius (step control
+                                     still uses the disk annulus) */
     float step_scale, disk_step, disk_inner, disk_outer, disk_thickness;
     float band_rel;               /* band half-width / b_crit used by oracle_lawS_band */
 } oracle_lawS_params;
@@ -106,6 +107,11 @@ void oracle_render(const float pos[3], const float basis[9], float tan_half_fov,
                    uint32_t camera_id, uint32_t sample, const oracle_lawS_params* p, int32_t mode,
                    int64_t first, int64_t count, int64_t stride, oracle_lawS_ray* out,
                    int32_t threads);
+
+/* Law S on n explicit rays (pos3/dir3 float xyz interleaved), `mode` 0 float, 1 double, 2 float
+ * with FP64 band, `threads` workers. */
+void oracle_march_batch(const float* pos3, const float* dir3, int64_t n, const oracle_lawS_params* p,
+                        int32_t mode, oracle_lawS_ray* out, int32_t threads);
 
 /* Atan2 built from IEEE operations only (range reduction + degree-4 polynomial in t^2); identical
  * in any IEEE-754 float implementation that honours fmaf. */
