@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import (CLASS_DISK, CLASS_ESCAPE, CLASS_HORIZON, DISK_NONE, DISK_THIN, FLAG_BAND64,
-                    FLAG_ESCAPED, FLAG_FALLBACK, FLAG_MAXSTEPS, FLAG_ROOT64, FLAG_SURROGATE,
+                    FLAG_ESCAPED, FLAG_MAXSTEPS, FLAG_ROOT64, FLAG_SURROGATE,
                     PREC_F32, PREC_F32_BAND64, PREC_F64, TRACE_CAPTURED, TRACE_ESCAPED,
                     TRACE_MAXSTEPS)
 from .camera import PRESETS, Camera, look_basis, orbit_camera
@@ -30,7 +30,7 @@ from .camera import PRESETS, Camera, look_basis, orbit_camera
 __all__ = ["Tracer", "BhsError", "TraceResult", "RayBuffers", "march_params", "trace",
            "Camera", "orbit_camera", "look_basis", "PRESETS", "critical_impact_parameter",
            "CLASS_HORIZON", "CLASS_DISK", "CLASS_ESCAPE", "FLAG_MAXSTEPS", "FLAG_ESCAPED",
-           "FLAG_BAND64", "FLAG_ROOT64", "FLAG_SURROGATE", "FLAG_FALLBACK", "PREC_F32",
+           "FLAG_BAND64", "FLAG_ROOT64", "FLAG_SURROGATE", "PREC_F32",
            "PREC_F32_BAND64", "PREC_F64", "DISK_NONE", "DISK_THIN", "TRACE_ESCAPED",
            "TRACE_CAPTURED", "TRACE_MAXSTEPS"]
 
@@ -252,7 +252,7 @@ class Tracer:
         s = _capi.Stats()
         self._check(self._lib.bhs_last_stats(self._ctx, C.byref(s)), "bhs_last_stats")
         return dict(rays=s.rays, steps=s.steps, crossings=s.crossings, escapes=s.escapes,
-                    band64=s.band64, fallback=s.fallback)
+                    band64=s.band64, fallback=s.fallback, surrogate=s.surrogate)
 
     def set_tuning(self, mode: int, block: int = 256, waves_per_simd: int = 8,
                    refill_min: int = 32):
@@ -328,7 +328,7 @@ class BatchGraph:
         self.tracer._check(self.tracer._lib.bhs_graph_stats(self.tracer._ctx, self._g, C.byref(s)),
                            "bhs_graph_stats")
         return dict(rays=s.rays, steps=s.steps, crossings=s.crossings, escapes=s.escapes,
-                    band64=s.band64, fallback=s.fallback)
+                    band64=s.band64, fallback=s.fallback, surrogate=s.surrogate)
 
     def close(self):
         if getattr(self, "_g", None):

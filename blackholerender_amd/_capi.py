@@ -17,7 +17,7 @@ STATUS_NAMES = {0: "BHS_OK", -1: "BHS_E_INVALID", -2: "BHS_E_DEVICE", -3: "BHS_E
 
 TRACE_ESCAPED, TRACE_CAPTURED, TRACE_MAXSTEPS = 0, 1, 2
 CLASS_HORIZON, CLASS_DISK, CLASS_ESCAPE = 0, 1, 2
-FLAG_MAXSTEPS, FLAG_ESCAPED, FLAG_BAND64, FLAG_ROOT64, FLAG_SURROGATE, FLAG_FALLBACK = 1, 2, 4, 8, 16, 32
+FLAG_MAXSTEPS, FLAG_ESCAPED, FLAG_BAND64, FLAG_ROOT64, FLAG_SURROGATE = 1, 2, 4, 8, 16
 PREC_F32, PREC_F32_BAND64, PREC_F64 = 0, 1, 2
 DISK_NONE, DISK_THIN = 0, 1
 
@@ -56,7 +56,26 @@ class Tuning(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("steps", C.c_uint64), ("crossings", C.c_uint64),
                 ("escapes", C.c_uint64), ("band64", C.c_uint64), ("fallback", C.c_uint64),
-                ("pad_", C.c_uint64 * 2)]
+                ("surrogate", C.c_uint64), ("pad_", C.c_uint64)]
+
+
+MLP_PARAMS = 34181
+
+
+class MlpInfo(C.Structure):
+    _fields_ = [("trained_for", MarchParams), ("dataset_rays", C.c_int64),
+                ("adam_steps", C.c_int64), ("tol_phi", C.c_float), ("radius_margin", C.c_float),
+                ("logit_margin", C.c_float), ("steps_slack", C.c_float),
+                ("steps_max", C.c_uint32 * 16), ("served_cells", C.c_int32), ("pad_", C.c_int32)]
+
+
+class MlpValidation(C.Structure):
+    _fields_ = [("rays", C.c_int64), ("in_domain", C.c_int64), ("served", C.c_int64),
+                ("escaping", C.c_int64), ("mismatches", C.c_int64),
+                ("phi_err_max", C.c_float), ("phi_err_p99", C.c_float),
+                ("phi_err_p50", C.c_float), ("r_err_max", C.c_float), ("r_err_p99", C.c_float),
+                ("r_err_p50", C.c_float), ("radius_margin", C.c_float),
+                ("served_cells", C.c_int32)]
 
 
 _lib = None
@@ -96,6 +115,28 @@ def load():
     L.bhs_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     L.bhs_autotune.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams), C.POINTER(RaySoA), i32,
                                C.POINTER(Tuning), C.POINTER(C.c_double)]
+    u32, fp = C.c_uint32, C.POINTER(C.c_float)
+    L.bhs_mlp_create.argtypes = [vp, C.POINTER(MarchParams), u32, C.POINTER(C.c_int)]
+    L.bhs_mlp_create.restype = vp
+    L.bhs_mlp_destroy.argtypes = [vp]
+    L.bhs_mlp_destroy.restype = None
+    L.bhs_mlp_make_dataset.argtypes = [vp, vp, i64, u32]
+    L.bhs_mlp_train.argtypes = [vp, vp, i32, i32, C.c_float, C.c_float, vp]
+    L.bhs_mlp_grads.argtypes = [vp, vp, vp, vp, vp, i32, vp, fp]
+    L.bhs_mlp_forward.argtypes = [vp, vp, vp, i64, vp, vp]
+    L.bhs_mlp_validate.argtypes = [vp, vp, i64, u32, C.c_float, C.POINTER(MlpValidation)]
+    L.bhs_mlp_get_params.argtypes = [vp, vp, vp, i64]
+    L.bhs_mlp_set_params.argtypes = [vp, vp, vp, i64]
+    L.bhs_mlp_get_info.argtypes = [vp, vp, C.POINTER(MlpInfo)]
+    L.bhs_mlp_save.argtypes = [vp, vp, C.c_char_p]
+    L.bhs_mlp_load.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
+    L.bhs_mlp_load.restype = vp
+    L.bhs_render_surrogate.argtypes = [vp, vp, C.POINTER(Camera), C.POINTER(MarchParams),
+                                       C.POINTER(RaySoA), vp]
+    for name in ("bhs_mlp_make_dataset", "bhs_mlp_train", "bhs_mlp_grads", "bhs_mlp_forward",
+                 "bhs_mlp_validate", "bhs_mlp_get_params", "bhs_mlp_set_params",
+                 "bhs_mlp_get_info", "bhs_mlp_save", "bhs_render_surrogate"):
+        getattr(L, name).restype = C.c_int
     for name in ("bhs_sync", "bhs_trace_rays", "bhs_render", "bhs_march_rays", "bhs_last_stats",
                  "bhs_set_timing", "bhs_timing", "bhs_set_tuning", "bhs_get_tuning",
                  "bhs_autotune", "bhs_render_batch", "bhs_graph_create", "bhs_graph_update",

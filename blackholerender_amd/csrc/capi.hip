@@ -11,35 +11,8 @@
 #include <vector>
 
 #include "../../include/bhs_trace.h"
+#include "capi_internal.hpp"
 #include "kernels.hpp"
-
-struct bhs_ctx {
-    int device = 0;
-    std::string err;
-    hipStream_t aux = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    hipStream_t last_stream = nullptr;
-    // Render scratch: [Stats | band_count (pad 256) | band_ids | roots]
-    void* scratch = nullptr;
-    size_t scratch_bytes = 0;
-    // Law-T staging (pos3, dir3, results).
-    void* tstage = nullptr;
-    size_t tstage_bytes = 0;
-    // GPU timers: per launch, events before the call, around the main kernel, after the call.
-    struct TimingRec { hipEvent_t call0, k0, k1, call1; };
-    bool timing = false;
-    std::vector<TimingRec> timers;
-    size_t timers_used = 0;
-    int num_cus = 256;
-    bhs::Tuning tuning{1, 256, 7, 32};
-    // Batch camera tables: device copy, pinned staging buffer and the event of its last upload.
-    void* table_dev = nullptr;
-    size_t table_dev_bytes = 0;
-    void* table_host = nullptr;
-    size_t table_host_bytes = 0;
-    hipEvent_t ev_upload = nullptr;
-    bool upload_pending = false;
-};
 
 // A captured batch launch (bhs_graph_*): owns its camera table and scratch so later calls on the
 // context cannot move them, and its own capture/fork streams and events.
@@ -58,9 +31,10 @@ struct bhs_graph {
     hipGraphExec_t exec = nullptr;
 };
 
-namespace {
+namespace bhs {
+namespace capi {
 
-int fail(bhs_ctx* ctx, int code, const char* fmt, const char* detail = nullptr) {
+int fail(bhs_ctx* ctx, int code, const char* fmt, const char* detail) {
     if (ctx) {
         char buf[512];
         std::snprintf(buf, sizeof buf, fmt, detail ? detail : "");
@@ -73,18 +47,6 @@ int hip_fail(bhs_ctx* ctx, hipError_t e, const char* where) {
     if (ctx) ctx->err = std::string(where) + ": " + hipGetErrorString(e);
     return (e == hipErrorOutOfMemory) ? BHS_E_ALLOC : BHS_E_DEVICE;
 }
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
 
 // Grows a device buffer, after draining work that may still use the old one.
 int ensure(bhs_ctx* ctx, void** buf, size_t* cap, size_t need, const char* what) {
@@ -198,8 +160,6 @@ bhs::SoA make_soa(const bhs_ray_soa* o, int64_t n) {
     return s;
 }
 
-constexpr size_t kStatsBytes = 256;   // Stats + band counter + work counter, padded
-
 // Next timer record of the context (null when timing is off or events cannot be created).
 bhs_ctx::TimingRec* next_timer(bhs_ctx* ctx) {
     if (!ctx->timing) return nullptr;
@@ -298,7 +258,10 @@ void pack_table(const BatchTable& t, char* dst) {
     std::memcpy(dst + table_cams_bytes(ncam), t.item_base.data(), t.item_base.size() * sizeof(uint32_t));
 }
 
-}  // namespace
+}  // namespace capi
+}  // namespace bhs
+
+using namespace bhs::capi;
 
 extern "C" {
 
@@ -786,6 +749,7 @@ int bhs_last_stats(bhs_ctx* ctx, bhs_stats* out_host) {
     out_host->escapes = s.escapes;
     out_host->band64 = s.band64;
     out_host->fallback = s.fallback;
+    out_host->surrogate = s.surrogate;
     return BHS_OK;
 }
 

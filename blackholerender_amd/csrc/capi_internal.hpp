@@ -1,0 +1,73 @@
+// capi_internal.hpp — the context object of the C-ABI and the helpers shared by its source files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/bhs_trace.h"
+#include "kernels.hpp"
+
+struct bhs_ctx {
+    int device = 0;
+    std::string err;
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t last_stream = nullptr;
+    // Render scratch: [Stats | band_count (pad 256) | band_ids | roots]
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // Law-T staging (pos3, dir3, results).
+    void* tstage = nullptr;
+    size_t tstage_bytes = 0;
+    // GPU timers: per launch, events before the call, around the main kernel, after the call.
+    struct TimingRec { hipEvent_t call0, k0, k1, call1; };
+    bool timing = false;
+    std::vector<TimingRec> timers;
+    size_t timers_used = 0;
+    int num_cus = 256;
+    bhs::Tuning tuning{1, 256, 7, 32};
+    // Batch camera tables: device copy, pinned staging buffer and the event of its last upload.
+    void* table_dev = nullptr;
+    size_t table_dev_bytes = 0;
+    void* table_host = nullptr;
+    size_t table_host_bytes = 0;
+    hipEvent_t ev_upload = nullptr;
+    bool upload_pending = false;
+};
+
+
+namespace bhs {
+namespace capi {
+
+constexpr size_t kStatsBytes = 256;   // Stats + band counter + work counter, padded
+
+// Records `fmt` (one %s for `detail`) as the context's last error and returns `code`.
+int fail(bhs_ctx* ctx, int code, const char* fmt, const char* detail = nullptr);
+int hip_fail(bhs_ctx* ctx, hipError_t e, const char* where);
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// Grows a device buffer, after draining work that may still use the old one.
+int ensure(bhs_ctx* ctx, void** buf, size_t* cap, size_t need, const char* what);
+int check_params(bhs_ctx* ctx, const bhs_march_params* p);
+int check_camera(bhs_ctx* ctx, const bhs_camera* cam);
+bhs::MarchConst make_const(const bhs_march_params* p);
+void fill_camera(bhs::MarchConst& c, const bhs_camera* cam);
+bhs::SoA make_soa(const bhs_ray_soa* o, int64_t n);
+bhs_ctx::TimingRec* next_timer(bhs_ctx* ctx);
+int prepare_scratch(bhs_ctx* ctx, int64_t n_rays, int64_t n_items, bool roots, bhs::Scratch* sc);
+
+}  // namespace capi
+}  // namespace bhs

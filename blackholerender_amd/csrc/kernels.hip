@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_root64(const Cams cams, const SoA ou
         const int64_t g = c.ray_offset + pix;
         const uint32_t nc = out.n_cross[g];
         const uint32_t fl = out.flags[g];
-        if (nc == 0 || (fl & BHS_FLAG_BAND64)) continue;
+        if (nc == 0 || (fl & (BHS_FLAG_BAND64 | BHS_FLAG_SURROGATE))) continue;
         const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
         // c3, b_axis and the redshift lapse of this ray, from its float setup.
         const Setup<float> s = lawS_setup(P, ray_dir(c, pix));
@@ -327,7 +327,8 @@ template <int BLOCK, bool ROOTS, int WPE, bool BATCH>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
 void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
                const uint32_t* __restrict__ item_base, uint32_t ncam, uint32_t n_items,
-               const SoA out, Scratch sc, uint32_t refill_min) {
+               const SoA out, Scratch sc, uint32_t refill_min, const uint32_t* __restrict__ list,
+               const uint32_t* __restrict__ list_count) {
     __shared__ MarchConst s_cam[BATCH ? kMaxBatchCams : 1];
     __shared__ MarchK<float> s_k[BATCH ? kMaxBatchCams : 1];
     __shared__ uint32_t s_base[BATCH ? kMaxBatchCams + 1 : 1];
@@ -341,6 +342,10 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
     }
     const MarchK<float> k1 = march_consts<float>(c1);
     const uint32_t lane = threadIdx.x & 63u;
+    if (list) {   // listed items: the count is known on the device only
+        const uint32_t cnt = *list_count;
+        n_items = cnt < n_items ? cnt : n_items;
+    }
     MarchState<float> st;
     Setup<float> s;
     uint32_t pix = 0, ci = 0;
@@ -360,7 +365,7 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
                     (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
                 const uint32_t q = base + rank;
                 if (q < n_items) {
-                    uint32_t local = q;
+                    uint32_t local = list ? list[q] : q;
                     if (BATCH) {
                         ci = find_cam(s_base, ncam, q);
                         local = q - s_base[ci];
@@ -516,25 +521,27 @@ dim3 item_grid(uint32_t n_items) {
 template <int BLOCK, bool ROOTS, bool BATCH>
 void launch_persist_w(dim3 g, int wps, hipStream_t s, const MarchConst& c1, const MarchConst* table,
                       const uint32_t* item_base, uint32_t ncam, uint32_t n_items, const SoA& out,
-                      const Scratch& sc, uint32_t rmin) {
+                      const Scratch& sc, uint32_t rmin, const uint32_t* list,
+                      const uint32_t* list_count) {
     if (wps >= 8)
         hipLaunchKernelGGL((k_persist<BLOCK, ROOTS, 8, BATCH>), g, dim3(BLOCK), 0, s, c1, table,
-                           item_base, ncam, n_items, out, sc, rmin);
+                           item_base, ncam, n_items, out, sc, rmin, list, list_count);
     else if (wps == 7)
         hipLaunchKernelGGL((k_persist<BLOCK, ROOTS, 7, BATCH>), g, dim3(BLOCK), 0, s, c1, table,
-                           item_base, ncam, n_items, out, sc, rmin);
+                           item_base, ncam, n_items, out, sc, rmin, list, list_count);
     else if (wps >= 5)
         hipLaunchKernelGGL((k_persist<BLOCK, ROOTS, 6, BATCH>), g, dim3(BLOCK), 0, s, c1, table,
-                           item_base, ncam, n_items, out, sc, rmin);
+                           item_base, ncam, n_items, out, sc, rmin, list, list_count);
     else
         hipLaunchKernelGGL((k_persist<BLOCK, ROOTS, 4, BATCH>), g, dim3(BLOCK), 0, s, c1, table,
-                           item_base, ncam, n_items, out, sc, rmin);
+                           item_base, ncam, n_items, out, sc, rmin, list, list_count);
 }
 
 template <bool BATCH>
 hipError_t launch_persist(const Tuning& t, int num_cus, hipStream_t s, const MarchConst& c1,
                           const MarchConst* table, const uint32_t* item_base, uint32_t ncam,
-                          uint32_t n_items, const SoA& out, const Scratch& sc) {
+                          uint32_t n_items, const SoA& out, const Scratch& sc,
+                          const uint32_t* list = nullptr, const uint32_t* list_count = nullptr) {
     const int block = t.block == 64 ? 64 : 256;
     const int wps = t.waves_per_simd < 1 ? 1 : (t.waves_per_simd > 8 ? 8 : t.waves_per_simd);
     const uint32_t waves = (uint32_t)num_cus * 4u * (uint32_t)wps;
@@ -544,11 +551,11 @@ hipError_t launch_persist(const Tuning& t, int num_cus, hipStream_t s, const Mar
     const dim3 g(blocks ? blocks : 1u);
     const bool roots = sc.roots != nullptr;
     if (block == 64) {
-        if (roots) launch_persist_w<64, true, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin);
-        else launch_persist_w<64, false, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin);
+        if (roots) launch_persist_w<64, true, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin, list, list_count);
+        else launch_persist_w<64, false, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin, list, list_count);
     } else {
-        if (roots) launch_persist_w<256, true, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin);
-        else launch_persist_w<256, false, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin);
+        if (roots) launch_persist_w<256, true, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin, list, list_count);
+        else launch_persist_w<256, false, BATCH>(g, wps, s, c1, table, item_base, ncam, n_items, out, sc, rmin, list, list_count);
     }
     return hipGetLastError();
 }
@@ -640,6 +647,30 @@ hipError_t launch_batch(const MarchConst* table, const uint32_t* item_base, uint
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+hipError_t launch_list(const MarchConst& c, const uint32_t* ids, const uint32_t* count, uint32_t cap,
+                       const SoA& out, const Scratch& sc, const Tuning& t, int num_cus,
+                       hipStream_t s) {
+    return launch_persist<false>(t, num_cus, s, c, nullptr, nullptr, 1u, cap, out, sc, ids, count);
+}
+
+hipError_t launch_band_list(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s,
+                            hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join) {
+    const OneCam one{c, camera_items(c.width, c.height)};
+    hipError_t e;
+    if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_band64<OneCam>, dim3(256), dim3(64), 0, aux, one, out, sc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipEventRecord(ev_join, aux);
+}
+
+hipError_t launch_root_refine(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s) {
+    if (c.precision == BHS_PREC_F64 || !c.root64 || !sc.roots) return hipSuccess;
+    const OneCam one{c, camera_items(c.width, c.height)};
+    hipLaunchKernelGGL(k_root64<OneCam>, item_grid(one.n_items), dim3(kBlock), 0, s, one, out, sc);
+    return hipGetLastError();
 }
 
 hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, const MarchConst& c,

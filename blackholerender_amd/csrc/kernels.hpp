@@ -22,7 +22,7 @@ struct SoA {
 
 // Device counters of one launch (layout of bhs_stats).
 struct Stats {
-    unsigned long long rays, steps, crossings, escapes, band64, fallback, pad0, pad1;
+    unsigned long long rays, steps, crossings, escapes, band64, fallback, surrogate, pad1;
 };
 
 // Scratch used by one render launch.
@@ -62,6 +62,17 @@ hipError_t launch_batch(const MarchConst* table, const uint32_t* item_base, uint
                         const Scratch& sc, const Tuning& t, int num_cus, hipStream_t s,
                         hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join,
                         hipEvent_t ev_begin, hipEvent_t ev_end);
+// FP32 persistent march of a listed subset of one camera's work items (ids[0 .. *count), count
+// read on the device, at most cap).
+hipError_t launch_list(const MarchConst& c, const uint32_t* ids, const uint32_t* count, uint32_t cap,
+                       const SoA& out, const Scratch& sc, const Tuning& t, int num_cus,
+                       hipStream_t s);
+// FP64 march of the band list sc.band_ids[0 .. *sc.band_count) on aux, forked from s at ev_fork
+// and joined by ev_join (which the caller waits for).
+hipError_t launch_band_list(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s,
+                            hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join);
+// FP64 root refinement of the crossings of integrator-marched FP32 rays (root64 cameras).
+hipError_t launch_root_refine(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s);
 hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, const MarchConst& c,
                              const SoA& out, const Scratch& sc, hipStream_t s, hipEvent_t ev_begin,
                              hipEvent_t ev_end);

@@ -1,0 +1,84 @@
+// mlp.hpp — launchers of the surrogate's data-generation, training and inference kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+#include "mlp_device.hpp"
+
+namespace bhs {
+namespace mlp {
+
+// Error map of a trained model: a (log r_obs, beta) cell grid over the feature square
+// [-1, 1]^2 of (x1, x2); a set bit marks a cell whose validation rays all met the escape-direction
+// tolerance. Step table: the largest law-S step count of the training rays per log r_obs bin.
+constexpr int kMapR = 32, kMapB = 64;
+constexpr int kMapWords = kMapR * kMapB / 32;
+constexpr int kStepBins = 16;
+
+// Per-model constants of the surrogate render kernel.
+struct SurConst {
+    uint32_t serve[kMapWords];        // error map bits, cell = ir * kMapB + ib
+    uint32_t steps_max[kStepBins];    // training step maximum per log r_obs bin
+    float beta_eps;                   // fallback band |beta - 1| < beta_eps
+    float radius_margin;              // relative margin around the annulus edges
+    float logit_margin;               // |logit| below this: reach undecided -> fallback
+    float steps_slack;                // served only if steps_max * slack < max_steps
+};
+
+// Training arguments of one Adam step.
+struct TrainArgs {
+    const float* X;          // n x 3 features
+    const float* Y;          // n x 5 targets
+    const float* M;          // n x 5 masks
+    int64_t n;
+    float* P;                // kParams parameters (updated in place)
+    float* G;                // kParams gradient (written)
+    float* m;                // Adam moments
+    float* v;
+    float* ws;               // train_workspace_floats(batch) floats
+    int32_t batch;
+    uint32_t key;            // minibatch hash key
+    int32_t ident;           // 1: minibatch row b = dataset row b (no sampling)
+    int32_t reg_count;       // loss normaliser of the regression terms (<= 0: batch)
+    int32_t adam;            // 0: gradient only
+    float lr, b1, b2, eps, c1, c2;   // c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)
+    float* loss_dev;         // device float receiving the minibatch loss (may be null)
+};
+
+hipError_t launch_sample_rays(int64_t n, uint32_t key, float* pos3, float* dir3, hipStream_t s);
+// Features / targets / masks of n marched rays; atomically raises steps_max[bin] (log r bins).
+hipError_t launch_targets(int64_t n, const float* pos3, const float* dir3, const SoA& out, float* X,
+                          float* Y, float* M, uint32_t* steps_max, hipStream_t s);
+hipError_t launch_init(uint32_t key, float* P, hipStream_t s);
+size_t train_workspace_floats(int32_t batch);
+hipError_t launch_train_step(const TrainArgs& a, hipStream_t s);
+
+// Inference weights: bf16 A fragments of layers 2-4 in MFMA lane order, FP32 layer 1 and biases.
+constexpr size_t kPackedBytes = 2 * 32768 + 4096 + 128 * 16 + 2 * 512 + 64;
+hipError_t launch_pack(const float* P, void* packed, hipStream_t s);
+// Raw network outputs of n feature rows (MFMA inference).
+hipError_t launch_forward(const float* X, int64_t n, const void* packed, float* Y, hipStream_t s);
+// Validation: escape-direction error (rad), relative crossing-radius error and error-map cell of
+// each ray's surrogate decode against integrator outputs of the same rays.
+hipError_t launch_validate(int64_t n, const float* pos3, const float* dir3, const MarchConst& c,
+                           const SurConst& k, const SoA& ref, const void* packed, float* err_phi,
+                           float* err_r, int32_t* cell, int num_cus, hipStream_t s);
+
+// Surrogate render of one camera: every eligible ray is served by the network, the others are
+// listed for the integrator (FP64 band list in sc.band_ids, FP32 list in fb_ids).
+struct SurLists {
+    uint32_t* fb_ids;         // FP32 fallback items
+    uint32_t* fb_count;
+    uint32_t fb_cap;
+};
+// ev_begin / ev_mid / ev_end (may be null) bracket the network kernel and the FP32 fallback march.
+hipError_t launch_surrogate(const MarchConst& c, const SurConst& k, const void* packed,
+                            const SoA& out, const Scratch& sc, const SurLists& fl, const Tuning& t,
+                            int num_cus, hipStream_t s, hipStream_t aux, hipEvent_t ev_fork,
+                            hipEvent_t ev_join, hipEvent_t ev_begin, hipEvent_t ev_mid,
+                            hipEvent_t ev_end);
+
+}  // namespace mlp
+}  // namespace bhs

@@ -1,0 +1,212 @@
+// mlp_device.hpp — the learned surrogate of the law-S march: network layout, the physics prior,
+// input features, output decoding and the activation, shared by training-data generation,
+// training, validation and inference.
+//
+// Network: 3 -> 128 -> 128 -> 128 -> 5, SiLU after each hidden layer. Layer 1 runs in FP32 on the
+// VALU; layers 2-4 run on bf16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation. Training
+// rounds the same operands to bf16, so training and inference compute the same function.
+//
+// Orbit-plane frame of one ray from camera position P with view direction d: e1 = P/|P|, e2 the
+// unit tangential part of the initial coordinate velocity v0 (initialVelocity, geodesic.glsl:27-34).
+// The ray sweeps the position angle phi from 0 (camera) upwards; it meets the equatorial plane
+// y = 0 at phi = psi + k pi, psi in [0, pi).
+//
+// Inputs:
+//   x1 = log r_obs mapped from [log 8, log 1000] to [-1, 1]
+//   x2 = sign(beta - 1) log(1 + |beta - 1| / 1e-3) / log(1 + 1e4), beta = b / b_crit
+//   x3 = psi mapped from [0, pi) to [-1, 1)
+// Outputs:
+//   y0  escape: residual of the law-S escape angle over the exact Schwarzschild swept angle,
+//       Phi = Phi_exact + 1e-3 y0, escape direction = cos(Phi) e1 + sin(Phi) e2
+//   y1, y2  u = 1/r at the crossings phi = psi and psi + pi: u = exp(y) / 8 (u >= 1e-3 in training)
+//   y3, y4  logits of "the march reaches crossing 1 / 2"
+//
+// Phi_exact = pi + ahat(beta) - J(u_obs, b): the swept angle from the camera through periapsis to
+// infinity, 2 int_0^{u0} - int_0^{u_obs} of du / sqrt(1/b^2 - u^2 (1 - 2u)), u0 the periapsis.
+// ahat is the infinity-to-infinity deflection as a 25-term Chebyshev series in log(beta - 1)
+// (fitted once to the exact quadrature, error 6e-7 rad), J the 6-node Gauss-Legendre value of the
+// finite-distance part after the substitution u = u0 (1 - t^2) (error < 1e-7 rad for r_obs >= 8).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lawS_device.hpp"
+
+namespace bhs {
+namespace mlp {
+
+constexpr int kIn = 3, kH = 128, kOut = 5;
+// Offsets into the FP32 parameter vector (row-major weights [out][in]).
+constexpr int kW1 = 0;
+constexpr int kB1 = kW1 + kH * kIn;
+constexpr int kW2 = kB1 + kH;
+constexpr int kB2 = kW2 + kH * kH;
+constexpr int kW3 = kB2 + kH;
+constexpr int kB3 = kW3 + kH * kH;
+constexpr int kW4 = kB3 + kH;
+constexpr int kB4 = kW4 + kOut * kH;
+constexpr int kParams = kB4 + kOut;   // 34181
+
+constexpr float kResScale = 1e-3f;   // escape residual unit (rad)
+constexpr float kUMin = 1e-3f;            // crossing targets below this (r > 1000) are clamped
+constexpr float kBetaEps = 1e-3f;
+constexpr float kLogRLo = 2.0794415f;      // log 8
+constexpr float kLogRHi = 6.9077553f;      // log 1000
+constexpr float kPi = 3.14159265358979f;
+constexpr float kSinMax = 0.95f;           // domain: sin(view angle) <= 0.95
+constexpr float kBetaMin = 0.3f;           // domain: beta >= 0.3 (near-radial rays: u(psi) is too
+                                           // steep in beta for the network's resolution)
+
+// Chebyshev coefficients of ahat(s), s = log(beta - 1) in [log 1e-3, log 200].
+constexpr int kAhatTerms = 25;
+constexpr float kAhatC[kAhatTerms] = {
+    2.252328873e+00f, -3.297978401e+00f, 1.115420222e+00f, 6.010319665e-02f,
+    -1.329363883e-01f, -2.116844989e-02f, 2.762471139e-02f, 7.664505392e-03f,
+    -6.423670799e-03f, -2.731988439e-03f, 1.508211251e-03f, 9.516093414e-04f,
+    -3.349816834e-04f, -3.234522010e-04f, 6.337290688e-05f, 1.071531224e-04f,
+    -6.627894436e-06f, -3.451645898e-05f, -2.213727157e-06f, 1.076196713e-05f,
+    2.081819275e-06f, -3.225621867e-06f, -1.096105734e-06f, 9.152259963e-07f,
+    4.800450029e-07f};
+constexpr float kAhatS0 = -6.9077553f, kAhatS1 = 5.2983174f;
+
+// bm = beta - 1 > 0.
+__device__ __forceinline__ float ahat(float bm) {
+    bm = bm < 1e-3f ? 1e-3f : (bm > 200.0f ? 200.0f : bm);
+    const float s = logf(bm);
+    const float t = (s - 0.5f * (kAhatS0 + kAhatS1)) / (0.5f * (kAhatS1 - kAhatS0));
+    float b1 = 0.0f, b2 = 0.0f;
+#pragma unroll
+    for (int k = kAhatTerms - 1; k >= 1; --k) {   // Clenshaw recurrence
+        const float b0 = __builtin_fmaf(2.0f * t, b1, kAhatC[k] - b2);
+        b2 = b1;
+        b1 = b0;
+    }
+    return __builtin_fmaf(t, b1, kAhatC[0] - b2);
+}
+
+// J = int_0^{u_obs} du / sqrt(1/b^2 - u^2 (1 - 2u)) for b > b_crit, u_obs below the periapsis u0.
+// With the roots u0 < u1 and u2 < 0 of 2u^3 - u^2 + 1/b^2 (u0 = 1/6 + cos(theta/3 - 2pi/3)/3,
+// cos theta = 1 - 54/b^2), the radicand is (u0 - u) R(u), R = 2(-u^2 + (1/2 - u0) u + 1/(2 b^2 u0)),
+// and u = u0 (1 - t^2) turns the integral into int_{t_obs}^1 2 sqrt(u0) / sqrt(R) dt, smooth in t.
+__device__ __forceinline__ float j_near(float b, float u_obs) {
+    constexpr float xg[3] = {0.2386191860831969f, 0.6612093864662645f, 0.9324695142031521f};
+    constexpr float wg[3] = {0.4679139345726910f, 0.3607615730481386f, 0.1713244923791704f};
+    const float ib2 = 1.0f / (b * b);
+    float ct = 1.0f - 54.0f * ib2;
+    ct = ct < -1.0f ? -1.0f : ct;
+    const float th = acosf(ct);
+    const float u0 = (1.0f / 6.0f) + cosf(th * (1.0f / 3.0f) - 2.0943951023931953f) * (1.0f / 3.0f);
+    const float q = 1.0f - u_obs / u0;
+    const float t0 = sqrtf(q > 0.0f ? q : 0.0f);
+    const float half = 0.5f * (1.0f - t0), mid = 0.5f * (1.0f + t0);
+    const float cu = 0.5f - u0, k0 = 0.5f * ib2 / u0;
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int sg = -1; sg <= 1; sg += 2) {
+            const float t = __builtin_fmaf((float)sg * xg[i], half, mid);
+            const float u = u0 * __builtin_fmaf(-t, t, 1.0f);
+            const float R = 2.0f * __builtin_fmaf(u, cu - u, k0);
+            s = __builtin_fmaf(wg[i], __builtin_amdgcn_rsqf(R), s);
+        }
+    }
+    return s * half * 2.0f * sqrtf(u0);
+}
+
+// Geometry of one camera ray for the surrogate.
+struct RayGeom {
+    V3<float> e1, e2;     // orbit-plane basis
+    float r;              // r_obs
+    float bm;             // beta - 1 (from the FP64 setup within 5% of b_crit)
+    float b;              // impact parameter
+    float psi;            // swept angle of the first equatorial crossing, [0, pi)
+    float alpha_v;        // angle of the initial coordinate velocity from e1
+    float sin_a;          // sin of the view angle from the radial direction
+    bool ingoing;
+    bool planar;          // degenerate orbit plane (radial ray) or camera on the equator
+};
+
+__device__ __forceinline__ RayGeom ray_geom(const V3<float>& P, const V3<float>& d,
+                                            const Setup<float>& s) {
+    RayGeom g;
+    g.r = sqrtf(dot3(P, P));
+    const float ir = 1.0f / g.r;
+    g.e1 = V3<float>{P.x * ir, P.y * ir, P.z * ir};
+    const V3<float>& v = s.v;
+    const float vr = dot3(v, g.e1);
+    V3<float> t{v.x - vr * g.e1.x, v.y - vr * g.e1.y, v.z - vr * g.e1.z};
+    const float tn = sqrtf(dot3(t, t));
+    g.planar = !(tn > 1e-6f);
+    const float it = g.planar ? 0.0f : 1.0f / tn;
+    g.e2 = V3<float>{t.x * it, t.y * it, t.z * it};
+    g.alpha_v = atan2f(tn, vr);
+    g.ingoing = vr < 0.0f;
+    g.b = s.b;
+    g.bm = s.b * (1.0f / kBcritF) - 1.0f;
+    if (fabsf(g.bm) < 0.05f) {   // near b_crit: beta - 1 from the double setup of the same ray
+        const V3<double> Pd{(double)P.x, (double)P.y, (double)P.z};
+        const V3<double> dd{(double)d.x, (double)d.y, (double)d.z};
+        const double b64 = lawS_setup(Pd, dd).b;
+        g.bm = (float)(b64 * (1.0 / 5.196152422706632) - 1.0);
+        g.b = (float)b64;
+    }
+    const float f = sqrtf(fmaxf(1.0f - 2.0f * ir, 1e-12f));
+    g.sin_a = s.b * f * ir;
+    float psi = atan2f(-g.e1.y, g.e2.y);
+    if (psi < 0.0f) psi += kPi;
+    if (psi >= kPi) psi -= kPi;
+    g.psi = psi;
+    g.planar = g.planar || fabsf(g.e1.y) < 1e-6f;
+    return g;
+}
+
+// Exact Schwarzschild swept angle from the camera to infinity (escaping side, bm > 0).
+__device__ __forceinline__ float phi_exact(const RayGeom& g) {
+    return kPi + ahat(g.bm) - j_near(g.b, 1.0f / g.r);
+}
+
+// Crossing target encoding and its inverse.
+__device__ __forceinline__ float u_encode(float u) { return logf(8.0f * (u > kUMin ? u : kUMin)); }
+__device__ __forceinline__ float u_decode(float y) { return 0.125f * __expf(y); }
+
+__device__ __forceinline__ void features(const RayGeom& g, float x[3]) {
+    x[0] = (logf(g.r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
+    const float m = log1pf(fabsf(g.bm) * (1.0f / kBetaEps)) * (1.0f / 9.2104404f);   // log(1+1e4)
+    x[1] = g.bm < 0.0f ? -m : m;
+    x[2] = g.psi * (2.0f / kPi) - 1.0f;
+}
+
+// Surrogate domain of a ray (independent of the trained model).
+__device__ __forceinline__ bool in_domain(const RayGeom& g) {
+    return g.ingoing && !g.planar && g.sin_a <= kSinMax && g.r >= 8.0f && g.r <= 1000.0f &&
+           fabsf(g.bm) > kBetaEps && g.bm >= kBetaMin - 1.0f;
+}
+
+// SiLU and its derivative (approximate exp and reciprocal: the bf16 rounding of the hidden
+// activations dominates their error).
+__device__ __forceinline__ float sigmoid_fast(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
+__device__ __forceinline__ float dsilu(float z) {
+    const float sg = sigmoid_fast(z);
+    return sg * __builtin_fmaf(z, 1.0f - sg, 1.0f);
+}
+
+// Round-to-nearest-even FP32 -> bf16 bits (finite inputs).
+__device__ __forceinline__ uint16_t bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_round(float f) { return __uint_as_float((uint32_t)bf16_bits(f) << 16); }
+
+// Layer-1 pre-activation in the evaluation order shared by training and inference.
+__device__ __forceinline__ float l1_pre(const float* w, float b, const float x[3]) {
+    return __builtin_fmaf(w[2], x[2], __builtin_fmaf(w[1], x[1], w[0] * x[0])) + b;
+}
+
+}  // namespace mlp
+}  // namespace bhs

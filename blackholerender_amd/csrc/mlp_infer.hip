@@ -1,0 +1,571 @@
+// mlp_infer.hip — surrogate inference: weight packing into MFMA fragment order, the wave-level bf16
+// MFMA forward pass, output decoding, the fused surrogate render kernel with its fallback lists,
+// and validation against integrator outputs.
+#include "mlp.hpp"
+
+#include "../../include/bhs_trace.h"
+
+namespace bhs {
+namespace mlp {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4;
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u4;
+
+// Packed layout (bytes): A fragments of layers 2 and 3 ([row tile 8][K chunk 4][lane 64][8 bf16]),
+// of layer 4 ([K chunk 4][lane 64][8 bf16], rows 5..15 zero), layer 1 as [128][w0 w1 w2 b1] FP32,
+// biases of layers 2, 3 (FP32 [128]) and 4 (FP32 [16], 5..15 zero).
+constexpr int kOffA2 = 0, kOffA3 = 32768, kOffA4 = 65536, kOffW1 = 69632, kOffB2 = 71680,
+              kOffB3 = 72192, kOffB4 = 72704;
+static_assert(kOffB4 + 64 == (int)kPackedBytes, "packed layout");
+static_assert(kPackedBytes % 16 == 0, "packed size");
+
+// Neuron of K chunk kc, lane group g, element j. v_mfma_f32_16x16x32_bf16 reads A row l % 16 and B
+// column l % 16 with K elements 8 (l / 16) + j from lane l, and leaves D rows 4 (l / 16) + jj of
+// column l % 16 in lane l. Ordering the K elements of chunk kc as neurons
+// 16 (2 kc + j / 4) + 4 g + j % 4 makes the D tiles of row tiles 2 kc and 2 kc + 1 exactly the B
+// fragment of chunk kc of the next layer: activations never move between lanes.
+__host__ __device__ constexpr int perm(int kc, int g, int j) {
+    return 16 * (2 * kc + j / 4) + 4 * g + (j & 3);
+}
+
+__global__ void k_pack(const float* __restrict__ P, uint8_t* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint16_t* h = (uint16_t*)out;
+    if (i < 2 * 16384) {
+        const int layer = i / 16384, e = i % 16384;
+        const int mt = e / 2048, kc = (e / 512) & 3, lane = (e / 8) & 63, j = e & 7;
+        const int m = 16 * mt + (lane & 15), k = perm(kc, lane >> 4, j);
+        h[(layer ? kOffA3 : kOffA2) / 2 + e] = bf16_bits(P[(layer ? kW3 : kW2) + m * kH + k]);
+    } else if (i < 2 * 16384 + 2048) {
+        const int e = i - 2 * 16384;
+        const int kc = e / 512, lane = (e / 8) & 63, j = e & 7;
+        const int m = lane & 15, k = perm(kc, lane >> 4, j);
+        h[kOffA4 / 2 + e] = m < kOut ? bf16_bits(P[kW4 + m * kH + k]) : (uint16_t)0;
+    } else if (i < 2 * 16384 + 2048 + kH) {
+        const int q = i - (2 * 16384 + 2048);
+        float* w1 = (float*)(out + kOffW1);
+        w1[4 * q + 0] = P[kW1 + 3 * q + 0];
+        w1[4 * q + 1] = P[kW1 + 3 * q + 1];
+        w1[4 * q + 2] = P[kW1 + 3 * q + 2];
+        w1[4 * q + 3] = P[kB1 + q];
+        ((float*)(out + kOffB2))[q] = P[kB2 + q];
+        ((float*)(out + kOffB3))[q] = P[kB3 + q];
+        if (q < 16) ((float*)(out + kOffB4))[q] = q < kOut ? P[kB4 + q] : 0.0f;
+    }
+}
+
+__device__ __forceinline__ void stage_weights(const uint8_t* __restrict__ g, uint8_t* s) {
+    const u4* src = (const u4*)g;
+    u4* dst = (u4*)s;
+    for (int i = threadIdx.x; i < (int)(kPackedBytes / 16); i += blockDim.x) dst[i] = src[i];
+}
+
+typedef __attribute__((ext_vector_type(2))) float f2;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+
+// Two floats rounded to bf16 (nearest even) in one 32-bit word, lo in bits 0-15.
+__device__ __forceinline__ uint32_t pack_bf(float lo, float hi) {
+    const f2 v = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
+__device__ __forceinline__ bf16x8 frag(const u4& v) { return __builtin_bit_cast(bf16x8, v); }
+
+// One hidden MFMA layer for NC column tiles of 16 rays: out = bf16(silu(W in + b)).
+template <int NC>
+__device__ __forceinline__ void hidden_layer(const uint8_t* __restrict__ w, int off_a, int off_b,
+                                             const u4 (&in)[NC][4], u4 (&out)[NC][4]) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const float* bias = (const float*)(w + off_b);
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt) {
+        // keep the scheduler from hoisting the A fragments of later row tiles (register pressure)
+        __builtin_amdgcn_sched_barrier(0);
+        bf16x8 A[4];
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc)
+            A[kc] = *(const bf16x8*)(w + off_a + ((mt * 4 + kc) * 64 + lane) * 16);
+        const f4 bq = *(const f4*)(bias + 16 * mt + 4 * g);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[kc], frag(in[c][kc]), acc, 0, 0, 0);
+            // rows 4g + jj of tile mt are elements 4 (mt % 2) + jj of chunk mt / 2
+            out[c][mt >> 1][2 * (mt & 1)] = pack_bf(silu(acc[0] + bq[0]), silu(acc[1] + bq[1]));
+            out[c][mt >> 1][2 * (mt & 1) + 1] = pack_bf(silu(acc[2] + bq[2]), silu(acc[3] + bq[3]));
+        }
+    }
+}
+
+// Network outputs of the 16 NC rays ray0 .. ray0 + 16 NC - 1 of the wave (lane l holds the features
+// of ray l in x), written to ysc[(ray - ray0) * 5 + o].
+template <int NC>
+__device__ __forceinline__ void mlp_tiles(const uint8_t* __restrict__ w, const float x[3], int ray0,
+                                          float* __restrict__ ysc) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
+    const f4* W1 = (const f4*)(w + kOffW1);
+    float xs[NC][3];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) xs[c][i] = __shfl(x[i], ray0 + 16 * c + n, 64);
+    u4 B[NC][4], H[NC][4];
+#pragma unroll
+    for (int kc = 0; kc < 4; ++kc)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            __builtin_amdgcn_sched_barrier(0);
+            const f4 wa = W1[perm(kc, g, 2 * p)], wb = W1[perm(kc, g, 2 * p + 1)];
+            const float va[3] = {wa[0], wa[1], wa[2]}, vb[3] = {wb[0], wb[1], wb[2]};
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                B[c][kc][p] = pack_bf(silu(l1_pre(va, wa[3], xs[c])), silu(l1_pre(vb, wb[3], xs[c])));
+        }
+    hidden_layer<NC>(w, kOffA2, kOffB2, B, H);
+    hidden_layer<NC>(w, kOffA3, kOffB3, H, B);
+    const float* b4 = (const float*)(w + kOffB4);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                *(const bf16x8*)(w + kOffA4 + (kc * 64 + lane) * 16), frag(B[c][kc]), acc, 0, 0, 0);
+        float* yr = ysc + (16 * c + n) * kOut;
+        if (g == 0) {
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) yr[jj] = acc[jj] + b4[jj];
+        } else if (g == 1) {
+            yr[4] = acc[0] + b4[4];
+        }
+    }
+}
+
+// Network forward pass of the wave's 64 rays in 64 / (16 NC) passes: lane l holds the features of
+// ray l in x and receives its 5 outputs in y. ysc: 80 NC floats of LDS owned by this wave. All 64
+// lanes must call it.
+constexpr int kNC = 2;
+__device__ __forceinline__ void mlp_wave(const uint8_t* __restrict__ w, const float x[3],
+                                         float* __restrict__ ysc, float y[5]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int h = 0; h < 4 / kNC; ++h) {
+        // The weights stay in LDS: without this the loads are hoisted out of the callers' loops
+        // and the whole network is held in (spilled) registers.
+        asm volatile("" ::: "memory");
+        mlp_tiles<kNC>(w, x, 16 * kNC * h, ysc);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int rl = lane - 16 * kNC * h;
+        const bool mine = rl >= 0 && rl < 16 * kNC;
+        const int ri = mine ? rl : 0;
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) {
+            const float v = ysc[ri * kOut + o];
+            if (mine) y[o] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Decoded surrogate result of one ray (the law-S output record; steps unknown, reported as 0).
+struct Dec {
+    bool serve;
+    bool escaping;
+    uint32_t cls, flags, n_cross;
+    float esc[3];
+    float dr[2], dphi[2], dg[2];
+    uint32_t n_eq;      // equatorial crossings before escape/capture (3 means three or more)
+    float r_eq[2];      // radii of the first two equatorial crossings
+};
+
+__device__ __forceinline__ bool cell_served(const SurConst& k, const float x[3]) {
+    int ir = (int)((x[0] + 1.0f) * (0.5f * kMapR));
+    int ib = (int)((x[1] + 1.0f) * (0.5f * kMapB));
+    ir = ir < 0 ? 0 : (ir >= kMapR ? kMapR - 1 : ir);
+    ib = ib < 0 ? 0 : (ib >= kMapB ? kMapB - 1 : ib);
+    const int cell = ir * kMapB + ib;
+    return (k.serve[cell >> 5] >> (cell & 31)) & 1u;
+}
+
+__device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k, const RayGeom& g) {
+    int bin = (int)((logf(g.r) - kLogRLo) * ((float)kStepBins / (kLogRHi - kLogRLo)));
+    bin = bin < 0 ? 0 : (bin >= kStepBins ? kStepBins - 1 : bin);
+    return (float)k.steps_max[bin] * k.steps_slack < (float)c.max_steps;
+}
+
+// Output record of one in-domain ray from the network outputs y, or serve = false when the ray must
+// go back to the integrator: reach undecided, a crossing radius within the margin of an annulus
+// edge, a possible third crossing that could be recorded, or captured within 1% of b_crit.
+__device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, const Setup<float>& s,
+                                      const RayGeom& g, const float y[5]) {
+    Dec o;
+    o.serve = false;
+    o.escaping = g.bm > 0.0f;
+    o.n_cross = 0;
+    float Phi = 0.0f;
+    if (o.escaping) {
+        Phi = phi_exact(g) + kResScale * y[0];
+        const float a = Phi - g.psi;
+        o.n_eq = a <= 0.0f ? 0u : (a <= kPi ? 1u : (a <= 2.0f * kPi ? 2u : 3u));
+    } else {
+        if (g.bm > -0.01f) return o;
+        if (fabsf(y[3]) < k.logit_margin || fabsf(y[4]) < k.logit_margin) return o;
+        const bool r1 = y[3] > 0.0f, r2 = y[4] > 0.0f;
+        if (r2 && !r1) return o;
+        o.n_eq = (r1 ? 1u : 0u) + (r2 ? 1u : 0u);
+    }
+    bool in[2] = {false, false};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        o.r_eq[q] = 1.0f / u_decode(y[1 + q]);
+        if ((uint32_t)q < o.n_eq && c.thin) {
+            const float r = o.r_eq[q];
+            if (fabsf(r - c.disk_inner) < k.radius_margin * c.disk_inner ||
+                fabsf(r - c.disk_outer) < k.radius_margin * c.disk_outer)
+                return o;
+            in[q] = r >= c.disk_inner && r <= c.disk_outer;
+        }
+    }
+    if (o.n_eq >= 3u && c.thin && !(in[0] && in[1])) return o;
+    float sp, cp;
+    sincosf(g.psi, &sp, &cp);
+    const V3<float> w{cp * g.e1.x + sp * g.e2.x, cp * g.e1.y + sp * g.e2.y,
+                      cp * g.e1.z + sp * g.e2.z};
+    o.dr[0] = o.dr[1] = o.dphi[0] = o.dphi[1] = o.dg[0] = o.dg[1] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        if (!in[q]) continue;
+        const float r = o.r_eq[q], sr = q ? -r : r;
+        const float ph = atan2_ieee(sr * w.z, sr * w.x);
+        const float gg = redshift(r, s.b_axis, s.lapse_g);
+        if (o.n_cross == 0) {
+            o.dr[0] = r;
+            o.dphi[0] = ph;
+            o.dg[0] = gg;
+        } else {
+            o.dr[1] = r;
+            o.dphi[1] = ph;
+            o.dg[1] = gg;
+        }
+        ++o.n_cross;
+    }
+    if (o.escaping) {
+        float sf, cf;
+        sincosf(Phi, &sf, &cf);
+        o.esc[0] = cf * g.e1.x + sf * g.e2.x;
+        o.esc[1] = cf * g.e1.y + sf * g.e2.y;
+        o.esc[2] = cf * g.e1.z + sf * g.e2.z;
+    } else {
+        o.esc[0] = o.esc[1] = o.esc[2] = 0.0f;
+    }
+    o.cls = o.n_cross >= 1 ? BHS_CLASS_DISK : (o.escaping ? BHS_CLASS_ESCAPE : BHS_CLASS_HORIZON);
+    o.flags = BHS_FLAG_SURROGATE | (o.escaping ? BHS_FLAG_ESCAPED : 0u);
+    o.serve = true;
+    return o;
+}
+
+__device__ __forceinline__ void store_dec(const SoA& out, int64_t i, const Dec& o, float b) {
+    const int64_t n = out.n;
+    if (out.cls) out.cls[i] = (uint8_t)o.cls;
+    if (out.flags) out.flags[i] = (uint8_t)o.flags;
+    if (out.n_cross) out.n_cross[i] = (uint8_t)o.n_cross;
+    if (out.steps) out.steps[i] = 0;
+    if (out.esc) {
+        out.esc[i] = o.esc[0];
+        out.esc[i + n] = o.esc[1];
+        out.esc[i + 2 * n] = o.esc[2];
+    }
+    if (out.disk) {
+        out.disk[i] = o.dr[0];
+        out.disk[i + n] = o.dr[1];
+        out.disk[i + 2 * n] = o.dphi[0];
+        out.disk[i + 3 * n] = o.dphi[1];
+        out.disk[i + 4 * n] = o.dg[0];
+        out.disk[i + 5 * n] = o.dg[1];
+    }
+    if (out.b) out.b[i] = b;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Appends the items of the lanes with `want` to a list with one atomic per wave. All lanes call it.
+__device__ __forceinline__ void push_list(uint32_t* count, uint32_t* ids, uint32_t cap, bool want,
+                                          uint32_t item) {
+    const uint64_t m = __ballot(want);
+    if (m == 0) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    if (want) {
+        const uint32_t rank =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (base + rank < cap) ids[base + rank] = item;
+    }
+}
+
+// Work item q of a camera (8x8 tiles, as the integrator kernels).
+__device__ __forceinline__ int64_t item_pixel(const MarchConst& c, uint32_t q) {
+    const uint32_t tiles_x = ((uint32_t)c.width + 7u) >> 3;
+    const uint32_t t = q >> 6, l = q & 63u;
+    const uint32_t col = (t % tiles_x) * 8u + (l & 7u), row = (t / tiles_x) * 8u + (l >> 3);
+    return (col < (uint32_t)c.width && row < (uint32_t)c.height) ? (int64_t)row * c.width + col : -1;
+}
+
+constexpr int kBlock = 256;
+
+// Surrogate render of one camera. Persistent blocks (two per CU: the weights take 71 KB of LDS)
+// walk 256-item chunks; per chunk each lane prepares one ray, the wave runs the network if any of
+// its lanes is in the served domain, and each lane writes its record or lists its ray for the
+// integrator: FP64 band list (the integrator's own band rule) or FP32 list.
+__global__ __launch_bounds__(kBlock, 2) void k_surrogate(const MarchConst c, const SurConst k,
+                                                         const uint8_t* __restrict__ packed,
+                                                         const SoA out, Scratch sc, SurLists fl,
+                                                         uint32_t n_items) {
+    __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
+    __shared__ float s_y[4][64 * kOut];
+    stage_weights(packed, s_w);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
+    const uint32_t chunks = (n_items + kBlock - 1) / kBlock;
+    for (uint32_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        const uint32_t q = ch * kBlock + threadIdx.x;
+        const int64_t pix = q < n_items ? item_pixel(c, q) : -1;
+        const bool valid = pix >= 0;
+        Setup<float> s;
+        RayGeom g;
+        float x[3] = {0.0f, 0.0f, 0.0f};
+        bool elig = false;
+        if (valid) {
+            const V3<float> d = ray_dir(c, pix);
+            s = lawS_setup(P, d);
+            g = ray_geom(P, d, s);
+            if (in_domain(g)) {
+                features(g, x);
+                elig = cell_served(k, x) && steps_ok(c, k, g);
+            }
+        }
+        float y[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (__any(elig)) mlp_wave(s_w, x, s_y[wave], y);
+        bool served = false;
+        if (elig) {
+            const Dec o = decode(c, k, s, g, y);
+            if (o.serve) {
+                store_dec(out, c.ray_offset + pix, o, s.b);
+                served = true;
+                n_served += 1;
+                n_cross += o.n_cross;
+                n_esc += o.escaping ? 1u : 0u;
+            }
+        }
+        const bool back = valid && !served;
+        const bool band = back && (c.precision == BHS_PREC_F64 ||
+                                   (c.precision == BHS_PREC_F32_BAND64 &&
+                                    __builtin_fabsf(s.b - kBcritF) < c.band_abs));
+        push_list(sc.band_count, sc.band_ids, sc.band_cap, band, q);
+        push_list(fl.fb_count, fl.fb_ids, fl.fb_cap, back && !band, q);
+        n_fb += back ? 1u : 0u;
+    }
+    n_served = wave_sum(n_served);
+    n_cross = wave_sum(n_cross);
+    n_esc = wave_sum(n_esc);
+    n_fb = wave_sum(n_fb);
+    if ((threadIdx.x & 63) == 0) {
+        if (n_served) {
+            atomicAdd(&sc.stats->rays, (unsigned long long)n_served);
+            atomicAdd(&sc.stats->surrogate, (unsigned long long)n_served);
+        }
+        if (n_cross) atomicAdd(&sc.stats->crossings, (unsigned long long)n_cross);
+        if (n_esc) atomicAdd(&sc.stats->escapes, (unsigned long long)n_esc);
+        if (n_fb) atomicAdd(&sc.stats->fallback, (unsigned long long)n_fb);
+    }
+}
+
+// Network outputs of explicit feature rows.
+__global__ __launch_bounds__(kBlock, 2) void k_forward(const float* __restrict__ X, int64_t n,
+                                                       const uint8_t* __restrict__ packed,
+                                                       float* __restrict__ Y) {
+    __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
+    __shared__ float s_y[4][64 * kOut];
+    stage_weights(packed, s_w);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    const int64_t chunks = (n + kBlock - 1) / kBlock;
+    for (int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        const int64_t i = ch * kBlock + threadIdx.x;
+        float x[3] = {0.0f, 0.0f, 0.0f};
+        if (i < n) {
+            x[0] = X[3 * i];
+            x[1] = X[3 * i + 1];
+            x[2] = X[3 * i + 2];
+        }
+        float y[5];
+        mlp_wave(s_w, x, s_y[wave], y);
+        if (i < n)
+            for (int o = 0; o < kOut; ++o) Y[kOut * i + o] = y[o];
+    }
+}
+
+// Validation of the decode against integrator outputs of the same rays (FP64 march, refined
+// crossings, crossings recorded at any radius). Per ray: err_phi = angle between the escape
+// directions (0 for served captured rays), err_r = largest relative radius error over the
+// equatorial crossings both report within [inner / 2, 1.5 outer] (-1 when none), 1e30 in both on a
+// disagreement of the rendered record (class, escape status or in-annulus crossing count), and the
+// error-map cell of the ray (-1 when the decode does not serve it).
+__global__ __launch_bounds__(kBlock, 2) void k_validate(int64_t n, const float* __restrict__ pos3,
+                                                        const float* __restrict__ dir3,
+                                                        const MarchConst c, const SurConst k,
+                                                        const SoA ref,
+                                                        const uint8_t* __restrict__ packed,
+                                                        float* __restrict__ err_phi,
+                                                        float* __restrict__ err_r,
+                                                        int32_t* __restrict__ cell) {
+    __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
+    __shared__ float s_y[4][64 * kOut];
+    stage_weights(packed, s_w);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    const int64_t chunks = (n + kBlock - 1) / kBlock;
+    for (int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        const int64_t i = ch * kBlock + threadIdx.x;
+        Setup<float> s;
+        RayGeom g;
+        float x[3] = {0.0f, 0.0f, 0.0f};
+        bool elig = false;
+        if (i < n) {
+            const V3<float> P{pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2]};
+            const V3<float> d{dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2]};
+            s = lawS_setup(P, d);
+            g = ray_geom(P, d, s);
+            if (in_domain(g)) {
+                features(g, x);
+                elig = true;
+            }
+        }
+        float y[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (__any(elig)) mlp_wave(s_w, x, s_y[wave], y);
+        if (i >= n) continue;
+        float ep = -1.0f, er = -1.0f;
+        int32_t ci = -1;
+        if (elig) {
+            const Dec o = decode(c, k, s, g, y);
+            if (o.serve) {
+                const uint32_t fl = ref.flags[i], nc = ref.n_cross[i];
+                const bool ref_esc = (fl & BHS_FLAG_ESCAPED) != 0;
+                uint32_t ref_in = 0;   // in-annulus crossings among the reference's first two
+                const bool ref_more = nc >= 2;   // a third equatorial crossing is not recorded
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    if ((uint32_t)q >= nc) continue;
+                    const float r = ref.disk[i + q * ref.n];
+                    if (r >= c.disk_inner && r <= c.disk_outer) ++ref_in;
+                }
+                bool bad = (fl & BHS_FLAG_MAXSTEPS) != 0 || ref_esc != o.escaping;
+                if (c.thin) {
+                    bad = bad || (ref_in != o.n_cross && !(ref_more && ref_in < o.n_cross));
+                    const uint32_t m = nc < o.n_eq ? nc : o.n_eq;
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        if ((uint32_t)q >= m) continue;
+                        const float r = ref.disk[i + q * ref.n];
+                        if (r < 0.5f * c.disk_inner || r > 1.5f * c.disk_outer) continue;
+                        const float e = fabsf(o.r_eq[q] - r) / r;
+                        er = e > er ? e : er;
+                    }
+                }
+                ep = 0.0f;
+                if (!bad && o.escaping) {
+                    const float ex = ref.esc[i], ey = ref.esc[i + ref.n], ez = ref.esc[i + 2 * ref.n];
+                    const float cx = o.esc[1] * ez - o.esc[2] * ey;
+                    const float cy = o.esc[2] * ex - o.esc[0] * ez;
+                    const float cz = o.esc[0] * ey - o.esc[1] * ex;
+                    const float dt = o.esc[0] * ex + o.esc[1] * ey + o.esc[2] * ez;
+                    ep = atan2f(sqrtf(cx * cx + cy * cy + cz * cz), dt);
+                }
+                if (bad) ep = er = 1e30f;
+                int ir = (int)((x[0] + 1.0f) * (0.5f * kMapR));
+                int ib = (int)((x[1] + 1.0f) * (0.5f * kMapB));
+                ir = ir < 0 ? 0 : (ir >= kMapR ? kMapR - 1 : ir);
+                ib = ib < 0 ? 0 : (ib >= kMapB ? kMapB - 1 : ib);
+                ci = ir * kMapB + ib;
+            }
+        }
+        err_phi[i] = ep;
+        err_r[i] = er;
+        cell[i] = ci;
+    }
+}
+
+int persistent_blocks(int64_t chunks, int num_cus) {
+    const int64_t cap = 2 * (int64_t)(num_cus > 0 ? num_cus : 256);
+    return (int)(chunks < cap ? (chunks > 0 ? chunks : 1) : cap);
+}
+
+}  // namespace
+
+hipError_t launch_pack(const float* P, void* packed, hipStream_t s) {
+    const int n = 2 * 16384 + 2048 + kH;
+    hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, s, P, (uint8_t*)packed);
+    return hipGetLastError();
+}
+
+hipError_t launch_forward(const float* X, int64_t n, const void* packed, float* Y, hipStream_t s) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int blocks = persistent_blocks((n + kBlock - 1) / kBlock, cus);
+    hipLaunchKernelGGL(k_forward, dim3(blocks), dim3(kBlock), 0, s, X, n, (const uint8_t*)packed, Y);
+    return hipGetLastError();
+}
+
+hipError_t launch_validate(int64_t n, const float* pos3, const float* dir3, const MarchConst& c,
+                           const SurConst& k, const SoA& ref, const void* packed, float* err_phi,
+                           float* err_r, int32_t* cell, int num_cus, hipStream_t s) {
+    const int blocks = persistent_blocks((n + kBlock - 1) / kBlock, num_cus);
+    hipLaunchKernelGGL(k_validate, dim3(blocks), dim3(kBlock), 0, s, n, pos3, dir3, c, k, ref,
+                       (const uint8_t*)packed, err_phi, err_r, cell);
+    return hipGetLastError();
+}
+
+hipError_t launch_surrogate(const MarchConst& c, const SurConst& k, const void* packed,
+                            const SoA& out, const Scratch& sc, const SurLists& fl,
+                            const Tuning& t, int num_cus, hipStream_t s, hipStream_t aux,
+                            hipEvent_t ev_fork, hipEvent_t ev_join, hipEvent_t ev_begin,
+                            hipEvent_t ev_mid, hipEvent_t ev_end) {
+    const uint32_t n_items = camera_items(c.width, c.height);
+    hipError_t e;
+    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
+    const int blocks = persistent_blocks((n_items + kBlock - 1) / kBlock, num_cus);
+    hipLaunchKernelGGL(k_surrogate, dim3(blocks), dim3(kBlock), 0, s, c, k,
+                       (const uint8_t*)packed, out, sc, fl, n_items);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
+    // Fallback: FP64 band list on aux, FP32 list on s (persistent, lane refill).
+    if ((e = launch_band_list(c, out, sc, s, aux, ev_fork, ev_join)) != hipSuccess) return e;
+    Tuning tl = t;
+    tl.mode = 1;
+    if ((e = launch_list(c, fl.fb_ids, fl.fb_count, fl.fb_cap, out, sc, tl, num_cus, s)) !=
+        hipSuccess)
+        return e;
+    if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(s, ev_join, 0)) != hipSuccess) return e;
+    return launch_root_refine(c, out, sc, s);
+}
+
+}  // namespace mlp
+}  // namespace bhs

@@ -63,8 +63,9 @@ enum {
     BHS_FLAG_ESCAPED = 2,     /* terminal status was escape (also set for disk rays that escaped) */
     BHS_FLAG_BAND64 = 4,      /* |b - b_crit| < band: the ray was marched in FP64 */
     BHS_FLAG_ROOT64 = 8,      /* disk crossings refined by FP64 root bracketing of the RK4 step */
-    BHS_FLAG_SURROGATE = 16,  /* outputs come from the MLP surrogate */
-    BHS_FLAG_FALLBACK = 32    /* surrogate path: ray sent back to the integrator */
+    BHS_FLAG_SURROGATE = 16   /* outputs come from the MLP surrogate (steps then reads 0); rays of
+                                 a surrogate render without it were marched by the integrator and
+                                 carry exactly the plain render's record */
 };
 
 enum { BHS_PREC_F32 = 0, BHS_PREC_F32_BAND64 = 1, BHS_PREC_F64 = 2 };
@@ -121,7 +122,7 @@ typedef struct bhs_ray_soa {
     uint16_t* steps_dev;      /* n   : march iterations */
     float* esc_dev;           /* 3n  : escape direction x,y,z planes (0 unless escaped) */
     float* disk_dev;          /* 6n  : r0, r1, phi0, phi1, g0, g1 of the first two crossings */
-    float* b_dev;             /* n   : impact parameter h = |x0 x v0| */
+    float* b_dev;             /* n   : impact parameter b = r sin(alpha) / sqrt(1 - 2/r) */
 } bhs_ray_soa;
 
 /* Aggregate counters of one launch (device-side, read after sync). */
@@ -132,7 +133,8 @@ typedef struct bhs_stats {
     uint64_t escapes;         /* rays that escaped (far-field correction applied) */
     uint64_t band64;          /* rays marched in FP64 */
     uint64_t fallback;        /* surrogate rays sent back to the integrator */
-    uint64_t pad_[2];
+    uint64_t surrogate;       /* rays served by the surrogate network */
+    uint64_t pad_;
 } bhs_stats;
 
 int bhs_version(void);
@@ -213,6 +215,84 @@ int bhs_autotune(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* pa
  * the launches recorded since the previous bhs_timing call, then starts a new window. */
 int bhs_set_timing(bhs_ctx* ctx, int on);
 int bhs_timing(bhs_ctx* ctx, double* march_ms_avg, double* total_ms_avg, int32_t* launches);
+
+/* ------------------------------------------------------------------------------------------------
+ * Learned surrogate of the law-S march (no counterpart in the reference).
+ *
+ * A 3 -> 128 -> 128 -> 128 -> 5 SiLU network maps (log r_obs, b/b_crit, psi) of a ray to the residual
+ * of its escape angle over the exact Schwarzschild swept angle, the radii of its first two
+ * equatorial crossings and the logits of reaching them. A model is bound to the march parameters
+ * it was trained for (step control and disk; max_steps, precision, root64 and band_rel may differ
+ * at render time). Training data come from the FP64 march itself; training runs on the GPU
+ * (bf16 MFMA forward/backward, FP32 accumulation, Adam in FP32); bhs_mlp_validate compares the
+ * decoded outputs with FP64 marches of fresh rays and builds the error map that decides, per
+ * (r_obs, b) cell, where the network may serve rays. bhs_render_surrogate serves the rays of
+ * served cells whose outcome is decided (reach, annulus edges, step budget) and sends all others
+ * back to the integrator (FP64 band rule of `params`, FP32 elsewhere) in the same call.
+ * ------------------------------------------------------------------------------------------------ */
+typedef struct bhs_mlp bhs_mlp;
+
+#define BHS_MLP_PARAMS 34181
+
+typedef struct bhs_mlp_info {
+    bhs_march_params trained_for;
+    int64_t dataset_rays;
+    int64_t adam_steps;
+    float tol_phi;            /* escape-direction tolerance of the error map (rad) */
+    float radius_margin;      /* relative margin around the annulus edges */
+    float logit_margin;       /* |logit| below this: reach undecided */
+    float steps_slack;        /* served only if the training step maximum * slack < max_steps */
+    uint32_t steps_max[16];   /* training step maximum per log r_obs bin over [8, 1000] */
+    int32_t served_cells;     /* error-map cells (of 32 x 64) that serve rays */
+    int32_t pad_;
+} bhs_mlp_info;
+
+typedef struct bhs_mlp_validation {
+    int64_t rays;             /* validation rays */
+    int64_t in_domain;        /* rays in the surrogate's domain */
+    int64_t served;           /* rays the render would serve (error map applied) */
+    int64_t escaping;         /* served rays that escape */
+    int64_t mismatches;       /* served rays whose rendered record disagrees with FP64 */
+    float phi_err_max, phi_err_p99, phi_err_p50;   /* escape-direction error (rad), served */
+    float r_err_max, r_err_p99, r_err_p50;         /* relative crossing-radius error, served */
+    float radius_margin;      /* margin chosen from the radius errors */
+    int32_t served_cells;
+} bhs_mlp_validation;
+
+/* New model (He-initialised from `seed`) for the march parameters `trained_for`. */
+bhs_mlp* bhs_mlp_create(bhs_ctx* ctx, const bhs_march_params* trained_for, uint32_t seed,
+                        int* status);
+void bhs_mlp_destroy(bhs_mlp* m);
+/* Training set: n_rays rays sampled from `seed` (r_obs log-uniform in [8, 1000], b log-spaced on
+ * both sides of b_crit), marched in FP64 with refined crossings recorded at any radius. */
+int bhs_mlp_make_dataset(bhs_ctx* ctx, bhs_mlp* m, int64_t n_rays, uint32_t seed);
+/* `steps` Adam steps on minibatches of `batch` dataset rays, learning rate cosine-annealed from
+ * lr_start to lr_end. loss_host (steps floats) receives the minibatch losses if non-NULL.
+ * Synchronous. */
+int bhs_mlp_train(bhs_ctx* ctx, bhs_mlp* m, int32_t steps, int32_t batch, float lr_start,
+                  float lr_end, float* loss_host);
+/* Loss and gradient (BHS_MLP_PARAMS floats) on n explicit rows: x_host n x 3 features, y_host
+ * n x 5 targets, mask_host n x 5 masks. Parameters unchanged. Synchronous. */
+int bhs_mlp_grads(bhs_ctx* ctx, bhs_mlp* m, const float* x_host, const float* y_host,
+                  const float* mask_host, int32_t n, float* grad_host, float* loss_host);
+/* Raw network outputs (n x 5) of n feature rows (n x 3), device arrays, MFMA inference. */
+int bhs_mlp_forward(bhs_ctx* ctx, bhs_mlp* m, const float* x_dev, int64_t n, float* y_dev,
+                    void* stream);
+/* Validation on n_rays fresh rays from `seed`; sets the radius margin and the error map (cells
+ * whose served rays all stay within tol_phi and render the FP64 record). Synchronous. */
+int bhs_mlp_validate(bhs_ctx* ctx, bhs_mlp* m, int64_t n_rays, uint32_t seed, float tol_phi,
+                     bhs_mlp_validation* out_host);
+int bhs_mlp_get_params(bhs_ctx* ctx, bhs_mlp* m, float* host, int64_t n);
+int bhs_mlp_set_params(bhs_ctx* ctx, bhs_mlp* m, const float* host, int64_t n);
+int bhs_mlp_get_info(bhs_ctx* ctx, bhs_mlp* m, bhs_mlp_info* out_host);
+/* Model file: parameters, error map, margins, step table and the march parameters it serves. */
+int bhs_mlp_save(bhs_ctx* ctx, bhs_mlp* m, const char* path);
+bhs_mlp* bhs_mlp_load(bhs_ctx* ctx, const char* path, int* status);
+/* Surrogate render of one camera with integrator fallback, outputs as bhs_render (steps = 0 and
+ * BHS_FLAG_SURROGATE on served rays). Stream-ordered. With timing on, the main-kernel interval is
+ * the network kernel and the call interval includes the fallback march. */
+int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
+                         const bhs_march_params* params, const bhs_ray_soa* out, void* stream);
 
 #ifdef __cplusplus
 }

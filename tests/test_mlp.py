@@ -1,0 +1,154 @@
+"""The learned surrogate: network oracle pins (CPU), GPU inference and gradients against the
+oracle, training, and the surrogate render's fallback path against the plain integrator.
+
+Bars: the FP64 oracle gradient agrees with central finite differences to 1e-4 relative; GPU
+inference (bf16 MFMA, FP32 accumulation) agrees with the oracle's bf16-emulating forward pass to
+2e-3 absolute (bf16 rounding flips of single activations) and 2e-5 at the median; GPU gradients
+agree with the oracle's bf16-emulating gradient to 1e-2 relative (L2 norm per parameter block).
+Rays the surrogate does not serve are marched by the integrator's own kernels and must be
+bit-identical to the plain render.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp as OM
+
+BLOCKS = [("W1", OM.W1, OM.B1), ("b1", OM.B1, OM.W2), ("W2", OM.W2, OM.B2), ("b2", OM.B2, OM.W3),
+          ("W3", OM.W3, OM.B3), ("b3", OM.B3, OM.W4), ("W4", OM.W4, OM.B4), ("b4", OM.B4, OM.NPARAMS)]
+
+
+def _batch(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (n, 3))
+    t = rng.normal(0, 0.5, (n, 5))
+    t[:, 3:] = rng.integers(0, 2, (n, 2))
+    m = (rng.random((n, 5)) > 0.25).astype(np.float64)
+    return x, t, m
+
+
+# ------------------------------------------------------------------------------------------- CPU
+def test_oracle_gradient_matches_finite_differences():
+    p = OM.init_params(3)
+    x, t, m = _batch(16, 0)
+    _, g = OM.loss_grad(p, x, t, m, "exact")
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    for name, a, b in BLOCKS:
+        for i in rng.choice(np.arange(a, b), min(5, b - a), replace=False):
+            pp = p.copy()
+            pp[i] += h
+            lp, _ = OM.loss_grad(pp, x, t, m, "exact")
+            pp[i] -= 2 * h
+            lm, _ = OM.loss_grad(pp, x, t, m, "exact")
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[i]) <= 1e-4 * max(abs(g[i]), 1e-3), (name, i, fd, g[i])
+
+
+def test_oracle_closed_forms():
+    assert OM.silu(0.0) == 0.0 and OM.dsilu(0.0) == 0.5
+    z = np.array([-30.0, -2.0, 0.5, 40.0])
+    assert np.allclose(OM.softplus(z), np.log1p(np.exp(z)))
+    # bf16 rounding: exact values stay, ties go to even, 1 + 3 * 2^-9 rounds up
+    assert OM.bf16(np.array([1.0, -2.5, 0.0]))[0] == 1.0
+    assert OM.bf16(np.array([1.0 + 2.0 ** -8]))[0] == 1.0
+    assert OM.bf16(np.array([1.0 + 3 * 2.0 ** -8]))[0] == 1.0 + 2.0 ** -6
+    assert OM.bf16(np.array([1.0 + 3 * 2.0 ** -9]))[0] == 1.0 + 2.0 ** -7
+
+
+def test_oracle_bf16_mode_is_close_to_exact():
+    p = OM.init_params(5)
+    x, t, m = _batch(64, 2)
+    _, ge = OM.loss_grad(p, x, t, m, "exact")
+    _, gb = OM.loss_grad(p, x, t, m, "bf16")
+    rel = np.linalg.norm(gb - ge) / np.linalg.norm(ge)
+    assert 0 < rel < 0.05
+
+
+# ------------------------------------------------------------------------------------------- GPU
+@pytest.fixture(scope="module")
+def model(tracer):
+    from blackholerender_amd.surrogate import Surrogate
+    s = Surrogate(tracer, seed=7)
+    yield s
+    s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_forward_matches_oracle(model):
+    p = OM.init_params(11).astype(np.float32)
+    model.params = p
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, (1000, 3)).astype(np.float32)   # not a multiple of 64: ragged tail
+    print("[mlp] forward 1000 rows", flush=True)
+    y = model.forward(x)
+    ref, _ = OM.forward(p.astype(np.float64), x.astype(np.float64), "bf16")
+    d = np.abs(y - ref)
+    assert np.all(np.isfinite(y))
+    assert d.max() <= 2e-3 * max(1.0, np.abs(ref).max()), d.max()
+    assert np.median(d) <= 2e-5, np.median(d)
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_oracle(model):
+    p = OM.init_params(12).astype(np.float32)
+    model.params = p
+    x, t, m = _batch(777, 9)
+    print("[mlp] gradient 777 rows", flush=True)
+    g, loss = model.grads(x, t, m)
+    lo, go = OM.loss_grad(p.astype(np.float64), x.astype(np.float32).astype(np.float64),
+                          t.astype(np.float32), m.astype(np.float32), "bf16")
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    for name, a, b in BLOCKS:
+        rel = np.linalg.norm(g[a:b] - go[a:b]) / max(np.linalg.norm(go[a:b]), 1e-30)
+        assert rel <= 1e-2, (name, rel)
+    # parameters untouched by a gradient-only call
+    assert np.array_equal(model.params, p)
+
+
+@pytest.mark.gpu
+def test_gpu_training_reduces_loss(tracer):
+    from blackholerender_amd.surrogate import Surrogate
+    s = Surrogate(tracer, seed=3)
+    print("[mlp] dataset 65536 rays", flush=True)
+    s.make_dataset(1 << 16, seed=5)
+    info = s.info()
+    assert info["dataset_rays"] == 1 << 16
+    assert 0 < max(info["steps_max"]) < 4096
+    print("[mlp] train 400 steps", flush=True)
+    loss = s.train(400, batch=2048, lr_start=2e-3, lr_end=2e-4)
+    assert np.all(np.isfinite(loss))
+    assert loss[-50:].mean() < 0.3 * loss[:10].mean(), (loss[:10].mean(), loss[-50:].mean())
+    s.close()
+
+
+@pytest.mark.gpu
+def test_unvalidated_model_serves_nothing_and_falls_back_bit_exact(tracer, model):
+    """An empty error map sends every ray through the fallback lists: the result must equal the
+    plain render bit for bit (FP32 list march + FP64 band list + root refinement)."""
+    import torch
+    import blackholerender_amd as bhs
+    from helpers import bit_mismatches
+    cam = bhs.orbit_camera(30.0, 85.0, 60.0, 200, 120)
+    for root64 in (0, 1):
+        params = bhs.march_params(root64=root64)
+        print(f"[mlp] fallback-only render root64={root64}", flush=True)
+        a = tracer.render(cam, params)
+        torch.cuda.synchronize()
+        ref = a.numpy()
+        b = model.render(cam, params)
+        torch.cuda.synchronize()
+        st = tracer.last_stats()
+        got = b.numpy()
+        mm = bit_mismatches(got, ref)
+        assert all(v == 0 for v in mm.values()), mm
+        assert st["surrogate"] == 0 and st["fallback"] == 200 * 120
+
+
+@pytest.mark.gpu
+def test_render_rejects_other_march_parameters(tracer, model):
+    import blackholerender_amd as bhs
+    cam = bhs.orbit_camera(30.0, 85.0, 60.0, 64, 64)
+    with pytest.raises(bhs.BhsError):
+        model.render(cam, bhs.march_params(disk_outer=30.0))
+    with pytest.raises(bhs.BhsError):
+        model.render(cam, bhs.march_params(max_steps=700))   # other preset step control
