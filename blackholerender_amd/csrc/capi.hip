@@ -287,6 +287,9 @@ bhs_ctx* bhs_create(int device, int* status) {
         cus > 0)
         ctx->num_cus = cus;
     if (hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->aux2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork2, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join2, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_upload, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
@@ -318,6 +321,9 @@ void bhs_destroy(bhs_ctx* ctx) {
         if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
         if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
         if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
+        if (ctx->ev_fork2) (void)hipEventDestroy(ctx->ev_fork2);
+        if (ctx->ev_join2) (void)hipEventDestroy(ctx->ev_join2);
+        if (ctx->aux2) (void)hipStreamDestroy(ctx->aux2);
     }
     delete ctx;
 }

@@ -14,6 +14,9 @@ struct bhs_ctx {
     std::string err;
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // second side stream (surrogate render: early FP32 fallback march beside the FP64 band)
+    hipStream_t aux2 = nullptr;
+    hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
     hipStream_t last_stream = nullptr;
     // Render scratch: [Stats | band_count (pad 256) | band_ids | roots]
     void* scratch = nullptr;

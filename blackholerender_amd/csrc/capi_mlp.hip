@@ -34,9 +34,13 @@ struct bhs_mlp {
     size_t ws_bytes = 0;
     void* loss = nullptr;
     size_t loss_bytes = 0;
-    // FP32 fallback list of the surrogate render: [count (256 B) | ids]
+    // fallback lists of the surrogate render: [counters (256 B) | late ids | early ids]
     void* fb = nullptr;
     size_t fb_bytes = 0;
+    // table mode: [x2 range (256 B) | n2 n3 nodes x 8 floats]; tab_n2 = 0: per-ray inference
+    void* tab = nullptr;
+    size_t tab_bytes = 0;
+    int32_t tab_n2 = 0, tab_n3 = 0;
     bhs::mlp::SurConst k{};
     float tol_phi = 1e-4f;
     int64_t adam_t = 0;
@@ -54,7 +58,8 @@ void free_model(bhs_mlp* m) {
     if (!m) return;
     DeviceGuard g(m->device);
     (void)hipDeviceSynchronize();
-    for (void* p : {(void*)m->P, m->packed, (void*)m->steps_dev, m->data, m->ws, m->loss, m->fb})
+    for (void* p : {(void*)m->P, m->packed, (void*)m->steps_dev, m->data, m->ws, m->loss, m->fb,
+                    m->tab})
         if (p) (void)hipFree(p);
     delete m;
 }
@@ -207,7 +212,7 @@ bhs_mlp* bhs_mlp_create(bhs_ctx* ctx, const bhs_march_params* trained_for, uint3
     std::memset(m->k.serve, 0, sizeof m->k.serve);
     for (int i = 0; i < M::kStepBins; ++i) m->k.steps_max[i] = 0xFFFFu;
     m->k.beta_eps = M::kBetaEps;
-    m->k.radius_margin = 0.02f;
+    for (int i = 0; i < M::kStepBins; ++i) m->k.radius_margin[i] = 0.1f;
     m->k.logit_margin = 2.0f;
     m->k.steps_slack = 1.15f;
     *st = BHS_OK;
@@ -412,17 +417,34 @@ int bhs_mlp_validate(bhs_ctx* ctx, bhs_mlp* m, int64_t n_rays, uint32_t seed, fl
             return hip_fail(ctx, e2, "bhs_mlp_validate");
         return BHS_OK;
     };
-    // Pass 1: radius errors near the annulus decide the edge margin.
+    // Pass 1: radius errors near the annulus decide the edge margin of each log r_obs bin (the
+    // largest error of the bin and its neighbours, x1.25, within [1e-3, 0.1]).
     if ((st = run()) != BHS_OK) {
         (void)hipFree(buf);
         return st;
     }
-    std::vector<float> tmp;
-    for (int64_t i = 0; i < n_rays; ++i)
-        if (er[i] >= 0.0f && er[i] < 1e29f) tmp.push_back(er[i]);
-    float margin = tmp.empty() ? 0.02f : 1.25f * quantile(tmp, 1.0);
-    margin = std::min(0.1f, std::max(1e-3f, margin));
-    k.radius_margin = margin;
+    std::vector<float> pos(3 * n_rays);
+    if ((e = hipMemcpy(pos.data(), rs.pos3, 12 * n_rays, hipMemcpyDeviceToHost)) != hipSuccess) {
+        (void)hipFree(buf);
+        return hip_fail(ctx, e, "bhs_mlp_validate copy");
+    }
+    float bin_max[M::kStepBins] = {};
+    for (int64_t i = 0; i < n_rays; ++i) {
+        if (!(er[i] >= 0.0f && er[i] < 1e29f)) continue;
+        const double r = std::sqrt((double)pos[3 * i] * pos[3 * i] + (double)pos[3 * i + 1] * pos[3 * i + 1] +
+                                   (double)pos[3 * i + 2] * pos[3 * i + 2]);
+        int bin = (int)((std::log(r) - M::kLogRLo) * (M::kStepBins / (M::kLogRHi - M::kLogRLo)));
+        bin = std::max(0, std::min(M::kStepBins - 1, bin));
+        bin_max[bin] = std::max(bin_max[bin], er[i]);
+    }
+    float margin = 0.0f;
+    for (int i = 0; i < M::kStepBins; ++i) {
+        float mx = bin_max[i];
+        if (i > 0) mx = std::max(mx, bin_max[i - 1]);
+        if (i + 1 < M::kStepBins) mx = std::max(mx, bin_max[i + 1]);
+        k.radius_margin[i] = std::min(0.1f, std::max(1e-3f, 1.25f * (mx > 0.0f ? mx : 0.08f)));
+        margin = std::max(margin, k.radius_margin[i]);
+    }
     // Pass 2: per-cell maxima with the final margin.
     if ((st = run()) != BHS_OK) {
         (void)hipFree(buf);
@@ -472,7 +494,7 @@ int bhs_mlp_validate(bhs_ctx* ctx, bhs_mlp* m, int64_t n_rays, uint32_t seed, fl
     v.radius_margin = margin;
     v.served_cells = served_cells;
     std::memcpy(m->k.serve, serve, sizeof serve);
-    m->k.radius_margin = margin;
+    std::memcpy(m->k.radius_margin, k.radius_margin, sizeof k.radius_margin);
     m->tol_phi = tol_phi;
     if (out_host) *out_host = v;
     ctx->last_stream = nullptr;
@@ -514,7 +536,7 @@ int bhs_mlp_get_info(bhs_ctx* ctx, bhs_mlp* m, bhs_mlp_info* out_host) {
     i.dataset_rays = m->n_data;
     i.adam_steps = m->adam_t;
     i.tol_phi = m->tol_phi;
-    i.radius_margin = m->k.radius_margin;
+    i.radius_margin = *std::max_element(m->k.radius_margin, m->k.radius_margin + M::kStepBins);
     i.logit_margin = m->k.logit_margin;
     i.steps_slack = m->k.steps_slack;
     std::memcpy(i.steps_max, m->k.steps_max, sizeof i.steps_max);
@@ -532,8 +554,8 @@ int bhs_mlp_save(bhs_ctx* ctx, bhs_mlp* m, const char* path) {
     int st;
     if ((st = bhs_mlp_get_params(ctx, m, p.data(), M::kParams)) != BHS_OK) return st;
     FileHead h{};
-    std::memcpy(h.magic, "BHSMLP01", 8);
-    h.version = 1;
+    std::memcpy(h.magic, "BHSMLP02", 8);
+    h.version = 2;
     h.params = M::kParams;
     h.trained_for = m->trained_for;
     h.dataset_rays = m->n_data;
@@ -565,7 +587,7 @@ bhs_mlp* bhs_mlp_load(bhs_ctx* ctx, const char* path, int* status) {
     const bool ok = std::fread(&h, sizeof h, 1, f) == 1 &&
                     std::fread(p.data(), sizeof(float), p.size(), f) == p.size();
     std::fclose(f);
-    if (!ok || std::memcmp(h.magic, "BHSMLP01", 8) != 0 || h.version != 1 ||
+    if (!ok || std::memcmp(h.magic, "BHSMLP02", 8) != 0 || h.version != 2 ||
         h.params != (uint32_t)M::kParams) {
         *st = fail(ctx, BHS_E_IO, "bhs_mlp_load: not a model file of this version: %s", path);
         return nullptr;
@@ -580,6 +602,20 @@ bhs_mlp* bhs_mlp_load(bhs_ctx* ctx, const char* path, int* status) {
     m->tol_phi = h.tol_phi;
     m->k = h.k;
     return m;
+}
+
+int bhs_mlp_set_table(bhs_ctx* ctx, bhs_mlp* m, int32_t n2, int32_t n3) {
+    if (!ctx || !m) return BHS_E_INVALID;
+    if (n2 == 0) {
+        m->tab_n2 = m->tab_n3 = 0;
+        return BHS_OK;
+    }
+    if (n2 < 2 || n3 < 2 || n2 > 16384 || n3 > 16384 || (int64_t)n2 * n3 > ((int64_t)1 << 24))
+        return fail(ctx, BHS_E_INVALID,
+                    "bhs_mlp_set_table: n2, n3 in [2, 16384] with n2 * n3 <= 2^24 (or n2 = 0)%s");
+    m->tab_n2 = n2;
+    m->tab_n3 = n3;
+    return BHS_OK;
 }
 
 int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
@@ -607,9 +643,21 @@ int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
     const bool roots = params->root64 && params->precision != BHS_PREC_F64;
     bhs::Scratch sc;
     if ((st = prepare_scratch(ctx, n, n_items, roots, &sc)) != BHS_OK) return st;
-    if ((st = grow(ctx, &m->fb, &m->fb_bytes, 256 + (size_t)n_items * 4, "fallback list")) != BHS_OK)
+    if ((st = grow(ctx, &m->fb, &m->fb_bytes, 256 + (size_t)n_items * 8, "fallback lists")) !=
+        BHS_OK)
         return st;
-    M::SurLists fl{(uint32_t*)((char*)m->fb + 256), (uint32_t*)m->fb, n_items};
+    uint32_t* cnt = (uint32_t*)m->fb;
+    uint32_t* late = (uint32_t*)((char*)m->fb + 256);
+    M::SurLists fl{late, cnt, late + n_items, cnt + 1, cnt + 2, n_items};
+    const M::SurStreams ss{ctx->aux, ctx->aux2, ctx->ev_fork, ctx->ev_join, ctx->ev_fork2,
+                           ctx->ev_join2};
+    M::SurTable tab{};
+    if (m->tab_n2 > 0) {
+        const size_t nodes = (size_t)m->tab_n2 * (size_t)m->tab_n3;
+        if ((st = grow(ctx, &m->tab, &m->tab_bytes, 256 + nodes * 32, "surrogate table")) != BHS_OK)
+            return st;
+        tab = M::SurTable{(float*)((char*)m->tab + 256), (int32_t*)m->tab, m->tab_n2, m->tab_n3};
+    }
     hipStream_t s = (hipStream_t)stream;
     if ((st = pack(ctx, m, s)) != BHS_OK) return st;
     hipError_t e;
@@ -618,8 +666,8 @@ int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
     if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess ||
         (e = bhs::launch_zero(m->fb, 256, s)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_render_surrogate clear");
-    if ((e = M::launch_surrogate(c, m->k, m->packed, make_soa(out, n), sc, fl, ctx->tuning,
-                                 ctx->num_cus, s, ctx->aux, ctx->ev_fork, ctx->ev_join,
+    if ((e = M::launch_surrogate(c, m->k, m->packed, m->tab_n2 > 0 ? &tab : nullptr,
+                                 make_soa(out, n), sc, fl, ctx->tuning, ctx->num_cus, s, ss,
                                  tr ? tr->k0 : nullptr, tr ? tr->k1 : nullptr, nullptr)) !=
         hipSuccess)
         return hip_fail(ctx, e, "bhs_render_surrogate launch");

@@ -345,6 +345,9 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
     if (list) {   // listed items: the count is known on the device only
         const uint32_t cnt = *list_count;
         n_items = cnt < n_items ? cnt : n_items;
+        // Fallback lists are few long rays beside other kernels: issue them first on their SIMD
+        // (below the FP64 band's priority 3).
+        __builtin_amdgcn_s_setprio(2);
     }
     MarchState<float> st;
     Setup<float> s;
@@ -544,7 +547,7 @@ hipError_t launch_persist(const Tuning& t, int num_cus, hipStream_t s, const Mar
                           const uint32_t* list = nullptr, const uint32_t* list_count = nullptr) {
     const int block = t.block == 64 ? 64 : 256;
     const int wps = t.waves_per_simd < 1 ? 1 : (t.waves_per_simd > 8 ? 8 : t.waves_per_simd);
-    const uint32_t waves = (uint32_t)num_cus * 4u * (uint32_t)wps;
+    const uint32_t waves = (uint32_t)num_cus * 4u * (uint32_t)wps;   // resident-wave budget
     const uint32_t need = (n_items + 63u) / 64u;
     const uint32_t blocks = (waves < need ? waves : need) * 64u / (uint32_t)block;
     const uint32_t rmin = t.refill_min < 1 ? 1u : (t.refill_min > 64 ? 64u : (uint32_t)t.refill_min);
@@ -664,6 +667,12 @@ hipError_t launch_band_list(const MarchConst& c, const SoA& out, const Scratch& 
     hipLaunchKernelGGL(k_band64<OneCam>, dim3(256), dim3(64), 0, aux, one, out, sc);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipEventRecord(ev_join, aux);
+}
+
+hipError_t launch_band_fork(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s,
+                            hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join) {
+    const OneCam one{c, camera_items(c.width, c.height)};
+    return fork_band(one, out, sc, s, aux, ev_fork, ev_join);
 }
 
 hipError_t launch_root_refine(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s) {

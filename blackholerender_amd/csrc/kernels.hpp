@@ -71,6 +71,10 @@ hipError_t launch_list(const MarchConst& c, const uint32_t* ids, const uint32_t*
 // and joined by ev_join (which the caller waits for).
 hipError_t launch_band_list(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s,
                             hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join);
+// The integrator's band scan of one camera on s, then the FP64 march of the listed rays on aux
+// between ev_fork and ev_join (as in launch_render).
+hipError_t launch_band_fork(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s,
+                            hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join);
 // FP64 root refinement of the crossings of integrator-marched FP32 rays (root64 cameras).
 hipError_t launch_root_refine(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s);
 hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, const MarchConst& c,

@@ -21,8 +21,8 @@ constexpr int kStepBins = 16;
 struct SurConst {
     uint32_t serve[kMapWords];        // error map bits, cell = ir * kMapB + ib
     uint32_t steps_max[kStepBins];    // training step maximum per log r_obs bin
+    float radius_margin[kStepBins];   // relative margin around the annulus edges per log r_obs bin
     float beta_eps;                   // fallback band |beta - 1| < beta_eps
-    float radius_margin;              // relative margin around the annulus edges
     float logit_margin;               // |logit| below this: reach undecided -> fallback
     float steps_slack;                // served only if steps_max * slack < max_steps
 };
@@ -69,15 +69,31 @@ hipError_t launch_validate(int64_t n, const float* pos3, const float* dir3, cons
 // Surrogate render of one camera: every eligible ray is served by the network, the others are
 // listed for the integrator (FP64 band list in sc.band_ids, FP32 list in fb_ids).
 struct SurLists {
-    uint32_t* fb_ids;         // FP32 fallback items
+    uint32_t* fb_ids;         // FP32 fallback items decided by the decode (after the network)
     uint32_t* fb_count;
+    uint32_t* early_ids;      // FP32 fallback items known before the network (outside the served
+    uint32_t* early_count;    // domain, unserved cell, step budget)
+    uint32_t* early_work;     // work counter of the early march
     uint32_t fb_cap;
 };
-// ev_begin / ev_mid / ev_end (may be null) bracket the network kernel and the FP32 fallback march.
+// Side stream of the early FP32 fallback march.
+struct SurStreams {
+    hipStream_t aux, aux2;
+    hipEvent_t ev_fork, ev_join, ev_fork2, ev_join2;
+};
+// Table mode: the network on an n2 x n3 grid over (x2, x3) at the camera's r_obs (node: n2 n3 x 8
+// floats; range: 2 ints of device scratch).
+struct SurTable {
+    float* node;
+    int32_t* range;
+    int32_t n2, n3;
+};
+// tab null: per-ray network inference. ev_begin / ev_mid / ev_end (may be null) bracket the
+// network part and the FP32 fallback march.
 hipError_t launch_surrogate(const MarchConst& c, const SurConst& k, const void* packed,
-                            const SoA& out, const Scratch& sc, const SurLists& fl, const Tuning& t,
-                            int num_cus, hipStream_t s, hipStream_t aux, hipEvent_t ev_fork,
-                            hipEvent_t ev_join, hipEvent_t ev_begin, hipEvent_t ev_mid,
+                            const SurTable* tab, const SoA& out, const Scratch& sc,
+                            const SurLists& fl, const Tuning& t, int num_cus, hipStream_t s,
+                            const SurStreams& ss, hipEvent_t ev_begin, hipEvent_t ev_mid,
                             hipEvent_t ev_end);
 
 }  // namespace mlp

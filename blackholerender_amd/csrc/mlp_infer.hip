@@ -196,10 +196,14 @@ __device__ __forceinline__ bool cell_served(const SurConst& k, const float x[3])
     return (k.serve[cell >> 5] >> (cell & 31)) & 1u;
 }
 
+// Log r_obs bin of the step table and the radius margins.
+__device__ __forceinline__ int r_bin(float r) {
+    const int bin = (int)((logf(r) - kLogRLo) * ((float)kStepBins / (kLogRHi - kLogRLo)));
+    return bin < 0 ? 0 : (bin >= kStepBins ? kStepBins - 1 : bin);
+}
+
 __device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k, const RayGeom& g) {
-    int bin = (int)((logf(g.r) - kLogRLo) * ((float)kStepBins / (kLogRHi - kLogRLo)));
-    bin = bin < 0 ? 0 : (bin >= kStepBins ? kStepBins - 1 : bin);
-    return (float)k.steps_max[bin] * k.steps_slack < (float)c.max_steps;
+    return (float)k.steps_max[r_bin(g.r)] * k.steps_slack < (float)c.max_steps;
 }
 
 // Output record of one in-domain ray from the network outputs y, or serve = false when the ray must
@@ -224,13 +228,14 @@ __device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, co
         o.n_eq = (r1 ? 1u : 0u) + (r2 ? 1u : 0u);
     }
     bool in[2] = {false, false};
+    const float margin = k.radius_margin[r_bin(g.r)];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         o.r_eq[q] = 1.0f / u_decode(y[1 + q]);
         if ((uint32_t)q < o.n_eq && c.thin) {
             const float r = o.r_eq[q];
-            if (fabsf(r - c.disk_inner) < k.radius_margin * c.disk_inner ||
-                fabsf(r - c.disk_outer) < k.radius_margin * c.disk_outer)
+            if (fabsf(r - c.disk_inner) < margin * c.disk_inner ||
+                fabsf(r - c.disk_outer) < margin * c.disk_outer)
                 return o;
             in[q] = r >= c.disk_inner && r <= c.disk_outer;
         }
@@ -300,20 +305,55 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// Appends the items of the lanes with `want` to a list with one atomic per wave. All lanes call it.
-__device__ __forceinline__ void push_list(uint32_t* count, uint32_t* ids, uint32_t cap, bool want,
-                                          uint32_t item) {
+// Appends the items of the lanes with `want` to a list with one atomic per block (wave ballots,
+// per-wave counts in LDS). Every thread of the block must call it; s_cnt: 5 words of LDS.
+__device__ __forceinline__ void push_block(uint32_t* count, uint32_t* ids, uint32_t cap, bool want,
+                                           uint32_t item, uint32_t* s_cnt) {
     const uint64_t m = __ballot(want);
-    if (m == 0) return;
-    const uint32_t lane = threadIdx.x & 63u;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, leader, 64);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) s_cnt[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        s_cnt[4] = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = s_cnt[4];
+    for (int w = 0; w < wave; ++w) base += s_cnt[w];
     if (want) {
         const uint32_t rank =
             __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         if (base + rank < cap) ids[base + rank] = item;
+    }
+    __syncthreads();
+}
+
+// Adds the block's counters to the launch statistics with one atomic per counter and block.
+__device__ __forceinline__ void add_block_stats(Stats* st, uint32_t served, uint32_t cross,
+                                                uint32_t esc, uint32_t fb, uint32_t (*s_st)[4]) {
+    const int wave = threadIdx.x >> 6;
+    served = wave_sum(served);
+    cross = wave_sum(cross);
+    esc = wave_sum(esc);
+    fb = wave_sum(fb);
+    if ((threadIdx.x & 63) == 0) {
+        s_st[wave][0] = served;
+        s_st[wave][1] = cross;
+        s_st[wave][2] = esc;
+        s_st[wave][3] = fb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t v[4] = {0, 0, 0, 0};
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            for (int k = 0; k < 4; ++k) v[k] += s_st[w][k];
+        if (v[0]) {
+            atomicAdd(&st->rays, (unsigned long long)v[0]);
+            atomicAdd(&st->surrogate, (unsigned long long)v[0]);
+        }
+        if (v[1]) atomicAdd(&st->crossings, (unsigned long long)v[1]);
+        if (v[2]) atomicAdd(&st->escapes, (unsigned long long)v[2]);
+        if (v[3]) atomicAdd(&st->fallback, (unsigned long long)v[3]);
     }
 }
 
@@ -327,16 +367,118 @@ __device__ __forceinline__ int64_t item_pixel(const MarchConst& c, uint32_t q) {
 
 constexpr int kBlock = 256;
 
-// Surrogate render of one camera. Persistent blocks (two per CU: the weights take 71 KB of LDS)
-// walk 256-item chunks; per chunk each lane prepares one ray, the wave runs the network if any of
-// its lanes is in the served domain, and each lane writes its record or lists its ray for the
-// integrator: FP64 band list (the integrator's own band rule) or FP32 list.
+// Order-preserving map of floats to ints (for atomic min/max) and back.
+__device__ __forceinline__ int32_t ord_int(float f) {
+    const int32_t i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float ord_float(int32_t i) {
+    return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF);
+}
+
+// Route of one ray of a surrogate render, decided before the network runs.
+enum Route : uint32_t {
+    kNone = 0,    // padding item
+    kBand = 1,    // integrator's FP64 band (BAND64 precision): FP64 march from the start
+    kEarly = 2,   // outside the served domain / unserved cell / step budget: FP32 march at once
+    kNet = 3,     // network candidate (the decode may still send it back)
+};
+
+struct RayPrep {
+    Setup<float> s;
+    RayGeom g;
+    float x[3];
+    Route route;
+};
+
+// Classification of item q; identical in the pre-pass and the network pass (same operations).
+__device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst& k,
+                                            const V3<float>& P, int64_t pix) {
+    RayPrep r;
+    r.x[0] = r.x[1] = r.x[2] = 0.0f;
+    r.route = kNone;
+    if (pix < 0) return r;
+    const V3<float> d = ray_dir(c, pix);
+    r.s = lawS_setup(P, d);
+    if (c.precision == BHS_PREC_F32_BAND64 && __builtin_fabsf(r.s.b - kBcritF) < c.band_abs) {
+        r.route = kBand;
+        return r;
+    }
+    r.g = ray_geom(P, d, r.s);
+    r.route = kEarly;
+    if (in_domain(r.g)) {
+        features(r.g, r.x);
+        if (cell_served(k, r.x) && steps_ok(c, k, r.g)) r.route = kNet;
+    }
+    return r;
+}
+
+// Fallback list of a ray that the surrogate does not serve: FP64 precision sends everything to the
+// FP64 band list after the network pass.
+__device__ __forceinline__ void push_fallback(const MarchConst& c, const Scratch& sc,
+                                              uint32_t* count, uint32_t* ids, uint32_t cap,
+                                              bool want, uint32_t q, uint32_t* s_cnt) {
+    if (c.precision == BHS_PREC_F64)
+        push_block(sc.band_count, sc.band_ids, sc.band_cap, want, q, s_cnt);
+    else
+        push_block(count, ids, cap, want, q, s_cnt);
+}
+
+// Pre-pass: lists the FP64 band rays and the early fallback rays, and (table mode, range non-null)
+// the x2 range of the network candidates.
+__global__ __launch_bounds__(kBlock) void k_sur_pre(const MarchConst c, const SurConst k,
+                                                    uint32_t n_items, Scratch sc, SurLists fl,
+                                                    int32_t* __restrict__ range) {
+    __shared__ uint32_t s_cnt[5];
+    __shared__ int32_t s_lo[kBlock / 64], s_hi[kBlock / 64];
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    int32_t lo = 0x7FFFFFFF, hi = (int32_t)0x80000000;
+    for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
+        const uint32_t q = q0 + threadIdx.x;
+        const RayPrep r = prep_ray(c, k, P, q < n_items ? item_pixel(c, q) : -1);
+        if (r.route == kNet) {
+            const int32_t v = ord_int(r.x[1]);
+            lo = v < lo ? v : lo;
+            hi = v > hi ? v : hi;
+        }
+        if (c.precision == BHS_PREC_F32_BAND64)
+            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
+        push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q, s_cnt);
+    }
+    if (!range) return;   // grid-uniform
+    for (int o = 32; o > 0; o >>= 1) {
+        const int32_t a2 = __shfl_xor(lo, o, 64), b2 = __shfl_xor(hi, o, 64);
+        lo = a2 < lo ? a2 : lo;
+        hi = b2 > hi ? b2 : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) {
+            lo = s_lo[w] < lo ? s_lo[w] : lo;
+            hi = s_hi[w] > hi ? s_hi[w] : hi;
+        }
+        if (lo <= hi) {
+            atomicMin(&range[0], lo);
+            atomicMax(&range[1], hi);
+        }
+    }
+}
+
+// Surrogate render, per-ray network. Persistent blocks (two per CU: the weights take 71 KB of LDS)
+// walk 256-item chunks; the wave runs the network if any of its lanes is a candidate, each
+// candidate lane decodes and writes its record or lists its ray for the integrator.
 __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const MarchConst c, const SurConst k,
                                                          const uint8_t* __restrict__ packed,
                                                          const SoA out, Scratch sc, SurLists fl,
                                                          uint32_t n_items) {
     __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
     __shared__ float s_y[4][64 * kOut];
+    __shared__ uint32_t s_cnt[5];
+    __shared__ uint32_t s_st[4][4];
     stage_weights(packed, s_w);
     __syncthreads();
     const int wave = threadIdx.x >> 6;
@@ -346,54 +488,26 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const MarchConst c, con
     for (uint32_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
         const uint32_t q = ch * kBlock + threadIdx.x;
         const int64_t pix = q < n_items ? item_pixel(c, q) : -1;
-        const bool valid = pix >= 0;
-        Setup<float> s;
-        RayGeom g;
-        float x[3] = {0.0f, 0.0f, 0.0f};
-        bool elig = false;
-        if (valid) {
-            const V3<float> d = ray_dir(c, pix);
-            s = lawS_setup(P, d);
-            g = ray_geom(P, d, s);
-            if (in_domain(g)) {
-                features(g, x);
-                elig = cell_served(k, x) && steps_ok(c, k, g);
-            }
-        }
+        const RayPrep r = prep_ray(c, k, P, pix);
+        const bool cand = r.route == kNet;
         float y[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        if (__any(elig)) mlp_wave(s_w, x, s_y[wave], y);
-        bool served = false;
-        if (elig) {
-            const Dec o = decode(c, k, s, g, y);
+        if (__any(cand)) mlp_wave(s_w, r.x, s_y[wave], y);
+        bool back = false;
+        if (cand) {
+            const Dec o = decode(c, k, r.s, r.g, y);
             if (o.serve) {
-                store_dec(out, c.ray_offset + pix, o, s.b);
-                served = true;
+                store_dec(out, c.ray_offset + pix, o, r.s.b);
                 n_served += 1;
                 n_cross += o.n_cross;
                 n_esc += o.escaping ? 1u : 0u;
+            } else {
+                back = true;
             }
         }
-        const bool back = valid && !served;
-        const bool band = back && (c.precision == BHS_PREC_F64 ||
-                                   (c.precision == BHS_PREC_F32_BAND64 &&
-                                    __builtin_fabsf(s.b - kBcritF) < c.band_abs));
-        push_list(sc.band_count, sc.band_ids, sc.band_cap, band, q);
-        push_list(fl.fb_count, fl.fb_ids, fl.fb_cap, back && !band, q);
-        n_fb += back ? 1u : 0u;
+        push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
+        n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
-    n_served = wave_sum(n_served);
-    n_cross = wave_sum(n_cross);
-    n_esc = wave_sum(n_esc);
-    n_fb = wave_sum(n_fb);
-    if ((threadIdx.x & 63) == 0) {
-        if (n_served) {
-            atomicAdd(&sc.stats->rays, (unsigned long long)n_served);
-            atomicAdd(&sc.stats->surrogate, (unsigned long long)n_served);
-        }
-        if (n_cross) atomicAdd(&sc.stats->crossings, (unsigned long long)n_cross);
-        if (n_esc) atomicAdd(&sc.stats->escapes, (unsigned long long)n_esc);
-        if (n_fb) atomicAdd(&sc.stats->fallback, (unsigned long long)n_fb);
-    }
+    add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
 }
 
 // Network outputs of explicit feature rows.
@@ -511,6 +625,124 @@ __global__ __launch_bounds__(kBlock, 2) void k_validate(int64_t n, const float* 
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Table mode: the network evaluated on an n2 x n3 grid over (x2, x3) at the camera's x1, rays
+// interpolate bilinearly. x2 spans the camera's in-domain rays (device-side min/max pass), x3 the
+// full [-1, 1]. Node (i, j) holds the 5 outputs at node[(j n2 + i) * 8 + o].
+
+__global__ void k_range_init(int32_t* range) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        range[0] = 0x7FFFFFFF;
+        range[1] = (int32_t)0x80000000;
+    }
+}
+
+struct TabGeom {
+    float x2_lo, inv_h2, h2, inv_h3, h3;
+    bool empty;
+};
+
+__device__ __forceinline__ TabGeom tab_geom(const int32_t* range, int32_t n2, int32_t n3) {
+    TabGeom t;
+    const int32_t a = range[0], b = range[1];
+    t.empty = a > b;
+    float lo = t.empty ? 0.0f : ord_float(a), hi = t.empty ? 0.0f : ord_float(b);
+    lo -= 1e-3f;
+    hi += 1e-3f;
+    t.x2_lo = lo;
+    t.h2 = (hi - lo) / (float)(n2 - 1);
+    t.inv_h2 = 1.0f / t.h2;
+    t.h3 = 2.0f / (float)(n3 - 1);
+    t.inv_h3 = 1.0f / t.h3;
+    return t;
+}
+
+// Network outputs at the table nodes.
+__global__ __launch_bounds__(kBlock, 2) void k_table(const MarchConst c,
+                                                     const uint8_t* __restrict__ packed,
+                                                     const int32_t* __restrict__ range, int32_t n2,
+                                                     int32_t n3, float* __restrict__ node) {
+    __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
+    __shared__ float s_y[4][64 * kOut];
+    const TabGeom t = tab_geom(range, n2, n3);
+    if (t.empty) return;   // block-uniform: no in-domain ray, nothing reads the table
+    stage_weights(packed, s_w);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    const float r = sqrtf(c.pos[0] * c.pos[0] + c.pos[1] * c.pos[1] + c.pos[2] * c.pos[2]);
+    const float x1 = (logf(r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
+    const int64_t n = (int64_t)n2 * n3;
+    const int64_t chunks = (n + kBlock - 1) / kBlock;
+    for (int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        const int64_t q = ch * kBlock + threadIdx.x;
+        const int32_t i = (int32_t)(q % n2), j = (int32_t)(q / n2);
+        const float x[3] = {x1, __builtin_fmaf((float)i, t.h2, t.x2_lo),
+                            __builtin_fmaf((float)j, t.h3, -1.0f)};
+        float y[5];
+        mlp_wave(s_w, x, s_y[wave], y);
+        if (q < n) {
+            float* o = node + q * 8;
+#pragma unroll
+            for (int k2 = 0; k2 < kOut; ++k2) o[k2] = y[k2];
+        }
+    }
+}
+
+__device__ __forceinline__ void tab_lookup(const float* __restrict__ node, const TabGeom& t,
+                                           int32_t n2, int32_t n3, const float x[3], float y[5]) {
+    float fx = (x[1] - t.x2_lo) * t.inv_h2, fy = (x[2] + 1.0f) * t.inv_h3;
+    fx = fminf(fmaxf(fx, 0.0f), (float)(n2 - 1) - 1e-3f);
+    fy = fminf(fmaxf(fy, 0.0f), (float)(n3 - 1) - 1e-3f);
+    const int32_t i = (int32_t)fx, j = (int32_t)fy;
+    const float a = fx - (float)i, b = fy - (float)j;
+    const float* p0 = node + ((int64_t)j * n2 + i) * 8;
+    const float* p1 = p0 + (int64_t)n2 * 8;
+    const f4 q00 = *(const f4*)p0, q01 = *(const f4*)(p0 + 8);
+    const f4 q10 = *(const f4*)p1, q11 = *(const f4*)(p1 + 8);
+    const float w00 = (1.0f - a) * (1.0f - b), w01 = a * (1.0f - b), w10 = (1.0f - a) * b,
+                w11 = a * b;
+#pragma unroll
+    for (int o = 0; o < 4; ++o)
+        y[o] = w00 * q00[o] + w01 * q01[o] + w10 * q10[o] + w11 * q11[o];
+    y[4] = w00 * p0[4] + w01 * p0[12] + w10 * p1[4] + w11 * p1[12];
+}
+
+// Surrogate render in table mode: as k_surrogate with the network replaced by the table lookup.
+__global__ __launch_bounds__(kBlock) void k_surrogate_tab(const MarchConst c, const SurConst k,
+                                                          const float* __restrict__ node,
+                                                          const int32_t* __restrict__ range,
+                                                          int32_t n2, int32_t n3, const SoA out,
+                                                          Scratch sc, SurLists fl,
+                                                          uint32_t n_items) {
+    __shared__ uint32_t s_cnt[5];
+    __shared__ uint32_t s_st[4][4];
+    const TabGeom t = tab_geom(range, n2, n3);
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
+    for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
+        const uint32_t q = q0 + threadIdx.x;
+        const int64_t pix = q < n_items ? item_pixel(c, q) : -1;
+        const RayPrep r = prep_ray(c, k, P, pix);
+        bool back = false;
+        if (r.route == kNet) {
+            float y[5];
+            tab_lookup(node, t, n2, n3, r.x, y);
+            const Dec o = decode(c, k, r.s, r.g, y);
+            if (o.serve) {
+                store_dec(out, c.ray_offset + pix, o, r.s.b);
+                n_served += 1;
+                n_cross += o.n_cross;
+                n_esc += o.escaping ? 1u : 0u;
+            } else {
+                back = true;
+            }
+        }
+        push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
+        n_fb += (back || r.route == kEarly) ? 1u : 0u;
+    }
+    add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
+}
+
 int persistent_blocks(int64_t chunks, int num_cus) {
     const int64_t cap = 2 * (int64_t)(num_cus > 0 ? num_cus : 256);
     return (int)(chunks < cap ? (chunks > 0 ? chunks : 1) : cap);
@@ -543,27 +775,69 @@ hipError_t launch_validate(int64_t n, const float* pos3, const float* dir3, cons
 }
 
 hipError_t launch_surrogate(const MarchConst& c, const SurConst& k, const void* packed,
-                            const SoA& out, const Scratch& sc, const SurLists& fl,
-                            const Tuning& t, int num_cus, hipStream_t s, hipStream_t aux,
-                            hipEvent_t ev_fork, hipEvent_t ev_join, hipEvent_t ev_begin,
-                            hipEvent_t ev_mid, hipEvent_t ev_end) {
+                            const SurTable* tab, const SoA& out, const Scratch& sc,
+                            const SurLists& fl, const Tuning& t, int num_cus, hipStream_t s,
+                            const SurStreams& ss, hipEvent_t ev_begin, hipEvent_t ev_mid,
+                            hipEvent_t ev_end) {
     const uint32_t n_items = camera_items(c.width, c.height);
-    hipError_t e;
-    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
-    const int blocks = persistent_blocks((n_items + kBlock - 1) / kBlock, num_cus);
-    hipLaunchKernelGGL(k_surrogate, dim3(blocks), dim3(kBlock), 0, s, c, k,
-                       (const uint8_t*)packed, out, sc, fl, n_items);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
-    // Fallback: FP64 band list on aux, FP32 list on s (persistent, lane refill).
-    if ((e = launch_band_list(c, out, sc, s, aux, ev_fork, ev_join)) != hipSuccess) return e;
+    const uint32_t gb0 = (n_items + kBlock - 1) / kBlock;
+    const uint32_t cap = (uint32_t)(4 * (num_cus > 0 ? num_cus : 256));
+    const dim3 grid(gb0 < cap ? (gb0 ? gb0 : 1u) : cap);
+    const bool f64 = c.precision == BHS_PREC_F64;
+    // The fallback lists are short and latency-bound: two resident waves per SIMD (131k lanes)
+    // leave the rest of the chip to the network pass running beside them.
     Tuning tl = t;
     tl.mode = 1;
-    if ((e = launch_list(c, fl.fb_ids, fl.fb_count, fl.fb_cap, out, sc, tl, num_cus, s)) !=
-        hipSuccess)
+    tl.block = 256;
+    tl.waves_per_simd = 2;
+    hipError_t e;
+    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
+    // Pre-pass, then the two side marches start at once: the FP64 band (BAND64) on aux and the
+    // early FP32 fallback on aux2 run beside the network pass; their serial rays are the longest
+    // chains of the frame.
+    if (tab) hipLaunchKernelGGL(k_range_init, dim3(1), dim3(64), 0, s, tab->range);
+    hipLaunchKernelGGL(k_sur_pre, grid, dim3(kBlock), 0, s, c, k, n_items, sc, fl,
+                       tab ? tab->range : (int32_t*)nullptr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (!f64) {
+        if ((e = hipEventRecord(ss.ev_fork2, s)) != hipSuccess ||
+            (e = hipStreamWaitEvent(ss.aux2, ss.ev_fork2, 0)) != hipSuccess)
+            return e;
+        Scratch se = sc;
+        se.work = fl.early_work;
+        if ((e = launch_list(c, fl.early_ids, fl.early_count, fl.fb_cap, out, se, tl, num_cus,
+                             ss.aux2)) != hipSuccess)
+            return e;
+        if ((e = hipEventRecord(ss.ev_join2, ss.aux2)) != hipSuccess) return e;
+        if (c.precision == BHS_PREC_F32_BAND64 &&
+            (e = launch_band_list(c, out, sc, s, ss.aux, ss.ev_fork, ss.ev_join)) != hipSuccess)
+            return e;
+    }
+    if (tab) {
+        const int64_t nodes = (int64_t)tab->n2 * tab->n3;
+        hipLaunchKernelGGL(k_table, dim3(persistent_blocks((nodes + kBlock - 1) / kBlock, num_cus)),
+                           dim3(kBlock), 0, s, c, (const uint8_t*)packed, (const int32_t*)tab->range,
+                           tab->n2, tab->n3, tab->node);
+        hipLaunchKernelGGL(k_surrogate_tab, grid, dim3(kBlock), 0, s, c, k, (const float*)tab->node,
+                           (const int32_t*)tab->range, tab->n2, tab->n3, out, sc, fl, n_items);
+    } else {
+        const int blocks = persistent_blocks(gb0, num_cus);
+        hipLaunchKernelGGL(k_surrogate, dim3(blocks), dim3(kBlock), 0, s, c, k,
+                           (const uint8_t*)packed, out, sc, fl, n_items);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
+    if (f64) {   // every unserved ray is in the band list: FP64 march now
+        if ((e = launch_band_list(c, out, sc, s, ss.aux, ss.ev_fork, ss.ev_join)) != hipSuccess)
+            return e;
+    } else if ((e = launch_list(c, fl.fb_ids, fl.fb_count, fl.fb_cap, out, sc, tl, num_cus, s)) !=
+               hipSuccess) {
         return e;
+    }
     if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(s, ev_join, 0)) != hipSuccess) return e;
+    if (f64 || c.precision == BHS_PREC_F32_BAND64)
+        if ((e = hipStreamWaitEvent(s, ss.ev_join, 0)) != hipSuccess) return e;
+    if (!f64 && (e = hipStreamWaitEvent(s, ss.ev_join2, 0)) != hipSuccess) return e;
     return launch_root_refine(c, out, sc, s);
 }
 

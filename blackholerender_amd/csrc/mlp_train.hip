@@ -202,9 +202,9 @@ __global__ void k_sum_splits(int S, int64_t cnt, const float* __restrict__ ws, f
     out[i] = s;
 }
 
-// Output layer (FP32), loss and its gradient for one sample per thread. Loss per sample (mean over
-// the batch): sum_o w_o m_o (y_o - t_o)^2 over the regressions (w = 4, 1, 1) / (batch regression
-// mask count) + 0.1 * mean of the two logistic losses.
+// Output layer (bf16 operands, FP32 accumulation, as the MFMA inference), loss and its gradient for
+// one sample per thread. Loss (mean over the batch): sum_o w_o m_o (y_o - t_o)^2 over the
+// regressions (w = 1, 8, 8) + 0.1 * mean of the two logistic losses.
 __global__ void k_head(int32_t B, const float* __restrict__ a3, const int32_t* __restrict__ idx,
                        const float* __restrict__ Y, const float* __restrict__ Mk,
                        const float* __restrict__ P, float reg_norm, float* __restrict__ dy,
@@ -223,7 +223,7 @@ __global__ void k_head(int32_t B, const float* __restrict__ a3, const int32_t* _
         const int64_t row = idx[b];
         const float* t = Y + 5 * row;
         const float* m = Mk + 5 * row;
-        const float wr[3] = {4.0f, 1.0f, 1.0f};
+        const float wr[3] = {1.0f, 8.0f, 8.0f};
         for (int o = 0; o < 3; ++o) {
             const float e = y[o] - t[o];
             l += wr[o] * m[o] * e * e * reg_norm;

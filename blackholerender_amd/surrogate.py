@@ -128,6 +128,11 @@ class Surrogate:
                     "bhs_mlp_get_info")
         return _struct_dict(i)
 
+    def set_table(self, n2: int, n3: int = 0):
+        """Table mode on an n2 x n3 (b/b_crit, psi) grid per render; n2 = 0: per-ray inference."""
+        self._check(self._lib.bhs_mlp_set_table(self.tracer._ctx, self._m, n2, n3),
+                    "bhs_mlp_set_table")
+
     def save(self, path: str):
         self._check(self._lib.bhs_mlp_save(self.tracer._ctx, self._m, path.encode()),
                     "bhs_mlp_save")

@@ -288,6 +288,10 @@ int bhs_mlp_get_info(bhs_ctx* ctx, bhs_mlp* m, bhs_mlp_info* out_host);
 /* Model file: parameters, error map, margins, step table and the march parameters it serves. */
 int bhs_mlp_save(bhs_ctx* ctx, bhs_mlp* m, const char* path);
 bhs_mlp* bhs_mlp_load(bhs_ctx* ctx, const char* path, int* status);
+/* Table mode (n2, n3 >= 2; n2 = 0 returns to per-ray inference): every bhs_render_surrogate then
+ * evaluates the network once per node of an n2 x n3 grid over (b/b_crit, psi) at the camera's r_obs,
+ * spanning the camera's in-domain rays, and each ray interpolates the outputs bilinearly. */
+int bhs_mlp_set_table(bhs_ctx* ctx, bhs_mlp* m, int32_t n2, int32_t n3);
 /* Surrogate render of one camera with integrator fallback, outputs as bhs_render (steps = 0 and
  * BHS_FLAG_SURROGATE on served rays). Stream-ordered. With timing on, the main-kernel interval is
  * the network kernel and the call interval includes the fallback march. */
