@@ -57,10 +57,13 @@ class Camera:
 
 def orbit_camera(r_obs: float, inclination_deg: float, fov_deg: float, width: int, height: int,
                  *, jitter_hashed: int = 0, seed: int = 0, camera_id: int = 0, sample: int = 0,
-                 target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)) -> Camera:
-    """Camera at r_obs * (0, cos i, sin i) looking at the hole, +y up (src/Camera.cpp:54-55)."""
+                 target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), azimuth_deg: float = 0.0) -> Camera:
+    """Camera at r_obs * (sin i sin a, cos i, sin i cos a) looking at the hole, +y up
+    (src/Camera.cpp:54-55; azimuth a = 0 is the reference's bookmark plane)."""
     inc = math.radians(inclination_deg)
-    pos = np.array([0.0, r_obs * math.cos(inc), r_obs * math.sin(inc)], dtype=np.float64)
+    az = math.radians(azimuth_deg)
+    pos = np.array([r_obs * math.sin(inc) * math.sin(az), r_obs * math.cos(inc),
+                    r_obs * math.sin(inc) * math.cos(az)], dtype=np.float64)
     basis = look_basis(pos, target, up)
     tan_h = math.tan(math.radians(fov_deg) * 0.5)
     return Camera(pos=pos.astype(np.float32), basis=basis, tan_half_fov=float(np.float32(tan_h)),

@@ -83,8 +83,8 @@ bool same_march(const bhs_march_params& a, const bhs_march_params& b) {
            a.disk_outer == b.disk_outer && a.disk_thickness == b.disk_thickness;
 }
 
-// Device ray set: pos3, dir3 (3n floats each) and the FP64 law-S outputs of the rays with refined
-// crossings recorded at any radius.
+// Device ray set: pos3, dir3 (3n floats each) and the FP64 law-S outputs of the rays with the
+// crossings recorded at any radius (root-refined when root64, else the march's chord crossings).
 struct RaySet {
     void* mem = nullptr;
     float* pos3 = nullptr;
@@ -95,7 +95,8 @@ struct RaySet {
     }
 };
 
-int march_set(bhs_ctx* ctx, bhs_mlp* m, int64_t n, uint32_t key, RaySet* rs, hipStream_t s) {
+int march_set(bhs_ctx* ctx, bhs_mlp* m, int64_t n, uint32_t key, bool root64, RaySet* rs,
+              hipStream_t s) {
     const size_t bytes = (size_t)n * (24 + 1 + 1 + 1 + 2 + 12 + 24 + 4) + 4096;
     hipError_t e = hipMalloc(&rs->mem, bytes);
     if (e != hipSuccess) {
@@ -123,7 +124,7 @@ int march_set(bhs_ctx* ctx, bhs_mlp* m, int64_t n, uint32_t key, RaySet* rs, hip
         return hip_fail(ctx, e, "bhs_mlp sample rays");
     bhs_march_params mp = m->trained_for;
     mp.precision = BHS_PREC_F64;
-    mp.root64 = 1;
+    mp.root64 = root64 ? 1 : 0;
     mp.cross_all = 1;
     if (mp.max_steps < 4096) mp.max_steps = 4096;
     bhs::MarchConst c = make_const(&mp);
@@ -240,7 +241,7 @@ int bhs_mlp_make_dataset(bhs_ctx* ctx, bhs_mlp* m, int64_t n_rays, uint32_t seed
     }
     RaySet rs;
     int st;
-    if ((st = march_set(ctx, m, n_rays, mix(seed, 0x51u), &rs, nullptr)) != BHS_OK) return st;
+    if ((st = march_set(ctx, m, n_rays, mix(seed, 0x51u), true, &rs, nullptr)) != BHS_OK) return st;
     float* X = (float*)m->data;
     float* Y = X + 3 * n_rays;
     float* Mk = Y + 5 * n_rays;
@@ -388,7 +389,9 @@ int bhs_mlp_validate(bhs_ctx* ctx, bhs_mlp* m, int64_t n_rays, uint32_t seed, fl
     DeviceGuard g(ctx->device);
     RaySet rs;
     int st;
-    if ((st = march_set(ctx, m, n_rays, mix(seed, 0xA5u), &rs, nullptr)) != BHS_OK) return st;
+    // The reference semantics: chord crossings, so the edge margins cover the march's own chord
+    // error as well as the surrogate's.
+    if ((st = march_set(ctx, m, n_rays, mix(seed, 0xA5u), false, &rs, nullptr)) != BHS_OK) return st;
     if ((st = pack(ctx, m, nullptr)) != BHS_OK) return st;
     void* buf = nullptr;
     hipError_t e = hipMalloc(&buf, (size_t)n_rays * 12);
@@ -554,8 +557,8 @@ int bhs_mlp_save(bhs_ctx* ctx, bhs_mlp* m, const char* path) {
     int st;
     if ((st = bhs_mlp_get_params(ctx, m, p.data(), M::kParams)) != BHS_OK) return st;
     FileHead h{};
-    std::memcpy(h.magic, "BHSMLP02", 8);
-    h.version = 2;
+    std::memcpy(h.magic, "BHSMLP03", 8);
+    h.version = 3;
     h.params = M::kParams;
     h.trained_for = m->trained_for;
     h.dataset_rays = m->n_data;
@@ -587,7 +590,7 @@ bhs_mlp* bhs_mlp_load(bhs_ctx* ctx, const char* path, int* status) {
     const bool ok = std::fread(&h, sizeof h, 1, f) == 1 &&
                     std::fread(p.data(), sizeof(float), p.size(), f) == p.size();
     std::fclose(f);
-    if (!ok || std::memcmp(h.magic, "BHSMLP02", 8) != 0 || h.version != 2 ||
+    if (!ok || std::memcmp(h.magic, "BHSMLP03", 8) != 0 || h.version != 3 ||
         h.params != (uint32_t)M::kParams) {
         *st = fail(ctx, BHS_E_IO, "bhs_mlp_load: not a model file of this version: %s", path);
         return nullptr;

@@ -18,14 +18,16 @@
 // Outputs:
 //   y0  escape: residual of the law-S escape angle over the exact Schwarzschild swept angle,
 //       Phi = Phi_exact + 1e-3 y0, escape direction = cos(Phi) e1 + sin(Phi) e2
-//   y1, y2  u = 1/r at the crossings phi = psi and psi + pi: u = exp(y) / 8 (u >= 1e-3 in training)
+//   y1, y2  crossings at phi = psi and psi + pi: u = 1/r = u_exact exp(1e-3 y), u_exact the exact
+//           orbit's value (Newton on the quadrature of the swept angle)
 //   y3, y4  logits of "the march reaches crossing 1 / 2"
 //
 // Phi_exact = pi + ahat(beta) - J(u_obs, b): the swept angle from the camera through periapsis to
 // infinity, 2 int_0^{u0} - int_0^{u_obs} of du / sqrt(1/b^2 - u^2 (1 - 2u)), u0 the periapsis.
 // ahat is the infinity-to-infinity deflection as a 25-term Chebyshev series in log(beta - 1)
-// (fitted once to the exact quadrature, error 6e-7 rad), J the 6-node Gauss-Legendre value of the
-// finite-distance part after the substitution u = u0 (1 - t^2) (error < 1e-7 rad for r_obs >= 8).
+// (fitted once to the exact quadrature, error 6e-7 rad), J the 8-node Gauss-Legendre value of the
+// finite-distance part after the substitution u = u0 (1 - t^2). The same quadratures, inverted by
+// Newton steps, give the exact crossing radii (relative error < 2e-6 against a 16-node reference).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,8 +51,8 @@ constexpr int kB4 = kW4 + kOut * kH;
 constexpr int kParams = kB4 + kOut;   // 34181
 
 constexpr float kResScale = 1e-3f;   // escape residual unit (rad)
-constexpr float kUMin = 1e-3f;            // crossing targets below this (r > 1000) are clamped
-constexpr float kBetaEps = 1e-3f;
+constexpr float kURes = 1e-3f;            // crossing residual unit: u = u_exact exp(1e-3 y)
+constexpr float kBetaEps = 0.02f;          // domain: |beta - 1| > 2% (the FP64 band)
 constexpr float kLogRLo = 2.0794415f;      // log 8
 constexpr float kLogRHi = 6.9077553f;      // log 1000
 constexpr float kPi = 3.14159265358979f;
@@ -85,34 +87,94 @@ __device__ __forceinline__ float ahat(float bm) {
     return __builtin_fmaf(t, b1, kAhatC[0] - b2);
 }
 
-// J = int_0^{u_obs} du / sqrt(1/b^2 - u^2 (1 - 2u)) for b > b_crit, u_obs below the periapsis u0.
-// With the roots u0 < u1 and u2 < 0 of 2u^3 - u^2 + 1/b^2 (u0 = 1/6 + cos(theta/3 - 2pi/3)/3,
-// cos theta = 1 - 54/b^2), the radicand is (u0 - u) R(u), R = 2(-u^2 + (1/2 - u0) u + 1/(2 b^2 u0)),
-// and u = u0 (1 - t^2) turns the integral into int_{t_obs}^1 2 sqrt(u0) / sqrt(R) dt, smooth in t.
-__device__ __forceinline__ float j_near(float b, float u_obs) {
-    constexpr float xg[3] = {0.2386191860831969f, 0.6612093864662645f, 0.9324695142031521f};
-    constexpr float wg[3] = {0.4679139345726910f, 0.3607615730481386f, 0.1713244923791704f};
+// 8-node Gauss-Legendre rule on [-1, 1] (symmetric half).
+constexpr float kGLx[4] = {0.1834346424956498f, 0.5255324099163290f, 0.7966664774136267f,
+                           0.9602898564975363f};
+constexpr float kGLw[4] = {0.3626837833783620f, 0.3137066458778873f, 0.2223810344533745f,
+                           0.1012285362903763f};
+
+// Escaping side (b > b_crit) of du/dphi = sqrt(Q), Q = 1/b^2 - u^2 (1 - 2u). With the roots
+// u0 < u1 and u2 < 0 of 2u^3 - u^2 + 1/b^2 (u0 = 1/6 + cos(theta/3 - 2pi/3)/3,
+// cos theta = 1 - 54/b^2), Q = (u0 - u) R(u), R = 2(-u^2 + (1/2 - u0) u + 1/(2 b^2 u0)), and
+// u = u0 (1 - t^2) turns the swept angle from periapsis into G(t) = int_0^t g(s) ds with the smooth
+// g(s) = 2 sqrt(u0) / sqrt(R(u0 (1 - s^2))); G(1) = I0 is half the total sweep.
+struct OrbitE {
+    float u0, cu, k0, g0;
+};
+
+__device__ __forceinline__ OrbitE orbit_e(float b) {
+    OrbitE o;
     const float ib2 = 1.0f / (b * b);
     float ct = 1.0f - 54.0f * ib2;
     ct = ct < -1.0f ? -1.0f : ct;
     const float th = acosf(ct);
-    const float u0 = (1.0f / 6.0f) + cosf(th * (1.0f / 3.0f) - 2.0943951023931953f) * (1.0f / 3.0f);
-    const float q = 1.0f - u_obs / u0;
-    const float t0 = sqrtf(q > 0.0f ? q : 0.0f);
-    const float half = 0.5f * (1.0f - t0), mid = 0.5f * (1.0f + t0);
-    const float cu = 0.5f - u0, k0 = 0.5f * ib2 / u0;
+    o.u0 = (1.0f / 6.0f) + cosf(th * (1.0f / 3.0f) - 2.0943951023931953f) * (1.0f / 3.0f);
+    o.cu = 0.5f - o.u0;
+    o.k0 = 0.5f * ib2 / o.u0;
+    o.g0 = 2.0f * sqrtf(o.u0);
+    return o;
+}
+
+__device__ __forceinline__ float g_e(const OrbitE& o, float s) {
+    const float u = o.u0 * __builtin_fmaf(-s, s, 1.0f);
+    return o.g0 * __builtin_amdgcn_rsqf(2.0f * __builtin_fmaf(u, o.cu - u, o.k0));
+}
+
+// int_a^b g(s) ds (8-node Gauss-Legendre).
+__device__ __forceinline__ float g_e_int(const OrbitE& o, float a, float b) {
+    const float half = 0.5f * (b - a), mid = 0.5f * (b + a);
     float s = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i)
+        s = __builtin_fmaf(kGLw[i],
+                           g_e(o, __builtin_fmaf(-kGLx[i], half, mid)) +
+                               g_e(o, __builtin_fmaf(kGLx[i], half, mid)),
+                           s);
+    return s * half;
+}
+
+// J = int_0^{u_obs} du / sqrt(Q) = int_{t_obs}^1 g(s) ds, t_obs = sqrt(1 - u_obs/u0).
+__device__ __forceinline__ float t_of_u(const OrbitE& o, float u) {
+    const float q = 1.0f - u / o.u0;
+    return sqrtf(q > 0.0f ? q : 0.0f);
+}
+
+// u at swept angle Th from periapsis (0 <= Th < I0): Newton on G(t) = Th from t = Th / I0.
+__device__ __forceinline__ float u_periapsis(const OrbitE& o, float I0, float Th) {
+    float t = Th / I0;
 #pragma unroll
-        for (int sg = -1; sg <= 1; sg += 2) {
-            const float t = __builtin_fmaf((float)sg * xg[i], half, mid);
-            const float u = u0 * __builtin_fmaf(-t, t, 1.0f);
-            const float R = 2.0f * __builtin_fmaf(u, cu - u, k0);
-            s = __builtin_fmaf(wg[i], __builtin_amdgcn_rsqf(R), s);
-        }
+    for (int it = 0; it < 2; ++it) {
+        t -= (g_e_int(o, 0.0f, t) - Th) / g_e(o, t);
+        t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
     }
-    return s * half * 2.0f * sqrtf(u0);
+    return o.u0 * __builtin_fmaf(-t, t, 1.0f);
+}
+
+// Captured side (b < b_crit): Q = q_min + d^2 + 2 d^3 with d = u - 1/3, q_min = 1/b^2 - 1/27 > 0.
+// d = a sinh(tau), a = sqrt(q_min), makes the swept angle int h(tau) dtau with the smooth
+// h = 1 / sqrt(1 + 2 a sinh^3(tau) / cosh^2(tau)).
+__device__ __forceinline__ float h_c(float a, float tau) {
+    const float e = __expf(tau), ie = __builtin_amdgcn_rcpf(e);
+    const float sh = 0.5f * (e - ie), ch = 0.5f * (e + ie);
+    return __builtin_amdgcn_rsqf(__builtin_fmaf(2.0f * a * sh * sh, sh * __builtin_amdgcn_rcpf(ch * ch), 1.0f));
+}
+
+__device__ __forceinline__ float h_c_int(float a, float lo, float hi) {
+    const float half = 0.5f * (hi - lo), mid = 0.5f * (hi + lo);
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        s = __builtin_fmaf(kGLw[i],
+                           h_c(a, __builtin_fmaf(-kGLx[i], half, mid)) +
+                               h_c(a, __builtin_fmaf(kGLx[i], half, mid)),
+                           s);
+    return s * half;
+}
+
+__device__ __forceinline__ float asinh_f(float x) {
+    const float ax = fabsf(x);
+    const float r = logf(ax + sqrtf(__builtin_fmaf(ax, ax, 1.0f)));
+    return x < 0.0f ? -r : r;
 }
 
 // Geometry of one camera ray for the surrogate.
@@ -162,14 +224,54 @@ __device__ __forceinline__ RayGeom ray_geom(const V3<float>& P, const V3<float>&
     return g;
 }
 
-// Exact Schwarzschild swept angle from the camera to infinity (escaping side, bm > 0).
-__device__ __forceinline__ float phi_exact(const RayGeom& g) {
-    return kPi + ahat(g.bm) - j_near(g.b, 1.0f / g.r);
+// Exact Schwarzschild orbit of a ray in the surrogate's domain: total swept angle to infinity
+// (escaping) or to the horizon test radius 2.02 (captured), and the radii of its equatorial
+// crossings at swept angles psi + k pi.
+struct Orbit {
+    bool escaping;
+    float phi_end;     // swept angle at escape (infinity) / at r = 2.02
+    // escaping
+    OrbitE e;
+    float I0, th_obs;  // half sweep, swept angle from camera to periapsis
+    // captured
+    float a, tau_obs;
+};
+
+__device__ __forceinline__ Orbit orbit(const RayGeom& g) {
+    Orbit o;
+    o.escaping = g.bm > 0.0f;
+    const float u_obs = 1.0f / g.r;
+    if (o.escaping) {
+        o.e = orbit_e(g.b);
+        o.I0 = g_e_int(o.e, 0.0f, 1.0f);
+        o.th_obs = g_e_int(o.e, 0.0f, t_of_u(o.e, u_obs));
+        // pi + ahat: the infinity-to-infinity sweep from its Chebyshev series (accurate to the
+        // band edge); the camera's part from the quadrature
+        o.phi_end = kPi + ahat(g.bm) - (o.I0 - o.th_obs);
+    } else {
+        const float ib2 = 1.0f / (g.b * g.b);
+        o.a = sqrtf(fmaxf(ib2 - (1.0f / 27.0f), 1e-12f));
+        const float ia = 1.0f / o.a;
+        o.tau_obs = asinh_f((u_obs - (1.0f / 3.0f)) * ia);
+        const float tau_h = asinh_f(((1.0f / 2.02f) - (1.0f / 3.0f)) * ia);
+        o.phi_end = h_c_int(o.a, o.tau_obs, tau_h);
+    }
+    return o;
 }
 
-// Crossing target encoding and its inverse.
-__device__ __forceinline__ float u_encode(float u) { return logf(8.0f * (u > kUMin ? u : kUMin)); }
-__device__ __forceinline__ float u_decode(float y) { return 0.125f * __expf(y); }
+// u at swept angle phi from the camera (phi < phi_end).
+__device__ __forceinline__ float orbit_u(const Orbit& o, float phi) {
+    if (o.escaping) {
+        const float th = fabsf(phi - o.th_obs);   // ingoing before periapsis, outgoing after
+        return u_periapsis(o.e, o.I0, th < o.I0 ? th : o.I0);
+    }
+    float tau = o.tau_obs + phi;
+#pragma unroll
+    for (int it = 0; it < 3; ++it) tau -= (h_c_int(o.a, o.tau_obs, tau) - phi) / h_c(o.a, tau);
+    const float e = __expf(tau);
+    return (1.0f / 3.0f) + o.a * 0.5f * (e - __builtin_amdgcn_rcpf(e));
+}
+
 
 __device__ __forceinline__ void features(const RayGeom& g, float x[3]) {
     x[0] = (logf(g.r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;

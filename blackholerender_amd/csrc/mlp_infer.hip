@@ -206,33 +206,30 @@ __device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k,
     return (float)k.steps_max[r_bin(g.r)] * k.steps_slack < (float)c.max_steps;
 }
 
-// Output record of one in-domain ray from the network outputs y, or serve = false when the ray must
-// go back to the integrator: reach undecided, a crossing radius within the margin of an annulus
-// edge, a possible third crossing that could be recorded, or captured within 1% of b_crit.
+// Output record of one in-domain ray from the exact orbit and the network outputs y, or
+// serve = false when the ray must go back to the integrator: a crossing radius within the margin
+// of an annulus edge, or a third crossing that could be recorded.
 __device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, const Setup<float>& s,
                                       const RayGeom& g, const float y[5]) {
     Dec o;
     o.serve = false;
     o.escaping = g.bm > 0.0f;
     o.n_cross = 0;
-    float Phi = 0.0f;
-    if (o.escaping) {
-        Phi = phi_exact(g) + kResScale * y[0];
-        const float a = Phi - g.psi;
-        o.n_eq = a <= 0.0f ? 0u : (a <= kPi ? 1u : (a <= 2.0f * kPi ? 2u : 3u));
-    } else {
-        if (g.bm > -0.01f) return o;
-        if (fabsf(y[3]) < k.logit_margin || fabsf(y[4]) < k.logit_margin) return o;
-        const bool r1 = y[3] > 0.0f, r2 = y[4] > 0.0f;
-        if (r2 && !r1) return o;
-        o.n_eq = (r1 ? 1u : 0u) + (r2 ? 1u : 0u);
-    }
+    const Orbit ob = orbit(g);
+    // validity of the crossings from the exact orbit's total sweep (to infinity with the learned
+    // residual, or to the horizon test radius)
+    const float Phi = ob.phi_end + (o.escaping ? kResScale * y[0] : 0.0f);
+    const float a = Phi - g.psi;
+    o.n_eq = a <= 0.0f ? 0u : (a <= kPi ? 1u : (a <= 2.0f * kPi ? 2u : 3u));
     bool in[2] = {false, false};
     const float margin = k.radius_margin[r_bin(g.r)];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        o.r_eq[q] = 1.0f / u_decode(y[1 + q]);
-        if ((uint32_t)q < o.n_eq && c.thin) {
+        o.r_eq[q] = 1e30f;
+        if ((uint32_t)q >= o.n_eq) continue;
+        const float u = orbit_u(ob, g.psi + (float)q * kPi) * __expf(kURes * y[1 + q]);
+        o.r_eq[q] = u > 0.0f ? 1.0f / u : 1e30f;
+        if (c.thin) {
             const float r = o.r_eq[q];
             if (fabsf(r - c.disk_inner) < margin * c.disk_inner ||
                 fabsf(r - c.disk_outer) < margin * c.disk_outer)

@@ -16,7 +16,8 @@ __device__ __forceinline__ float urand(uint32_t key, uint32_t i, uint32_t k) {
 
 // ------------------------------------------------------------------------------------------------
 // Training rays: r_obs log-uniform in [8, 1000] M; beta = b/b_crit log-spaced on both sides of 1
-// (40% above: 1 + 10^U(-3, log10(beta_max - 1)); 20% below: 1 - 10^U(-3, log10 0.7)) plus 40%
+// (40% above: 1 + 10^U(log10 0.02, log10(beta_max - 1)); 20% below: 1 - 10^U(log10 0.02,
+// log10 0.7)) plus 40%
 // uniform in [0.3, beta_max]; b at most 0.95 of the tangential limit r/sqrt(1 - 2/r); the first
 // equatorial crossing at swept angle psi uniform in [0, pi); the orbit plane tilted by U(15, 89) deg
 // about a random line of nodes. The view direction is ingoing with impact parameter b.
@@ -33,9 +34,9 @@ __global__ void k_sample_rays(int64_t n, uint32_t key, float* __restrict__ pos3,
     float beta;
     if (u < 0.4f) {
         const float tmax = log10f(beta_max - 1.0f);
-        beta = 1.0f + exp10f(-3.0f + (tmax + 3.0f) * urand(key, ii, 2));
+        beta = 1.0f + exp10f(-1.69897f + (tmax + 1.69897f) * urand(key, ii, 2));
     } else if (u < 0.6f) {
-        beta = 1.0f - exp10f(-3.0f + 2.8450980f * urand(key, ii, 2));   // 10^-3 .. 0.7
+        beta = 1.0f - exp10f(-1.69897f + 1.5440680f * urand(key, ii, 2));   // 0.02 .. 0.7
     } else {
         beta = 0.3f + (beta_max - 0.3f) * urand(key, ii, 2);
     }
@@ -80,25 +81,38 @@ __global__ void k_targets(int64_t n, const float* __restrict__ pos3, const float
     const uint32_t fl = out.flags[i], nc = out.n_cross[i];
     const bool maxed = (fl & 1u) != 0;
     const bool escaped = (fl & 2u) != 0 && g.bm > 0.0f;
-    float y0 = 0.0f;
-    if (escaped && dom) {
-        const V3<float> e{out.esc[i], out.esc[i + out.n], out.esc[i + 2 * out.n]};
-        const float phim = atan2f(dot3(e, g.e2), dot3(e, g.e1));
-        const float base = phi_exact(g);
-        const float k = rintf((base - phim) * (0.5f / kPi));
-        y0 = (phim + 2.0f * kPi * k - base) * (1.0f / kResScale);
+    float y0 = 0.0f, yu[2] = {0.0f, 0.0f}, mu[2] = {0.0f, 0.0f};
+    if (dom) {
+        const Orbit ob = orbit(g);
+        if (escaped) {
+            const V3<float> e{out.esc[i], out.esc[i + out.n], out.esc[i + 2 * out.n]};
+            const float phim = atan2f(dot3(e, g.e2), dot3(e, g.e1));
+            const float k = rintf((ob.phi_end - phim) * (0.5f / kPi));
+            y0 = (phim + 2.0f * kPi * k - ob.phi_end) * (1.0f / kResScale);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const float phi = g.psi + (float)q * kPi;
+            if ((uint32_t)q >= nc || !(phi < ob.phi_end)) continue;
+            const float ue = orbit_u(ob, phi);
+            const float ut = 1.0f / out.disk[i + q * out.n];
+            const float t = logf(ut / ue) * (1.0f / kURes);
+            if (ue > 0.0f && fabsf(t) < 1000.0f) {
+                yu[q] = t;
+                mu[q] = 1.0f;
+            }
+        }
     }
-    const float r0 = out.disk[i], r1 = out.disk[i + out.n];
     const float m = dom ? 1.0f : 0.0f;
     for (int k = 0; k < 3; ++k) X[3 * i + k] = x[k];
     Y[5 * i + 0] = y0;
-    Y[5 * i + 1] = nc >= 1 ? u_encode(1.0f / r0) : 0.0f;
-    Y[5 * i + 2] = nc >= 2 ? u_encode(1.0f / r1) : 0.0f;
+    Y[5 * i + 1] = yu[0];
+    Y[5 * i + 2] = yu[1];
     Y[5 * i + 3] = nc >= 1 ? 1.0f : 0.0f;
     Y[5 * i + 4] = nc >= 2 ? 1.0f : 0.0f;
     M[5 * i + 0] = (escaped && !maxed) ? m : 0.0f;
-    M[5 * i + 1] = nc >= 1 ? m : 0.0f;
-    M[5 * i + 2] = nc >= 2 ? m : 0.0f;
+    M[5 * i + 1] = mu[0] * m;
+    M[5 * i + 2] = mu[1] * m;
     M[5 * i + 3] = !maxed ? m : 0.0f;    // out-of-steps rays: reach unknown
     M[5 * i + 4] = !maxed ? m : 0.0f;
     if (dom && steps_max) {

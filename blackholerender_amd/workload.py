@@ -45,6 +45,25 @@ C2 = dict(name="c2_1920x1080", r_obs=30.0, inclination_deg=85.0, fov_deg=60.0, w
           seed=20260601)
 
 
+# The full form: 3840x2160, 8 jittered samples per pixel, 4 simultaneous camera poses (azimuths 0,
+# 90, 180, 270 deg), r_obs = 15 M, inclination 80 deg, vertical FOV 60 deg, Ultra preset (700 steps,
+# step scale 0.035, disk step 0.10), thin disk [6, 30] M: 32 cameras, 265,420,800 rays.
+C5 = dict(name="c5_3840x2160_8spp_4poses", r_obs=15.0, inclination_deg=80.0, fov_deg=60.0,
+          width=3840, height=2160, max_steps=700, disk_inner=6.0, disk_outer=30.0,
+          jitter_hashed=1, seed=20260605, spp=8, azimuths=(0.0, 90.0, 180.0, 270.0))
+
+
+def cameras_c5(cfg=C5, width=None, height=None):
+    """The 32 cameras of the full form (pose-major, sample-minor; camera_id = pose)."""
+    from .camera import orbit_camera
+    w = width or cfg["width"]
+    h = height or cfg["height"]
+    return [orbit_camera(cfg["r_obs"], cfg["inclination_deg"], cfg["fov_deg"], w, h,
+                         jitter_hashed=1, seed=cfg["seed"], camera_id=p, sample=s,
+                         azimuth_deg=az)
+            for p, az in enumerate(cfg["azimuths"]) for s in range(cfg["spp"])]
+
+
 def camera_for(cfg, **over):
     from .camera import orbit_camera
     c = dict(cfg)
