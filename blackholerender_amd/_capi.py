@@ -132,11 +132,13 @@ def load():
     L.bhs_mlp_load.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
     L.bhs_mlp_load.restype = vp
     L.bhs_mlp_set_table.argtypes = [vp, vp, i32, i32]
+    L.bhs_render_surrogate_batch.argtypes = [vp, vp, vp, vp, i32, C.POINTER(RaySoA), vp]
     L.bhs_render_surrogate.argtypes = [vp, vp, C.POINTER(Camera), C.POINTER(MarchParams),
                                        C.POINTER(RaySoA), vp]
     for name in ("bhs_mlp_make_dataset", "bhs_mlp_train", "bhs_mlp_grads", "bhs_mlp_forward",
                  "bhs_mlp_validate", "bhs_mlp_get_params", "bhs_mlp_set_params",
-                 "bhs_mlp_get_info", "bhs_mlp_save", "bhs_mlp_set_table", "bhs_render_surrogate"):
+                 "bhs_mlp_get_info", "bhs_mlp_save", "bhs_mlp_set_table", "bhs_render_surrogate",
+                 "bhs_render_surrogate_batch"):
         getattr(L, name).restype = C.c_int
     for name in ("bhs_sync", "bhs_trace_rays", "bhs_render", "bhs_march_rays", "bhs_last_stats",
                  "bhs_set_timing", "bhs_timing", "bhs_set_tuning", "bhs_get_tuning",

@@ -204,14 +204,6 @@ int prepare_scratch(bhs_ctx* ctx, int64_t n_rays, int64_t n_items, bool roots, b
 
 // Host-side camera table of a batch: constants, output offsets (prefix sums of width*height) and
 // item prefix sums. Checks every camera and its params.
-struct BatchTable {
-    std::vector<bhs::MarchConst> cams;
-    std::vector<uint32_t> item_base;
-    int64_t n_rays = 0;
-    uint32_t n_items = 0;
-    bool any_band = false, any_root = false;
-};
-
 int build_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* params, int32_t ncam,
                 BatchTable* t) {
     if (!cams || !params) return fail(ctx, BHS_E_INVALID, "batch: cameras or params is NULL%s");
@@ -256,6 +248,36 @@ void pack_table(const BatchTable& t, char* dst) {
     const int32_t ncam = (int32_t)t.cams.size();
     std::memcpy(dst, t.cams.data(), t.cams.size() * sizeof(bhs::MarchConst));
     std::memcpy(dst + table_cams_bytes(ncam), t.item_base.data(), t.item_base.size() * sizeof(uint32_t));
+}
+
+int upload_batch(bhs_ctx* ctx, const BatchTable& t, hipStream_t s, void** table_dev) {
+    const int32_t ncam = (int32_t)t.cams.size();
+    const size_t tb = table_bytes(ncam);
+    int st;
+    if ((st = ensure(ctx, &ctx->table_dev, &ctx->table_dev_bytes, tb, "camera table")) != BHS_OK)
+        return st;
+    if (ctx->table_host_bytes < tb) {
+        if (ctx->upload_pending) (void)hipEventSynchronize(ctx->ev_upload);
+        if (ctx->table_host) (void)hipHostFree(ctx->table_host);
+        ctx->table_host = nullptr;
+        ctx->table_host_bytes = 0;
+        if (hipHostMalloc(&ctx->table_host, tb, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->table_host = nullptr;
+            return fail(ctx, BHS_E_ALLOC, "pinned allocation of the camera table failed%s");
+        }
+        ctx->table_host_bytes = tb;
+    }
+    if (ctx->upload_pending) (void)hipEventSynchronize(ctx->ev_upload);   // staging buffer is free
+    pack_table(t, (char*)ctx->table_host);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(ctx->table_dev, ctx->table_host, tb, hipMemcpyHostToDevice, s)) !=
+            hipSuccess ||
+        (e = hipEventRecord(ctx->ev_upload, s)) != hipSuccess)
+        return hip_fail(ctx, e, "camera table upload");
+    ctx->upload_pending = true;
+    *table_dev = ctx->table_dev;
+    return BHS_OK;
 }
 
 }  // namespace capi
@@ -439,32 +461,14 @@ int bhs_render_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_param
     if (t.any_root && (!out->n_cross_dev || !out->flags_dev || !out->disk_dev))
         return fail(ctx, BHS_E_INVALID, "bhs_render_batch: root64 needs n_cross, flags and disk outputs%s");
     DeviceGuard g(ctx->device);
-    const size_t tb = table_bytes(ncam);
-    if ((st = ensure(ctx, &ctx->table_dev, &ctx->table_dev_bytes, tb, "camera table")) != BHS_OK) return st;
-    if (ctx->table_host_bytes < tb) {
-        if (ctx->upload_pending) (void)hipEventSynchronize(ctx->ev_upload);
-        if (ctx->table_host) (void)hipHostFree(ctx->table_host);
-        ctx->table_host = nullptr;
-        ctx->table_host_bytes = 0;
-        if (hipHostMalloc(&ctx->table_host, tb, 0) != hipSuccess) {
-            (void)hipGetLastError();
-            ctx->table_host = nullptr;
-            return fail(ctx, BHS_E_ALLOC, "pinned allocation of the camera table failed%s");
-        }
-        ctx->table_host_bytes = tb;
-    }
+    hipStream_t s = (hipStream_t)stream;
+    void* table_dev = nullptr;
+    if ((st = upload_batch(ctx, t, s, &table_dev)) != BHS_OK) return st;
     bhs::Scratch sc;
     if ((st = prepare_scratch(ctx, t.n_rays, t.n_items, t.any_root, &sc)) != BHS_OK) return st;
-    hipStream_t s = (hipStream_t)stream;
     hipError_t e;
-    if (ctx->upload_pending) (void)hipEventSynchronize(ctx->ev_upload);   // staging buffer is free
-    pack_table(t, (char*)ctx->table_host);
     bhs_ctx::TimingRec* tr = next_timer(ctx);
     if (tr) (void)hipEventRecord(tr->call0, s);
-    if ((e = hipMemcpyAsync(ctx->table_dev, ctx->table_host, tb, hipMemcpyHostToDevice, s)) != hipSuccess ||
-        (e = hipEventRecord(ctx->ev_upload, s)) != hipSuccess)
-        return hip_fail(ctx, e, "bhs_render_batch upload");
-    ctx->upload_pending = true;
     if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_render_batch clear");
     const bhs::MarchConst* table = (const bhs::MarchConst*)ctx->table_dev;

@@ -72,5 +72,20 @@ bhs::SoA make_soa(const bhs_ray_soa* o, int64_t n);
 bhs_ctx::TimingRec* next_timer(bhs_ctx* ctx);
 int prepare_scratch(bhs_ctx* ctx, int64_t n_rays, int64_t n_items, bool roots, bhs::Scratch* sc);
 
+// Host-side camera table of a batch: constants, output offsets (prefix sums of width*height) and
+// work-item prefix sums (camera_items per camera).
+struct BatchTable {
+    std::vector<bhs::MarchConst> cams;
+    std::vector<uint32_t> item_base;
+    int64_t n_rays = 0;
+    uint32_t n_items = 0;
+    bool any_band = false, any_root = false;
+};
+int build_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* params, int32_t ncam,
+                BatchTable* t);
+size_t table_cams_bytes(int32_t ncam);
+// Stream-ordered upload of the table into the context's device table (pinned staging).
+int upload_batch(bhs_ctx* ctx, const BatchTable& t, hipStream_t s, void** table_dev);
+
 }  // namespace capi
 }  // namespace bhs

@@ -37,7 +37,7 @@ struct bhs_mlp {
     // fallback lists of the surrogate render: [counters (256 B) | late ids | early ids]
     void* fb = nullptr;
     size_t fb_bytes = 0;
-    // table mode: [x2 range (256 B) | n2 n3 nodes x 8 floats]; tab_n2 = 0: per-ray inference
+    // table mode: [x2 ranges (512 B) | ncam n2 n3 nodes x 8 floats]; tab_n2 = 0: per-ray inference
     void* tab = nullptr;
     size_t tab_bytes = 0;
     int32_t tab_n2 = 0, tab_n3 = 0;
@@ -152,6 +152,30 @@ float quantile(std::vector<float>& v, double q) {
     const size_t k = std::min(v.size() - 1, (size_t)(q * (double)(v.size() - 1) + 0.5));
     std::nth_element(v.begin(), v.begin() + (long)k, v.end());
     return v[k];
+}
+
+// Table-mode buffers for ncam cameras (tab->n2 = 0 when the model runs per ray).
+int table_for(bhs_ctx* ctx, bhs_mlp* m, int32_t ncam, M::SurTable* tab) {
+    *tab = M::SurTable{nullptr, nullptr, 0, 0};
+    if (m->tab_n2 <= 0) return BHS_OK;
+    const size_t nodes = (size_t)m->tab_n2 * (size_t)m->tab_n3 * (size_t)ncam;
+    int st;
+    if ((st = grow(ctx, &m->tab, &m->tab_bytes, 512 + nodes * 32, "surrogate table")) != BHS_OK)
+        return st;
+    *tab = M::SurTable{(float*)((char*)m->tab + 512), (float*)m->tab, m->tab_n2, m->tab_n3};
+    return BHS_OK;
+}
+
+// Fallback lists for n_items work items: [counters (256 B) | late ids | early ids].
+int lists_for(bhs_ctx* ctx, bhs_mlp* m, uint32_t n_items, M::SurLists* fl) {
+    int st;
+    if ((st = grow(ctx, &m->fb, &m->fb_bytes, 256 + (size_t)n_items * 8, "fallback lists")) !=
+        BHS_OK)
+        return st;
+    uint32_t* cnt = (uint32_t*)m->fb;
+    uint32_t* late = (uint32_t*)((char*)m->fb + 256);
+    *fl = M::SurLists{late, cnt, late + n_items, cnt + 1, cnt + 2, n_items};
+    return BHS_OK;
 }
 
 // File header of a saved model.
@@ -646,21 +670,12 @@ int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
     const bool roots = params->root64 && params->precision != BHS_PREC_F64;
     bhs::Scratch sc;
     if ((st = prepare_scratch(ctx, n, n_items, roots, &sc)) != BHS_OK) return st;
-    if ((st = grow(ctx, &m->fb, &m->fb_bytes, 256 + (size_t)n_items * 8, "fallback lists")) !=
-        BHS_OK)
-        return st;
-    uint32_t* cnt = (uint32_t*)m->fb;
-    uint32_t* late = (uint32_t*)((char*)m->fb + 256);
-    M::SurLists fl{late, cnt, late + n_items, cnt + 1, cnt + 2, n_items};
+    M::SurLists fl;
+    if ((st = lists_for(ctx, m, n_items, &fl)) != BHS_OK) return st;
     const M::SurStreams ss{ctx->aux, ctx->aux2, ctx->ev_fork, ctx->ev_join, ctx->ev_fork2,
                            ctx->ev_join2};
     M::SurTable tab{};
-    if (m->tab_n2 > 0) {
-        const size_t nodes = (size_t)m->tab_n2 * (size_t)m->tab_n3;
-        if ((st = grow(ctx, &m->tab, &m->tab_bytes, 256 + nodes * 32, "surrogate table")) != BHS_OK)
-            return st;
-        tab = M::SurTable{(float*)((char*)m->tab + 256), (int32_t*)m->tab, m->tab_n2, m->tab_n3};
-    }
+    if ((st = table_for(ctx, m, 1, &tab)) != BHS_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     if ((st = pack(ctx, m, s)) != BHS_OK) return st;
     hipError_t e;
@@ -669,11 +684,62 @@ int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
     if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess ||
         (e = bhs::launch_zero(m->fb, 256, s)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_render_surrogate clear");
-    if ((e = M::launch_surrogate(c, m->k, m->packed, m->tab_n2 > 0 ? &tab : nullptr,
+    if ((e = M::launch_surrogate(c, m->k, m->packed, tab.n2 > 0 ? &tab : nullptr,
                                  make_soa(out, n), sc, fl, ctx->tuning, ctx->num_cus, s, ss,
                                  tr ? tr->k0 : nullptr, tr ? tr->k1 : nullptr, nullptr)) !=
         hipSuccess)
         return hip_fail(ctx, e, "bhs_render_surrogate launch");
+    if (tr) (void)hipEventRecord(tr->call1, s);
+    ctx->last_stream = s;
+    return BHS_OK;
+}
+
+int bhs_render_surrogate_batch(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cams,
+                               const bhs_march_params* params, int32_t ncam,
+                               const bhs_ray_soa* out, void* stream) {
+    if (!ctx || !m) return BHS_E_INVALID;
+    if (!out) return fail(ctx, BHS_E_INVALID, "bhs_render_surrogate_batch: out is NULL%s");
+    BatchTable t;
+    int st;
+    if ((st = build_batch(ctx, cams, params, ncam, &t)) != BHS_OK) return st;
+    for (int32_t k = 0; k < ncam; ++k) {
+        if (params[k].cross_all)
+            return fail(ctx, BHS_E_INVALID, "bhs_render_surrogate_batch: cross_all must be 0%s");
+        if (!same_march(params[k], m->trained_for))
+            return fail(ctx, BHS_E_INVALID,
+                        "bhs_render_surrogate_batch: the model was trained for other step-control "
+                        "/ disk parameters%s");
+    }
+    if (t.any_root && (!out->n_cross_dev || !out->flags_dev || !out->disk_dev))
+        return fail(ctx, BHS_E_INVALID,
+                    "bhs_render_surrogate_batch: root64 needs n_cross, flags and disk outputs%s");
+    DeviceGuard g(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    void* table_dev = nullptr;
+    if ((st = upload_batch(ctx, t, s, &table_dev)) != BHS_OK) return st;
+    bhs::Scratch sc;
+    if ((st = prepare_scratch(ctx, t.n_rays, t.n_items, t.any_root, &sc)) != BHS_OK) return st;
+    M::SurLists fl;
+    if ((st = lists_for(ctx, m, t.n_items, &fl)) != BHS_OK) return st;
+    M::SurTable tab;
+    if ((st = table_for(ctx, m, ncam, &tab)) != BHS_OK) return st;
+    const M::SurStreams ss{ctx->aux, ctx->aux2, ctx->ev_fork, ctx->ev_join, ctx->ev_fork2,
+                           ctx->ev_join2};
+    if ((st = pack(ctx, m, s)) != BHS_OK) return st;
+    hipError_t e;
+    bhs_ctx::TimingRec* tr = next_timer(ctx);
+    if (tr) (void)hipEventRecord(tr->call0, s);
+    if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess ||
+        (e = bhs::launch_zero(m->fb, 256, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_render_surrogate_batch clear");
+    const bhs::MarchConst* table = (const bhs::MarchConst*)table_dev;
+    const uint32_t* base = (const uint32_t*)((char*)table_dev + table_cams_bytes(ncam));
+    if ((e = M::launch_surrogate_batch(table, base, (uint32_t)ncam, t.n_items, t.any_band, m->k,
+                                       m->packed, tab.n2 > 0 ? &tab : nullptr,
+                                       make_soa(out, t.n_rays), sc, fl, ctx->tuning,
+                                       ctx->num_cus, s, ss, tr ? tr->k0 : nullptr,
+                                       tr ? tr->k1 : nullptr, nullptr)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_render_surrogate_batch launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
     ctx->last_stream = s;
     return BHS_OK;

@@ -368,10 +368,11 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
                     (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
                 const uint32_t q = base + rank;
                 if (q < n_items) {
-                    uint32_t local = list ? list[q] : q;
+                    const uint32_t item = list ? list[q] : q;
+                    uint32_t local = item;
                     if (BATCH) {
-                        ci = find_cam(s_base, ncam, q);
-                        local = q - s_base[ci];
+                        ci = find_cam(s_base, ncam, item);
+                        local = item - s_base[ci];
                     }
                     const MarchConst& c = BATCH ? s_cam[ci] : c1;
                     const int64_t p = item_pixel(c, local);
@@ -679,6 +680,38 @@ hipError_t launch_root_refine(const MarchConst& c, const SoA& out, const Scratch
     if (c.precision == BHS_PREC_F64 || !c.root64 || !sc.roots) return hipSuccess;
     const OneCam one{c, camera_items(c.width, c.height)};
     hipLaunchKernelGGL(k_root64<OneCam>, item_grid(one.n_items), dim3(kBlock), 0, s, one, out, sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_list_batch(const MarchConst* table, const uint32_t* item_base, uint32_t ncam,
+                             const uint32_t* ids, const uint32_t* count, uint32_t cap,
+                             const SoA& out, const Scratch& sc, const Tuning& t, int num_cus,
+                             hipStream_t s) {
+    MarchConst none;
+    __builtin_memset(&none, 0, sizeof none);
+    return launch_persist<true>(t, num_cus, s, none, table, item_base, ncam, cap, out, sc, ids,
+                                count);
+}
+
+hipError_t launch_band_list_batch(const MarchConst* table, const uint32_t* item_base,
+                                  uint32_t ncam, uint32_t n_items, const SoA& out,
+                                  const Scratch& sc, hipStream_t s, hipStream_t aux,
+                                  hipEvent_t ev_fork, hipEvent_t ev_join) {
+    const CamTable cams{table, item_base, ncam, n_items};
+    hipError_t e;
+    if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_band64<CamTable>, dim3(256), dim3(64), 0, aux, cams, out, sc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipEventRecord(ev_join, aux);
+}
+
+hipError_t launch_root_refine_batch(const MarchConst* table, const uint32_t* item_base,
+                                    uint32_t ncam, uint32_t n_items, const SoA& out,
+                                    const Scratch& sc, hipStream_t s) {
+    if (!sc.roots) return hipSuccess;
+    const CamTable cams{table, item_base, ncam, n_items};
+    hipLaunchKernelGGL(k_root64<CamTable>, item_grid(n_items), dim3(kBlock), 0, s, cams, out, sc);
     return hipGetLastError();
 }
 

@@ -77,6 +77,18 @@ hipError_t launch_band_fork(const MarchConst& c, const SoA& out, const Scratch& 
                             hipStream_t aux, hipEvent_t ev_fork, hipEvent_t ev_join);
 // FP64 root refinement of the crossings of integrator-marched FP32 rays (root64 cameras).
 hipError_t launch_root_refine(const MarchConst& c, const SoA& out, const Scratch& sc, hipStream_t s);
+// Batch versions (device camera table and item prefix sums; items are batch-global).
+hipError_t launch_list_batch(const MarchConst* table, const uint32_t* item_base, uint32_t ncam,
+                             const uint32_t* ids, const uint32_t* count, uint32_t cap,
+                             const SoA& out, const Scratch& sc, const Tuning& t, int num_cus,
+                             hipStream_t s);
+hipError_t launch_band_list_batch(const MarchConst* table, const uint32_t* item_base,
+                                  uint32_t ncam, uint32_t n_items, const SoA& out,
+                                  const Scratch& sc, hipStream_t s, hipStream_t aux,
+                                  hipEvent_t ev_fork, hipEvent_t ev_join);
+hipError_t launch_root_refine_batch(const MarchConst* table, const uint32_t* item_base,
+                                    uint32_t ncam, uint32_t n_items, const SoA& out,
+                                    const Scratch& sc, hipStream_t s);
 hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, const MarchConst& c,
                              const SoA& out, const Scratch& sc, hipStream_t s, hipEvent_t ev_begin,
                              hipEvent_t ev_end);

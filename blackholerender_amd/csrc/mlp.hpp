@@ -81,11 +81,11 @@ struct SurStreams {
     hipStream_t aux, aux2;
     hipEvent_t ev_fork, ev_join, ev_fork2, ev_join2;
 };
-// Table mode: the network on an n2 x n3 grid over (x2, x3) at the camera's r_obs (node: n2 n3 x 8
-// floats; range: 2 ints of device scratch).
+// Table mode: per camera, the network on an n2 x n3 grid over (x2, x3) at the camera's r_obs
+// (node: ncam n2 n3 x 8 floats; range: 2 ncam floats of device scratch).
 struct SurTable {
     float* node;
-    int32_t* range;
+    float* range;
     int32_t n2, n3;
 };
 // tab null: per-ray network inference. ev_begin / ev_mid / ev_end (may be null) bracket the
@@ -95,6 +95,16 @@ hipError_t launch_surrogate(const MarchConst& c, const SurConst& k, const void* 
                             const SurLists& fl, const Tuning& t, int num_cus, hipStream_t s,
                             const SurStreams& ss, hipEvent_t ev_begin, hipEvent_t ev_mid,
                             hipEvent_t ev_end);
+
+// Batch of cameras (device camera table and item prefix sums as bhs_render_batch; no FP64
+// precision).
+hipError_t launch_surrogate_batch(const MarchConst* table, const uint32_t* item_base,
+                                  uint32_t ncam, uint32_t n_items, bool any_band,
+                                  const SurConst& k, const void* packed, const SurTable* tab,
+                                  const SoA& out, const Scratch& sc, const SurLists& fl,
+                                  const Tuning& t, int num_cus, hipStream_t s,
+                                  const SurStreams& ss, hipEvent_t ev_begin, hipEvent_t ev_mid,
+                                  hipEvent_t ev_end);
 
 }  // namespace mlp
 }  // namespace bhs

@@ -13,7 +13,7 @@
 //
 // Inputs:
 //   x1 = log r_obs mapped from [log 8, log 1000] to [-1, 1]
-//   x2 = sign(beta - 1) log(1 + |beta - 1| / 1e-3) / log(1 + 1e4), beta = b / b_crit
+//   x2 = sign(beta - 1) log(1 + |beta - 1| / 0.02) / log(1 + 1e4), beta = b / b_crit
 //   x3 = psi mapped from [0, pi) to [-1, 1)
 // Outputs:
 //   y0  escape: residual of the law-S escape angle over the exact Schwarzschild swept angle,
@@ -273,10 +273,15 @@ __device__ __forceinline__ float orbit_u(const Orbit& o, float phi) {
 }
 
 
+// Input feature of beta - 1 (scale 0.02, normalised by log(1 + 1e4)).
+__device__ __forceinline__ float x2_of(float bm) {
+    const float m = log1pf(fabsf(bm) * (1.0f / kBetaEps)) * (1.0f / 9.2104404f);
+    return bm < 0.0f ? -m : m;
+}
+
 __device__ __forceinline__ void features(const RayGeom& g, float x[3]) {
     x[0] = (logf(g.r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
-    const float m = log1pf(fabsf(g.bm) * (1.0f / kBetaEps)) * (1.0f / 9.2104404f);   // log(1+1e4)
-    x[1] = g.bm < 0.0f ? -m : m;
+    x[1] = x2_of(g.bm);
     x[2] = g.psi * (2.0f / kPi) - 1.0f;
 }
 

@@ -3,6 +3,8 @@
 // and validation against integrator outputs.
 #include "mlp.hpp"
 
+#include <type_traits>
+
 #include "../../include/bhs_trace.h"
 
 namespace bhs {
@@ -364,14 +366,30 @@ __device__ __forceinline__ int64_t item_pixel(const MarchConst& c, uint32_t q) {
 
 constexpr int kBlock = 256;
 
-// Order-preserving map of floats to ints (for atomic min/max) and back.
-__device__ __forceinline__ int32_t ord_int(float f) {
-    const int32_t i = __float_as_int(f);
-    return i >= 0 ? i : i ^ 0x7FFFFFFF;
-}
-__device__ __forceinline__ float ord_float(int32_t i) {
-    return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF);
-}
+// Camera sources of the surrogate kernels: one camera by value, or the device camera table of a
+// batch with the prefix sums of the per-camera work items (all multiples of 64, so a wave's 64
+// items always belong to one camera).
+struct SurOneCam {
+    MarchConst c;
+    __device__ __forceinline__ uint32_t find(uint32_t) const { return 0u; }
+    __device__ __forceinline__ const MarchConst& cam(uint32_t) const { return c; }
+    __device__ __forceinline__ uint32_t base(uint32_t) const { return 0u; }
+};
+struct SurCamTable {
+    const MarchConst* cams;
+    const uint32_t* item_base;
+    uint32_t ncam;
+    __device__ __forceinline__ uint32_t find(uint32_t q) const {
+        uint32_t lo = 0, hi = ncam;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (item_base[mid] <= q) lo = mid; else hi = mid;
+        }
+        return lo;
+    }
+    __device__ __forceinline__ const MarchConst& cam(uint32_t k) const { return cams[k]; }
+    __device__ __forceinline__ uint32_t base(uint32_t k) const { return item_base[k]; }
+};
 
 // Route of one ray of a surrogate render, decided before the network runs.
 enum Route : uint32_t {
@@ -388,13 +406,13 @@ struct RayPrep {
     Route route;
 };
 
-// Classification of item q; identical in the pre-pass and the network pass (same operations).
-__device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst& k,
-                                            const V3<float>& P, int64_t pix) {
+// Classification of a ray; identical in the pre-pass and the network pass (same operations).
+__device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst& k, int64_t pix) {
     RayPrep r;
     r.x[0] = r.x[1] = r.x[2] = 0.0f;
     r.route = kNone;
     if (pix < 0) return r;
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
     const V3<float> d = ray_dir(c, pix);
     r.s = lawS_setup(P, d);
     if (c.precision == BHS_PREC_F32_BAND64 && __builtin_fabsf(r.s.b - kBcritF) < c.band_abs) {
@@ -421,54 +439,26 @@ __device__ __forceinline__ void push_fallback(const MarchConst& c, const Scratch
         push_block(count, ids, cap, want, q, s_cnt);
 }
 
-// Pre-pass: lists the FP64 band rays and the early fallback rays, and (table mode, range non-null)
-// the x2 range of the network candidates.
-__global__ __launch_bounds__(kBlock) void k_sur_pre(const MarchConst c, const SurConst k,
-                                                    uint32_t n_items, Scratch sc, SurLists fl,
-                                                    int32_t* __restrict__ range) {
+// Pre-pass: lists the FP64 band rays and the early fallback rays.
+template <typename Cams>
+__global__ __launch_bounds__(kBlock) void k_sur_pre(const Cams cams, const SurConst k,
+                                                    uint32_t n_items, Scratch sc, SurLists fl) {
     __shared__ uint32_t s_cnt[5];
-    __shared__ int32_t s_lo[kBlock / 64], s_hi[kBlock / 64];
-    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-    int32_t lo = 0x7FFFFFFF, hi = (int32_t)0x80000000;
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
         const uint32_t q = q0 + threadIdx.x;
-        const RayPrep r = prep_ray(c, k, P, q < n_items ? item_pixel(c, q) : -1);
-        if (r.route == kNet) {
-            const int32_t v = ord_int(r.x[1]);
-            lo = v < lo ? v : lo;
-            hi = v > hi ? v : hi;
-        }
-        if (c.precision == BHS_PREC_F32_BAND64)
-            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
+        const uint32_t ci = cams.find(q < n_items ? q : n_items - 1);
+        const MarchConst& c = cams.cam(ci);
+        const RayPrep r = prep_ray(c, k, q < n_items ? item_pixel(c, q - cams.base(ci)) : -1);
+        push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
         push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q, s_cnt);
-    }
-    if (!range) return;   // grid-uniform
-    for (int o = 32; o > 0; o >>= 1) {
-        const int32_t a2 = __shfl_xor(lo, o, 64), b2 = __shfl_xor(hi, o, 64);
-        lo = a2 < lo ? a2 : lo;
-        hi = b2 > hi ? b2 : hi;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        s_lo[threadIdx.x >> 6] = lo;
-        s_hi[threadIdx.x >> 6] = hi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < kBlock / 64; ++w) {
-            lo = s_lo[w] < lo ? s_lo[w] : lo;
-            hi = s_hi[w] > hi ? s_hi[w] : hi;
-        }
-        if (lo <= hi) {
-            atomicMin(&range[0], lo);
-            atomicMax(&range[1], hi);
-        }
     }
 }
 
 // Surrogate render, per-ray network. Persistent blocks (two per CU: the weights take 71 KB of LDS)
 // walk 256-item chunks; the wave runs the network if any of its lanes is a candidate, each
 // candidate lane decodes and writes its record or lists its ray for the integrator.
-__global__ __launch_bounds__(kBlock, 2) void k_surrogate(const MarchConst c, const SurConst k,
+template <typename Cams>
+__global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const SurConst k,
                                                          const uint8_t* __restrict__ packed,
                                                          const SoA out, Scratch sc, SurLists fl,
                                                          uint32_t n_items) {
@@ -479,18 +469,158 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const MarchConst c, con
     stage_weights(packed, s_w);
     __syncthreads();
     const int wave = threadIdx.x >> 6;
-    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
     const uint32_t chunks = (n_items + kBlock - 1) / kBlock;
     for (uint32_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
         const uint32_t q = ch * kBlock + threadIdx.x;
-        const int64_t pix = q < n_items ? item_pixel(c, q) : -1;
-        const RayPrep r = prep_ray(c, k, P, pix);
+        const uint32_t ci = cams.find(q < n_items ? q : n_items - 1);
+        const MarchConst& c = cams.cam(ci);
+        const int64_t pix = q < n_items ? item_pixel(c, q - cams.base(ci)) : -1;
+        const RayPrep r = prep_ray(c, k, pix);
         const bool cand = r.route == kNet;
         float y[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (__any(cand)) mlp_wave(s_w, r.x, s_y[wave], y);
         bool back = false;
         if (cand) {
+            const Dec o = decode(c, k, r.s, r.g, y);
+            if (o.serve) {
+                store_dec(out, c.ray_offset + pix, o, r.s.b);
+                n_served += 1;
+                n_cross += o.n_cross;
+                n_esc += o.escaping ? 1u : 0u;
+            } else {
+                back = true;
+            }
+        }
+        push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
+        n_fb += (back || r.route == kEarly) ? 1u : 0u;
+    }
+    add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Table mode: per camera, the network evaluated on an n2 x n3 grid over (x2, x3) at the camera's
+// x1; rays interpolate bilinearly. x3 spans [-1, 1]; x2 spans [x2(beta = 0.3), x2(beta_max)] with
+// beta_max from the image corners (the largest view angle from the inward radial over a pinhole
+// frustum is at a corner), both widened by 1e-3. Node (i, j) of camera k holds the 5 outputs at
+// node[((k n3 + j) n2 + i) * 8 + o]; range[2k], range[2k + 1] hold the x2 bounds.
+
+template <typename Cams>
+__global__ void k_tab_range(const Cams cams, uint32_t ncam, float* __restrict__ range) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncam) return;
+    const MarchConst& c = cams.cam(k);
+    const float r = sqrtf(c.pos[0] * c.pos[0] + c.pos[1] * c.pos[1] + c.pos[2] * c.pos[2]);
+    const float ir = 1.0f / r;
+    const float aspect = (float)c.width / (float)c.height;
+    float smax = 0.0f;
+    for (int corner = 0; corner < 4; ++corner) {
+        const float cx = (corner & 1 ? 1.0f : -1.0f) * c.tan_half_fov * aspect;
+        const float cy = (corner & 2 ? 1.0f : -1.0f) * c.tan_half_fov;
+        // d = B (cx, cy, -1) / |.|, B columns right, up, back
+        const float dx = c.basis[0] * cx + c.basis[3] * cy - c.basis[6];
+        const float dy = c.basis[1] * cx + c.basis[4] * cy - c.basis[7];
+        const float dz = c.basis[2] * cx + c.basis[5] * cy - c.basis[8];
+        const float dn = sqrtf(dx * dx + dy * dy + dz * dz);
+        const float ca = -(dx * c.pos[0] + dy * c.pos[1] + dz * c.pos[2]) * ir / dn;
+        const float sa = ca <= 0.0f ? 1.0f : sqrtf(fmaxf(0.0f, 1.0f - ca * ca));
+        smax = fmaxf(smax, sa);
+    }
+    smax = fminf(smax, kSinMax);
+    const float f = sqrtf(fmaxf(1.0f - 2.0f * ir, 1e-12f));
+    const float bm_max = r * smax / f * (1.0f / kBcritF) * 1.0001f - 1.0f;
+    range[2 * k] = x2_of(kBetaMin - 1.0f) - 1e-3f;
+    range[2 * k + 1] = fmaxf(x2_of(bm_max), 0.0f) + 1e-3f;
+}
+
+struct TabGeom {
+    float x2_lo, inv_h2, h2, inv_h3, h3;
+};
+
+__device__ __forceinline__ TabGeom tab_geom(const float* range, int32_t n2, int32_t n3) {
+    TabGeom t;
+    t.x2_lo = range[0];
+    t.h2 = (range[1] - range[0]) / (float)(n2 - 1);
+    t.inv_h2 = 1.0f / t.h2;
+    t.h3 = 2.0f / (float)(n3 - 1);
+    t.inv_h3 = 1.0f / t.h3;
+    return t;
+}
+
+// Network outputs at the table nodes of every camera (blocks walk (camera, chunk) pairs).
+template <typename Cams>
+__global__ __launch_bounds__(kBlock, 2) void k_table(const Cams cams, uint32_t ncam,
+                                                     const uint8_t* __restrict__ packed,
+                                                     const float* __restrict__ range, int32_t n2,
+                                                     int32_t n3, float* __restrict__ node) {
+    __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
+    __shared__ float s_y[4][64 * kOut];
+    stage_weights(packed, s_w);
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    const int64_t n = (int64_t)n2 * n3;
+    const int64_t per = (n + kBlock - 1) / kBlock;
+    for (int64_t p = blockIdx.x; p < per * ncam; p += gridDim.x) {
+        const uint32_t k = (uint32_t)(p / per);
+        const int64_t q = (p % per) * kBlock + threadIdx.x;
+        const MarchConst& c = cams.cam(k);
+        const TabGeom t = tab_geom(range + 2 * k, n2, n3);
+        const float r = sqrtf(c.pos[0] * c.pos[0] + c.pos[1] * c.pos[1] + c.pos[2] * c.pos[2]);
+        const float x1 = (logf(r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
+        const int32_t i = (int32_t)(q % n2), j = (int32_t)(q / n2);
+        const float x[3] = {x1, __builtin_fmaf((float)i, t.h2, t.x2_lo),
+                            __builtin_fmaf((float)j, t.h3, -1.0f)};
+        float y[5];
+        mlp_wave(s_w, x, s_y[wave], y);
+        if (q < n) {
+            float* o = node + ((int64_t)k * n + q) * 8;
+#pragma unroll
+            for (int k2 = 0; k2 < kOut; ++k2) o[k2] = y[k2];
+        }
+    }
+}
+
+__device__ __forceinline__ void tab_lookup(const float* __restrict__ node, const TabGeom& t,
+                                           int32_t n2, int32_t n3, const float x[3], float y[5]) {
+    float fx = (x[1] - t.x2_lo) * t.inv_h2, fy = (x[2] + 1.0f) * t.inv_h3;
+    fx = fminf(fmaxf(fx, 0.0f), (float)(n2 - 1) - 1e-3f);
+    fy = fminf(fmaxf(fy, 0.0f), (float)(n3 - 1) - 1e-3f);
+    const int32_t i = (int32_t)fx, j = (int32_t)fy;
+    const float a = fx - (float)i, b = fy - (float)j;
+    const float* p0 = node + ((int64_t)j * n2 + i) * 8;
+    const float* p1 = p0 + (int64_t)n2 * 8;
+    const f4 q00 = *(const f4*)p0, q01 = *(const f4*)(p0 + 8);
+    const f4 q10 = *(const f4*)p1, q11 = *(const f4*)(p1 + 8);
+    const float w00 = (1.0f - a) * (1.0f - b), w01 = a * (1.0f - b), w10 = (1.0f - a) * b,
+                w11 = a * b;
+#pragma unroll
+    for (int o = 0; o < 4; ++o)
+        y[o] = w00 * q00[o] + w01 * q01[o] + w10 * q10[o] + w11 * q11[o];
+    y[4] = w00 * p0[4] + w01 * p0[12] + w10 * p1[4] + w11 * p1[12];
+}
+
+// Surrogate render in table mode: as k_surrogate with the network replaced by the table lookup.
+template <typename Cams>
+__global__ __launch_bounds__(kBlock) void k_surrogate_tab(const Cams cams, const SurConst k,
+                                                          const float* __restrict__ node,
+                                                          const float* __restrict__ range,
+                                                          int32_t n2, int32_t n3, const SoA out,
+                                                          Scratch sc, SurLists fl,
+                                                          uint32_t n_items) {
+    __shared__ uint32_t s_cnt[5];
+    __shared__ uint32_t s_st[4][4];
+    uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
+    const int64_t nodes = (int64_t)n2 * n3 * 8;
+    for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
+        const uint32_t q = q0 + threadIdx.x;
+        const uint32_t ci = cams.find(q < n_items ? q : n_items - 1);
+        const MarchConst& c = cams.cam(ci);
+        const int64_t pix = q < n_items ? item_pixel(c, q - cams.base(ci)) : -1;
+        const RayPrep r = prep_ray(c, k, pix);
+        bool back = false;
+        if (r.route == kNet) {
+            float y[5];
+            tab_lookup(node + ci * nodes, tab_geom(range + 2 * ci, n2, n3), n2, n3, r.x, y);
             const Dec o = decode(c, k, r.s, r.g, y);
             if (o.serve) {
                 store_dec(out, c.ray_offset + pix, o, r.s.b);
@@ -622,124 +752,6 @@ __global__ __launch_bounds__(kBlock, 2) void k_validate(int64_t n, const float* 
     }
 }
 
-// ------------------------------------------------------------------------------------------------
-// Table mode: the network evaluated on an n2 x n3 grid over (x2, x3) at the camera's x1, rays
-// interpolate bilinearly. x2 spans the camera's in-domain rays (device-side min/max pass), x3 the
-// full [-1, 1]. Node (i, j) holds the 5 outputs at node[(j n2 + i) * 8 + o].
-
-__global__ void k_range_init(int32_t* range) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        range[0] = 0x7FFFFFFF;
-        range[1] = (int32_t)0x80000000;
-    }
-}
-
-struct TabGeom {
-    float x2_lo, inv_h2, h2, inv_h3, h3;
-    bool empty;
-};
-
-__device__ __forceinline__ TabGeom tab_geom(const int32_t* range, int32_t n2, int32_t n3) {
-    TabGeom t;
-    const int32_t a = range[0], b = range[1];
-    t.empty = a > b;
-    float lo = t.empty ? 0.0f : ord_float(a), hi = t.empty ? 0.0f : ord_float(b);
-    lo -= 1e-3f;
-    hi += 1e-3f;
-    t.x2_lo = lo;
-    t.h2 = (hi - lo) / (float)(n2 - 1);
-    t.inv_h2 = 1.0f / t.h2;
-    t.h3 = 2.0f / (float)(n3 - 1);
-    t.inv_h3 = 1.0f / t.h3;
-    return t;
-}
-
-// Network outputs at the table nodes.
-__global__ __launch_bounds__(kBlock, 2) void k_table(const MarchConst c,
-                                                     const uint8_t* __restrict__ packed,
-                                                     const int32_t* __restrict__ range, int32_t n2,
-                                                     int32_t n3, float* __restrict__ node) {
-    __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
-    __shared__ float s_y[4][64 * kOut];
-    const TabGeom t = tab_geom(range, n2, n3);
-    if (t.empty) return;   // block-uniform: no in-domain ray, nothing reads the table
-    stage_weights(packed, s_w);
-    __syncthreads();
-    const int wave = threadIdx.x >> 6;
-    const float r = sqrtf(c.pos[0] * c.pos[0] + c.pos[1] * c.pos[1] + c.pos[2] * c.pos[2]);
-    const float x1 = (logf(r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
-    const int64_t n = (int64_t)n2 * n3;
-    const int64_t chunks = (n + kBlock - 1) / kBlock;
-    for (int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
-        const int64_t q = ch * kBlock + threadIdx.x;
-        const int32_t i = (int32_t)(q % n2), j = (int32_t)(q / n2);
-        const float x[3] = {x1, __builtin_fmaf((float)i, t.h2, t.x2_lo),
-                            __builtin_fmaf((float)j, t.h3, -1.0f)};
-        float y[5];
-        mlp_wave(s_w, x, s_y[wave], y);
-        if (q < n) {
-            float* o = node + q * 8;
-#pragma unroll
-            for (int k2 = 0; k2 < kOut; ++k2) o[k2] = y[k2];
-        }
-    }
-}
-
-__device__ __forceinline__ void tab_lookup(const float* __restrict__ node, const TabGeom& t,
-                                           int32_t n2, int32_t n3, const float x[3], float y[5]) {
-    float fx = (x[1] - t.x2_lo) * t.inv_h2, fy = (x[2] + 1.0f) * t.inv_h3;
-    fx = fminf(fmaxf(fx, 0.0f), (float)(n2 - 1) - 1e-3f);
-    fy = fminf(fmaxf(fy, 0.0f), (float)(n3 - 1) - 1e-3f);
-    const int32_t i = (int32_t)fx, j = (int32_t)fy;
-    const float a = fx - (float)i, b = fy - (float)j;
-    const float* p0 = node + ((int64_t)j * n2 + i) * 8;
-    const float* p1 = p0 + (int64_t)n2 * 8;
-    const f4 q00 = *(const f4*)p0, q01 = *(const f4*)(p0 + 8);
-    const f4 q10 = *(const f4*)p1, q11 = *(const f4*)(p1 + 8);
-    const float w00 = (1.0f - a) * (1.0f - b), w01 = a * (1.0f - b), w10 = (1.0f - a) * b,
-                w11 = a * b;
-#pragma unroll
-    for (int o = 0; o < 4; ++o)
-        y[o] = w00 * q00[o] + w01 * q01[o] + w10 * q10[o] + w11 * q11[o];
-    y[4] = w00 * p0[4] + w01 * p0[12] + w10 * p1[4] + w11 * p1[12];
-}
-
-// Surrogate render in table mode: as k_surrogate with the network replaced by the table lookup.
-__global__ __launch_bounds__(kBlock) void k_surrogate_tab(const MarchConst c, const SurConst k,
-                                                          const float* __restrict__ node,
-                                                          const int32_t* __restrict__ range,
-                                                          int32_t n2, int32_t n3, const SoA out,
-                                                          Scratch sc, SurLists fl,
-                                                          uint32_t n_items) {
-    __shared__ uint32_t s_cnt[5];
-    __shared__ uint32_t s_st[4][4];
-    const TabGeom t = tab_geom(range, n2, n3);
-    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-    uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
-    for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
-        const uint32_t q = q0 + threadIdx.x;
-        const int64_t pix = q < n_items ? item_pixel(c, q) : -1;
-        const RayPrep r = prep_ray(c, k, P, pix);
-        bool back = false;
-        if (r.route == kNet) {
-            float y[5];
-            tab_lookup(node, t, n2, n3, r.x, y);
-            const Dec o = decode(c, k, r.s, r.g, y);
-            if (o.serve) {
-                store_dec(out, c.ray_offset + pix, o, r.s.b);
-                n_served += 1;
-                n_cross += o.n_cross;
-                n_esc += o.escaping ? 1u : 0u;
-            } else {
-                back = true;
-            }
-        }
-        push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
-        n_fb += (back || r.route == kEarly) ? 1u : 0u;
-    }
-    add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
-}
-
 int persistent_blocks(int64_t chunks, int num_cus) {
     const int64_t cap = 2 * (int64_t)(num_cus > 0 ? num_cus : 256);
     return (int)(chunks < cap ? (chunks > 0 ? chunks : 1) : cap);
@@ -771,30 +783,49 @@ hipError_t launch_validate(int64_t n, const float* pos3, const float* dir3, cons
     return hipGetLastError();
 }
 
-hipError_t launch_surrogate(const MarchConst& c, const SurConst& k, const void* packed,
-                            const SurTable* tab, const SoA& out, const Scratch& sc,
-                            const SurLists& fl, const Tuning& t, int num_cus, hipStream_t s,
-                            const SurStreams& ss, hipEvent_t ev_begin, hipEvent_t ev_mid,
-                            hipEvent_t ev_end) {
-    const uint32_t n_items = camera_items(c.width, c.height);
+namespace {
+
+// The surrogate render of one camera (Cams = SurOneCam) or of a batch (SurCamTable, table and
+// item_base on the device).
+template <typename Cams>
+hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const MarchConst& c0,
+                      const MarchConst* table, const uint32_t* item_base, bool any_band,
+                      const SurConst& k, const void* packed, const SurTable* tab, const SoA& out,
+                      const Scratch& sc, const SurLists& fl, const Tuning& t, int num_cus,
+                      hipStream_t s, const SurStreams& ss, hipEvent_t ev_begin, hipEvent_t ev_mid,
+                      hipEvent_t ev_end) {
+    constexpr bool kOne = std::is_same<Cams, SurOneCam>::value;
     const uint32_t gb0 = (n_items + kBlock - 1) / kBlock;
     const uint32_t cap = (uint32_t)(4 * (num_cus > 0 ? num_cus : 256));
     const dim3 grid(gb0 < cap ? (gb0 ? gb0 : 1u) : cap);
-    const bool f64 = c.precision == BHS_PREC_F64;
+    const bool f64 = kOne && c0.precision == BHS_PREC_F64;
     // The fallback lists are short and latency-bound: two resident waves per SIMD (131k lanes)
     // leave the rest of the chip to the network pass running beside them.
     Tuning tl = t;
     tl.mode = 1;
     tl.block = 256;
     tl.waves_per_simd = 2;
+    auto march_list = [&](const uint32_t* ids, const uint32_t* count, const Scratch& scr,
+                          hipStream_t st) {
+        if constexpr (kOne)
+            return launch_list(c0, ids, count, fl.fb_cap, out, scr, tl, num_cus, st);
+        else
+            return launch_list_batch(table, item_base, ncam, ids, count, fl.fb_cap, out, scr, tl,
+                                     num_cus, st);
+    };
+    auto band = [&]() {
+        if constexpr (kOne)
+            return launch_band_list(c0, out, sc, s, ss.aux, ss.ev_fork, ss.ev_join);
+        else
+            return launch_band_list_batch(table, item_base, ncam, n_items, out, sc, s, ss.aux,
+                                          ss.ev_fork, ss.ev_join);
+    };
     hipError_t e;
     if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
     // Pre-pass, then the two side marches start at once: the FP64 band (BAND64) on aux and the
     // early FP32 fallback on aux2 run beside the network pass; their serial rays are the longest
     // chains of the frame.
-    if (tab) hipLaunchKernelGGL(k_range_init, dim3(1), dim3(64), 0, s, tab->range);
-    hipLaunchKernelGGL(k_sur_pre, grid, dim3(kBlock), 0, s, c, k, n_items, sc, fl,
-                       tab ? tab->range : (int32_t*)nullptr);
+    hipLaunchKernelGGL(k_sur_pre<Cams>, grid, dim3(kBlock), 0, s, cams, k, n_items, sc, fl);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (!f64) {
         if ((e = hipEventRecord(ss.ev_fork2, s)) != hipSuccess ||
@@ -802,40 +833,67 @@ hipError_t launch_surrogate(const MarchConst& c, const SurConst& k, const void* 
             return e;
         Scratch se = sc;
         se.work = fl.early_work;
-        if ((e = launch_list(c, fl.early_ids, fl.early_count, fl.fb_cap, out, se, tl, num_cus,
-                             ss.aux2)) != hipSuccess)
-            return e;
+        if ((e = march_list(fl.early_ids, fl.early_count, se, ss.aux2)) != hipSuccess) return e;
         if ((e = hipEventRecord(ss.ev_join2, ss.aux2)) != hipSuccess) return e;
-        if (c.precision == BHS_PREC_F32_BAND64 &&
-            (e = launch_band_list(c, out, sc, s, ss.aux, ss.ev_fork, ss.ev_join)) != hipSuccess)
-            return e;
+        if (any_band && (e = band()) != hipSuccess) return e;
     }
     if (tab) {
         const int64_t nodes = (int64_t)tab->n2 * tab->n3;
-        hipLaunchKernelGGL(k_table, dim3(persistent_blocks((nodes + kBlock - 1) / kBlock, num_cus)),
-                           dim3(kBlock), 0, s, c, (const uint8_t*)packed, (const int32_t*)tab->range,
-                           tab->n2, tab->n3, tab->node);
-        hipLaunchKernelGGL(k_surrogate_tab, grid, dim3(kBlock), 0, s, c, k, (const float*)tab->node,
-                           (const int32_t*)tab->range, tab->n2, tab->n3, out, sc, fl, n_items);
+        hipLaunchKernelGGL(k_tab_range<Cams>, dim3((ncam + 63) / 64), dim3(64), 0, s, cams, ncam,
+                           tab->range);
+        hipLaunchKernelGGL(k_table<Cams>,
+                           dim3(persistent_blocks((nodes + kBlock - 1) / kBlock * ncam, num_cus)),
+                           dim3(kBlock), 0, s, cams, ncam, (const uint8_t*)packed,
+                           (const float*)tab->range, tab->n2, tab->n3, tab->node);
+        hipLaunchKernelGGL(k_surrogate_tab<Cams>, grid, dim3(kBlock), 0, s, cams, k,
+                           (const float*)tab->node, (const float*)tab->range, tab->n2, tab->n3,
+                           out, sc, fl, n_items);
     } else {
-        const int blocks = persistent_blocks(gb0, num_cus);
-        hipLaunchKernelGGL(k_surrogate, dim3(blocks), dim3(kBlock), 0, s, c, k,
-                           (const uint8_t*)packed, out, sc, fl, n_items);
+        hipLaunchKernelGGL(k_surrogate<Cams>, dim3(persistent_blocks(gb0, num_cus)), dim3(kBlock),
+                           0, s, cams, k, (const uint8_t*)packed, out, sc, fl, n_items);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
     if (f64) {   // every unserved ray is in the band list: FP64 march now
-        if ((e = launch_band_list(c, out, sc, s, ss.aux, ss.ev_fork, ss.ev_join)) != hipSuccess)
-            return e;
-    } else if ((e = launch_list(c, fl.fb_ids, fl.fb_count, fl.fb_cap, out, sc, tl, num_cus, s)) !=
-               hipSuccess) {
+        if ((e = band()) != hipSuccess) return e;
+    } else if ((e = march_list(fl.fb_ids, fl.fb_count, sc, s)) != hipSuccess) {
         return e;
     }
     if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
-    if (f64 || c.precision == BHS_PREC_F32_BAND64)
+    if (f64 || any_band)
         if ((e = hipStreamWaitEvent(s, ss.ev_join, 0)) != hipSuccess) return e;
     if (!f64 && (e = hipStreamWaitEvent(s, ss.ev_join2, 0)) != hipSuccess) return e;
-    return launch_root_refine(c, out, sc, s);
+    if constexpr (kOne)
+        return launch_root_refine(c0, out, sc, s);
+    else
+        return launch_root_refine_batch(table, item_base, ncam, n_items, out, sc, s);
+}
+
+}  // namespace
+
+hipError_t launch_surrogate(const MarchConst& c, const SurConst& k, const void* packed,
+                            const SurTable* tab, const SoA& out, const Scratch& sc,
+                            const SurLists& fl, const Tuning& t, int num_cus, hipStream_t s,
+                            const SurStreams& ss, hipEvent_t ev_begin, hipEvent_t ev_mid,
+                            hipEvent_t ev_end) {
+    const SurOneCam one{c};
+    return launch_sur(one, 1u, camera_items(c.width, c.height), c, nullptr, nullptr,
+                      c.precision == BHS_PREC_F32_BAND64, k, packed, tab, out, sc, fl, t, num_cus,
+                      s, ss, ev_begin, ev_mid, ev_end);
+}
+
+hipError_t launch_surrogate_batch(const MarchConst* table, const uint32_t* item_base,
+                                  uint32_t ncam, uint32_t n_items, bool any_band,
+                                  const SurConst& k, const void* packed, const SurTable* tab,
+                                  const SoA& out, const Scratch& sc, const SurLists& fl,
+                                  const Tuning& t, int num_cus, hipStream_t s,
+                                  const SurStreams& ss, hipEvent_t ev_begin, hipEvent_t ev_mid,
+                                  hipEvent_t ev_end) {
+    const SurCamTable cams{table, item_base, ncam};
+    MarchConst none;
+    __builtin_memset(&none, 0, sizeof none);
+    return launch_sur(cams, ncam, n_items, none, table, item_base, any_band, k, packed, tab, out,
+                      sc, fl, t, num_cus, s, ss, ev_begin, ev_mid, ev_end);
 }
 
 }  // namespace mlp

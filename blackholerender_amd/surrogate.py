@@ -137,6 +137,21 @@ class Surrogate:
         self._check(self._lib.bhs_mlp_save(self.tracer._ctx, self._m, path.encode()),
                     "bhs_mlp_save")
 
+    def render_batch(self, cams, params, out=None, stream=None):
+        """Surrogate render of up to 64 cameras in one call (bhs_render_surrogate_batch), outputs
+        laid out as Tracer.render_batch; returns the RayBuffers."""
+        from . import RayBuffers
+        t = self.tracer
+        cs, ps = t._batch_structs(cams, params)
+        n = sum(c.width * c.height for c in cams)
+        if out is None:
+            out = RayBuffers(n, f"cuda:{t.device}")
+        soa = out.soa()
+        self._check(self._lib.bhs_render_surrogate_batch(
+            t._ctx, self._m, C.addressof(cs), C.addressof(ps), len(cams), C.byref(soa),
+            C.c_void_p(t._stream_ptr(stream))), "bhs_render_surrogate_batch")
+        return out
+
     def render(self, cam: Camera, params: _capi.MarchParams | None = None, out=None,
                stream=None):
         """Surrogate render with integrator fallback (stream-ordered); returns the RayBuffers."""

@@ -297,6 +297,12 @@ int bhs_mlp_set_table(bhs_ctx* ctx, bhs_mlp* m, int32_t n2, int32_t n3);
  * the network kernel and the call interval includes the fallback march. */
 int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
                          const bhs_march_params* params, const bhs_ray_soa* out, void* stream);
+/* Surrogate render of ncam <= 64 cameras in one call, laid out as bhs_render_batch (camera k's
+ * rays start at the prefix sum of width * height; no BHS_PREC_F64). The FP64 band and the
+ * fallback marches of all cameras run together, so their serial tails are paid once per call. */
+int bhs_render_surrogate_batch(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cams,
+                               const bhs_march_params* params, int32_t ncam,
+                               const bhs_ray_soa* out, void* stream);
 
 #ifdef __cplusplus
 }

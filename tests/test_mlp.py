@@ -154,3 +154,109 @@ def test_render_rejects_other_march_parameters(tracer, model):
         model.render(cam, bhs.march_params(disk_outer=30.0))
     with pytest.raises(bhs.BhsError):
         model.render(cam, bhs.march_params(max_steps=700))   # other preset step control
+
+
+# ------------------------------------------------------------------------- trained models (GPU)
+MODEL_HIGH = "lawS_450_disk6-20.bhsmlp"
+
+
+def _model_path(name):
+    import os
+    return os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "blackholerender_amd", "models", name)
+
+
+def _angle(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), (a * b).sum(1))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("table", [0, 1])
+def test_trained_model_frame_accuracy(tracer, table):
+    """480x270 frame of the 1080p camera (r_obs 30, i 85, FOV 60, High preset, disk [6, 20]):
+    served escape directions within 1e-4 rad of FP64, classes and crossing counts equal to the
+    FP64 reference march, every unserved ray bit-identical to the plain render."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd.surrogate import Surrogate
+    from helpers import bit_mismatches
+    sur = Surrogate.load(tracer, _model_path(MODEL_HIGH))
+    if table:
+        sur.set_table(256, 64)
+    cam = bhs.orbit_camera(30.0, 85.0, 60.0, 480, 270, jitter_hashed=1, seed=11)
+    params = bhs.march_params()
+    print(f"[mlp] trained model frame table={table}", flush=True)
+    s = sur.render(cam, params)
+    torch.cuda.synchronize()
+    st = tracer.last_stats()
+    S = s.numpy()
+    P = tracer.render(cam, params).numpy()
+    R = tracer.render(cam, bhs.march_params(precision=bhs.PREC_F64)).numpy()
+    sv = (S["flags"] & bhs.FLAG_SURROGATE) > 0
+    assert sv.mean() > 0.95, sv.mean()
+    assert st["surrogate"] == sv.sum()
+    esc = sv & ((S["flags"] & 2) > 0)
+    assert np.array_equal(esc, sv & ((R["flags"] & 2) > 0))
+    assert _angle(S["esc"][esc], R["esc"][esc]).max() <= 1e-4
+    assert np.array_equal(S["cls"][sv], R["cls"][sv])
+    assert np.array_equal(S["n_cross"][sv], R["n_cross"][sv])
+    cr = sv & (S["n_cross"] > 0)
+    rel = np.abs(S["disk_r"][cr, 0] / R["disk_r"][cr, 0] - 1.0)
+    assert rel.max() < 5e-3, rel.max()
+    mm = bit_mismatches({k: v[~sv] for k, v in S.items()}, {k: v[~sv] for k, v in P.items()})
+    assert all(v == 0 for v in mm.values()), mm
+    sur.close()
+
+
+@pytest.mark.gpu
+def test_batch_equals_single_calls(tracer):
+    """bhs_render_surrogate_batch over three cameras of different sizes and poses equals the three
+    single-camera calls bit for bit (table mode)."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd.surrogate import Surrogate
+    from helpers import bit_mismatches
+    sur = Surrogate.load(tracer, _model_path(MODEL_HIGH))
+    sur.set_table(256, 64)
+    cams = [bhs.orbit_camera(30.0, 85.0, 60.0, 200, 120, jitter_hashed=1, seed=3, camera_id=0),
+            bhs.orbit_camera(60.0, 70.0, 40.0, 96, 160, jitter_hashed=1, seed=3, camera_id=1,
+                             azimuth_deg=90.0),
+            bhs.orbit_camera(12.0, 80.0, 75.0, 128, 64, jitter_hashed=1, seed=3, camera_id=2)]
+    params = bhs.march_params(root64=1)
+    print("[mlp] surrogate batch of 3 cameras", flush=True)
+    b = sur.render_batch(cams, params)
+    torch.cuda.synchronize()
+    B = b.numpy()
+    off = tracer.batch_offsets(cams)
+    for k, cam in enumerate(cams):
+        one = sur.render(cam, params)
+        torch.cuda.synchronize()
+        O = one.numpy()
+        part = {key: v[off[k]:off[k + 1]] for key, v in B.items()}
+        mm = bit_mismatches(part, O)
+        assert all(v == 0 for v in mm.values()), (k, mm)
+    sur.close()
+
+
+@pytest.mark.gpu
+def test_network_gain_over_prior_alone(tracer):
+    """What the trained network adds to the exact-orbit prior: on the same validation rays, the
+    share of in-domain rays whose (r_obs, b) cell meets the 1e-4 rad escape tolerance, with the
+    trained weights and with all weights zero (outputs 0: the prior alone)."""
+    from blackholerender_amd.surrogate import Surrogate, MLP_PARAMS
+    net = Surrogate.load(tracer, _model_path(MODEL_HIGH))
+    prior = Surrogate.load(tracer, _model_path(MODEL_HIGH))
+    prior.params = np.zeros(MLP_PARAMS, dtype=np.float32)
+    print("[mlp] validation with and without the network", flush=True)
+    vn = net.validate(1 << 20, seed=77)
+    vp = prior.validate(1 << 20, seed=77)
+    share_n = vn["served"] / vn["in_domain"]
+    share_p = vp["served"] / vp["in_domain"]
+    print(f"served share: network {share_n:.4f}, prior alone {share_p:.4f}")
+    assert vn["mismatches"] == 0 and vp["mismatches"] == 0
+    assert vn["phi_err_max"] <= 1e-4
+    assert share_n > share_p + 0.05, (share_n, share_p)
+    net.close()
+    prior.close()
