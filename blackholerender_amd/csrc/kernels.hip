@@ -513,6 +513,14 @@ __global__ void k_zero(uint32_t* p, size_t words) {
         p[i] = 0u;
 }
 
+// Grid of the FP64 band march (64-thread blocks, grid-stride over the device-side list): enough
+// waves for band lists of millions of rays (large batches); surplus waves find no work and exit.
+dim3 band_grid(uint32_t cap) {
+    uint32_t b = (cap + 63u) / 64u;
+    b = b > 4096u ? 4096u : b;
+    return dim3(b < 256u ? 256u : b);
+}
+
 // Grid of the item-indexed helper kernels (grid-stride loops).
 dim3 item_grid(uint32_t n_items) {
     uint32_t b = (n_items + kBlock - 1) / kBlock;
@@ -573,7 +581,7 @@ hipError_t fork_band(const Cams& cams, const SoA& out, const Scratch& sc, hipStr
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_band64<Cams>, dim3(256), dim3(64), 0, aux, cams, out, sc);
+    hipLaunchKernelGGL(k_band64<Cams>, band_grid(sc.band_cap), dim3(64), 0, aux, cams, out, sc);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipEventRecord(ev_join, aux);
 }
@@ -665,7 +673,7 @@ hipError_t launch_band_list(const MarchConst& c, const SoA& out, const Scratch& 
     hipError_t e;
     if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_band64<OneCam>, dim3(256), dim3(64), 0, aux, one, out, sc);
+    hipLaunchKernelGGL(k_band64<OneCam>, band_grid(sc.band_cap), dim3(64), 0, aux, one, out, sc);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipEventRecord(ev_join, aux);
 }
@@ -701,7 +709,7 @@ hipError_t launch_band_list_batch(const MarchConst* table, const uint32_t* item_
     hipError_t e;
     if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_band64<CamTable>, dim3(256), dim3(64), 0, aux, cams, out, sc);
+    hipLaunchKernelGGL(k_band64<CamTable>, band_grid(sc.band_cap), dim3(64), 0, aux, cams, out, sc);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return hipEventRecord(ev_join, aux);
 }

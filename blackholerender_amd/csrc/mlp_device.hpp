@@ -181,7 +181,7 @@ __device__ __forceinline__ float asinh_f(float x) {
 struct RayGeom {
     V3<float> e1, e2;     // orbit-plane basis
     float r;              // r_obs
-    float bm;             // beta - 1 (from the FP64 setup within 5% of b_crit)
+    float bm;             // beta - 1
     float b;              // impact parameter
     float psi;            // swept angle of the first equatorial crossing, [0, pi)
     float alpha_v;        // angle of the initial coordinate velocity from e1
@@ -205,15 +205,10 @@ __device__ __forceinline__ RayGeom ray_geom(const V3<float>& P, const V3<float>&
     g.e2 = V3<float>{t.x * it, t.y * it, t.z * it};
     g.alpha_v = atan2f(tn, vr);
     g.ingoing = vr < 0.0f;
+    // beta - 1 in float: outside the 2% band its relative rounding (~1e-7 / 0.02) moves the exact
+    // sweep by < 1e-5 rad
     g.b = s.b;
     g.bm = s.b * (1.0f / kBcritF) - 1.0f;
-    if (fabsf(g.bm) < 0.05f) {   // near b_crit: beta - 1 from the double setup of the same ray
-        const V3<double> Pd{(double)P.x, (double)P.y, (double)P.z};
-        const V3<double> dd{(double)d.x, (double)d.y, (double)d.z};
-        const double b64 = lawS_setup(Pd, dd).b;
-        g.bm = (float)(b64 * (1.0 / 5.196152422706632) - 1.0);
-        g.b = (float)b64;
-    }
     const float f = sqrtf(fmaxf(1.0f - 2.0f * ir, 1e-12f));
     g.sin_a = s.b * f * ir;
     float psi = atan2f(-g.e1.y, g.e2.y);

@@ -461,7 +461,7 @@ template <typename Cams>
 __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const SurConst k,
                                                          const uint8_t* __restrict__ packed,
                                                          const SoA out, Scratch sc, SurLists fl,
-                                                         uint32_t n_items) {
+                                                         uint32_t n_items, bool routed) {
     __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
     __shared__ float s_y[4][64 * kOut];
     __shared__ uint32_t s_cnt[5];
@@ -493,6 +493,11 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
             }
         }
         push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
+        if (!routed) {   // no pre-pass: band and early lists from this pass
+            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
+            push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q,
+                          s_cnt);
+        }
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
@@ -606,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_surrogate_tab(const Cams cams, const
                                                           const float* __restrict__ range,
                                                           int32_t n2, int32_t n3, const SoA out,
                                                           Scratch sc, SurLists fl,
-                                                          uint32_t n_items) {
+                                                          uint32_t n_items, bool routed) {
     __shared__ uint32_t s_cnt[5];
     __shared__ uint32_t s_st[4][4];
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
@@ -632,6 +637,11 @@ __global__ __launch_bounds__(kBlock) void k_surrogate_tab(const Cams cams, const
             }
         }
         push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
+        if (!routed) {   // no pre-pass: band and early lists from this pass
+            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
+            push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q,
+                          s_cnt);
+        }
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
@@ -785,6 +795,8 @@ hipError_t launch_validate(int64_t n, const float* pos3, const float* dir3, cons
 
 namespace {
 
+constexpr uint32_t kRoutedItems = 1u << 24;
+
 // The surrogate render of one camera (Cams = SurOneCam) or of a batch (SurCamTable, table and
 // item_base on the device).
 template <typename Cams>
@@ -799,12 +811,17 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     const uint32_t cap = (uint32_t)(4 * (num_cus > 0 ? num_cus : 256));
     const dim3 grid(gb0 < cap ? (gb0 ? gb0 : 1u) : cap);
     const bool f64 = kOne && c0.precision == BHS_PREC_F64;
-    // The fallback lists are short and latency-bound: two resident waves per SIMD (131k lanes)
-    // leave the rest of the chip to the network pass running beside them.
+    // Latency mode (up to kRoutedItems work items): a pre-pass routes the rays first, so the side
+    // marches, whose serial rays are the longest chains of a frame, run beside the network pass;
+    // their lists are short, and two resident waves per SIMD (131k lanes) leave the rest of the
+    // chip to the network pass. Throughput mode (larger calls): the network pass routes the rays
+    // itself (the pre-pass would cost a quarter of the surrogate's work) and the fallback lists,
+    // millions of rays long, are marched after it at the tuned occupancy.
+    const bool routed = !f64 && n_items <= kRoutedItems;
     Tuning tl = t;
     tl.mode = 1;
     tl.block = 256;
-    tl.waves_per_simd = 2;
+    if (routed) tl.waves_per_simd = 2;
     auto march_list = [&](const uint32_t* ids, const uint32_t* count, const Scratch& scr,
                           hipStream_t st) {
         if constexpr (kOne)
@@ -820,22 +837,25 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
             return launch_band_list_batch(table, item_base, ncam, n_items, out, sc, s, ss.aux,
                                           ss.ev_fork, ss.ev_join);
     };
-    hipError_t e;
-    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
-    // Pre-pass, then the two side marches start at once: the FP64 band (BAND64) on aux and the
-    // early FP32 fallback on aux2 run beside the network pass; their serial rays are the longest
-    // chains of the frame.
-    hipLaunchKernelGGL(k_sur_pre<Cams>, grid, dim3(kBlock), 0, s, cams, k, n_items, sc, fl);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (!f64) {
-        if ((e = hipEventRecord(ss.ev_fork2, s)) != hipSuccess ||
-            (e = hipStreamWaitEvent(ss.aux2, ss.ev_fork2, 0)) != hipSuccess)
-            return e;
+    // Side marches: the FP64 band (BAND64) on aux and the early FP32 fallback on aux2.
+    auto side = [&]() -> hipError_t {
+        hipError_t e2;
+        if ((e2 = hipEventRecord(ss.ev_fork2, s)) != hipSuccess ||
+            (e2 = hipStreamWaitEvent(ss.aux2, ss.ev_fork2, 0)) != hipSuccess)
+            return e2;
         Scratch se = sc;
         se.work = fl.early_work;
-        if ((e = march_list(fl.early_ids, fl.early_count, se, ss.aux2)) != hipSuccess) return e;
-        if ((e = hipEventRecord(ss.ev_join2, ss.aux2)) != hipSuccess) return e;
-        if (any_band && (e = band()) != hipSuccess) return e;
+        if ((e2 = march_list(fl.early_ids, fl.early_count, se, ss.aux2)) != hipSuccess) return e2;
+        if ((e2 = hipEventRecord(ss.ev_join2, ss.aux2)) != hipSuccess) return e2;
+        if (any_band && (e2 = band()) != hipSuccess) return e2;
+        return hipSuccess;
+    };
+    hipError_t e;
+    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
+    if (routed) {
+        hipLaunchKernelGGL(k_sur_pre<Cams>, grid, dim3(kBlock), 0, s, cams, k, n_items, sc, fl);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = side()) != hipSuccess) return e;
     }
     if (tab) {
         const int64_t nodes = (int64_t)tab->n2 * tab->n3;
@@ -847,13 +867,15 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                            (const float*)tab->range, tab->n2, tab->n3, tab->node);
         hipLaunchKernelGGL(k_surrogate_tab<Cams>, grid, dim3(kBlock), 0, s, cams, k,
                            (const float*)tab->node, (const float*)tab->range, tab->n2, tab->n3,
-                           out, sc, fl, n_items);
+                           out, sc, fl, n_items, routed);
     } else {
         hipLaunchKernelGGL(k_surrogate<Cams>, dim3(persistent_blocks(gb0, num_cus)), dim3(kBlock),
-                           0, s, cams, k, (const uint8_t*)packed, out, sc, fl, n_items);
+                           0, s, cams, k, (const uint8_t*)packed, out, sc, fl, n_items,
+                           routed);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
+    if (!routed && !f64 && (e = side()) != hipSuccess) return e;
     if (f64) {   // every unserved ray is in the band list: FP64 march now
         if ((e = band()) != hipSuccess) return e;
     } else if ((e = march_list(fl.fb_ids, fl.fb_count, sc, s)) != hipSuccess) {

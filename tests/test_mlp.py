@@ -260,3 +260,32 @@ def test_network_gain_over_prior_alone(tracer):
     assert share_n > share_p + 0.05, (share_n, share_p)
     net.close()
     prior.close()
+
+
+@pytest.mark.gpu
+def test_throughput_mode_batch_equals_single_calls(tracer):
+    """Above 2^24 work items the batched call routes rays inside the network pass (no pre-pass);
+    two 3000x2800 cameras (16.8 M items) must still equal the two single-camera calls (latency
+    mode) bit for bit."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd.surrogate import Surrogate
+    sur = Surrogate.load(tracer, _model_path(MODEL_HIGH))
+    sur.set_table(256, 64)
+    cams = [bhs.orbit_camera(30.0, 85.0, 60.0, 3000, 2800, jitter_hashed=1, seed=5, camera_id=k,
+                             azimuth_deg=45.0 * k) for k in range(2)]
+    params = bhs.march_params()
+    print("[mlp] throughput-mode batch 2 x 3000x2800", flush=True)
+    b = sur.render_batch(cams, params)
+    torch.cuda.synchronize()
+    st = tracer.last_stats()
+    assert st["rays"] == 2 * 3000 * 2800 and st["surrogate"] > 0.9 * st["rays"]
+    off = tracer.batch_offsets(cams)
+    for k, cam in enumerate(cams):
+        one = sur.render(cam, params)
+        torch.cuda.synchronize()
+        for name in ("cls", "flags", "n_cross", "steps", "b"):
+            assert torch.equal(getattr(b, name)[off[k]:off[k + 1]], getattr(one, name)), (k, name)
+        for name in ("esc", "disk"):
+            assert torch.equal(getattr(b, name)[:, off[k]:off[k + 1]], getattr(one, name)), (k, name)
+    sur.close()
