@@ -1,13 +1,20 @@
-"""Benchmark: law-S thin-disk geodesic tracing at 1920x1080, 450 steps (r_obs = 30 M, inclination
-85 deg, FOV 60 deg, disk [6, 20] M, seeded per-pixel jitter), on one MI355X.
+"""Benchmark: Schwarzschild geodesic tracing of the 1080p camera (1920x1080, 450 steps, r_obs = 30 M,
+inclination 85 deg, FOV 60 deg, thin disk [6, 20] M, seeded per-pixel jitter) on one MI355X.
 
-One step = one full frame (2,073,600 rays) marched through the C-ABI (bhs_render) with the FP32
-march and the FP64 band near b_crit. Prints one JSON line (see --help for flags).
+One step = one frame (2,073,600 rays; --frames N: N jittered frames per call) through the C-ABI:
+  surrogate leg (value): bhs_render_surrogate / bhs_render_surrogate_batch with the committed model
+      blackholerender_amd/models/lawS_450_disk6-20.bhsmlp in table mode: exact-orbit prior + learned
+      residual for the served rays, the plain integrator (FP32, FP64 band) for all others;
+  plain leg: bhs_render (autotuned persistent FP32 march + FP64 band), timed the same way.
+Accuracy (outside the timed region): the last surrogate frame against an FP64 render of the same
+rays: escape-direction error of the served rays, class / crossing-count agreement of all rays.
 
-roofline: achieved FP32 TFLOP/s of the main march kernel = algorithmic flops of the FP32 rays of one
-launch (blackholerender_amd.workload model, from the kernel's own step/crossing/escape counts) /
-the kernel's average duration from HIP events recorded on its stream inside the timed region.
-cpu_baseline: the CPU oracle's FP64 law-S march of the same frame on this host's cores.
+roofline: the plain march kernel (the path's VALU-bound integrator): algorithmic FP32 flops of its
+rays (blackholerender_amd.workload model, from the kernel's own step/crossing/escape counts) / its
+average duration from HIP events recorded on its stream inside the timed region. traffic: HBM
+bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/traffic.json), if measured.
+cpu_baseline: the CPU oracle's FP64 law-S march of a sample of the same frame on this host's cores.
+--config c5: the full form instead (3840x2160, 8 spp, 4 poses, 700 steps, disk [6, 30]).
 """
 from __future__ import annotations
 
@@ -69,13 +76,72 @@ def run_cpu_baseline(cfg, max_seconds: float = 20.0) -> dict:
                        f"{threads} threads, {dt:.2f} s, CPU {cpu_model()}")}
 
 
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def _angle(a, b):
+    import numpy as np
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), (a * b).sum(1))
+
+
+def _accuracy(sur_out, plain_out, ref_out, n):
+    """Served-ray escape error and record agreement of a surrogate frame against FP64."""
+    import numpy as np
+    import blackholerender_amd as bhs
+    S, R = sur_out, ref_out
+    sv = (S["flags"] & bhs.FLAG_SURROGATE) > 0
+    esc = sv & ((S["flags"] & 2) > 0) & ((R["flags"] & 2) > 0)
+    e = _angle(S["esc"][esc], R["esc"][esc]) if esc.any() else np.zeros(1)
+    pe = ((plain_out["flags"] & 2) > 0) & ((R["flags"] & 2) > 0)
+    ep = _angle(plain_out["esc"][pe], R["esc"][pe])
+    fb = ~sv
+    return {
+        "served_fraction": round(float(sv.mean()), 5),
+        "escape_err_max_rad": float(e.max()), "escape_err_p9999_rad": float(np.quantile(e, 0.9999)),
+        "escape_err_p50_rad": float(np.median(e)),
+        "class_mismatch_served": int((S["cls"][sv] != R["cls"][sv]).sum()),
+        "n_cross_mismatch_served": int((S["n_cross"][sv] != R["n_cross"][sv]).sum()),
+        "fallback_identical_to_plain": bool(all(np.array_equal(S[k][fb], plain_out[k][fb])
+                                                for k in ("cls", "flags", "n_cross", "steps"))),
+        "plain_escape_err_max_rad": float(ep.max()),
+        "plain_class_mismatch": int((plain_out["cls"] != R["cls"]).sum()),
+        "rays": n,
+    }
+
+
+def _traffic():
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2", choices=["c2", "c5"])
+    ap.add_argument("--frames", type=int, default=1,
+                    help="jittered 1080p frames per call (one step = one call)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--precision", default="band", choices=["f32", "band", "f64"])
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 comparison")
+    ap.add_argument("--plain-only", action="store_true", help="time only the plain integrator")
+    ap.add_argument("--per-ray", action="store_true",
+                    help="surrogate with per-ray network inference instead of the table mode")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default)")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shape 'mode,block,waves_per_simd,refill_min' (skips the "
@@ -102,14 +168,24 @@ def main(argv=None) -> int:
     import torch
     import blackholerender_amd as bhs
     from blackholerender_amd import workload as W
+    from blackholerender_amd.surrogate import Surrogate
 
-    cfg = W.C2
-    prec = {"f32": bhs.PREC_F32, "band": bhs.PREC_F32_BAND64, "f64": bhs.PREC_F64}[args.precision]
     tracer = bhs.Tracer(0)
-    cam = W.camera_for(cfg)
-    params = W.params_for(cfg, precision=prec)
-    n = cam.n_rays
-    out = bhs.RayBuffers(n, "cuda:0")
+    if args.config == "c5":
+        cfg = W.C5
+        cams = W.cameras_c5()
+        model = "lawS_700_disk6-30.bhsmlp"
+        workload = ("full form: 3840x2160, 8 jittered spp, 4 poses (32 cameras, 265,420,800 rays "
+                    "per step), 700 steps, r_obs=15M, incl 80deg, FOV 60deg, thin disk [6,30]M")
+    else:
+        cfg = W.C2
+        cams = [W.camera_for(cfg, sample=k) for k in range(args.frames)]
+        model = "lawS_450_disk6-20.bhsmlp"
+        workload = ("1920x1080 (2,073,600 rays per frame, %d frame(s) per step), 450 steps, "
+                    "r_obs=30M, incl 85deg, FOV 60deg, thin disk [6,20]M" % args.frames)
+    params = W.params_for(cfg)
+    single = len(cams) == 1
+    n = sum(c.width * c.height for c in cams)
     stream = torch.cuda.current_stream()
     if args.tuning:
         tracer.set_tuning(*[int(v) for v in args.tuning.split(",")])
@@ -117,64 +193,104 @@ def main(argv=None) -> int:
     elif args.no_tune:
         tuned = tracer.get_tuning()
     else:
-        tuned = tracer.autotune(cam, params, out, reps=3)
-    for _ in range(args.warmup):
-        tracer.render(cam, params, out, stream)
+        tuned = tracer.autotune(cams[0], params, reps=3)
+
+    # ---- plain leg
+    pout = bhs.RayBuffers(n, "cuda:0")
+    if single:
+        plain = lambda: tracer.render(cams[0], params, pout, stream)
+    else:
+        plain = lambda: tracer.render_batch(cams, params, pout, stream)
+    tracer.set_timing(False)
+    plain()
     torch.cuda.synchronize()
     tracer.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tracer.render(cam, params, out, stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    tm = tracer.timing()
+    wall_p = _timed(plain, args.steps, args.warmup)
+    tm_p = tracer.timing()
     tracer.set_timing(False)
-    stats = tracer.last_stats()
-    wall = t1 - t0
-    ms_per_step = wall / args.steps * 1e3
-
+    plain()
+    pstats = tracer.last_stats()
     # algorithmic work of the FP32 rays of one launch (the main kernel's rays)
-    fl = out.flags.to(torch.int32)
+    fl = pout.flags.to(torch.int32)
     f32 = (fl & bhs.FLAG_BAND64) == 0
-    steps = out.steps_i32()
-    ncr = out.n_cross.to(torch.int32)
-    rays32 = int(f32.sum())
-    steps32 = int(steps[f32].sum())
-    esc32 = int(((fl & bhs.FLAG_ESCAPED) != 0)[f32].sum())
-    cross32 = int(ncr[f32].sum())
-    rec32 = int(torch.clamp(ncr, max=2)[f32].sum())
-    flops = W.march_flops(rays32, steps32, esc32, cross32, rec32)
-    tflops = flops / (tm["march_ms"] * 1e-3) / 1e12 if tm["march_ms"] > 0 else 0.0
+    steps = pout.steps_i32()
+    ncr = pout.n_cross.to(torch.int32)
+    flops = W.march_flops(int(f32.sum()), int(steps[f32].sum()),
+                          int(((fl & bhs.FLAG_ESCAPED) != 0)[f32].sum()), int(ncr[f32].sum()),
+                          int(torch.clamp(ncr, max=2)[f32].sum()))
+    tflops = flops / (tm_p["march_ms"] * 1e-3) / 1e12 if tm_p["march_ms"] > 0 else 0.0
+    plain_rate = n * args.steps / wall_p / 1e6
+    mean_steps = int(steps.sum()) / n
 
     res = {
         "metric": "Mrays/s at 1080p/450 steps: surrogate+fallback vs plain, <=1e-4 rad vs FP64",
-        "value": round(n * args.steps / wall / 1e6, 2),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if prec != bhs.PREC_F64 else "f64",
-        "data": "synthetic (seeded per-pixel jitter)",
-        "config": {"workload": "plain law-S march, 1920x1080 (2,073,600 rays), 450 steps, "
-                               "r_obs=30M, incl 85deg, FOV 60deg, thin disk [6,20]M, "
-                               f"precision={args.precision}",
-                   "path": "plain", "rays_per_step": n, "parallelism": "single GPU",
-                   "launch": tuned},
-        "roofline": {"bound": "valu", "achieved": round(tflops, 3),
-                     "peak": W.PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(tflops / W.PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_render_f32" if prec != bhs.PREC_F64 else "k_render_f64",
-                     "kernel_ms": round(tm["march_ms"], 5), "call_ms": round(tm["total_ms"], 5),
-                     "flops_per_launch": flops},
-        "march": {"mean_steps": round(int(steps.sum()) / n, 3), "band64_rays": stats["band64"],
-                  "escapes": stats["escapes"], "crossings": stats["crossings"]},
+        "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded per-pixel jitter; committed trained model)",
     }
-    if not args.no_cpu:
+    roof = {"bound": "valu", "achieved": round(tflops, 3), "peak": W.PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / W.PEAK_FP32_TFLOPS, 4),
+            "traffic": None, "kernel": "k_persist (plain FP32 law-S march)",
+            "kernel_ms": round(tm_p["march_ms"], 5), "flops_per_launch": flops}
+    tr = _traffic()
+    if tr and args.config == "c2" and single:
+        roof["traffic"] = tr.get("bytes_per_launch")
+        roof["traffic_source"] = tr.get("source")
+    plain_rec = {"value": round(plain_rate, 2), "ms_per_step": round(wall_p / args.steps * 1e3, 4),
+                 "kernel_ms": round(tm_p["march_ms"], 5), "call_ms": round(tm_p["total_ms"], 5),
+                 "mean_steps": round(mean_steps, 3), "band64_rays": pstats["band64"]}
+
+    if args.plain_only:
+        res.update({"value": plain_rec["value"], "ms_per_step": plain_rec["ms_per_step"],
+                    "config": {"workload": workload, "path": "plain", "launch": tuned},
+                    "roofline": roof, "plain": plain_rec})
+    else:
+        # ---- surrogate leg
+        sur = Surrogate.load(tracer, os.path.join(ROOT, "blackholerender_amd", "models", model))
+        if not args.per_ray:
+            sur.set_table(256, 64)
+        sout = bhs.RayBuffers(n, "cuda:0")
+        if single:
+            surf = lambda: sur.render(cams[0], params, sout, stream)
+        else:
+            surf = lambda: sur.render_batch(cams, params, sout, stream)
+        surf()
+        torch.cuda.synchronize()
+        tracer.set_timing(True)
+        wall_s = _timed(surf, args.steps, args.warmup)
+        tm_s = tracer.timing()
+        tracer.set_timing(False)
+        surf()
+        sstats = tracer.last_stats()
+        sur_rate = n * args.steps / wall_s / 1e6
+        res.update({
+            "value": round(sur_rate, 2),
+            "ms_per_step": round(wall_s / args.steps * 1e3, 4),
+            "speedup_vs_plain": round(sur_rate / plain_rate, 3),
+            "config": {"workload": workload, "path": "surrogate+fallback",
+                       "surrogate": "table 256x64" if not args.per_ray else "per-ray network",
+                       "model": model, "rays_per_step": n, "parallelism": "single GPU",
+                       "launch": tuned},
+            "roofline": roof,
+            "plain": plain_rec,
+            "surrogate": {"network_part_ms": round(tm_s["march_ms"], 5),
+                          "call_ms": round(tm_s["total_ms"], 5),
+                          "served": sstats["surrogate"], "fallback": sstats["fallback"],
+                          "band64": sstats["band64"],
+                          "fallback_steps": sstats["steps"]},
+        })
+        if not args.no_accuracy:
+            cam0 = cams[0]
+            n0 = cam0.width * cam0.height
+            p64 = W.params_for(cfg, precision=bhs.PREC_F64)
+            ref = tracer.render(cam0, p64)
+            pl0 = tracer.render(cam0, params)
+            s0 = sur.render(cam0, params)
+            torch.cuda.synchronize()
+            res["accuracy"] = _accuracy(s0.numpy(), pl0.numpy(), ref.numpy(), n0)
+            res["accuracy"]["frame"] = "camera 0 of the step vs FP64 law-S march of the same rays"
+        sur.close()
+    if not args.no_cpu and args.config == "c2":
         res["cpu_baseline"] = run_cpu_baseline(cfg)
     print(json.dumps(res), flush=True)
     tracer.close()
