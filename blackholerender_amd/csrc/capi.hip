@@ -189,6 +189,7 @@ void layout_scratch(void* mem, int64_t n_items, bool roots, bhs::Scratch* sc) {
     sc->stats = (bhs::Stats*)base;
     sc->band_count = (uint32_t*)(base + sizeof(bhs::Stats));
     sc->work = (uint32_t*)(base + sizeof(bhs::Stats) + 4);
+    sc->band_work = (uint32_t*)(base + sizeof(bhs::Stats) + 8);
     sc->band_ids = (uint32_t*)(base + kStatsBytes);
     sc->band_cap = (uint32_t)n_items;
     sc->roots = roots ? (bhs::RootState<float>*)(base + kStatsBytes + band_bytes) : nullptr;

@@ -44,7 +44,7 @@ struct bhs_ctx {
 namespace bhs {
 namespace capi {
 
-constexpr size_t kStatsBytes = 256;   // Stats + band counter + work counter, padded
+constexpr size_t kStatsBytes = 256;   // Stats + band counter + work counters, padded
 
 // Records `fmt` (one %s for `detail`) as the context's last error and returns `code`.
 int fail(bhs_ctx* ctx, int code, const char* fmt, const char* detail = nullptr);

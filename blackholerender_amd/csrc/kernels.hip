@@ -3,6 +3,8 @@
 // law-T (bhs::physics::trace) kernel.
 #include "kernels.hpp"
 
+#include <type_traits>
+
 #include "../../include/bhs_trace.h"
 
 namespace bhs {
@@ -132,11 +134,42 @@ struct GlobalSink {
     }
 };
 
+// Sink of the FP64 band march: records straight to the SoA output; with ROOTS the two crossing
+// states stay in the lane's registers (slots selected by constant indices) until the ray ends.
+template <bool ROOTS>
+struct BandSink {
+    const SoA& o;
+    int64_t i;
+    RootState<double>& rs0;
+    RootState<double>& rs1;
+    __device__ __forceinline__ void escape(float ex, float ey, float ez) {
+        if (o.esc) {
+            o.esc[i] = ex;
+            o.esc[i + o.n] = ey;
+            o.esc[i + 2 * o.n] = ez;
+        }
+    }
+    __device__ __forceinline__ void cross(uint32_t slot, float r, float phi, float g,
+                                          const RootState<double>& q) {
+        if (o.disk) {
+            const int64_t j = i + (int64_t)slot * o.n;
+            o.disk[j] = r;
+            o.disk[j + 2 * o.n] = phi;
+            o.disk[j + 4 * o.n] = g;
+        }
+        if (ROOTS) {
+            if (slot == 0) rs0 = q; else rs1 = q;
+        }
+    }
+};
+
 // Scalar outputs of a finished ray, and zeros for the records it did not produce.
-__device__ __forceinline__ void store_finish(const SoA& o, int64_t i, const MarchState<float>& st,
-                                             float b) {
+template <typename R>
+__device__ __forceinline__ void store_finish(const SoA& o, int64_t i, const MarchState<R>& st,
+                                             float b, uint32_t extra_flags = 0u) {
     uint32_t cls, flags;
     march_class(st, cls, flags);
+    flags |= extra_flags;
     const int64_t n = o.n;
     if (o.cls) o.cls[i] = (uint8_t)cls;
     if (o.flags) o.flags[i] = (uint8_t)flags;
@@ -183,49 +216,182 @@ __device__ __forceinline__ RayOut march_f64(const MarchConst& c, const V3<float>
 }
 
 // ------------------------------------------------------------------------------------------------
-// Lists the work items of the FP64 band (float setup, |b - b_crit| < band_abs).
+// Lists the work items of the FP64 band (float setup, |b - b_crit| < band_abs): one atomic per
+// wave (ballot of the band lanes, per-lane prefix count); list order is unspecified.
 template <typename Cams>
 __global__ __launch_bounds__(kBlock) void k_band_scan(const Cams cams, Scratch sc) {
-    for (uint32_t q = blockIdx.x * kBlock + threadIdx.x; q < cams.n_items;
-         q += gridDim.x * kBlock) {
-        const uint32_t ci = cams.find(q);
-        const MarchConst& c = cams.cam(ci);
-        if (c.precision != BHS_PREC_F32_BAND64) continue;
-        const int64_t pix = item_pixel(c, q - cams.base(ci));
-        if (pix < 0) continue;
-        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-        const Setup<float> s = lawS_setup(P, ray_dir(c, pix));
-        if (in_band(c, s.b)) {
-            const uint32_t k = atomicAdd(sc.band_count, 1u);
+    for (uint32_t q0 = blockIdx.x * kBlock; q0 < cams.n_items; q0 += gridDim.x * kBlock) {
+        const uint32_t q = q0 + threadIdx.x;
+        bool want = false;
+        if (q < cams.n_items) {
+            const uint32_t ci = cams.find(q);
+            const MarchConst& c = cams.cam(ci);
+            const int64_t pix = item_pixel(c, q - cams.base(ci));
+            if (c.precision == BHS_PREC_F32_BAND64 && pix >= 0) {
+                const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+                want = in_band(c, lawS_setup(P, ray_dir(c, pix)).b);
+            }
+        }
+        const uint64_t m = __ballot(want);
+        if (m == 0) continue;
+        uint32_t base = 0;
+        if ((threadIdx.x & 63u) == 0) base = atomicAdd(sc.band_count, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, 0);
+        if (want) {
+            const uint32_t k = base + __builtin_amdgcn_mbcnt_hi(
+                                          (uint32_t)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
             if (k < sc.band_cap) sc.band_ids[k] = q;
         }
     }
 }
 
-// FP64 march of the band list (grid-stride; the count is read on the device).
-template <typename Cams>
-__global__ __launch_bounds__(64) void k_band64(const Cams cams, const SoA out, Scratch sc) {
+// FP64 refinement of crossing `slot` of a band ray (as march_f64): the refined record replaces the
+// chord one when the double step brackets the root.
+__device__ __forceinline__ void refine_band_crossing(const SoA& o, int64_t i, uint32_t slot,
+                                                     const RootState<double>& q,
+                                                     const Setup<double>& s) {
+    double hx, hz;
+    if (!root64(q.x0, q.v0, q.dt, s.c3, hx, hz)) return;
+    const double rh = __builtin_sqrt(__builtin_fma(hz, hz, hx * hx));
+    const int64_t j = i + (int64_t)slot * o.n;
+    o.disk[j] = (float)rh;
+    o.disk[j + 2 * o.n] = atan2_ieee((float)hz, (float)hx);
+    o.disk[j + 4 * o.n] = (float)redshift(rh, s.b_axis, s.lapse_g);
+}
+
+// Work-queue refill of a wave's idle lanes (every lane of the wave calls it with the same
+// `more`): when at least `refill_min` lanes are idle, one atomic on `counter` claims that many
+// queue positions and idle lane j (in lane order) gets position base + j. Returns true for a lane
+// that received a position below n (in q); clears `more` once the queue is exhausted.
+__device__ __forceinline__ bool claim_items(uint32_t* counter, uint32_t n, uint32_t refill_min,
+                                            bool active, bool& more, uint32_t& q) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (!more || n_idle < refill_min) return false;
+    uint32_t base = 0;
+    if ((threadIdx.x & 63u) == 0) base = atomicAdd(counter, n_idle);
+    base = (uint32_t)__shfl((int)base, 0);
+    if (base + n_idle >= n) more = false;
+    if (active) return false;
+    q = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                         __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+    return q < n;
+}
+
+// Refill threshold of the band march: its rays run hundreds of FP64 iterations, so a refill (one
+// setup) is cheap next to an idle lane; 32 keeps the queue atomics to one per 32 rays.
+constexpr uint32_t kBandRefill = 32;
+
+// FP64 march of the band list (the count is read on the device). A list longer than the grid's
+// lanes (large batches) is marched by resident waves pulling rays from it (queue counter
+// sc.band_work); finished lanes are refilled as in k_persist, so a wave is not held by its slowest
+// ray. A shorter list is marched one ray per lane. One camera (OneCam, constants in kernel arguments) or a
+// batch (CamTable: camera table, FP64 march constants and item prefix sums staged in LDS). ROOTS:
+// the FP64 root refinement of march_f64 for cameras with root64. Exit: the list is finite, every
+// ray ends within max_steps iterations and the counter only grows, so every wave leaves.
+template <typename Cams, bool ROOTS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ROOTS ? 2 : 3)))
+void k_band64(const Cams cams, const SoA out, Scratch sc) {
+    constexpr bool kBatch = std::is_same<Cams, CamTable>::value;
+    __shared__ MarchConst s_cam[kBatch ? kMaxBatchCams : 1];
+    __shared__ MarchK<double> s_k[kBatch ? kMaxBatchCams : 1];
+    __shared__ uint32_t s_base[kBatch ? kMaxBatchCams + 1 : 1];
+    uint32_t ncam = 1;
+    if constexpr (kBatch) {
+        ncam = cams.ncam;
+        for (uint32_t i = threadIdx.x; i < ncam; i += 64) {
+            s_cam[i] = cams.cams[i];
+            s_k[i] = march_consts<double>(cams.cams[i]);
+        }
+        for (uint32_t i = threadIdx.x; i <= ncam; i += 64) s_base[i] = cams.item_base[i];
+        __syncthreads();
+    }
     // Few waves with long serial chains run beside the FP32 march: issue them first on their SIMD.
     __builtin_amdgcn_s_setprio(3);
-    const uint32_t cnt = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
-    for (uint32_t base = blockIdx.x * 64; base < cnt; base += gridDim.x * 64) {
-        const uint32_t k = base + threadIdx.x;
-        uint32_t steps = 0, cross = 0, esc = 0, one = 0;
-        if (k < cnt) {
-            const uint32_t q = sc.band_ids[k];
-            const uint32_t ci = cams.find(q);
-            const MarchConst& c = cams.cam(ci);
-            const int64_t pix = item_pixel(c, q - cams.base(ci));
-            const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-            const RayOut o = march_f64(c, P, ray_dir(c, pix));
-            store_ray(out, c.ray_offset + pix, o);
-            steps = (uint32_t)o.steps;
-            cross = o.n_cross;
-            esc = (o.flags & 2u) ? 1u : 0u;
-            one = 1;
+    uint32_t ci0 = 0;
+    const MarchConst& c1 = cams.cam(0);
+    const MarchK<double> k1 = march_consts<double>(c1);
+    const uint32_t n = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
+    if (n <= gridDim.x * 64u) {
+        // A list the grid covers at once (the band of a frame): one ray per lane, no queue. Its
+        // longest ray sets the latency, which the queue loop lengthens.
+        const uint32_t q = blockIdx.x * 64u + threadIdx.x;
+        uint32_t one = 0, steps = 0, cross = 0, esc = 0;
+        if (q < n) {
+            const uint32_t item = sc.band_ids[q];
+            uint32_t local = item;
+            if constexpr (kBatch) {
+                ci0 = find_cam(s_base, ncam, item);
+                local = item - s_base[ci0];
+            }
+            const MarchConst& c = kBatch ? s_cam[ci0] : c1;
+            const int64_t p = item_pixel(c, local);
+            if (p >= 0) {
+                const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+                const RayOut o = march_f64(c, P, ray_dir(c, p));
+                store_ray(out, c.ray_offset + p, o);
+                one = 1;
+                steps = (uint32_t)o.steps;
+                cross = o.n_cross;
+                esc = (o.flags & 2u) ? 1u : 0u;
+            }
         }
         add_stats(sc.stats, one, steps, cross, esc, one);
+        return;
     }
+    MarchState<double> st;
+    Setup<double> s;
+    RootState<double> rs0, rs1;
+    uint32_t pix = 0, ci = 0;
+    bool active = false;
+    bool more = true;   // wave-uniform: the list may still hold rays
+    uint32_t n_rays = 0, n_steps = 0, n_cross = 0, n_esc = 0;
+    while (true) {
+        uint32_t q = 0;
+        if (claim_items(sc.band_work, n, kBandRefill, active, more, q)) {
+            const uint32_t item = sc.band_ids[q];
+            uint32_t local = item;
+            if constexpr (kBatch) {
+                ci = find_cam(s_base, ncam, item);
+                local = item - s_base[ci];
+            }
+            const MarchConst& c = kBatch ? s_cam[ci] : c1;
+            const int64_t p = item_pixel(c, local);
+            if (p >= 0) {
+                pix = (uint32_t)p;
+                const V3<float> df = ray_dir(c, p);
+                const V3<double> P{(double)c.pos[0], (double)c.pos[1], (double)c.pos[2]};
+                const V3<double> d{(double)df.x, (double)df.y, (double)df.z};
+                s = lawS_setup(P, d);
+                march_init(st, P, s);
+                active = true;
+            }
+        }
+        if (!__any(active)) {
+            if (!more) break;
+            continue;
+        }
+        const MarchConst& c = kBatch ? s_cam[ci] : c1;
+        const int64_t gi = c.ray_offset + pix;
+        BandSink<ROOTS> sink{out, gi, rs0, rs1};
+        const bool fin = march_step(st, s, kBatch ? s_k[ci] : k1, sink, active);
+        if (active && fin) {
+            uint32_t extra = BHS_FLAG_BAND64;
+            if (ROOTS && c.root64 && st.n_cross > 0) {
+                refine_band_crossing(out, gi, 0u, rs0, s);
+                if (st.n_cross > 1) refine_band_crossing(out, gi, 1u, rs1, s);
+                extra |= BHS_FLAG_ROOT64;
+            }
+            store_finish(out, gi, st, (float)s.b, extra);
+            n_rays += 1;
+            n_steps += (uint32_t)st.i;
+            n_cross += st.n_cross;
+            n_esc += st.status == 1 ? 1u : 0u;
+            active = false;
+        }
+    }
+    add_stats(sc.stats, n_rays, n_steps, n_cross, n_esc, n_rays);
 }
 
 // FP64 root refinement of the crossings recorded by the FP32 march (all items).
@@ -341,7 +507,6 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
         __syncthreads();
     }
     const MarchK<float> k1 = march_consts<float>(c1);
-    const uint32_t lane = threadIdx.x & 63u;
     if (list) {   // listed items: the count is known on the device only
         const uint32_t cnt = *list_count;
         n_items = cnt < n_items ? cnt : n_items;
@@ -356,35 +521,23 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
     bool more = true;   // wave-uniform: the queue may still hold items
     uint32_t n_rays = 0, n_steps = 0, n_cross = 0, n_esc = 0;
     while (true) {
-        const uint64_t idle = __ballot(!active);
-        const uint32_t n_idle = (uint32_t)__popcll(idle);
-        if (more && n_idle >= refill_min) {
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(sc.work, n_idle);
-            base = (uint32_t)__shfl((int)base, 0);
-            if (base + n_idle >= n_items) more = false;
-            if (!active) {
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-                    (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                const uint32_t q = base + rank;
-                if (q < n_items) {
-                    const uint32_t item = list ? list[q] : q;
-                    uint32_t local = item;
-                    if (BATCH) {
-                        ci = find_cam(s_base, ncam, item);
-                        local = item - s_base[ci];
-                    }
-                    const MarchConst& c = BATCH ? s_cam[ci] : c1;
-                    const int64_t p = item_pixel(c, local);
-                    if (p >= 0) {
-                        pix = (uint32_t)p;
-                        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-                        s = lawS_setup(P, ray_dir(c, p));
-                        if (!(c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.b))) {
-                            march_init(st, P, s);
-                            active = true;
-                        }
-                    }
+        uint32_t q = 0;
+        if (claim_items(sc.work, n_items, refill_min, active, more, q)) {
+            const uint32_t item = list ? list[q] : q;
+            uint32_t local = item;
+            if (BATCH) {
+                ci = find_cam(s_base, ncam, item);
+                local = item - s_base[ci];
+            }
+            const MarchConst& c = BATCH ? s_cam[ci] : c1;
+            const int64_t p = item_pixel(c, local);
+            if (p >= 0) {
+                pix = (uint32_t)p;
+                const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+                s = lawS_setup(P, ray_dir(c, p));
+                if (!(c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.b))) {
+                    march_init(st, P, s);
+                    active = true;
                 }
             }
         }
@@ -513,8 +666,9 @@ __global__ void k_zero(uint32_t* p, size_t words) {
         p[i] = 0u;
 }
 
-// Grid of the FP64 band march (64-thread blocks, grid-stride over the device-side list): enough
-// waves for band lists of millions of rays (large batches); surplus waves find no work and exit.
+// Grid of the FP64 band march (64-thread blocks, resident waves pulling from the device-side
+// list): up to 4096 waves (4 per SIMD on 256 CUs) for band lists of millions of rays; surplus waves
+// find the list exhausted and exit.
 dim3 band_grid(uint32_t cap) {
     uint32_t b = (cap + 63u) / 64u;
     b = b > 4096u ? 4096u : b;
@@ -572,6 +726,24 @@ hipError_t launch_persist(const Tuning& t, int num_cus, hipStream_t s, const Mar
     return hipGetLastError();
 }
 
+// FP64 march of the band list on `aux`; the ROOTS variant (crossing states in registers) only for
+// launches with FP64 root refinement.
+template <typename Cams>
+hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hipStream_t aux) {
+    bool roots;
+    if constexpr (std::is_same<Cams, OneCam>::value)
+        roots = cams.c.root64 != 0;
+    else
+        roots = sc.roots != nullptr;   // batches allocate root states iff a camera has root64
+    if (roots)
+        hipLaunchKernelGGL((k_band64<Cams, true>), band_grid(sc.band_cap), dim3(64), 0, aux, cams,
+                           out, sc);
+    else
+        hipLaunchKernelGGL((k_band64<Cams, false>), band_grid(sc.band_cap), dim3(64), 0, aux, cams,
+                           out, sc);
+    return hipGetLastError();
+}
+
 // The band fork: scan on s, FP64 march on aux between ev_fork and ev_join.
 template <typename Cams>
 hipError_t fork_band(const Cams& cams, const SoA& out, const Scratch& sc, hipStream_t s,
@@ -581,8 +753,7 @@ hipError_t fork_band(const Cams& cams, const SoA& out, const Scratch& sc, hipStr
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_band64<Cams>, band_grid(sc.band_cap), dim3(64), 0, aux, cams, out, sc);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_band64(cams, out, sc, aux)) != hipSuccess) return e;
     return hipEventRecord(ev_join, aux);
 }
 
@@ -673,8 +844,7 @@ hipError_t launch_band_list(const MarchConst& c, const SoA& out, const Scratch& 
     hipError_t e;
     if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_band64<OneCam>, band_grid(sc.band_cap), dim3(64), 0, aux, one, out, sc);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_band64(one, out, sc, aux)) != hipSuccess) return e;
     return hipEventRecord(ev_join, aux);
 }
 
@@ -709,8 +879,7 @@ hipError_t launch_band_list_batch(const MarchConst* table, const uint32_t* item_
     hipError_t e;
     if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_band64<CamTable>, band_grid(sc.band_cap), dim3(64), 0, aux, cams, out, sc);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_band64(cams, out, sc, aux)) != hipSuccess) return e;
     return hipEventRecord(ev_join, aux);
 }
 

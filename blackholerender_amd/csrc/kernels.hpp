@@ -29,6 +29,7 @@ struct Stats {
 struct Scratch {
     Stats* stats;
     uint32_t* work;              // work-queue counter of the persistent kernels
+    uint32_t* band_work;         // work-queue counter of the FP64 band march
     uint32_t* band_count;        // number of rays in the FP64 band list
     uint32_t* band_ids;          // work items (camera tile order) of the FP64 band rays
     uint32_t band_cap;           // capacity of band_ids (>= the launch's work items)

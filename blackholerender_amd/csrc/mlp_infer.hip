@@ -399,6 +399,11 @@ enum Route : uint32_t {
     kNet = 3,     // network candidate (the decode may still send it back)
 };
 
+// Lists already built when the network pass runs (bit mask): the FP64 band (by the pre-pass or
+// the integrator's band scan) and the early fallback list (by the pre-pass).
+constexpr uint32_t kListedBand = 1u;
+constexpr uint32_t kListedEarly = 2u;
+
 struct RayPrep {
     Setup<float> s;
     RayGeom g;
@@ -461,7 +466,7 @@ template <typename Cams>
 __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const SurConst k,
                                                          const uint8_t* __restrict__ packed,
                                                          const SoA out, Scratch sc, SurLists fl,
-                                                         uint32_t n_items, bool routed) {
+                                                         uint32_t n_items, uint32_t listed) {
     __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
     __shared__ float s_y[4][64 * kOut];
     __shared__ uint32_t s_cnt[5];
@@ -493,11 +498,11 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
             }
         }
         push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
-        if (!routed) {   // no pre-pass: band and early lists from this pass
+        if (!(listed & kListedBand))
             push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
+        if (!(listed & kListedEarly))
             push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q,
                           s_cnt);
-        }
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
@@ -611,7 +616,7 @@ __global__ __launch_bounds__(kBlock) void k_surrogate_tab(const Cams cams, const
                                                           const float* __restrict__ range,
                                                           int32_t n2, int32_t n3, const SoA out,
                                                           Scratch sc, SurLists fl,
-                                                          uint32_t n_items, bool routed) {
+                                                          uint32_t n_items, uint32_t listed) {
     __shared__ uint32_t s_cnt[5];
     __shared__ uint32_t s_st[4][4];
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
@@ -637,11 +642,11 @@ __global__ __launch_bounds__(kBlock) void k_surrogate_tab(const Cams cams, const
             }
         }
         push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
-        if (!routed) {   // no pre-pass: band and early lists from this pass
+        if (!(listed & kListedBand))
             push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
+        if (!(listed & kListedEarly))
             push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q,
                           s_cnt);
-        }
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
@@ -815,9 +820,12 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     // marches, whose serial rays are the longest chains of a frame, run beside the network pass;
     // their lists are short, and two resident waves per SIMD (131k lanes) leave the rest of the
     // chip to the network pass. Throughput mode (larger calls): the network pass routes the rays
-    // itself (the pre-pass would cost a quarter of the surrogate's work) and the fallback lists,
-    // millions of rays long, are marched after it at the tuned occupancy.
+    // itself and the lists, millions of rays long, are marched after it at the tuned occupancy
+    // (the FP64 band beside the FP32 lists, whose tails it fills). Starting the band before the
+    // network pass gains nothing there: both are VALU-bound, and the pre-pass or band scan is
+    // extra work.
     const bool routed = !f64 && n_items <= kRoutedItems;
+    const uint32_t listed = routed ? (kListedBand | kListedEarly) : 0u;
     Tuning tl = t;
     tl.mode = 1;
     tl.block = 256;
@@ -837,7 +845,7 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
             return launch_band_list_batch(table, item_base, ncam, n_items, out, sc, s, ss.aux,
                                           ss.ev_fork, ss.ev_join);
     };
-    // Side marches: the FP64 band (BAND64) on aux and the early FP32 fallback on aux2.
+    // Side marches: the early FP32 fallback on aux2 and the FP64 band on aux.
     auto side = [&]() -> hipError_t {
         hipError_t e2;
         if ((e2 = hipEventRecord(ss.ev_fork2, s)) != hipSuccess ||
@@ -867,11 +875,11 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                            (const float*)tab->range, tab->n2, tab->n3, tab->node);
         hipLaunchKernelGGL(k_surrogate_tab<Cams>, grid, dim3(kBlock), 0, s, cams, k,
                            (const float*)tab->node, (const float*)tab->range, tab->n2, tab->n3,
-                           out, sc, fl, n_items, routed);
+                           out, sc, fl, n_items, listed);
     } else {
         hipLaunchKernelGGL(k_surrogate<Cams>, dim3(persistent_blocks(gb0, num_cus)), dim3(kBlock),
                            0, s, cams, k, (const uint8_t*)packed, out, sc, fl, n_items,
-                           routed);
+                           listed);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
