@@ -1,18 +1,24 @@
-"""Benchmark: Schwarzschild geodesic tracing of the 1080p camera (1920x1080, 450 steps, r_obs = 30 M,
-inclination 85 deg, FOV 60 deg, thin disk [6, 20] M, seeded per-pixel jitter) on one MI355X.
+"""Benchmark: Schwarzschild geodesic tracing at 1920x1080, 450 steps (r_obs = 30 M, inclination
+85 deg, FOV 60 deg, thin disk [6, 20] M, seeded sub-pixel jitter per frame) on one MI355X.
 
-One step = one frame (2,073,600 rays; --frames N: N jittered frames per call) through the C-ABI:
-  surrogate leg (value): bhs_render_surrogate / bhs_render_surrogate_batch with the committed model
-      blackholerender_amd/models/lawS_450_disk6-20.bhsmlp in table mode: exact-orbit prior + learned
-      residual for the served rays, the plain integrator (FP32, FP64 band) for all others;
-  plain leg: bhs_render (autotuned persistent FP32 march + FP64 band), timed the same way.
-Accuracy (outside the timed region): the last surrogate frame against an FP64 render of the same
-rays: escape-direction error of the served rays, class / crossing-count agreement of all rays.
+One step = one call through the C-ABI over a batch of 32 jittered 1080p frames (66,355,200 rays;
+--frames N for another count, 1 = one frame per call):
+  surrogate leg (value): bhs_render_surrogate_batch with the committed model
+      blackholerender_amd/models/lawS_450_disk6-20.bhsmlp in table mode (per pose: the network
+      residuals and the exact Schwarzschild orbit tabulated over (b/b_crit, psi)), the plain
+      integrator (FP32, FP64 band) for every ray the surrogate does not serve;
+  plain leg: bhs_render_batch (autotuned persistent FP32 march + FP64 band) over the same frames,
+      timed the same way.
+single_frame: both legs again at one frame per call (latency-bound).
+Accuracy (outside the timed region): the first frame's surrogate render against FP64 renders of
+the same rays: escape-direction error of the served rays, class / crossing-count agreement, and
+crossing-radius error against the FP64 march's chord and refined crossings.
 
 roofline: the plain march kernel (the path's VALU-bound integrator): algorithmic FP32 flops of its
 rays (blackholerender_amd.workload model, from the kernel's own step/crossing/escape counts) / its
 average duration from HIP events recorded on its stream inside the timed region. traffic: HBM
-bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/traffic.json), if measured.
+bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/traffic.json), if measured for the
+workload.
 cpu_baseline: the CPU oracle's FP64 law-S march of a sample of the same frame on this host's cores.
 --config c5: the full form instead (3840x2160, 8 spp, 4 poses, 700 steps, disk [6, 30]).
 """
@@ -95,8 +101,23 @@ def _angle(a, b):
     return np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), (a * b).sum(1))
 
 
-def _accuracy(sur_out, plain_out, ref_out, n):
-    """Served-ray escape error and record agreement of a surrogate frame against FP64."""
+def _radius_err(A, R, sel):
+    """Relative error of the recorded crossing radii of rays `sel` (both records present)."""
+    import numpy as np
+    errs = []
+    for k in range(2):
+        m = sel & (A["n_cross"] > k) & (R["n_cross"] > k)
+        if m.any():
+            errs.append(np.abs(A["disk_r"][m, k].astype(np.float64) - R["disk_r"][m, k]) /
+                        R["disk_r"][m, k])
+    e = np.concatenate(errs) if errs else np.zeros(1)
+    return {"max": float(e.max()), "p999": float(np.quantile(e, 0.999)),
+            "p50": float(np.median(e)), "n": int(e.size)}
+
+
+def _accuracy(sur_out, plain_out, ref_out, ref_root_out, n):
+    """Served-ray escape error, crossing-radius error and record agreement of a surrogate frame
+    against FP64 (chord crossings, and refined crossings ref_root_out)."""
     import numpy as np
     import blackholerender_amd as bhs
     S, R = sur_out, ref_out
@@ -116,6 +137,9 @@ def _accuracy(sur_out, plain_out, ref_out, n):
                                                 for k in ("cls", "flags", "n_cross", "steps"))),
         "plain_escape_err_max_rad": float(ep.max()),
         "plain_class_mismatch": int((plain_out["cls"] != R["cls"]).sum()),
+        "disk_r_rel_err_served_vs_fp64_refined": _radius_err(S, ref_root_out, sv),
+        "disk_r_rel_err_served_vs_fp64_chord": _radius_err(S, R, sv),
+        "disk_r_rel_err_plain_vs_fp64_chord": _radius_err(plain_out, R, sv),
         "rays": n,
     }
 
@@ -135,13 +159,17 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=["c2", "c5"])
-    ap.add_argument("--frames", type=int, default=1,
+    ap.add_argument("--frames", type=int, default=32,
                     help="jittered 1080p frames per call (one step = one call)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the one-frame-per-call record (single_frame)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 comparison")
     ap.add_argument("--plain-only", action="store_true", help="time only the plain integrator")
     ap.add_argument("--per-ray", action="store_true",
                     help="surrogate with per-ray network inference instead of the table mode")
+    ap.add_argument("--table", default="1024x256",
+                    help="table-mode grid N2xN3 over (b/b_crit, psi) features")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default)")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shape 'mode,block,waves_per_simd,refill_min' (skips the "
@@ -181,12 +209,10 @@ def main(argv=None) -> int:
         cfg = W.C2
         cams = [W.camera_for(cfg, sample=k) for k in range(args.frames)]
         model = "lawS_450_disk6-20.bhsmlp"
-        workload = ("1920x1080 (2,073,600 rays per frame, %d frame(s) per step), 450 steps, "
-                    "r_obs=30M, incl 85deg, FOV 60deg, thin disk [6,20]M" % args.frames)
+        workload = ("1920x1080 frames (2,073,600 rays each, seeded sub-pixel jitter per frame), "
+                    "%d frame(s) per step = %s rays, 450 steps, r_obs=30M, incl 85deg, FOV 60deg, "
+                    "thin disk [6,20]M" % (args.frames, format(2073600 * args.frames, ",")))
     params = W.params_for(cfg)
-    single = len(cams) == 1
-    n = sum(c.width * c.height for c in cams)
-    stream = torch.cuda.current_stream()
     if args.tuning:
         tracer.set_tuning(*[int(v) for v in args.tuning.split(",")])
         tuned = tracer.get_tuning()
@@ -194,8 +220,78 @@ def main(argv=None) -> int:
         tuned = tracer.get_tuning()
     else:
         tuned = tracer.autotune(cams[0], params, reps=3)
+    sur = None
+    tab = None
+    if not args.plain_only:
+        sur = Surrogate.load(tracer, os.path.join(ROOT, "blackholerender_amd", "models", model))
+        if not args.per_ray:
+            tab = tuple(int(v) for v in args.table.lower().split("x"))
+            sur.set_table(*tab)
 
-    # ---- plain leg
+    plain_rec, sur_rec, roof = _legs(tracer, sur, cams, params, args.steps, args.warmup)
+    res = {
+        "metric": "Mrays/s at 1080p/450 steps: surrogate+fallback vs plain, <=1e-4 rad vs FP64",
+        "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded per-pixel jitter; committed trained model)",
+    }
+    tr = _traffic()
+    key = f"{args.config}x{len(cams)}"
+    if tr and key in tr.get("workloads", {}):
+        roof["traffic"] = tr["workloads"][key]["bytes_per_launch"]
+        roof["traffic_source"] = tr["workloads"][key]["source"]
+    n = sum(c.width * c.height for c in cams)
+    config = {"workload": workload, "rays_per_step": n, "parallelism": "single GPU",
+              "launch": tuned}
+    if sur is None:
+        res.update({"value": plain_rec["value"], "ms_per_step": plain_rec["ms_per_step"],
+                    "config": dict(config, path="plain"), "roofline": roof, "plain": plain_rec})
+    else:
+        res.update({
+            "value": sur_rec.pop("value"),
+            "ms_per_step": sur_rec.pop("ms_per_step"),
+            "speedup_vs_plain": sur_rec.pop("speedup_vs_plain"),
+            "config": dict(config, path="surrogate+fallback", model=model,
+                           surrogate=(f"table {tab[0]}x{tab[1]}" if tab else "per-ray network")),
+            "roofline": roof, "plain": plain_rec, "surrogate": sur_rec,
+        })
+        if args.config == "c2" and len(cams) > 1 and not args.no_single:
+            # the same workload one frame per call (latency-bound: the longest serial rays of a
+            # frame, FP64 band and fallback marches, set its time)
+            p1, s1, _ = _legs(tracer, sur, cams[:1], params, args.steps, args.warmup)
+            res["single_frame"] = {"value": s1["value"], "ms_per_step": s1["ms_per_step"],
+                                   "speedup_vs_plain": s1["speedup_vs_plain"],
+                                   "plain_value": p1["value"],
+                                   "plain_ms_per_step": p1["ms_per_step"]}
+        if not args.no_accuracy:
+            cam0 = cams[0]
+            n0 = cam0.width * cam0.height
+            p64 = W.params_for(cfg, precision=bhs.PREC_F64)
+            ref = tracer.render(cam0, p64)
+            ref_root = tracer.render(cam0, W.params_for(cfg, precision=bhs.PREC_F64, root64=1))
+            pl0 = tracer.render(cam0, params)
+            s0 = sur.render(cam0, params)
+            torch.cuda.synchronize()
+            res["accuracy"] = _accuracy(s0.numpy(), pl0.numpy(), ref.numpy(), ref_root.numpy(), n0)
+            res["accuracy"]["frame"] = "camera 0 of the step vs FP64 law-S march of the same rays"
+        sur.close()
+    if not args.no_cpu and args.config == "c2":
+        res["cpu_baseline"] = run_cpu_baseline(cfg)
+    print(json.dumps(res), flush=True)
+    tracer.close()
+    return 0
+
+
+def _legs(tracer, sur, cams, params, steps, warmup):
+    """Times the plain integrator and (sur given) the surrogate on the cameras (one call per step:
+    bhs_render / bhs_render_surrogate for one camera, the batch calls for several). Returns the
+    plain record, the surrogate record (None without sur) and the plain kernel's roofline."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd import workload as W
+    n = sum(c.width * c.height for c in cams)
+    stream = torch.cuda.current_stream()
+    single = len(cams) == 1
     pout = bhs.RayBuffers(n, "cuda:0")
     if single:
         plain = lambda: tracer.render(cams[0], params, pout, stream)
@@ -205,7 +301,7 @@ def main(argv=None) -> int:
     plain()
     torch.cuda.synchronize()
     tracer.set_timing(True)
-    wall_p = _timed(plain, args.steps, args.warmup)
+    wall_p = _timed(plain, steps, warmup)
     tm_p = tracer.timing()
     tracer.set_timing(False)
     plain()
@@ -213,88 +309,45 @@ def main(argv=None) -> int:
     # algorithmic work of the FP32 rays of one launch (the main kernel's rays)
     fl = pout.flags.to(torch.int32)
     f32 = (fl & bhs.FLAG_BAND64) == 0
-    steps = pout.steps_i32()
+    st = pout.steps_i32()
     ncr = pout.n_cross.to(torch.int32)
-    flops = W.march_flops(int(f32.sum()), int(steps[f32].sum()),
+    flops = W.march_flops(int(f32.sum()), int(st[f32].sum()),
                           int(((fl & bhs.FLAG_ESCAPED) != 0)[f32].sum()), int(ncr[f32].sum()),
                           int(torch.clamp(ncr, max=2)[f32].sum()))
     tflops = flops / (tm_p["march_ms"] * 1e-3) / 1e12 if tm_p["march_ms"] > 0 else 0.0
-    plain_rate = n * args.steps / wall_p / 1e6
-    mean_steps = int(steps.sum()) / n
-
-    res = {
-        "metric": "Mrays/s at 1080p/450 steps: surrogate+fallback vs plain, <=1e-4 rad vs FP64",
-        "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded per-pixel jitter; committed trained model)",
-    }
+    plain_rate = n * steps / wall_p / 1e6
+    plain_rec = {"value": round(plain_rate, 2),
+                 "ms_per_step": round(wall_p / steps * 1e3, 4),
+                 "kernel_ms": round(tm_p["march_ms"], 5), "call_ms": round(tm_p["total_ms"], 5),
+                 "mean_steps": round(int(st.sum()) / n, 3), "band64_rays": pstats["band64"]}
     roof = {"bound": "valu", "achieved": round(tflops, 3), "peak": W.PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(tflops / W.PEAK_FP32_TFLOPS, 4),
             "traffic": None, "kernel": "k_persist (plain FP32 law-S march)",
             "kernel_ms": round(tm_p["march_ms"], 5), "flops_per_launch": flops}
-    tr = _traffic()
-    if tr and args.config == "c2" and single:
-        roof["traffic"] = tr.get("bytes_per_launch")
-        roof["traffic_source"] = tr.get("source")
-    plain_rec = {"value": round(plain_rate, 2), "ms_per_step": round(wall_p / args.steps * 1e3, 4),
-                 "kernel_ms": round(tm_p["march_ms"], 5), "call_ms": round(tm_p["total_ms"], 5),
-                 "mean_steps": round(mean_steps, 3), "band64_rays": pstats["band64"]}
-
-    if args.plain_only:
-        res.update({"value": plain_rec["value"], "ms_per_step": plain_rec["ms_per_step"],
-                    "config": {"workload": workload, "path": "plain", "launch": tuned},
-                    "roofline": roof, "plain": plain_rec})
+    del pout
+    if sur is None:
+        return plain_rec, None, roof
+    sout = bhs.RayBuffers(n, "cuda:0")
+    if single:
+        surf = lambda: sur.render(cams[0], params, sout, stream)
     else:
-        # ---- surrogate leg
-        sur = Surrogate.load(tracer, os.path.join(ROOT, "blackholerender_amd", "models", model))
-        if not args.per_ray:
-            sur.set_table(256, 64)
-        sout = bhs.RayBuffers(n, "cuda:0")
-        if single:
-            surf = lambda: sur.render(cams[0], params, sout, stream)
-        else:
-            surf = lambda: sur.render_batch(cams, params, sout, stream)
-        surf()
-        torch.cuda.synchronize()
-        tracer.set_timing(True)
-        wall_s = _timed(surf, args.steps, args.warmup)
-        tm_s = tracer.timing()
-        tracer.set_timing(False)
-        surf()
-        sstats = tracer.last_stats()
-        sur_rate = n * args.steps / wall_s / 1e6
-        res.update({
-            "value": round(sur_rate, 2),
-            "ms_per_step": round(wall_s / args.steps * 1e3, 4),
-            "speedup_vs_plain": round(sur_rate / plain_rate, 3),
-            "config": {"workload": workload, "path": "surrogate+fallback",
-                       "surrogate": "table 256x64" if not args.per_ray else "per-ray network",
-                       "model": model, "rays_per_step": n, "parallelism": "single GPU",
-                       "launch": tuned},
-            "roofline": roof,
-            "plain": plain_rec,
-            "surrogate": {"network_part_ms": round(tm_s["march_ms"], 5),
-                          "call_ms": round(tm_s["total_ms"], 5),
-                          "served": sstats["surrogate"], "fallback": sstats["fallback"],
-                          "band64": sstats["band64"],
-                          "fallback_steps": sstats["steps"]},
-        })
-        if not args.no_accuracy:
-            cam0 = cams[0]
-            n0 = cam0.width * cam0.height
-            p64 = W.params_for(cfg, precision=bhs.PREC_F64)
-            ref = tracer.render(cam0, p64)
-            pl0 = tracer.render(cam0, params)
-            s0 = sur.render(cam0, params)
-            torch.cuda.synchronize()
-            res["accuracy"] = _accuracy(s0.numpy(), pl0.numpy(), ref.numpy(), n0)
-            res["accuracy"]["frame"] = "camera 0 of the step vs FP64 law-S march of the same rays"
-        sur.close()
-    if not args.no_cpu and args.config == "c2":
-        res["cpu_baseline"] = run_cpu_baseline(cfg)
-    print(json.dumps(res), flush=True)
-    tracer.close()
-    return 0
+        surf = lambda: sur.render_batch(cams, params, sout, stream)
+    surf()
+    torch.cuda.synchronize()
+    tracer.set_timing(True)
+    wall_s = _timed(surf, steps, warmup)
+    tm_s = tracer.timing()
+    tracer.set_timing(False)
+    surf()
+    sstats = tracer.last_stats()
+    rate = n * steps / wall_s / 1e6
+    sur_rec = {"value": round(rate, 2), "ms_per_step": round(wall_s / steps * 1e3, 4),
+               "speedup_vs_plain": round(rate / plain_rate, 3),
+               "network_part_ms": round(tm_s["march_ms"], 5),
+               "call_ms": round(tm_s["total_ms"], 5),
+               "served": sstats["surrogate"], "fallback": sstats["fallback"],
+               "band64": sstats["band64"], "fallback_steps": sstats["steps"]}
+    return plain_rec, sur_rec, roof
 
 
 if __name__ == "__main__":

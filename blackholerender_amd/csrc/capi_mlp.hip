@@ -37,7 +37,8 @@ struct bhs_mlp {
     // fallback lists of the surrogate render: [counters (256 B) | late ids | early ids]
     void* fb = nullptr;
     size_t fb_bytes = 0;
-    // table mode: [x2 ranges (512 B) | ncam n2 n3 nodes x 8 floats]; tab_n2 = 0: per-ray inference
+    // table mode: [x2 ranges (512 B) | slots (256 B) | pad to 1 KiB | ncam x kPhiNodes swept
+    // angles | ncam n2 n3 nodes x 4 floats]; tab_n2 = 0: per-ray inference
     void* tab = nullptr;
     size_t tab_bytes = 0;
     int32_t tab_n2 = 0, tab_n3 = 0;
@@ -156,13 +157,17 @@ float quantile(std::vector<float>& v, double q) {
 
 // Table-mode buffers for ncam cameras (tab->n2 = 0 when the model runs per ray).
 int table_for(bhs_ctx* ctx, bhs_mlp* m, int32_t ncam, M::SurTable* tab) {
-    *tab = M::SurTable{nullptr, nullptr, 0, 0};
+    *tab = M::SurTable{nullptr, nullptr, nullptr, nullptr, 0, 0};
     if (m->tab_n2 <= 0) return BHS_OK;
     const size_t nodes = (size_t)m->tab_n2 * (size_t)m->tab_n3 * (size_t)ncam;
+    const size_t phi_bytes = (size_t)ncam * M::kPhiNodes * 4;
     int st;
-    if ((st = grow(ctx, &m->tab, &m->tab_bytes, 512 + nodes * 32, "surrogate table")) != BHS_OK)
+    if ((st = grow(ctx, &m->tab, &m->tab_bytes, 1024 + phi_bytes + nodes * 16,
+                   "surrogate table")) != BHS_OK)
         return st;
-    *tab = M::SurTable{(float*)((char*)m->tab + 512), (float*)m->tab, m->tab_n2, m->tab_n3};
+    char* base = (char*)m->tab;
+    *tab = M::SurTable{(float*)(base + 1024 + phi_bytes), (float*)(base + 1024), (float*)base,
+                       (uint32_t*)(base + 512), m->tab_n2, m->tab_n3};
     return BHS_OK;
 }
 

@@ -283,63 +283,86 @@ __device__ __forceinline__ bool claim_items(uint32_t* counter, uint32_t n, uint3
 // setup) is cheap next to an idle lane; 32 keeps the queue atomics to one per 32 rays.
 constexpr uint32_t kBandRefill = 32;
 
-// FP64 march of the band list (the count is read on the device). A list longer than the grid's
-// lanes (large batches) is marched by resident waves pulling rays from it (queue counter
-// sc.band_work); finished lanes are refilled as in k_persist, so a wave is not held by its slowest
-// ray. A shorter list is marched one ray per lane. One camera (OneCam, constants in kernel arguments) or a
-// batch (CamTable: camera table, FP64 march constants and item prefix sums staged in LDS). ROOTS:
-// the FP64 root refinement of march_f64 for cameras with root64. Exit: the list is finite, every
-// ray ends within max_steps iterations and the counter only grows, so every wave leaves.
-template <typename Cams, bool ROOTS>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(ROOTS ? 2 : 3)))
+// Block of the band kernels: 256 threads, so the LDS camera table of a batch (~9 KB) does not cap
+// the resident waves.
+constexpr uint32_t kBandBlock = 256;
+
+// Camera table of the band kernels in LDS (batches) or the kernel argument (one camera).
+template <typename Cams>
+struct BandCams {
+    static constexpr bool kBatch = std::is_same<Cams, CamTable>::value;
+    MarchConst cam[kBatch ? kMaxBatchCams : 1];
+    uint32_t base[kBatch ? kMaxBatchCams + 1 : 1];
+};
+
+// FP64 march of the band list, first part: list positions [0, min(n, lanes of the grid)) (the
+// count n is read on the device), one ray per lane. Covers the band of a frame or of a moderate
+// batch at once; its longest ray sets its time.
+template <typename Cams>
+__global__ __launch_bounds__(kBandBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void k_band64(const Cams cams, const SoA out, Scratch sc) {
-    constexpr bool kBatch = std::is_same<Cams, CamTable>::value;
-    __shared__ MarchConst s_cam[kBatch ? kMaxBatchCams : 1];
-    __shared__ MarchK<double> s_k[kBatch ? kMaxBatchCams : 1];
-    __shared__ uint32_t s_base[kBatch ? kMaxBatchCams + 1 : 1];
+    constexpr bool kBatch = BandCams<Cams>::kBatch;
+    __shared__ BandCams<Cams> s_c;
     uint32_t ncam = 1;
     if constexpr (kBatch) {
         ncam = cams.ncam;
-        for (uint32_t i = threadIdx.x; i < ncam; i += 64) {
-            s_cam[i] = cams.cams[i];
-            s_k[i] = march_consts<double>(cams.cams[i]);
-        }
-        for (uint32_t i = threadIdx.x; i <= ncam; i += 64) s_base[i] = cams.item_base[i];
+        for (uint32_t i = threadIdx.x; i < ncam; i += kBandBlock) s_c.cam[i] = cams.cams[i];
+        for (uint32_t i = threadIdx.x; i <= ncam; i += kBandBlock) s_c.base[i] = cams.item_base[i];
         __syncthreads();
     }
     // Few waves with long serial chains run beside the FP32 march: issue them first on their SIMD.
     __builtin_amdgcn_s_setprio(3);
-    uint32_t ci0 = 0;
+    const uint32_t n = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
+    const uint32_t q = blockIdx.x * kBandBlock + threadIdx.x;
+    uint32_t one = 0, steps = 0, cross = 0, esc = 0;
+    if (q < n) {
+        const uint32_t item = sc.band_ids[q];
+        uint32_t local = item, ci = 0;
+        if constexpr (kBatch) {
+            ci = find_cam(s_c.base, ncam, item);
+            local = item - s_c.base[ci];
+        }
+        const MarchConst& c = kBatch ? s_c.cam[ci] : cams.cam(0);
+        const int64_t p = item_pixel(c, local);
+        if (p >= 0) {
+            const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+            const RayOut o = march_f64(c, P, ray_dir(c, p));
+            store_ray(out, c.ray_offset + p, o);
+            one = 1;
+            steps = (uint32_t)o.steps;
+            cross = o.n_cross;
+            esc = (o.flags & 2u) ? 1u : 0u;
+        }
+    }
+    add_stats(sc.stats, one, steps, cross, esc, one);
+}
+
+// FP64 march of the band list, rest: positions [first, n) of a list longer than k_band64's lanes
+// (large batches), by resident waves pulling rays from it (queue counter sc.band_work); finished
+// lanes are refilled as in k_persist, so a wave is not held by its slowest ray. ROOTS: the FP64
+// root refinement of march_f64 for cameras with root64. Exit: the list is finite, every ray ends
+// within max_steps iterations and the counter only grows, so every wave leaves.
+template <typename Cams, bool ROOTS>
+__global__ __launch_bounds__(kBandBlock) __attribute__((amdgpu_waves_per_eu(ROOTS ? 2 : 3)))
+void k_band64_queue(const Cams cams, const SoA out, Scratch sc, uint32_t first) {
+    constexpr bool kBatch = BandCams<Cams>::kBatch;
+    __shared__ BandCams<Cams> s_c;
+    __shared__ MarchK<double> s_k[kBatch ? kMaxBatchCams : 1];
+    const uint32_t n = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
+    if (n <= first) return;   // uniform: the list fitted k_band64
+    uint32_t ncam = 1;
+    if constexpr (kBatch) {
+        ncam = cams.ncam;
+        for (uint32_t i = threadIdx.x; i < ncam; i += kBandBlock) {
+            s_c.cam[i] = cams.cams[i];
+            s_k[i] = march_consts<double>(cams.cams[i]);
+        }
+        for (uint32_t i = threadIdx.x; i <= ncam; i += kBandBlock) s_c.base[i] = cams.item_base[i];
+        __syncthreads();
+    }
+    __builtin_amdgcn_s_setprio(3);
     const MarchConst& c1 = cams.cam(0);
     const MarchK<double> k1 = march_consts<double>(c1);
-    const uint32_t n = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
-    if (n <= gridDim.x * 64u) {
-        // A list the grid covers at once (the band of a frame): one ray per lane, no queue. Its
-        // longest ray sets the latency, which the queue loop lengthens.
-        const uint32_t q = blockIdx.x * 64u + threadIdx.x;
-        uint32_t one = 0, steps = 0, cross = 0, esc = 0;
-        if (q < n) {
-            const uint32_t item = sc.band_ids[q];
-            uint32_t local = item;
-            if constexpr (kBatch) {
-                ci0 = find_cam(s_base, ncam, item);
-                local = item - s_base[ci0];
-            }
-            const MarchConst& c = kBatch ? s_cam[ci0] : c1;
-            const int64_t p = item_pixel(c, local);
-            if (p >= 0) {
-                const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-                const RayOut o = march_f64(c, P, ray_dir(c, p));
-                store_ray(out, c.ray_offset + p, o);
-                one = 1;
-                steps = (uint32_t)o.steps;
-                cross = o.n_cross;
-                esc = (o.flags & 2u) ? 1u : 0u;
-            }
-        }
-        add_stats(sc.stats, one, steps, cross, esc, one);
-        return;
-    }
     MarchState<double> st;
     Setup<double> s;
     RootState<double> rs0, rs1;
@@ -349,14 +372,14 @@ void k_band64(const Cams cams, const SoA out, Scratch sc) {
     uint32_t n_rays = 0, n_steps = 0, n_cross = 0, n_esc = 0;
     while (true) {
         uint32_t q = 0;
-        if (claim_items(sc.band_work, n, kBandRefill, active, more, q)) {
-            const uint32_t item = sc.band_ids[q];
+        if (claim_items(sc.band_work, n - first, kBandRefill, active, more, q)) {
+            const uint32_t item = sc.band_ids[first + q];
             uint32_t local = item;
             if constexpr (kBatch) {
-                ci = find_cam(s_base, ncam, item);
-                local = item - s_base[ci];
+                ci = find_cam(s_c.base, ncam, item);
+                local = item - s_c.base[ci];
             }
-            const MarchConst& c = kBatch ? s_cam[ci] : c1;
+            const MarchConst& c = kBatch ? s_c.cam[ci] : c1;
             const int64_t p = item_pixel(c, local);
             if (p >= 0) {
                 pix = (uint32_t)p;
@@ -372,7 +395,7 @@ void k_band64(const Cams cams, const SoA out, Scratch sc) {
             if (!more) break;
             continue;
         }
-        const MarchConst& c = kBatch ? s_cam[ci] : c1;
+        const MarchConst& c = kBatch ? s_c.cam[ci] : c1;
         const int64_t gi = c.ray_offset + pix;
         BandSink<ROOTS> sink{out, gi, rs0, rs1};
         const bool fin = march_step(st, s, kBatch ? s_k[ci] : k1, sink, active);
@@ -666,13 +689,12 @@ __global__ void k_zero(uint32_t* p, size_t words) {
         p[i] = 0u;
 }
 
-// Grid of the FP64 band march (64-thread blocks, resident waves pulling from the device-side
-// list): up to 4096 waves (4 per SIMD on 256 CUs) for band lists of millions of rays; surplus waves
-// find the list exhausted and exit.
+// Grid of the FP64 band march (256-thread blocks over the device-side list): up to 4096 waves (4 per
+// SIMD on 256 CUs); surplus lanes find no ray and exit.
 dim3 band_grid(uint32_t cap) {
-    uint32_t b = (cap + 63u) / 64u;
-    b = b > 4096u ? 4096u : b;
-    return dim3(b < 256u ? 256u : b);
+    uint32_t b = (cap + 255u) / 256u;
+    b = b > 1024u ? 1024u : b;
+    return dim3(b < 64u ? 64u : b);
 }
 
 // Grid of the item-indexed helper kernels (grid-stride loops).
@@ -726,21 +748,28 @@ hipError_t launch_persist(const Tuning& t, int num_cus, hipStream_t s, const Mar
     return hipGetLastError();
 }
 
-// FP64 march of the band list on `aux`; the ROOTS variant (crossing states in registers) only for
-// launches with FP64 root refinement.
+// FP64 march of the band list on `aux`: k_band64 over the first positions, then (lists longer
+// than its lanes only; it exits at once otherwise) k_band64_queue over the rest, its ROOTS variant
+// (crossing states in registers) only for launches with FP64 root refinement.
 template <typename Cams>
 hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hipStream_t aux) {
+    const dim3 g = band_grid(sc.band_cap);
+    hipLaunchKernelGGL(k_band64<Cams>, g, dim3(kBandBlock), 0, aux, cams, out, sc);
+    hipError_t e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const uint32_t first = g.x * kBandBlock;
+    if (sc.band_cap <= first) return hipSuccess;
     bool roots;
     if constexpr (std::is_same<Cams, OneCam>::value)
         roots = cams.c.root64 != 0;
     else
         roots = sc.roots != nullptr;   // batches allocate root states iff a camera has root64
     if (roots)
-        hipLaunchKernelGGL((k_band64<Cams, true>), band_grid(sc.band_cap), dim3(64), 0, aux, cams,
-                           out, sc);
+        hipLaunchKernelGGL((k_band64_queue<Cams, true>), g, dim3(kBandBlock), 0, aux, cams, out,
+                           sc, first);
     else
-        hipLaunchKernelGGL((k_band64<Cams, false>), band_grid(sc.band_cap), dim3(64), 0, aux, cams,
-                           out, sc);
+        hipLaunchKernelGGL((k_band64_queue<Cams, false>), g, dim3(kBandBlock), 0, aux, cams, out,
+                           sc, first);
     return hipGetLastError();
 }
 

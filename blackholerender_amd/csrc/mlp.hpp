@@ -83,9 +83,18 @@ struct SurStreams {
 };
 // Table mode: per camera, the network on an n2 x n3 grid over (x2, x3) at the camera's r_obs
 // (node: ncam n2 n3 x 8 floats; range: 2 ncam floats of device scratch).
+// Table mode, per camera: an n2 x n3 grid of nodes over (x2, x3) holding the network's escape
+// residual and the two crossings' u with their residuals applied, and a kPhiNodes-point table of
+// the exact swept angle at escape / capture over x2 (both over the camera's x2 range).
+constexpr int32_t kPhiNodes = 4096;
+// Cameras of one pose (position, orientation, field of view and image size; the jitter may
+// differ) share one table: slot[k] is the first camera of camera k's pose, and the table of
+// slot s holds camera s's data.
 struct SurTable {
-    float* node;
-    float* range;
+    float* node;      // ncam x n2 x n3 x 4 floats
+    float* phi;       // ncam x kPhiNodes floats
+    float* range;     // ncam x 2 floats (x2 bounds)
+    uint32_t* slot;   // ncam table slots
     int32_t n2, n3;
 };
 // tab null: per-ray network inference. ev_begin / ev_mid / ev_end (may be null) bracket the

@@ -274,6 +274,12 @@ __device__ __forceinline__ float x2_of(float bm) {
     return bm < 0.0f ? -m : m;
 }
 
+// Inverse of x2_of: beta - 1 of a feature value.
+__device__ __forceinline__ float bm_of_x2(float x2) {
+    const float m = kBetaEps * expm1f(fabsf(x2) * 9.2104404f);
+    return x2 < 0.0f ? -m : m;
+}
+
 __device__ __forceinline__ void features(const RayGeom& g, float x[3]) {
     x[0] = (logf(g.r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
     x[1] = x2_of(g.bm);

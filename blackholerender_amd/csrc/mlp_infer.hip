@@ -208,19 +208,20 @@ __device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k,
     return (float)k.steps_max[r_bin(g.r)] * k.steps_slack < (float)c.max_steps;
 }
 
-// Output record of one in-domain ray from the exact orbit and the network outputs y, or
-// serve = false when the ray must go back to the integrator: a crossing radius within the margin
-// of an annulus edge, or a third crossing that could be recorded.
-__device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, const Setup<float>& s,
-                                      const RayGeom& g, const float y[5]) {
+// Output record of one in-domain ray from its swept angle at escape / capture Phi (learned
+// residual included) and the u = 1/r of its equatorial crossings (u_at(q), q = 0, 1, asked only
+// for crossings before Phi), or serve = false when the ray must go back to the integrator: a
+// crossing radius within the margin of an annulus edge, or a third crossing that could be
+// recorded.
+template <typename UAt>
+__device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& k,
+                                           const Setup<float>& s, const RayGeom& g, float Phi,
+                                           UAt u_at) {
     Dec o;
     o.serve = false;
     o.escaping = g.bm > 0.0f;
     o.n_cross = 0;
-    const Orbit ob = orbit(g);
-    // validity of the crossings from the exact orbit's total sweep (to infinity with the learned
-    // residual, or to the horizon test radius)
-    const float Phi = ob.phi_end + (o.escaping ? kResScale * y[0] : 0.0f);
+    // validity of the crossings from the total sweep
     const float a = Phi - g.psi;
     o.n_eq = a <= 0.0f ? 0u : (a <= kPi ? 1u : (a <= 2.0f * kPi ? 2u : 3u));
     bool in[2] = {false, false};
@@ -229,7 +230,7 @@ __device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, co
     for (int q = 0; q < 2; ++q) {
         o.r_eq[q] = 1e30f;
         if ((uint32_t)q >= o.n_eq) continue;
-        const float u = orbit_u(ob, g.psi + (float)q * kPi) * __expf(kURes * y[1 + q]);
+        const float u = u_at(q);
         o.r_eq[q] = u > 0.0f ? 1.0f / u : 1e30f;
         if (c.thin) {
             const float r = o.r_eq[q];
@@ -275,6 +276,18 @@ __device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, co
     o.flags = BHS_FLAG_SURROGATE | (o.escaping ? BHS_FLAG_ESCAPED : 0u);
     o.serve = true;
     return o;
+}
+
+// Decode from the exact orbit of the ray and the network outputs y (per-ray inference,
+// validation).
+__device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, const Setup<float>& s,
+                                      const RayGeom& g, const float y[5]) {
+    const Orbit ob = orbit(g);
+    // the exact sweep to infinity with the learned residual, or to the horizon test radius
+    const float Phi = ob.phi_end + (g.bm > 0.0f ? kResScale * y[0] : 0.0f);
+    return decode_with(c, k, s, g, Phi, [&](int q) {
+        return orbit_u(ob, g.psi + (float)q * kPi) * __expf(kURes * y[1 + q]);
+    });
 }
 
 __device__ __forceinline__ void store_dec(const SoA& out, int64_t i, const Dec& o, float b) {
@@ -509,17 +522,38 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
 }
 
 // ------------------------------------------------------------------------------------------------
-// Table mode: per camera, the network evaluated on an n2 x n3 grid over (x2, x3) at the camera's
-// x1; rays interpolate bilinearly. x3 spans [-1, 1]; x2 spans [x2(beta = 0.3), x2(beta_max)] with
-// beta_max from the image corners (the largest view angle from the inward radial over a pinhole
-// frustum is at a corner), both widened by 1e-3. Node (i, j) of camera k holds the 5 outputs at
-// node[((k n3 + j) n2 + i) * 8 + o]; range[2k], range[2k + 1] hold the x2 bounds.
+// Table mode: per camera, everything of the decode that depends on (x2, x3) only (x1 is the
+// camera's): the network's escape residual and the two crossings' u (exact orbit with the learned
+// residual applied) on an n2 x n3 grid, interpolated bilinearly, and the exact swept angle at
+// escape / capture (a function of x2 alone, with a logarithmic rise towards the band) on a
+// kPhiNodes-point grid, interpolated linearly. A ray then costs its setup, two lookups and the
+// output trigonometry instead of the orbit quadratures and Newton solves. x3 spans [-1, 1]; x2 spans
+// [x2(beta = 0.3), x2(beta_max)] with beta_max from the image corners (the largest view angle from
+// the inward radial over a pinhole frustum is at a corner), both widened by 1e-3. Node (i, j) of
+// camera k is node[((k n3 + j) n2 + i) * 4 + o]; range[2k], range[2k + 1] hold the x2 bounds.
+
+__device__ __forceinline__ bool same_pose(const MarchConst& a, const MarchConst& b) {
+    bool eq = a.width == b.width && a.height == b.height && a.tan_half_fov == b.tan_half_fov;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) eq = eq && a.pos[i] == b.pos[i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) eq = eq && a.basis[i] == b.basis[i];
+    return eq;
+}
 
 template <typename Cams>
-__global__ void k_tab_range(const Cams cams, uint32_t ncam, float* __restrict__ range) {
+__global__ void k_tab_range(const Cams cams, uint32_t ncam, float* __restrict__ range,
+                            uint32_t* __restrict__ slot) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncam) return;
     const MarchConst& c = cams.cam(k);
+    uint32_t first = k;
+    for (uint32_t j = 0; j < k; ++j)
+        if (same_pose(cams.cam(j), c)) {
+            first = j;
+            break;
+        }
+    slot[k] = first;
     const float r = sqrtf(c.pos[0] * c.pos[0] + c.pos[1] * c.pos[1] + c.pos[2] * c.pos[2]);
     const float ir = 1.0f / r;
     const float aspect = (float)c.width / (float)c.height;
@@ -557,12 +591,33 @@ __device__ __forceinline__ TabGeom tab_geom(const float* range, int32_t n2, int3
     return t;
 }
 
-// Network outputs at the table nodes of every camera (blocks walk (camera, chunk) pairs).
+// Orbit of the exact Schwarzschild prior at feature x2 for a camera at radius r (the geometry
+// fields orbit() reads).
+__device__ __forceinline__ Orbit node_orbit(float r, float x2) {
+    RayGeom g;
+    g.r = r;
+    g.bm = bm_of_x2(x2);
+    g.b = (1.0f + g.bm) * kBcritF;
+    return orbit(g);
+}
+
+__device__ __forceinline__ float finite_or_zero(float v) {
+    return __builtin_isfinite(v) ? v : 0.0f;
+}
+
+__device__ __forceinline__ float cam_radius(const MarchConst& c) {
+    return sqrtf(c.pos[0] * c.pos[0] + c.pos[1] * c.pos[1] + c.pos[2] * c.pos[2]);
+}
+
+// Table nodes of every camera (blocks walk (camera, chunk) pairs): at node (i, j) = (x2_i, x3_j),
+// [escape residual y0, u of crossing 1, u of crossing 2 (exact orbit times exp(1e-3 y1|y2)), 0].
 template <typename Cams>
 __global__ __launch_bounds__(kBlock, 2) void k_table(const Cams cams, uint32_t ncam,
                                                      const uint8_t* __restrict__ packed,
-                                                     const float* __restrict__ range, int32_t n2,
-                                                     int32_t n3, float* __restrict__ node) {
+                                                     const float* __restrict__ range,
+                                                     const uint32_t* __restrict__ slot,
+                                                     int32_t n2, int32_t n3,
+                                                     float* __restrict__ node) {
     __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
     __shared__ float s_y[4][64 * kOut];
     stage_weights(packed, s_w);
@@ -572,10 +627,10 @@ __global__ __launch_bounds__(kBlock, 2) void k_table(const Cams cams, uint32_t n
     const int64_t per = (n + kBlock - 1) / kBlock;
     for (int64_t p = blockIdx.x; p < per * ncam; p += gridDim.x) {
         const uint32_t k = (uint32_t)(p / per);
+        if (slot[k] != k) continue;   // block-uniform: a camera of an earlier camera's pose
         const int64_t q = (p % per) * kBlock + threadIdx.x;
-        const MarchConst& c = cams.cam(k);
         const TabGeom t = tab_geom(range + 2 * k, n2, n3);
-        const float r = sqrtf(c.pos[0] * c.pos[0] + c.pos[1] * c.pos[1] + c.pos[2] * c.pos[2]);
+        const float r = cam_radius(cams.cam(k));
         const float x1 = (logf(r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
         const int32_t i = (int32_t)(q % n2), j = (int32_t)(q / n2);
         const float x[3] = {x1, __builtin_fmaf((float)i, t.h2, t.x2_lo),
@@ -583,44 +638,80 @@ __global__ __launch_bounds__(kBlock, 2) void k_table(const Cams cams, uint32_t n
         float y[5];
         mlp_wave(s_w, x, s_y[wave], y);
         if (q < n) {
-            float* o = node + ((int64_t)k * n + q) * 8;
-#pragma unroll
-            for (int k2 = 0; k2 < kOut; ++k2) o[k2] = y[k2];
+            const Orbit ob = node_orbit(r, x[1]);
+            const float psi = (x[2] + 1.0f) * (0.5f * kPi);
+            f4 v;
+            v[0] = y[0];
+            v[1] = finite_or_zero(orbit_u(ob, psi) * __expf(kURes * y[1]));
+            v[2] = finite_or_zero(orbit_u(ob, psi + kPi) * __expf(kURes * y[2]));
+            v[3] = 0.0f;
+            *(f4*)(node + ((int64_t)k * n + q) * 4) = v;
         }
     }
 }
 
+// Exact swept angle at escape / capture over the camera's x2 range (kPhiNodes points).
+template <typename Cams>
+__global__ __launch_bounds__(kBlock) void k_tab_phi(const Cams cams, uint32_t ncam,
+                                                    const float* __restrict__ range,
+                                                    const uint32_t* __restrict__ slot,
+                                                    float* __restrict__ phi) {
+    const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= (int64_t)ncam * kPhiNodes) return;
+    const uint32_t k = (uint32_t)(q / kPhiNodes);
+    if (slot[k] != k) return;
+    const int32_t i = (int32_t)(q % kPhiNodes);
+    const float lo = range[2 * k], hi = range[2 * k + 1];
+    const float x2 = __builtin_fmaf((float)i, (hi - lo) / (float)(kPhiNodes - 1), lo);
+    phi[q] = finite_or_zero(node_orbit(cam_radius(cams.cam(k)), x2).phi_end);
+}
+
+// Bilinear lookup of a node grid at (x2, x3): y0, u1, u2.
 __device__ __forceinline__ void tab_lookup(const float* __restrict__ node, const TabGeom& t,
-                                           int32_t n2, int32_t n3, const float x[3], float y[5]) {
+                                           int32_t n2, int32_t n3, const float x[3],
+                                           float v[3]) {
     float fx = (x[1] - t.x2_lo) * t.inv_h2, fy = (x[2] + 1.0f) * t.inv_h3;
     fx = fminf(fmaxf(fx, 0.0f), (float)(n2 - 1) - 1e-3f);
     fy = fminf(fmaxf(fy, 0.0f), (float)(n3 - 1) - 1e-3f);
     const int32_t i = (int32_t)fx, j = (int32_t)fy;
     const float a = fx - (float)i, b = fy - (float)j;
-    const float* p0 = node + ((int64_t)j * n2 + i) * 8;
-    const float* p1 = p0 + (int64_t)n2 * 8;
-    const f4 q00 = *(const f4*)p0, q01 = *(const f4*)(p0 + 8);
-    const f4 q10 = *(const f4*)p1, q11 = *(const f4*)(p1 + 8);
+    const float* p0 = node + ((int64_t)j * n2 + i) * 4;
+    const float* p1 = p0 + (int64_t)n2 * 4;
+    const f4 q00 = *(const f4*)p0, q01 = *(const f4*)(p0 + 4);
+    const f4 q10 = *(const f4*)p1, q11 = *(const f4*)(p1 + 4);
     const float w00 = (1.0f - a) * (1.0f - b), w01 = a * (1.0f - b), w10 = (1.0f - a) * b,
                 w11 = a * b;
 #pragma unroll
-    for (int o = 0; o < 4; ++o)
-        y[o] = w00 * q00[o] + w01 * q01[o] + w10 * q10[o] + w11 * q11[o];
-    y[4] = w00 * p0[4] + w01 * p0[12] + w10 * p1[4] + w11 * p1[12];
+    for (int o = 0; o < 3; ++o)
+        v[o] = w00 * q00[o] + w01 * q01[o] + w10 * q10[o] + w11 * q11[o];
+}
+
+// Linear lookup of the swept-angle table at x2.
+__device__ __forceinline__ float phi_lookup(const float* __restrict__ phi, const float* range,
+                                            float x2) {
+    const float lo = range[0], hi = range[1];
+    float f = (x2 - lo) * ((float)(kPhiNodes - 1) / (hi - lo));
+    f = fminf(fmaxf(f, 0.0f), (float)(kPhiNodes - 1) - 1e-3f);
+    const int32_t i = (int32_t)f;
+    const float a = f - (float)i;
+    return __builtin_fmaf(a, phi[i + 1] - phi[i], phi[i]);
 }
 
 // Surrogate render in table mode: as k_surrogate with the network replaced by the table lookup.
 template <typename Cams>
-__global__ __launch_bounds__(kBlock) void k_surrogate_tab(const Cams cams, const SurConst k,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
+void k_surrogate_tab(const Cams cams, const SurConst k,
                                                           const float* __restrict__ node,
+                                                          const float* __restrict__ phi,
                                                           const float* __restrict__ range,
+                                                          const uint32_t* __restrict__ slot,
                                                           int32_t n2, int32_t n3, const SoA out,
                                                           Scratch sc, SurLists fl,
                                                           uint32_t n_items, uint32_t listed) {
     __shared__ uint32_t s_cnt[5];
     __shared__ uint32_t s_st[4][4];
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
-    const int64_t nodes = (int64_t)n2 * n3 * 8;
+    const int64_t nodes = (int64_t)n2 * n3 * 4;
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
         const uint32_t q = q0 + threadIdx.x;
         const uint32_t ci = cams.find(q < n_items ? q : n_items - 1);
@@ -629,9 +720,12 @@ __global__ __launch_bounds__(kBlock) void k_surrogate_tab(const Cams cams, const
         const RayPrep r = prep_ray(c, k, pix);
         bool back = false;
         if (r.route == kNet) {
-            float y[5];
-            tab_lookup(node + ci * nodes, tab_geom(range + 2 * ci, n2, n3), n2, n3, r.x, y);
-            const Dec o = decode(c, k, r.s, r.g, y);
+            const uint32_t ts = slot[ci];
+            float v[3];
+            tab_lookup(node + ts * nodes, tab_geom(range + 2 * ts, n2, n3), n2, n3, r.x, v);
+            const float Phi = phi_lookup(phi + (int64_t)ts * kPhiNodes, range + 2 * ts, r.x[1]) +
+                              (r.g.bm > 0.0f ? kResScale * v[0] : 0.0f);
+            const Dec o = decode_with(c, k, r.s, r.g, Phi, [&](int q) { return v[1 + q]; });
             if (o.serve) {
                 store_dec(out, c.ray_offset + pix, o, r.s.b);
                 n_served += 1;
@@ -813,7 +907,7 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                       hipEvent_t ev_end) {
     constexpr bool kOne = std::is_same<Cams, SurOneCam>::value;
     const uint32_t gb0 = (n_items + kBlock - 1) / kBlock;
-    const uint32_t cap = (uint32_t)(4 * (num_cus > 0 ? num_cus : 256));
+    const uint32_t cap = (uint32_t)(8 * (num_cus > 0 ? num_cus : 256));
     const dim3 grid(gb0 < cap ? (gb0 ? gb0 : 1u) : cap);
     const bool f64 = kOne && c0.precision == BHS_PREC_F64;
     // Latency mode (up to kRoutedItems work items): a pre-pass routes the rays first, so the side
@@ -868,14 +962,20 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     if (tab) {
         const int64_t nodes = (int64_t)tab->n2 * tab->n3;
         hipLaunchKernelGGL(k_tab_range<Cams>, dim3((ncam + 63) / 64), dim3(64), 0, s, cams, ncam,
-                           tab->range);
+                           tab->range, tab->slot);
+        hipLaunchKernelGGL(k_tab_phi<Cams>,
+                           dim3((uint32_t)(((int64_t)ncam * kPhiNodes + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, cams, ncam, (const float*)tab->range,
+                           (const uint32_t*)tab->slot, tab->phi);
         hipLaunchKernelGGL(k_table<Cams>,
                            dim3(persistent_blocks((nodes + kBlock - 1) / kBlock * ncam, num_cus)),
                            dim3(kBlock), 0, s, cams, ncam, (const uint8_t*)packed,
-                           (const float*)tab->range, tab->n2, tab->n3, tab->node);
+                           (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
+                           tab->n3, tab->node);
         hipLaunchKernelGGL(k_surrogate_tab<Cams>, grid, dim3(kBlock), 0, s, cams, k,
-                           (const float*)tab->node, (const float*)tab->range, tab->n2, tab->n3,
-                           out, sc, fl, n_items, listed);
+                           (const float*)tab->node, (const float*)tab->phi,
+                           (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
+                           tab->n3, out, sc, fl, n_items, listed);
     } else {
         hipLaunchKernelGGL(k_surrogate<Cams>, dim3(persistent_blocks(gb0, num_cus)), dim3(kBlock),
                            0, s, cams, k, (const uint8_t*)packed, out, sc, fl, n_items,

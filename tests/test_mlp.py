@@ -177,14 +177,16 @@ def _angle(a, b):
 def test_trained_model_frame_accuracy(tracer, table):
     """480x270 frame of the 1080p camera (r_obs 30, i 85, FOV 60, High preset, disk [6, 20]):
     served escape directions within 1e-4 rad of FP64, classes and crossing counts equal to the
-    FP64 reference march, every unserved ray bit-identical to the plain render."""
+    FP64 reference march, crossing radii within 1e-3 (relative) of the FP64 march's refined
+    crossings, every unserved ray bit-identical to the plain render. Per-ray inference and the
+    1024 x 256 table mode."""
     import torch
     import blackholerender_amd as bhs
     from blackholerender_amd.surrogate import Surrogate
     from helpers import bit_mismatches
     sur = Surrogate.load(tracer, _model_path(MODEL_HIGH))
     if table:
-        sur.set_table(256, 64)
+        sur.set_table(1024, 256)
     cam = bhs.orbit_camera(30.0, 85.0, 60.0, 480, 270, jitter_hashed=1, seed=11)
     params = bhs.march_params()
     print(f"[mlp] trained model frame table={table}", flush=True)
@@ -205,6 +207,11 @@ def test_trained_model_frame_accuracy(tracer, table):
     cr = sv & (S["n_cross"] > 0)
     rel = np.abs(S["disk_r"][cr, 0] / R["disk_r"][cr, 0] - 1.0)
     assert rel.max() < 5e-3, rel.max()
+    RR = tracer.render(cam, bhs.march_params(precision=bhs.PREC_F64, root64=1)).numpy()
+    for k in range(2):
+        ck = sv & (S["n_cross"] > k)
+        rel = np.abs(S["disk_r"][ck, k] / RR["disk_r"][ck, k] - 1.0)
+        assert rel.max() < 1e-3, (k, rel.max())
     mm = bit_mismatches({k: v[~sv] for k, v in S.items()}, {k: v[~sv] for k, v in P.items()})
     assert all(v == 0 for v in mm.values()), mm
     sur.close()
@@ -212,8 +219,9 @@ def test_trained_model_frame_accuracy(tracer, table):
 
 @pytest.mark.gpu
 def test_batch_equals_single_calls(tracer):
-    """bhs_render_surrogate_batch over three cameras of different sizes and poses equals the three
-    single-camera calls bit for bit (table mode)."""
+    """bhs_render_surrogate_batch over four cameras (three sizes and poses, the fourth a jittered
+    copy of the first's pose, which shares its table) equals the single-camera calls bit for bit
+    (table mode)."""
     import torch
     import blackholerender_amd as bhs
     from blackholerender_amd.surrogate import Surrogate
@@ -223,9 +231,10 @@ def test_batch_equals_single_calls(tracer):
     cams = [bhs.orbit_camera(30.0, 85.0, 60.0, 200, 120, jitter_hashed=1, seed=3, camera_id=0),
             bhs.orbit_camera(60.0, 70.0, 40.0, 96, 160, jitter_hashed=1, seed=3, camera_id=1,
                              azimuth_deg=90.0),
-            bhs.orbit_camera(12.0, 80.0, 75.0, 128, 64, jitter_hashed=1, seed=3, camera_id=2)]
+            bhs.orbit_camera(12.0, 80.0, 75.0, 128, 64, jitter_hashed=1, seed=3, camera_id=2),
+            bhs.orbit_camera(30.0, 85.0, 60.0, 200, 120, jitter_hashed=1, seed=4, camera_id=3)]
     params = bhs.march_params(root64=1)
-    print("[mlp] surrogate batch of 3 cameras", flush=True)
+    print("[mlp] surrogate batch of 4 cameras", flush=True)
     b = sur.render_batch(cams, params)
     torch.cuda.synchronize()
     B = b.numpy()
