@@ -174,12 +174,14 @@ int table_for(bhs_ctx* ctx, bhs_mlp* m, int32_t ncam, M::SurTable* tab) {
 // Fallback lists for n_items work items: [counters (256 B) | late ids | early ids].
 int lists_for(bhs_ctx* ctx, bhs_mlp* m, uint32_t n_items, M::SurLists* fl) {
     int st;
-    if ((st = grow(ctx, &m->fb, &m->fb_bytes, 256 + (size_t)n_items * 8, "fallback lists")) !=
+    if ((st = grow(ctx, &m->fb, &m->fb_bytes, 256 + (size_t)n_items * 12, "fallback lists")) !=
         BHS_OK)
         return st;
+    // [counters: late, early, early work, late band, late band work | late | early | late band]
     uint32_t* cnt = (uint32_t*)m->fb;
     uint32_t* late = (uint32_t*)((char*)m->fb + 256);
-    *fl = M::SurLists{late, cnt, late + n_items, cnt + 1, cnt + 2, n_items};
+    *fl = M::SurLists{late,        cnt,     late + n_items, cnt + 1, cnt + 2,
+                      late + 2 * (size_t)n_items, cnt + 3, cnt + 4, n_items};
     return BHS_OK;
 }
 
@@ -222,7 +224,7 @@ bhs_mlp* bhs_mlp_create(bhs_ctx* ctx, const bhs_march_params* trained_for, uint3
     hipError_t e;
     if ((e = hipMalloc((void**)&m->P, 4 * sizeof(float) * M::kParams)) != hipSuccess ||
         (e = hipMalloc(&m->packed, M::kPackedBytes)) != hipSuccess ||
-        (e = hipMalloc((void**)&m->steps_dev, 256)) != hipSuccess) {
+        (e = hipMalloc((void**)&m->steps_dev, sizeof m->k.steps_max)) != hipSuccess) {
         (void)hipGetLastError();
         free_model(m);
         *st = fail(ctx, BHS_E_ALLOC, "bhs_mlp_create: device allocation failed%s");
@@ -240,7 +242,7 @@ bhs_mlp* bhs_mlp_create(bhs_ctx* ctx, const bhs_march_params* trained_for, uint3
     }
     // Nothing is served until bhs_mlp_validate has built the error map.
     std::memset(m->k.serve, 0, sizeof m->k.serve);
-    for (int i = 0; i < M::kStepBins; ++i) m->k.steps_max[i] = 0xFFFFu;
+    for (uint32_t& v : m->k.steps_max) v = 0xFFFFu;
     m->k.beta_eps = M::kBetaEps;
     for (int i = 0; i < M::kStepBins; ++i) m->k.radius_margin[i] = 0.1f;
     m->k.logit_margin = 2.0f;
@@ -275,7 +277,7 @@ int bhs_mlp_make_dataset(bhs_ctx* ctx, bhs_mlp* m, int64_t n_rays, uint32_t seed
     float* Y = X + 3 * n_rays;
     float* Mk = Y + 5 * n_rays;
     // pos3/dir3 of the set are not kept: the features, targets and masks are all training needs
-    if ((e = bhs::launch_zero(m->steps_dev, 256, nullptr)) != hipSuccess ||
+    if ((e = bhs::launch_zero(m->steps_dev, sizeof m->k.steps_max, nullptr)) != hipSuccess ||
         (e = M::launch_targets(n_rays, rs.pos3, rs.dir3, rs.soa, X, Y, Mk, m->steps_dev, nullptr)) !=
             hipSuccess ||
         (e = hipDeviceSynchronize()) != hipSuccess)
@@ -571,7 +573,12 @@ int bhs_mlp_get_info(bhs_ctx* ctx, bhs_mlp* m, bhs_mlp_info* out_host) {
     i.radius_margin = *std::max_element(m->k.radius_margin, m->k.radius_margin + M::kStepBins);
     i.logit_margin = m->k.logit_margin;
     i.steps_slack = m->k.steps_slack;
-    std::memcpy(i.steps_max, m->k.steps_max, sizeof i.steps_max);
+    for (int r = 0; r < M::kStepBins; ++r) {   // per log r_obs bin: the largest over x2
+        uint32_t mx = 0;
+        for (int b = 0; b < M::kStepBinsB; ++b)
+            mx = std::max(mx, m->k.steps_max[r * M::kStepBinsB + b]);
+        i.steps_max[r] = mx;
+    }
     int cells = 0;
     for (int w = 0; w < M::kMapWords; ++w) cells += __builtin_popcount(m->k.serve[w]);
     i.served_cells = cells;
@@ -586,8 +593,8 @@ int bhs_mlp_save(bhs_ctx* ctx, bhs_mlp* m, const char* path) {
     int st;
     if ((st = bhs_mlp_get_params(ctx, m, p.data(), M::kParams)) != BHS_OK) return st;
     FileHead h{};
-    std::memcpy(h.magic, "BHSMLP03", 8);
-    h.version = 3;
+    std::memcpy(h.magic, "BHSMLP04", 8);
+    h.version = 4;
     h.params = M::kParams;
     h.trained_for = m->trained_for;
     h.dataset_rays = m->n_data;
@@ -619,7 +626,7 @@ bhs_mlp* bhs_mlp_load(bhs_ctx* ctx, const char* path, int* status) {
     const bool ok = std::fread(&h, sizeof h, 1, f) == 1 &&
                     std::fread(p.data(), sizeof(float), p.size(), f) == p.size();
     std::fclose(f);
-    if (!ok || std::memcmp(h.magic, "BHSMLP03", 8) != 0 || h.version != 3 ||
+    if (!ok || std::memcmp(h.magic, "BHSMLP04", 8) != 0 || h.version != 4 ||
         h.params != (uint32_t)M::kParams) {
         *st = fail(ctx, BHS_E_IO, "bhs_mlp_load: not a model file of this version: %s", path);
         return nullptr;

@@ -12,15 +12,30 @@ namespace mlp {
 
 // Error map of a trained model: a (log r_obs, beta) cell grid over the feature square
 // [-1, 1]^2 of (x1, x2); a set bit marks a cell whose validation rays all met the escape-direction
-// tolerance. Step table: the largest law-S step count of the training rays per log r_obs bin.
+// tolerance. Step table: the largest law-S step count of the training rays per (log r_obs, x2)
+// bin, kStepBins x kStepBinsB over [log 8, log 1000] x [-1, 1] (near b_crit the step count rises
+// steeply with 1 / |beta - 1|).
 constexpr int kMapR = 32, kMapB = 64;
 constexpr int kMapWords = kMapR * kMapB / 32;
-constexpr int kStepBins = 16;
+constexpr int kStepBins = 16, kStepBinsB = 16;
+
+// Log r_obs bin of the radius margins and the step table.
+__device__ __forceinline__ int r_bin(float r) {
+    const int bin = (int)((logf(r) - kLogRLo) * ((float)kStepBins / (kLogRHi - kLogRLo)));
+    return bin < 0 ? 0 : (bin >= kStepBins ? kStepBins - 1 : bin);
+}
+
+// Step-table bin of a ray at r_obs with feature x2.
+__device__ __forceinline__ int step_bin(float r, float x2) {
+    int b = (int)((x2 + 1.0f) * (0.5f * (float)kStepBinsB));
+    b = b < 0 ? 0 : (b >= kStepBinsB ? kStepBinsB - 1 : b);
+    return r_bin(r) * kStepBinsB + b;
+}
 
 // Per-model constants of the surrogate render kernel.
 struct SurConst {
     uint32_t serve[kMapWords];        // error map bits, cell = ir * kMapB + ib
-    uint32_t steps_max[kStepBins];    // training step maximum per log r_obs bin
+    uint32_t steps_max[kStepBins * kStepBinsB];   // training step maximum per (r_obs, x2) bin
     float radius_margin[kStepBins];   // relative margin around the annulus edges per log r_obs bin
     float beta_eps;                   // fallback band |beta - 1| < beta_eps
     float logit_margin;               // |logit| below this: reach undecided -> fallback
@@ -74,6 +89,9 @@ struct SurLists {
     uint32_t* early_ids;      // FP32 fallback items known before the network (outside the served
     uint32_t* early_count;    // domain, unserved cell, step budget)
     uint32_t* early_work;     // work counter of the early march
+    uint32_t* lb_ids;         // band rays the decode sends back while the band march already runs
+    uint32_t* lb_count;       // (latency mode): FP64 march after the network pass
+    uint32_t* lb_work;        // work counter of that march
     uint32_t fb_cap;
 };
 // Side stream of the early FP32 fallback march.

@@ -13,7 +13,7 @@
 //
 // Inputs:
 //   x1 = log r_obs mapped from [log 8, log 1000] to [-1, 1]
-//   x2 = sign(beta - 1) log(1 + |beta - 1| / 0.02) / log(1 + 1e4), beta = b / b_crit
+//   x2 = sign(beta - 1) log(1 + |beta - 1| / 0.002) / log(1 + 1e5), beta = b / b_crit
 //   x3 = psi mapped from [0, pi) to [-1, 1)
 // Outputs:
 //   y0  escape: residual of the law-S escape angle over the exact Schwarzschild swept angle,
@@ -52,7 +52,9 @@ constexpr int kParams = kB4 + kOut;   // 34181
 
 constexpr float kResScale = 1e-3f;   // escape residual unit (rad)
 constexpr float kURes = 1e-3f;            // crossing residual unit: u = u_exact exp(1e-3 y)
-constexpr float kBetaEps = 0.02f;          // domain: |beta - 1| > 2% (the FP64 band)
+constexpr float kBetaEps = 0.002f;         // domain: |beta - 1| > 0.2%
+constexpr float kX2Scale = 0.002f;         // x2 feature: log(1 + |beta - 1| / kX2Scale) / kX2Norm
+constexpr float kX2Norm = 11.512935464920229f;   // log(1 + 1e5)
 constexpr float kLogRLo = 2.0794415f;      // log 8
 constexpr float kLogRHi = 6.9077553f;      // log 1000
 constexpr float kPi = 3.14159265358979f;
@@ -205,8 +207,8 @@ __device__ __forceinline__ RayGeom ray_geom(const V3<float>& P, const V3<float>&
     g.e2 = V3<float>{t.x * it, t.y * it, t.z * it};
     g.alpha_v = atan2f(tn, vr);
     g.ingoing = vr < 0.0f;
-    // beta - 1 in float: outside the 2% band its relative rounding (~1e-7 / 0.02) moves the exact
-    // sweep by < 1e-5 rad
+    // beta - 1 in float: its rounding (~6e-8) moves the exact sweep by ~6e-8 / |beta - 1| rad,
+    // 3e-5 rad at the domain edge (the error map validates what it does to served rays)
     g.b = s.b;
     g.bm = s.b * (1.0f / kBcritF) - 1.0f;
     const float f = sqrtf(fmaxf(1.0f - 2.0f * ir, 1e-12f));
@@ -268,15 +270,15 @@ __device__ __forceinline__ float orbit_u(const Orbit& o, float phi) {
 }
 
 
-// Input feature of beta - 1 (scale 0.02, normalised by log(1 + 1e4)).
+// Input feature of beta - 1 (scale kX2Scale, normalised by kX2Norm).
 __device__ __forceinline__ float x2_of(float bm) {
-    const float m = log1pf(fabsf(bm) * (1.0f / kBetaEps)) * (1.0f / 9.2104404f);
+    const float m = log1pf(fabsf(bm) * (1.0f / kX2Scale)) * (1.0f / kX2Norm);
     return bm < 0.0f ? -m : m;
 }
 
 // Inverse of x2_of: beta - 1 of a feature value.
 __device__ __forceinline__ float bm_of_x2(float x2) {
-    const float m = kBetaEps * expm1f(fabsf(x2) * 9.2104404f);
+    const float m = kX2Scale * expm1f(fabsf(x2) * kX2Norm);
     return x2 < 0.0f ? -m : m;
 }
 

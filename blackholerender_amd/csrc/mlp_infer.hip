@@ -198,14 +198,9 @@ __device__ __forceinline__ bool cell_served(const SurConst& k, const float x[3])
     return (k.serve[cell >> 5] >> (cell & 31)) & 1u;
 }
 
-// Log r_obs bin of the step table and the radius margins.
-__device__ __forceinline__ int r_bin(float r) {
-    const int bin = (int)((logf(r) - kLogRLo) * ((float)kStepBins / (kLogRHi - kLogRLo)));
-    return bin < 0 ? 0 : (bin >= kStepBins ? kStepBins - 1 : bin);
-}
-
-__device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k, const RayGeom& g) {
-    return (float)k.steps_max[r_bin(g.r)] * k.steps_slack < (float)c.max_steps;
+__device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k, const RayGeom& g,
+                                         float x2) {
+    return (float)k.steps_max[step_bin(g.r, x2)] * k.steps_slack < (float)c.max_steps;
 }
 
 // Output record of one in-domain ray from its swept angle at escape / capture Phi (learned
@@ -422,26 +417,27 @@ struct RayPrep {
     RayGeom g;
     float x[3];
     Route route;
+    bool band;   // in the integrator's FP64 band (BAND64 precision)
 };
 
-// Classification of a ray; identical in the pre-pass and the network pass (same operations).
+// Classification of a ray; identical in the pre-pass and the network pass (same operations). A
+// ray of the FP64 band that the model serves is a network candidate; the other band rays are
+// marched in FP64.
 __device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst& k, int64_t pix) {
     RayPrep r;
     r.x[0] = r.x[1] = r.x[2] = 0.0f;
     r.route = kNone;
+    r.band = false;
     if (pix < 0) return r;
     const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
     const V3<float> d = ray_dir(c, pix);
     r.s = lawS_setup(P, d);
-    if (c.precision == BHS_PREC_F32_BAND64 && __builtin_fabsf(r.s.b - kBcritF) < c.band_abs) {
-        r.route = kBand;
-        return r;
-    }
+    r.band = c.precision == BHS_PREC_F32_BAND64 && __builtin_fabsf(r.s.b - kBcritF) < c.band_abs;
     r.g = ray_geom(P, d, r.s);
-    r.route = kEarly;
+    r.route = r.band ? kBand : kEarly;
     if (in_domain(r.g)) {
         features(r.g, r.x);
-        if (cell_served(k, r.x) && steps_ok(c, k, r.g)) r.route = kNet;
+        if (cell_served(k, r.x) && steps_ok(c, k, r.g, r.x[1])) r.route = kNet;
     }
     return r;
 }
@@ -455,6 +451,21 @@ __device__ __forceinline__ void push_fallback(const MarchConst& c, const Scratch
         push_block(sc.band_count, sc.band_ids, sc.band_cap, want, q, s_cnt);
     else
         push_block(count, ids, cap, want, q, s_cnt);
+}
+
+// A network candidate the decode sends back (back): a band ray to the FP64 band (the late band
+// list when the band march already runs, band_running, else the call's band list), any other to
+// the FP32 late list; FP64 precision sends everything to the band list. Every thread of the block
+// calls it.
+__device__ __forceinline__ void push_back(const MarchConst& c, const Scratch& sc,
+                                          const SurLists& fl, bool back, bool band,
+                                          bool band_running, uint32_t q, uint32_t* s_cnt) {
+    const bool to_band = back && (band || c.precision == BHS_PREC_F64);
+    push_block(fl.fb_count, fl.fb_ids, fl.fb_cap, back && !to_band, q, s_cnt);
+    if (band_running)
+        push_block(fl.lb_count, fl.lb_ids, fl.fb_cap, to_band, q, s_cnt);
+    else
+        push_block(sc.band_count, sc.band_ids, sc.band_cap, to_band, q, s_cnt);
 }
 
 // Pre-pass: lists the FP64 band rays and the early fallback rays.
@@ -510,7 +521,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
                 back = true;
             }
         }
-        push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
+        push_back(c, sc, fl, back, r.band, (listed & kListedBand) != 0, q, s_cnt);
         if (!(listed & kListedBand))
             push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
         if (!(listed & kListedEarly))
@@ -735,7 +746,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
                 back = true;
             }
         }
-        push_fallback(c, sc, fl.fb_count, fl.fb_ids, fl.fb_cap, back, q, s_cnt);
+        push_back(c, sc, fl, back, r.band, (listed & kListedBand) != 0, q, s_cnt);
         if (!(listed & kListedBand))
             push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
         if (!(listed & kListedEarly))
@@ -983,6 +994,20 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
+    if (routed && any_band) {
+        // band rays the decode sent back (their band march had started): FP64 march on aux2
+        Scratch sb = sc;
+        sb.band_ids = fl.lb_ids;
+        sb.band_count = fl.lb_count;
+        sb.band_work = fl.lb_work;
+        sb.band_cap = fl.fb_cap;
+        if constexpr (kOne)
+            e = launch_band_list(c0, out, sb, s, ss.aux2, ss.ev_fork2, ss.ev_join2);
+        else
+            e = launch_band_list_batch(table, item_base, ncam, n_items, out, sb, s, ss.aux2,
+                                       ss.ev_fork2, ss.ev_join2);
+        if (e != hipSuccess) return e;
+    }
     if (!routed && !f64 && (e = side()) != hipSuccess) return e;
     if (f64) {   // every unserved ray is in the band list: FP64 march now
         if ((e = band()) != hipSuccess) return e;

@@ -16,7 +16,7 @@ __device__ __forceinline__ float urand(uint32_t key, uint32_t i, uint32_t k) {
 
 // ------------------------------------------------------------------------------------------------
 // Training rays: r_obs log-uniform in [8, 1000] M; beta = b/b_crit log-spaced on both sides of 1
-// (40% above: 1 + 10^U(log10 0.02, log10(beta_max - 1)); 20% below: 1 - 10^U(log10 0.02,
+// (40% above: 1 + 10^U(log10 0.002, log10(beta_max - 1)); 20% below: 1 - 10^U(log10 0.002,
 // log10 0.7)) plus 40%
 // uniform in [0.3, beta_max]; b at most 0.95 of the tangential limit r/sqrt(1 - 2/r); the first
 // equatorial crossing at swept angle psi uniform in [0, pi); the orbit plane tilted by U(15, 89) deg
@@ -34,9 +34,9 @@ __global__ void k_sample_rays(int64_t n, uint32_t key, float* __restrict__ pos3,
     float beta;
     if (u < 0.4f) {
         const float tmax = log10f(beta_max - 1.0f);
-        beta = 1.0f + exp10f(-1.69897f + (tmax + 1.69897f) * urand(key, ii, 2));
+        beta = 1.0f + exp10f(-2.69897f + (tmax + 2.69897f) * urand(key, ii, 2));
     } else if (u < 0.6f) {
-        beta = 1.0f - exp10f(-1.69897f + 1.5440680f * urand(key, ii, 2));   // 0.02 .. 0.7
+        beta = 1.0f - exp10f(-2.69897f + 2.5440680f * urand(key, ii, 2));   // 0.002 .. 0.7
     } else {
         beta = 0.3f + (beta_max - 0.3f) * urand(key, ii, 2);
     }
@@ -65,7 +65,7 @@ __global__ void k_sample_rays(int64_t n, uint32_t key, float* __restrict__ pos3,
 
 // Features, targets and masks from the integrator's outputs (FP64 march, root-refined crossings,
 // crossings recorded at any radius). Rays outside the surrogate's domain get all-zero masks.
-// steps_max[bin] collects the largest step count per log r_obs bin of the domain rays.
+// steps_max[step_bin] collects the largest step count per (log r_obs, x2) bin of the domain rays.
 __global__ void k_targets(int64_t n, const float* __restrict__ pos3, const float* __restrict__ dir3,
                           const SoA out, float* __restrict__ X, float* __restrict__ Y,
                           float* __restrict__ M, uint32_t* __restrict__ steps_max) {
@@ -115,11 +115,7 @@ __global__ void k_targets(int64_t n, const float* __restrict__ pos3, const float
     M[5 * i + 2] = mu[1] * m;
     M[5 * i + 3] = !maxed ? m : 0.0f;    // out-of-steps rays: reach unknown
     M[5 * i + 4] = !maxed ? m : 0.0f;
-    if (dom && steps_max) {
-        int bin = (int)((logf(g.r) - kLogRLo) * ((float)16 / (kLogRHi - kLogRLo)));
-        bin = bin < 0 ? 0 : (bin > 15 ? 15 : bin);
-        atomicMax(&steps_max[bin], (uint32_t)out.steps[i]);
-    }
+    if (dom && steps_max) atomicMax(&steps_max[step_bin(g.r, x[1])], (uint32_t)out.steps[i]);
 }
 
 // ------------------------------------------------------------------------------------------------

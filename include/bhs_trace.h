@@ -242,7 +242,8 @@ typedef struct bhs_mlp_info {
     float radius_margin;      /* relative margin around the annulus edges */
     float logit_margin;       /* |logit| below this: reach undecided */
     float steps_slack;        /* served only if the training step maximum * slack < max_steps */
-    uint32_t steps_max[16];   /* training step maximum per log r_obs bin over [8, 1000] */
+    uint32_t steps_max[16];   /* training step maximum per log r_obs bin over [8, 1000] (the
+                                 largest over b; the model keeps a (r_obs, b) table) */
     int32_t served_cells;     /* error-map cells (of 32 x 64) that serve rays */
     int32_t pad_;
 } bhs_mlp_info;
