@@ -186,32 +186,32 @@ struct RayGeom {
     float bm;             // beta - 1
     float b;              // impact parameter
     float psi;            // swept angle of the first equatorial crossing, [0, pi)
-    float alpha_v;        // angle of the initial coordinate velocity from e1
     float sin_a;          // sin of the view angle from the radial direction
     bool ingoing;
     bool planar;          // degenerate orbit plane (radial ray) or camera on the equator
 };
 
+// (Hardware square-root and reciprocal approximations: the frame and the view angle feed the
+// network and the decode, which need ~1e-6 relative, not the integrator's exact rounding.)
 __device__ __forceinline__ RayGeom ray_geom(const V3<float>& P, const V3<float>& d,
                                             const Setup<float>& s) {
     RayGeom g;
-    g.r = sqrtf(dot3(P, P));
-    const float ir = 1.0f / g.r;
+    g.r = __builtin_amdgcn_sqrtf(dot3(P, P));
+    const float ir = __builtin_amdgcn_rcpf(g.r);
     g.e1 = V3<float>{P.x * ir, P.y * ir, P.z * ir};
     const V3<float>& v = s.v;
     const float vr = dot3(v, g.e1);
     V3<float> t{v.x - vr * g.e1.x, v.y - vr * g.e1.y, v.z - vr * g.e1.z};
-    const float tn = sqrtf(dot3(t, t));
+    const float tn = __builtin_amdgcn_sqrtf(dot3(t, t));
     g.planar = !(tn > 1e-6f);
-    const float it = g.planar ? 0.0f : 1.0f / tn;
+    const float it = g.planar ? 0.0f : __builtin_amdgcn_rcpf(tn);
     g.e2 = V3<float>{t.x * it, t.y * it, t.z * it};
-    g.alpha_v = atan2f(tn, vr);
     g.ingoing = vr < 0.0f;
     // beta - 1 in float: its rounding (~6e-8) moves the exact sweep by ~6e-8 / |beta - 1| rad,
     // 3e-5 rad at the domain edge (the error map validates what it does to served rays)
     g.b = s.b;
     g.bm = s.b * (1.0f / kBcritF) - 1.0f;
-    const float f = sqrtf(fmaxf(1.0f - 2.0f * ir, 1e-12f));
+    const float f = __builtin_amdgcn_sqrtf(fmaxf(1.0f - 2.0f * ir, 1e-12f));
     g.sin_a = s.b * f * ir;
     float psi = atan2f(-g.e1.y, g.e2.y);
     if (psi < 0.0f) psi += kPi;

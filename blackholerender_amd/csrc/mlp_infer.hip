@@ -208,6 +208,17 @@ __device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k,
 // for crossings before Phi), or serve = false when the ray must go back to the integrator: a
 // crossing radius within the margin of an annulus edge, or a third crossing that could be
 // recorded.
+// redshiftFactor (disk.glsl:37-42) with hardware reciprocal and square-root approximations (the
+// integrator's version rounds every operation exactly; a served record needs ~1e-6 relative).
+__device__ __forceinline__ float redshift_fast(float r, float b_axis, float lapse) {
+    const float omega = __builtin_amdgcn_rsqf(r * r * r);
+    float t = 1.0f - 3.0f * __builtin_amdgcn_rcpf(r);
+    if (t < 0.02f) t = 0.02f;
+    float m = __builtin_fmaf(-omega, b_axis, 1.0f);
+    if (m < 0.05f) m = 0.05f;
+    return __builtin_amdgcn_sqrtf(t) * __builtin_amdgcn_rcpf(lapse * m);
+}
+
 template <typename UAt>
 __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& k,
                                            const Setup<float>& s, const RayGeom& g, float Phi,
@@ -226,7 +237,7 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
         o.r_eq[q] = 1e30f;
         if ((uint32_t)q >= o.n_eq) continue;
         const float u = u_at(q);
-        o.r_eq[q] = u > 0.0f ? 1.0f / u : 1e30f;
+        o.r_eq[q] = u > 0.0f ? __builtin_amdgcn_rcpf(u) : 1e30f;
         if (c.thin) {
             const float r = o.r_eq[q];
             if (fabsf(r - c.disk_inner) < margin * c.disk_inner ||
@@ -246,7 +257,7 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
         if (!in[q]) continue;
         const float r = o.r_eq[q], sr = q ? -r : r;
         const float ph = atan2_ieee(sr * w.z, sr * w.x);
-        const float gg = redshift(r, s.b_axis, s.lapse_g);
+        const float gg = redshift_fast(r, s.b_axis, s.lapse_g);
         if (o.n_cross == 0) {
             o.dr[0] = r;
             o.dphi[0] = ph;
@@ -592,13 +603,15 @@ struct TabGeom {
     float x2_lo, inv_h2, h2, inv_h3, h3;
 };
 
+// Node spacing of a camera's grid (approximate reciprocals: the nodes and the lookups use the
+// same values).
 __device__ __forceinline__ TabGeom tab_geom(const float* range, int32_t n2, int32_t n3) {
     TabGeom t;
     t.x2_lo = range[0];
-    t.h2 = (range[1] - range[0]) / (float)(n2 - 1);
-    t.inv_h2 = 1.0f / t.h2;
-    t.h3 = 2.0f / (float)(n3 - 1);
-    t.inv_h3 = 1.0f / t.h3;
+    t.h2 = (range[1] - range[0]) * __builtin_amdgcn_rcpf((float)(n2 - 1));
+    t.inv_h2 = (float)(n2 - 1) * __builtin_amdgcn_rcpf(range[1] - range[0]);
+    t.h3 = 2.0f * __builtin_amdgcn_rcpf((float)(n3 - 1));
+    t.inv_h3 = 0.5f * (float)(n3 - 1);
     return t;
 }
 
@@ -701,7 +714,7 @@ __device__ __forceinline__ void tab_lookup(const float* __restrict__ node, const
 __device__ __forceinline__ float phi_lookup(const float* __restrict__ phi, const float* range,
                                             float x2) {
     const float lo = range[0], hi = range[1];
-    float f = (x2 - lo) * ((float)(kPhiNodes - 1) / (hi - lo));
+    float f = (x2 - lo) * ((float)(kPhiNodes - 1) * __builtin_amdgcn_rcpf(hi - lo));
     f = fminf(fmaxf(f, 0.0f), (float)(kPhiNodes - 1) - 1e-3f);
     const int32_t i = (int32_t)f;
     const float a = f - (float)i;
