@@ -375,14 +375,6 @@ __device__ __forceinline__ void add_block_stats(Stats* st, uint32_t served, uint
     }
 }
 
-// Work item q of a camera (8x8 tiles, as the integrator kernels).
-__device__ __forceinline__ int64_t item_pixel(const MarchConst& c, uint32_t q) {
-    const uint32_t tiles_x = ((uint32_t)c.width + 7u) >> 3;
-    const uint32_t t = q >> 6, l = q & 63u;
-    const uint32_t col = (t % tiles_x) * 8u + (l & 7u), row = (t / tiles_x) * 8u + (l >> 3);
-    return (col < (uint32_t)c.width && row < (uint32_t)c.height) ? (int64_t)row * c.width + col : -1;
-}
-
 constexpr int kBlock = 256;
 
 // Camera sources of the surrogate kernels: one camera by value, or the device camera table of a
@@ -409,6 +401,29 @@ struct SurCamTable {
     __device__ __forceinline__ const MarchConst& cam(uint32_t k) const { return cams[k]; }
     __device__ __forceinline__ uint32_t base(uint32_t k) const { return item_base[k]; }
 };
+
+// Camera and pixel of the calling lane's item in the wave whose first item is q_wave: a wave's 64
+// items are one 8x8 tile of one camera, so the camera search, the camera's constants and the tile
+// position are wave-uniform (scalar) and only the pixel inside the tile is per lane.
+struct WaveItem {
+    uint32_t ci;
+    int64_t pix;   // -1: padding or past the end
+};
+template <typename Cams>
+__device__ __forceinline__ WaveItem wave_item(const Cams& cams, uint32_t q_wave, uint32_t n_items) {
+    const uint32_t qw = __builtin_amdgcn_readfirstlane(q_wave);
+    const uint32_t lane = threadIdx.x & 63u;
+    WaveItem w;
+    w.ci = cams.find(qw < n_items ? qw : n_items - 1);
+    const MarchConst& c = cams.cam(w.ci);
+    const uint32_t t = (qw - cams.base(w.ci)) >> 6;
+    const uint32_t tiles_x = ((uint32_t)c.width + 7u) >> 3;
+    const uint32_t col = (t % tiles_x) * 8u + (lane & 7u), row = (t / tiles_x) * 8u + (lane >> 3);
+    w.pix = (qw + lane < n_items && col < (uint32_t)c.width && row < (uint32_t)c.height)
+                ? (int64_t)row * c.width + col
+                : -1;
+    return w;
+}
 
 // Route of one ray of a surrogate render, decided before the network runs.
 enum Route : uint32_t {
@@ -486,9 +501,9 @@ __global__ __launch_bounds__(kBlock) void k_sur_pre(const Cams cams, const SurCo
     __shared__ uint32_t s_cnt[5];
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
         const uint32_t q = q0 + threadIdx.x;
-        const uint32_t ci = cams.find(q < n_items ? q : n_items - 1);
-        const MarchConst& c = cams.cam(ci);
-        const RayPrep r = prep_ray(c, k, q < n_items ? item_pixel(c, q - cams.base(ci)) : -1);
+        const WaveItem w = wave_item(cams, q0 + (threadIdx.x & ~63u), n_items);
+        const MarchConst& c = cams.cam(w.ci);
+        const RayPrep r = prep_ray(c, k, w.pix);
         push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
         push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q, s_cnt);
     }
@@ -513,9 +528,9 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
     const uint32_t chunks = (n_items + kBlock - 1) / kBlock;
     for (uint32_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
         const uint32_t q = ch * kBlock + threadIdx.x;
-        const uint32_t ci = cams.find(q < n_items ? q : n_items - 1);
-        const MarchConst& c = cams.cam(ci);
-        const int64_t pix = q < n_items ? item_pixel(c, q - cams.base(ci)) : -1;
+        const WaveItem w = wave_item(cams, ch * kBlock + (threadIdx.x & ~63u), n_items);
+        const MarchConst& c = cams.cam(w.ci);
+        const int64_t pix = w.pix;
         const RayPrep r = prep_ray(c, k, pix);
         const bool cand = r.route == kNet;
         float y[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -738,9 +753,10 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
     const int64_t nodes = (int64_t)n2 * n3 * 4;
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
         const uint32_t q = q0 + threadIdx.x;
-        const uint32_t ci = cams.find(q < n_items ? q : n_items - 1);
+        const WaveItem w = wave_item(cams, q0 + (threadIdx.x & ~63u), n_items);
+        const uint32_t ci = w.ci;
         const MarchConst& c = cams.cam(ci);
-        const int64_t pix = q < n_items ? item_pixel(c, q - cams.base(ci)) : -1;
+        const int64_t pix = w.pix;
         const RayPrep r = prep_ray(c, k, pix);
         bool back = false;
         if (r.route == kNet) {
