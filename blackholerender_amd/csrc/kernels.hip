@@ -295,9 +295,9 @@ struct BandCams {
     uint32_t base[kBatch ? kMaxBatchCams + 1 : 1];
 };
 
-// FP64 march of the band list, first part: list positions [0, min(n, lanes of the grid)) (the
-// count n is read on the device), one ray per lane. Covers the band of a frame or of a moderate
-// batch at once; its longest ray sets its time.
+// FP64 march of a band list that the grid covers at once (n at most the grid's lanes; the count n
+// is read on the device), one ray per lane: the band of a frame or of a moderate batch, whose
+// longest ray sets the time. A longer list is left to k_band64_queue.
 template <typename Cams>
 __global__ __launch_bounds__(kBandBlock) __attribute__((amdgpu_waves_per_eu(4)))
 void k_band64(const Cams cams, const SoA out, Scratch sc) {
@@ -313,6 +313,7 @@ void k_band64(const Cams cams, const SoA out, Scratch sc) {
     // Few waves with long serial chains run beside the FP32 march: issue them first on their SIMD.
     __builtin_amdgcn_s_setprio(3);
     const uint32_t n = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
+    if (n > gridDim.x * kBandBlock) return;   // uniform: k_band64_queue's list
     const uint32_t q = blockIdx.x * kBandBlock + threadIdx.x;
     uint32_t one = 0, steps = 0, cross = 0, esc = 0;
     if (q < n) {
@@ -337,19 +338,19 @@ void k_band64(const Cams cams, const SoA out, Scratch sc) {
     add_stats(sc.stats, one, steps, cross, esc, one);
 }
 
-// FP64 march of the band list, rest: positions [first, n) of a list longer than k_band64's lanes
-// (large batches), by resident waves pulling rays from it (queue counter sc.band_work); finished
-// lanes are refilled as in k_persist, so a wave is not held by its slowest ray. ROOTS: the FP64
+// FP64 march of a band list longer than k_band64's lanes (`lanes`; large batches): every position,
+// by resident waves pulling rays from it (queue counter sc.band_work); finished lanes are refilled
+// as in k_persist, so a wave is not held by its slowest ray. ROOTS: the FP64
 // root refinement of march_f64 for cameras with root64. Exit: the list is finite, every ray ends
 // within max_steps iterations and the counter only grows, so every wave leaves.
 template <typename Cams, bool ROOTS>
 __global__ __launch_bounds__(kBandBlock) __attribute__((amdgpu_waves_per_eu(ROOTS ? 2 : 3)))
-void k_band64_queue(const Cams cams, const SoA out, Scratch sc, uint32_t first) {
+void k_band64_queue(const Cams cams, const SoA out, Scratch sc, uint32_t lanes) {
     constexpr bool kBatch = BandCams<Cams>::kBatch;
     __shared__ BandCams<Cams> s_c;
     __shared__ MarchK<double> s_k[kBatch ? kMaxBatchCams : 1];
     const uint32_t n = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
-    if (n <= first) return;   // uniform: the list fitted k_band64
+    if (n <= lanes) return;   // uniform: k_band64's list
     uint32_t ncam = 1;
     if constexpr (kBatch) {
         ncam = cams.ncam;
@@ -372,8 +373,8 @@ void k_band64_queue(const Cams cams, const SoA out, Scratch sc, uint32_t first) 
     uint32_t n_rays = 0, n_steps = 0, n_cross = 0, n_esc = 0;
     while (true) {
         uint32_t q = 0;
-        if (claim_items(sc.band_work, n - first, kBandRefill, active, more, q)) {
-            const uint32_t item = sc.band_ids[first + q];
+        if (claim_items(sc.band_work, n, kBandRefill, active, more, q)) {
+            const uint32_t item = sc.band_ids[q];
             uint32_t local = item;
             if constexpr (kBatch) {
                 ci = find_cam(s_c.base, ncam, item);
@@ -748,8 +749,8 @@ hipError_t launch_persist(const Tuning& t, int num_cus, hipStream_t s, const Mar
     return hipGetLastError();
 }
 
-// FP64 march of the band list on `aux`: k_band64 over the first positions, then (lists longer
-// than its lanes only; it exits at once otherwise) k_band64_queue over the rest, its ROOTS variant
+// FP64 march of the band list on `aux`: k_band64 if the list fits its lanes, else k_band64_queue
+// (the other exits at once: the list length is known on the device only), its ROOTS variant
 // (crossing states in registers) only for launches with FP64 root refinement.
 template <typename Cams>
 hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hipStream_t aux) {
@@ -757,8 +758,8 @@ hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hi
     hipLaunchKernelGGL(k_band64<Cams>, g, dim3(kBandBlock), 0, aux, cams, out, sc);
     hipError_t e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const uint32_t first = g.x * kBandBlock;
-    if (sc.band_cap <= first) return hipSuccess;
+    const uint32_t lanes = g.x * kBandBlock;
+    if (sc.band_cap <= lanes) return hipSuccess;
     bool roots;
     if constexpr (std::is_same<Cams, OneCam>::value)
         roots = cams.c.root64 != 0;
@@ -766,10 +767,10 @@ hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hi
         roots = sc.roots != nullptr;   // batches allocate root states iff a camera has root64
     if (roots)
         hipLaunchKernelGGL((k_band64_queue<Cams, true>), g, dim3(kBandBlock), 0, aux, cams, out,
-                           sc, first);
+                           sc, lanes);
     else
         hipLaunchKernelGGL((k_band64_queue<Cams, false>), g, dim3(kBandBlock), 0, aux, cams, out,
-                           sc, first);
+                           sc, lanes);
     return hipGetLastError();
 }
 
