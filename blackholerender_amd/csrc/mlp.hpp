@@ -100,10 +100,11 @@ struct SurStreams {
     hipEvent_t ev_fork, ev_join, ev_fork2, ev_join2;
 };
 // Table mode: per camera, the network on an n2 x n3 grid over (x2, x3) at the camera's r_obs
-// (node: ncam n2 n3 x 8 floats; range: 2 ncam floats of device scratch).
+// (node: ncam n2 n3 x 4 floats; phi: ncam x kPhiNodes floats; range, slot: per camera).
 // Table mode, per camera: an n2 x n3 grid of nodes over (x2, x3) holding the network's escape
-// residual and the two crossings' u with their residuals applied, and a kPhiNodes-point table of
-// the exact swept angle at escape / capture over x2 (both over the camera's x2 range).
+// residual, the first two crossings' u with their residuals applied and the third crossing's u of
+// the exact orbit, and a kPhiNodes-point table of the exact swept angle at escape / capture over x2
+// (both over the camera's x2 range).
 constexpr int32_t kPhiNodes = 4096;
 // Cameras of one pose (position, orientation, field of view and image size; the jitter may
 // differ) share one table: slot[k] is the first camera of camera k's pose, and the table of

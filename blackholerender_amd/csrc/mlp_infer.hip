@@ -185,7 +185,7 @@ struct Dec {
     uint32_t cls, flags, n_cross;
     float esc[3];
     float dr[2], dphi[2], dg[2];
-    uint32_t n_eq;      // equatorial crossings before escape/capture (3 means three or more)
+    uint32_t n_eq;      // equatorial crossings before escape/capture (4 means four or more)
     float r_eq[2];      // radii of the first two equatorial crossings
 };
 
@@ -203,11 +203,15 @@ __device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k,
     return (float)k.steps_max[step_bin(g.r, x2)] * k.steps_slack < (float)c.max_steps;
 }
 
+// Relative margin around the annulus edges for a third crossing (exact orbit, no learned residual).
+constexpr float kThirdMargin = 0.05f;
+
 // Output record of one in-domain ray from its swept angle at escape / capture Phi (learned
-// residual included) and the u = 1/r of its equatorial crossings (u_at(q), q = 0, 1, asked only
-// for crossings before Phi), or serve = false when the ray must go back to the integrator: a
-// crossing radius within the margin of an annulus edge, or a third crossing that could be
-// recorded.
+// residual included) and the u = 1/r of its equatorial crossings (u_at(q), q = 0, 1 with the
+// learned residual, q = 2 the exact orbit's, asked only for crossings before Phi), or serve =
+// false when the ray must go back to the integrator: a crossing radius within the margin of an
+// annulus edge, a third crossing in the annulus that would be recorded (the first two are not
+// both in it), or a fourth crossing. A third crossing is only counted (the march records two).
 // redshiftFactor (disk.glsl:37-42) with hardware reciprocal and square-root approximations (the
 // integrator's version rounds every operation exactly; a served record needs ~1e-6 relative).
 __device__ __forceinline__ float redshift_fast(float r, float b_axis, float lapse) {
@@ -229,7 +233,9 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
     o.n_cross = 0;
     // validity of the crossings from the total sweep
     const float a = Phi - g.psi;
-    o.n_eq = a <= 0.0f ? 0u : (a <= kPi ? 1u : (a <= 2.0f * kPi ? 2u : 3u));
+    o.n_eq = a <= 0.0f ? 0u
+                       : (a <= kPi ? 1u : (a <= 2.0f * kPi ? 2u : (a <= 3.0f * kPi ? 3u : 4u)));
+    if (o.n_eq >= 4u && c.thin) return o;
     bool in[2] = {false, false};
     const float margin = k.radius_margin[r_bin(g.r)];
 #pragma unroll
@@ -246,7 +252,17 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
             in[q] = r >= c.disk_inner && r <= c.disk_outer;
         }
     }
-    if (o.n_eq >= 3u && c.thin && !(in[0] && in[1])) return o;
+    bool in3 = false;
+    if (o.n_eq == 3u && c.thin) {
+        const float u = u_at(2);
+        const float r = u > 0.0f ? __builtin_amdgcn_rcpf(u) : 1e30f;
+        const float m3 = fmaxf(margin, kThirdMargin);
+        if (fabsf(r - c.disk_inner) < m3 * c.disk_inner ||
+            fabsf(r - c.disk_outer) < m3 * c.disk_outer)
+            return o;
+        in3 = r >= c.disk_inner && r <= c.disk_outer;
+        if (in3 && !(in[0] && in[1])) return o;
+    }
     float sp, cp;
     sincosf(g.psi, &sp, &cp);
     const V3<float> w{cp * g.e1.x + sp * g.e2.x, cp * g.e1.y + sp * g.e2.y,
@@ -269,6 +285,7 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
         }
         ++o.n_cross;
     }
+    if (in3) ++o.n_cross;
     if (o.escaping) {
         float sf, cf;
         sincosf(Phi, &sf, &cf);
@@ -292,7 +309,8 @@ __device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, co
     // the exact sweep to infinity with the learned residual, or to the horizon test radius
     const float Phi = ob.phi_end + (g.bm > 0.0f ? kResScale * y[0] : 0.0f);
     return decode_with(c, k, s, g, Phi, [&](int q) {
-        return orbit_u(ob, g.psi + (float)q * kPi) * __expf(kURes * y[1 + q]);
+        const float u = orbit_u(ob, g.psi + (float)q * kPi);
+        return q < 2 ? u * __expf(kURes * y[1 + q]) : u;
     });
 }
 
@@ -560,14 +578,15 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
 
 // ------------------------------------------------------------------------------------------------
 // Table mode: per camera, everything of the decode that depends on (x2, x3) only (x1 is the
-// camera's): the network's escape residual and the two crossings' u (exact orbit with the learned
-// residual applied) on an n2 x n3 grid, interpolated bilinearly, and the exact swept angle at
-// escape / capture (a function of x2 alone, with a logarithmic rise towards the band) on a
-// kPhiNodes-point grid, interpolated linearly. A ray then costs its setup, two lookups and the
-// output trigonometry instead of the orbit quadratures and Newton solves. x3 spans [-1, 1]; x2 spans
-// [x2(beta = 0.3), x2(beta_max)] with beta_max from the image corners (the largest view angle from
-// the inward radial over a pinhole frustum is at a corner), both widened by 1e-3. Node (i, j) of
-// camera k is node[((k n3 + j) n2 + i) * 4 + o]; range[2k], range[2k + 1] hold the x2 bounds.
+// camera's): the network's escape residual, the first two crossings' u (exact orbit with the
+// learned residual applied) and the third crossing's (exact orbit) on an n2 x n3 grid,
+// interpolated bilinearly, and the exact swept angle at escape / capture (a function of x2 alone,
+// with a logarithmic rise towards the band) on a kPhiNodes-point grid, interpolated linearly. A ray
+// then costs its setup, two lookups and the output trigonometry instead of the orbit quadratures
+// and Newton solves. x3 spans [-1, 1]; x2 spans [x2(beta = 0.3), x2(beta_max)] with beta_max from
+// the image corners (the largest view angle from the inward radial over a pinhole frustum is at a
+// corner), both widened by 1e-3. Node (i, j) of camera k is node[((k n3 + j) n2 + i) * 4 + o];
+// range[2k], range[2k + 1] hold the x2 bounds.
 
 __device__ __forceinline__ bool same_pose(const MarchConst& a, const MarchConst& b) {
     bool eq = a.width == b.width && a.height == b.height && a.tan_half_fov == b.tan_half_fov;
@@ -649,7 +668,8 @@ __device__ __forceinline__ float cam_radius(const MarchConst& c) {
 }
 
 // Table nodes of every camera (blocks walk (camera, chunk) pairs): at node (i, j) = (x2_i, x3_j),
-// [escape residual y0, u of crossing 1, u of crossing 2 (exact orbit times exp(1e-3 y1|y2)), 0].
+// [escape residual y0, u of crossings 1 and 2 (exact orbit times exp(1e-3 y1|y2)), u of crossing 3
+// (exact orbit)].
 template <typename Cams>
 __global__ __launch_bounds__(kBlock, 2) void k_table(const Cams cams, uint32_t ncam,
                                                      const uint8_t* __restrict__ packed,
@@ -683,7 +703,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_table(const Cams cams, uint32_t n
             v[0] = y[0];
             v[1] = finite_or_zero(orbit_u(ob, psi) * __expf(kURes * y[1]));
             v[2] = finite_or_zero(orbit_u(ob, psi + kPi) * __expf(kURes * y[2]));
-            v[3] = 0.0f;
+            v[3] = finite_or_zero(orbit_u(ob, psi + 2.0f * kPi));
             *(f4*)(node + ((int64_t)k * n + q) * 4) = v;
         }
     }
@@ -705,10 +725,10 @@ __global__ __launch_bounds__(kBlock) void k_tab_phi(const Cams cams, uint32_t nc
     phi[q] = finite_or_zero(node_orbit(cam_radius(cams.cam(k)), x2).phi_end);
 }
 
-// Bilinear lookup of a node grid at (x2, x3): y0, u1, u2.
+// Bilinear lookup of a node grid at (x2, x3): y0, u of crossings 1, 2, 3.
 __device__ __forceinline__ void tab_lookup(const float* __restrict__ node, const TabGeom& t,
                                            int32_t n2, int32_t n3, const float x[3],
-                                           float v[3]) {
+                                           float v[4]) {
     float fx = (x[1] - t.x2_lo) * t.inv_h2, fy = (x[2] + 1.0f) * t.inv_h3;
     fx = fminf(fmaxf(fx, 0.0f), (float)(n2 - 1) - 1e-3f);
     fy = fminf(fmaxf(fy, 0.0f), (float)(n3 - 1) - 1e-3f);
@@ -721,7 +741,7 @@ __device__ __forceinline__ void tab_lookup(const float* __restrict__ node, const
     const float w00 = (1.0f - a) * (1.0f - b), w01 = a * (1.0f - b), w10 = (1.0f - a) * b,
                 w11 = a * b;
 #pragma unroll
-    for (int o = 0; o < 3; ++o)
+    for (int o = 0; o < 4; ++o)
         v[o] = w00 * q00[o] + w01 * q01[o] + w10 * q10[o] + w11 * q11[o];
 }
 
@@ -761,7 +781,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
         bool back = false;
         if (r.route == kNet) {
             const uint32_t ts = slot[ci];
-            float v[3];
+            float v[4];
             tab_lookup(node + ts * nodes, tab_geom(range + 2 * ts, n2, n3), n2, n3, r.x, v);
             const float Phi = phi_lookup(phi + (int64_t)ts * kPhiNodes, range + 2 * ts, r.x[1]) +
                               (r.g.bm > 0.0f ? kResScale * v[0] : 0.0f);

@@ -29,6 +29,9 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=None, help="model file to write")
     ap.add_argument("--chunk", type=int, default=1000, help="steps per progress line")
+    ap.add_argument("--revalidate", default=None, metavar="MODEL",
+                    help="no training: load MODEL, rebuild its error map and margins by the "
+                         "validation pass (after a change of the decode), save to --out")
     a = ap.parse_args(argv)
 
     from . import Tracer, march_params
@@ -36,6 +39,14 @@ def main(argv=None):
 
     t0 = time.time()
     tr = Tracer(0)
+    if a.revalidate:
+        sur = Surrogate.load(tr, a.revalidate)
+        v = sur.validate(a.validate, seed=a.seed + 1000, tol_phi=a.tol)
+        print("validation", json.dumps(v), flush=True)
+        if a.out:
+            sur.save(a.out)
+            print(f"saved {a.out}", flush=True)
+        return v
     params = march_params(max_steps=a.max_steps, disk_inner=a.disk[0], disk_outer=a.disk[1])
     sur = Surrogate(tr, params, seed=a.seed)
     sur.make_dataset(a.rays, seed=a.seed)
