@@ -224,8 +224,8 @@ __device__ __forceinline__ float redshift_fast(float r, float b_axis, float laps
 }
 
 template <typename UAt>
-__device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& k,
-                                           const Setup<float>& s, const RayGeom& g, float Phi,
+__device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& k, float b_axis,
+                                           float lapse_g, const RayGeom& g, float Phi,
                                            UAt u_at) {
     Dec o;
     o.serve = false;
@@ -273,7 +273,7 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
         if (!in[q]) continue;
         const float r = o.r_eq[q], sr = q ? -r : r;
         const float ph = atan2_ieee(sr * w.z, sr * w.x);
-        const float gg = redshift_fast(r, s.b_axis, s.lapse_g);
+        const float gg = redshift_fast(r, b_axis, lapse_g);
         if (o.n_cross == 0) {
             o.dr[0] = r;
             o.dphi[0] = ph;
@@ -303,12 +303,12 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
 
 // Decode from the exact orbit of the ray and the network outputs y (per-ray inference,
 // validation).
-__device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, const Setup<float>& s,
-                                      const RayGeom& g, const float y[5]) {
+__device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, float b_axis,
+                                      float lapse_g, const RayGeom& g, const float y[5]) {
     const Orbit ob = orbit(g);
     // the exact sweep to infinity with the learned residual, or to the horizon test radius
     const float Phi = ob.phi_end + (g.bm > 0.0f ? kResScale * y[0] : 0.0f);
-    return decode_with(c, k, s, g, Phi, [&](int q) {
+    return decode_with(c, k, b_axis, lapse_g, g, Phi, [&](int q) {
         const float u = orbit_u(ob, g.psi + (float)q * kPi);
         return q < 2 ? u * __expf(kURes * y[1 + q]) : u;
     });
@@ -426,6 +426,7 @@ struct SurCamTable {
 struct WaveItem {
     uint32_t ci;
     int64_t pix;   // -1: padding or past the end
+    uint32_t col, row;
 };
 template <typename Cams>
 __device__ __forceinline__ WaveItem wave_item(const Cams& cams, uint32_t q_wave, uint32_t n_items) {
@@ -436,9 +437,10 @@ __device__ __forceinline__ WaveItem wave_item(const Cams& cams, uint32_t q_wave,
     const MarchConst& c = cams.cam(w.ci);
     const uint32_t t = (qw - cams.base(w.ci)) >> 6;
     const uint32_t tiles_x = ((uint32_t)c.width + 7u) >> 3;
-    const uint32_t col = (t % tiles_x) * 8u + (lane & 7u), row = (t / tiles_x) * 8u + (lane >> 3);
-    w.pix = (qw + lane < n_items && col < (uint32_t)c.width && row < (uint32_t)c.height)
-                ? (int64_t)row * c.width + col
+    w.col = (t % tiles_x) * 8u + (lane & 7u);
+    w.row = (t / tiles_x) * 8u + (lane >> 3);
+    w.pix = (qw + lane < n_items && w.col < (uint32_t)c.width && w.row < (uint32_t)c.height)
+                ? (int64_t)w.row * c.width + w.col
                 : -1;
     return w;
 }
@@ -457,27 +459,93 @@ constexpr uint32_t kListedBand = 1u;
 constexpr uint32_t kListedEarly = 2u;
 
 struct RayPrep {
-    Setup<float> s;
     RayGeom g;
+    float b_axis, lapse_g;   // impact parameter about the spin axis, lapse (redshift)
     float x[3];
     Route route;
     bool band;   // in the integrator's FP64 band (BAND64 precision)
 };
 
+// Geometry of the camera ray of pixel (col, row) for the surrogate: the view direction as
+// ray_dir (blackhole.frag:68-72) and the orbit-plane frame, impact parameter and redshift inputs
+// of lawS_setup (geodesic.glsl:27-40), from hardware reciprocal / square-root approximations
+// (relative error ~1e-7). The tangential part of the initial coordinate velocity is the view
+// direction's, so the frame needs no velocity; |e1 x d| = |d - (d.e1) e1| is sin(alpha).
+__device__ __forceinline__ void sur_geom(const MarchConst& c, int64_t pix, uint32_t col,
+                                         uint32_t row, RayPrep& r) {
+    float px, py;
+    if (c.jitter_hashed) {
+        const uint32_t pi2 = 2u * (uint32_t)pix;
+        px = (float)col + (float)(pcg(c.jitter_base + pi2) >> 8) * 0x1p-24f;
+        py = (float)row + (float)(pcg(c.jitter_base + pi2 + 1u) >> 8) * 0x1p-24f;
+    } else {
+        px = ((float)col + 0.5f) + c.jitter[0];
+        py = ((float)row + 0.5f) + c.jitter[1];
+    }
+    const float wf = (float)c.width, hf = (float)c.height;
+    const float ndx = __builtin_fmaf(px * __builtin_amdgcn_rcpf(wf), 2.0f, -1.0f);
+    const float ndy = __builtin_fmaf(py * __builtin_amdgcn_rcpf(hf), 2.0f, -1.0f);
+    const V3<float> d0{(ndx * c.tan_half_fov) * (wf * __builtin_amdgcn_rcpf(hf)),
+                       ndy * c.tan_half_fov, -1.0f};
+    const float* B = c.basis;
+    V3<float> d{__builtin_fmaf(B[6], d0.z, __builtin_fmaf(B[3], d0.y, B[0] * d0.x)),
+                __builtin_fmaf(B[7], d0.z, __builtin_fmaf(B[4], d0.y, B[1] * d0.x)),
+                __builtin_fmaf(B[8], d0.z, __builtin_fmaf(B[5], d0.y, B[2] * d0.x))};
+    const float id = __builtin_amdgcn_rsqf(dot3(d, d));
+    d.x *= id; d.y *= id; d.z *= id;
+    RayGeom& g = r.g;
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    g.r = __builtin_amdgcn_sqrtf(dot3(P, P));
+    const float ir = __builtin_amdgcn_rcpf(g.r);
+    g.e1 = V3<float>{P.x * ir, P.y * ir, P.z * ir};
+    const float nr = dot3(d, g.e1);
+    const V3<float> t{d.x - nr * g.e1.x, d.y - nr * g.e1.y, d.z - nr * g.e1.z};
+    const float tn = __builtin_amdgcn_sqrtf(dot3(t, t));
+    g.planar = !(tn > 1e-6f) || fabsf(g.e1.y) < 1e-6f;
+    const float it = tn > 1e-6f ? __builtin_amdgcn_rcpf(tn) : 0.0f;
+    g.e2 = V3<float>{t.x * it, t.y * it, t.z * it};
+    g.ingoing = nr < 0.0f;
+    const float lapse2 = 1.0f - 2.0f * ir;
+    const float f = __builtin_amdgcn_sqrtf(fmaxf(lapse2, 1e-12f));
+    g.sin_a = tn;
+    g.b = g.r * tn * __builtin_amdgcn_rcpf(f);
+    g.bm = g.b * (1.0f / kBcritF) - 1.0f;
+    float psi = atan2f(-g.e1.y, g.e2.y);
+    if (psi < 0.0f) psi += kPi;
+    if (psi >= kPi) psi -= kPi;
+    g.psi = psi;
+    r.b_axis = -(P.z * d.x - P.x * d.z) * __builtin_amdgcn_rsqf(fmaxf(lapse2, 1e-6f));
+    r.lapse_g = __builtin_amdgcn_sqrtf(fmaxf(lapse2, 1e-4f));
+}
+
+// Rays with |beta - 1| below this get the integrator's exact setup: there the swept angle moves by
+// ~|db / b| / |beta - 1| rad, and the approximate setup's few 1e-7 would approach 1e-4 rad.
+constexpr float kExactBeta = 0.05f;
+
 // Classification of a ray; identical in the pre-pass and the network pass (same operations). A
 // ray of the FP64 band that the model serves is a network candidate; the other band rays are
-// marched in FP64.
-__device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst& k, int64_t pix) {
+// marched in FP64. Band membership must agree with the integrator's exact setup: near-critical
+// rays (kExactBeta) and rays within 1e-5 b_crit of the band edge are classified by it.
+__device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst& k,
+                                            const WaveItem& w) {
+    const int64_t pix = w.pix;
     RayPrep r;
     r.x[0] = r.x[1] = r.x[2] = 0.0f;
     r.route = kNone;
     r.band = false;
     if (pix < 0) return r;
-    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-    const V3<float> d = ray_dir(c, pix);
-    r.s = lawS_setup(P, d);
-    r.band = c.precision == BHS_PREC_F32_BAND64 && __builtin_fabsf(r.s.b - kBcritF) < c.band_abs;
-    r.g = ray_geom(P, d, r.s);
+    sur_geom(c, pix, w.col, w.row, r);
+    const bool band64 = c.precision == BHS_PREC_F32_BAND64;
+    if (fabsf(r.g.bm) < kExactBeta ||
+        (band64 && fabsf(fabsf(r.g.b - kBcritF) - c.band_abs) < 1e-5f * kBcritF)) {
+        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+        const V3<float> d = ray_dir(c, pix);
+        const Setup<float> s = lawS_setup(P, d);
+        r.g = ray_geom(P, d, s);
+        r.b_axis = s.b_axis;
+        r.lapse_g = s.lapse_g;
+    }
+    r.band = band64 && __builtin_fabsf(r.g.b - kBcritF) < c.band_abs;
     r.route = r.band ? kBand : kEarly;
     if (in_domain(r.g)) {
         features(r.g, r.x);
@@ -521,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_sur_pre(const Cams cams, const SurCo
         const uint32_t q = q0 + threadIdx.x;
         const WaveItem w = wave_item(cams, q0 + (threadIdx.x & ~63u), n_items);
         const MarchConst& c = cams.cam(w.ci);
-        const RayPrep r = prep_ray(c, k, w.pix);
+        const RayPrep r = prep_ray(c, k, w);
         push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
         push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q, s_cnt);
     }
@@ -549,15 +617,15 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
         const WaveItem w = wave_item(cams, ch * kBlock + (threadIdx.x & ~63u), n_items);
         const MarchConst& c = cams.cam(w.ci);
         const int64_t pix = w.pix;
-        const RayPrep r = prep_ray(c, k, pix);
+        const RayPrep r = prep_ray(c, k, w);
         const bool cand = r.route == kNet;
         float y[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (__any(cand)) mlp_wave(s_w, r.x, s_y[wave], y);
         bool back = false;
         if (cand) {
-            const Dec o = decode(c, k, r.s, r.g, y);
+            const Dec o = decode(c, k, r.b_axis, r.lapse_g, r.g, y);
             if (o.serve) {
-                store_dec(out, c.ray_offset + pix, o, r.s.b);
+                store_dec(out, c.ray_offset + pix, o, r.g.b);
                 n_served += 1;
                 n_cross += o.n_cross;
                 n_esc += o.escaping ? 1u : 0u;
@@ -777,7 +845,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
         const uint32_t ci = w.ci;
         const MarchConst& c = cams.cam(ci);
         const int64_t pix = w.pix;
-        const RayPrep r = prep_ray(c, k, pix);
+        const RayPrep r = prep_ray(c, k, w);
         bool back = false;
         if (r.route == kNet) {
             const uint32_t ts = slot[ci];
@@ -785,9 +853,10 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
             tab_lookup(node + ts * nodes, tab_geom(range + 2 * ts, n2, n3), n2, n3, r.x, v);
             const float Phi = phi_lookup(phi + (int64_t)ts * kPhiNodes, range + 2 * ts, r.x[1]) +
                               (r.g.bm > 0.0f ? kResScale * v[0] : 0.0f);
-            const Dec o = decode_with(c, k, r.s, r.g, Phi, [&](int q) { return v[1 + q]; });
+            const Dec o = decode_with(c, k, r.b_axis, r.lapse_g, r.g, Phi,
+                                      [&](int q) { return v[1 + q]; });
             if (o.serve) {
-                store_dec(out, c.ray_offset + pix, o, r.s.b);
+                store_dec(out, c.ray_offset + pix, o, r.g.b);
                 n_served += 1;
                 n_cross += o.n_cross;
                 n_esc += o.escaping ? 1u : 0u;
@@ -873,7 +942,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_validate(int64_t n, const float* 
         float ep = -1.0f, er = -1.0f;
         int32_t ci = -1;
         if (elig) {
-            const Dec o = decode(c, k, s, g, y);
+            const Dec o = decode(c, k, s.b_axis, s.lapse_g, g, y);
             if (o.serve) {
                 const uint32_t fl = ref.flags[i], nc = ref.n_cross[i];
                 const bool ref_esc = (fl & BHS_FLAG_ESCAPED) != 0;
