@@ -263,16 +263,21 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
         in3 = r >= c.disk_inner && r <= c.disk_outer;
         if (in3 && !(in[0] && in[1])) return o;
     }
-    float sp, cp;
-    sincosf(g.psi, &sp, &cp);
-    const V3<float> w{cp * g.e1.x + sp * g.e2.x, cp * g.e1.y + sp * g.e2.y,
-                      cp * g.e1.z + sp * g.e2.z};
     o.dr[0] = o.dr[1] = o.dphi[0] = o.dphi[1] = o.dg[0] = o.dg[1] = 0.0f;
+    // Azimuth of the first crossing: the direction cos(psi) e1 + sin(psi) e2 with zero y is
+    // +-(e2.y e1 - e1.y e2) (psi in [0, pi): the sign that makes sin(psi) >= 0); the second crossing
+    // lies opposite.
+    float ph0 = 0.0f;
+    if (in[0] || in[1]) {
+        const float sg = g.e1.y > 0.0f ? -1.0f : 1.0f;
+        ph0 = atan2_ieee(sg * (g.e2.y * g.e1.z - g.e1.y * g.e2.z),
+                         sg * (g.e2.y * g.e1.x - g.e1.y * g.e2.x));
+    }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         if (!in[q]) continue;
-        const float r = o.r_eq[q], sr = q ? -r : r;
-        const float ph = atan2_ieee(sr * w.z, sr * w.x);
+        const float r = o.r_eq[q];
+        const float ph = q == 0 ? ph0 : (ph0 > 0.0f ? ph0 - kPi : ph0 + kPi);
         const float gg = redshift_fast(r, b_axis, lapse_g);
         if (o.n_cross == 0) {
             o.dr[0] = r;
@@ -287,8 +292,9 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
     }
     if (in3) ++o.n_cross;
     if (o.escaping) {
+        // hardware sine / cosine (absolute error ~1e-6 over the swept angles' range)
         float sf, cf;
-        sincosf(Phi, &sf, &cf);
+        __sincosf(Phi, &sf, &cf);
         o.esc[0] = cf * g.e1.x + sf * g.e2.x;
         o.esc[1] = cf * g.e1.y + sf * g.e2.y;
         o.esc[2] = cf * g.e1.z + sf * g.e2.z;
