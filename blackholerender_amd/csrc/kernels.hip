@@ -338,9 +338,9 @@ void k_band64(const Cams cams, const SoA out, Scratch sc) {
     add_stats(sc.stats, one, steps, cross, esc, one);
 }
 
-// FP64 march of a band list longer than k_band64's lanes (`lanes`; large batches): every position,
-// by resident waves pulling rays from it (queue counter sc.band_work); finished lanes are refilled
-// as in k_persist, so a wave is not held by its slowest ray. ROOTS: the FP64
+// FP64 march of a band list longer than `lanes` (k_band64's lanes, or 0 when this kernel runs
+// alone): every position, by resident waves pulling rays from it (queue counter sc.band_work);
+// finished lanes are refilled as in k_persist, so a wave is not held by its slowest ray. ROOTS: the FP64
 // root refinement of march_f64 for cameras with root64. Exit: the list is finite, every ray ends
 // within max_steps iterations and the counter only grows, so every wave leaves.
 template <typename Cams, bool ROOTS>
@@ -749,17 +749,25 @@ hipError_t launch_persist(const Tuning& t, int num_cus, hipStream_t s, const Mar
     return hipGetLastError();
 }
 
-// FP64 march of the band list on `aux`: k_band64 if the list fits its lanes, else k_band64_queue
-// (the other exits at once: the list length is known on the device only), its ROOTS variant
-// (crossing states in registers) only for launches with FP64 root refinement.
+// Launches of more work items than this march their band list with k_band64_queue alone.
+constexpr uint32_t kBandQueueItems = 1u << 24;
+
+// FP64 march of the band list on `aux`. Up to kBandQueueItems work items: k_band64 (its list
+// usually fits its lanes), then k_band64_queue, which exits at once unless the list is longer (the
+// length is known on the device only). Larger launches: k_band64_queue alone (a k_band64 that
+// finds the list too long would hold the stream until the other kernels free room for its blocks).
+// The ROOTS variant (crossing states in registers) only for launches with FP64 root refinement.
 template <typename Cams>
 hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hipStream_t aux) {
     const dim3 g = band_grid(sc.band_cap);
-    hipLaunchKernelGGL(k_band64<Cams>, g, dim3(kBandBlock), 0, aux, cams, out, sc);
     hipError_t e;
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    const uint32_t lanes = g.x * kBandBlock;
-    if (sc.band_cap <= lanes) return hipSuccess;
+    uint32_t lanes = 0;
+    if (sc.band_cap <= kBandQueueItems) {
+        hipLaunchKernelGGL(k_band64<Cams>, g, dim3(kBandBlock), 0, aux, cams, out, sc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        lanes = g.x * kBandBlock;
+        if (sc.band_cap <= lanes) return hipSuccess;
+    }
     bool roots;
     if constexpr (std::is_same<Cams, OneCam>::value)
         roots = cams.c.root64 != 0;

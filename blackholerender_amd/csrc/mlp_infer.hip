@@ -1047,18 +1047,17 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     const bool f64 = kOne && c0.precision == BHS_PREC_F64;
     // Latency mode (up to kRoutedItems work items): a pre-pass routes the rays first, so the side
     // marches, whose serial rays are the longest chains of a frame, run beside the network pass;
-    // their lists are short, and two resident waves per SIMD (131k lanes) leave the rest of the
-    // chip to the network pass. Throughput mode (larger calls): the network pass routes the rays
-    // itself and the lists, millions of rays long, are marched after it at the tuned occupancy
-    // (the FP64 band beside the FP32 lists, whose tails it fills). Starting the band before the
-    // network pass gains nothing there: both are VALU-bound, and the pre-pass or band scan is
-    // extra work.
+    // their lists are short. Throughput mode (larger calls): the network pass routes the rays
+    // itself and the lists, millions of rays long, are marched after it (the FP64 band beside the
+    // FP32 lists). Starting the band before the network pass gains nothing there: both are
+    // VALU-bound, and the pre-pass or band scan is extra work. The FP32 lists run at two resident
+    // waves per SIMD (131k lanes), which leaves room for the network pass or the FP64 band.
     const bool routed = !f64 && n_items <= kRoutedItems;
     const uint32_t listed = routed ? (kListedBand | kListedEarly) : 0u;
     Tuning tl = t;
     tl.mode = 1;
     tl.block = 256;
-    if (routed) tl.waves_per_simd = 2;
+    tl.waves_per_simd = 2;
     auto march_list = [&](const uint32_t* ids, const uint32_t* count, const Scratch& scr,
                           hipStream_t st) {
         if constexpr (kOne)
@@ -1080,11 +1079,12 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
         if ((e2 = hipEventRecord(ss.ev_fork2, s)) != hipSuccess ||
             (e2 = hipStreamWaitEvent(ss.aux2, ss.ev_fork2, 0)) != hipSuccess)
             return e2;
+        // the band first: its waves (long FP64 chains) take their slots before the lists'
+        if (any_band && (e2 = band()) != hipSuccess) return e2;
         Scratch se = sc;
         se.work = fl.early_work;
         if ((e2 = march_list(fl.early_ids, fl.early_count, se, ss.aux2)) != hipSuccess) return e2;
         if ((e2 = hipEventRecord(ss.ev_join2, ss.aux2)) != hipSuccess) return e2;
-        if (any_band && (e2 = band()) != hipSuccess) return e2;
         return hipSuccess;
     };
     hipError_t e;
