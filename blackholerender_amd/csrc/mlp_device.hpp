@@ -294,6 +294,13 @@ __device__ __forceinline__ bool in_domain(const RayGeom& g) {
            fabsf(g.bm) > kBetaEps && g.bm >= kBetaMin - 1.0f;
 }
 
+// Plunging rays (beta < kBetaMin: captured almost radially): decided by the exact orbit alone,
+// without the network (served when they meet the equatorial plane at most once, well away from
+// the annulus).
+__device__ __forceinline__ bool plunging(const RayGeom& g) {
+    return g.ingoing && !g.planar && g.r >= 8.0f && g.r <= 1000.0f && g.bm < kBetaMin - 1.0f;
+}
+
 // SiLU and its derivative (approximate exp and reciprocal: the bf16 rounding of the hidden
 // activations dominates their error).
 __device__ __forceinline__ float sigmoid_fast(float z) {

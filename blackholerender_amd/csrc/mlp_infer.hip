@@ -320,6 +320,42 @@ __device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, fl
     });
 }
 
+// Relative margin around the annulus for the crossing of a plunging ray.
+constexpr float kPlungeMargin = 0.05f;
+
+// Record of a plunging ray from the exact orbit alone (no learned residual): captured; served as
+// such when it meets the equatorial plane before the horizon test radius at most once, at a radius
+// outside the annulus widened by kPlungeMargin (the crossing then records nothing).
+__device__ __forceinline__ Dec decode_plunge(const MarchConst& c, const RayGeom& g) {
+    Dec o;
+    o.serve = false;
+    o.escaping = false;
+    o.n_cross = 0;
+    o.r_eq[0] = o.r_eq[1] = 1e30f;
+    if (c.thin) {
+        const Orbit ob = orbit(g);
+        const float a = ob.phi_end - g.psi;
+        o.n_eq = a <= 0.0f ? 0u : (a <= kPi ? 1u : 2u);
+        if (o.n_eq >= 2u) return o;
+        if (o.n_eq == 1u) {
+            const float u = orbit_u(ob, g.psi);
+            const float r = u > 0.0f ? __builtin_amdgcn_rcpf(u) : 1e30f;
+            o.r_eq[0] = r;
+            if (r > c.disk_inner * (1.0f - kPlungeMargin) &&
+                r < c.disk_outer * (1.0f + kPlungeMargin))
+                return o;
+        }
+    } else {
+        o.n_eq = 0;
+    }
+    o.esc[0] = o.esc[1] = o.esc[2] = 0.0f;
+    o.dr[0] = o.dr[1] = o.dphi[0] = o.dphi[1] = o.dg[0] = o.dg[1] = 0.0f;
+    o.cls = BHS_CLASS_HORIZON;
+    o.flags = BHS_FLAG_SURROGATE;
+    o.serve = true;
+    return o;
+}
+
 __device__ __forceinline__ void store_dec(const SoA& out, int64_t i, const Dec& o, float b) {
     const int64_t n = out.n;
     if (out.cls) out.cls[i] = (uint8_t)o.cls;
@@ -457,6 +493,7 @@ enum Route : uint32_t {
     kBand = 1,    // integrator's FP64 band (BAND64 precision): FP64 march from the start
     kEarly = 2,   // outside the served domain / unserved cell / step budget: FP32 march at once
     kNet = 3,     // network candidate (the decode may still send it back)
+    kPlunge = 4,  // plunging ray: exact orbit alone (decode_plunge may send it back)
 };
 
 // Lists already built when the network pass runs (bit mask): the FP64 band (by the pre-pass or
@@ -556,6 +593,8 @@ __device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst&
     if (in_domain(r.g)) {
         features(r.g, r.x);
         if (cell_served(k, r.x) && steps_ok(c, k, r.g, r.x[1])) r.route = kNet;
+    } else if (plunging(r.g) && steps_ok(c, k, r.g, x2_of(r.g.bm))) {
+        r.route = kPlunge;
     }
     return r;
 }
@@ -628,8 +667,8 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
         float y[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (__any(cand)) mlp_wave(s_w, r.x, s_y[wave], y);
         bool back = false;
-        if (cand) {
-            const Dec o = decode(c, k, r.b_axis, r.lapse_g, r.g, y);
+        if (cand || r.route == kPlunge) {
+            const Dec o = cand ? decode(c, k, r.b_axis, r.lapse_g, r.g, y) : decode_plunge(c, r.g);
             if (o.serve) {
                 store_dec(out, c.ray_offset + pix, o, r.g.b);
                 n_served += 1;
@@ -866,6 +905,14 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
                 n_served += 1;
                 n_cross += o.n_cross;
                 n_esc += o.escaping ? 1u : 0u;
+            } else {
+                back = true;
+            }
+        } else if (r.route == kPlunge) {
+            const Dec o = decode_plunge(c, r.g);
+            if (o.serve) {
+                store_dec(out, c.ray_offset + pix, o, r.g.b);
+                n_served += 1;
             } else {
                 back = true;
             }

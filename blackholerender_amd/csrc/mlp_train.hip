@@ -17,10 +17,11 @@ __device__ __forceinline__ float urand(uint32_t key, uint32_t i, uint32_t k) {
 // ------------------------------------------------------------------------------------------------
 // Training rays: r_obs log-uniform in [8, 1000] M; beta = b/b_crit log-spaced on both sides of 1
 // (40% above: 1 + 10^U(log10 0.002, log10(beta_max - 1)); 20% below: 1 - 10^U(log10 0.002,
-// log10 0.7)) plus 40%
-// uniform in [0.3, beta_max]; b at most 0.95 of the tangential limit r/sqrt(1 - 2/r); the first
-// equatorial crossing at swept angle psi uniform in [0, pi); the orbit plane tilted by U(15, 89) deg
-// about a random line of nodes. The view direction is ingoing with impact parameter b.
+// log10 0.7)), 35% uniform in [0.3, beta_max] and 5% uniform in [0, 0.3) (plunging rays: outside
+// the network's domain, they only feed the step table); b at most 0.95 of the tangential limit
+// r/sqrt(1 - 2/r); the first equatorial crossing at swept angle psi uniform in [0, pi); the orbit
+// plane tilted by U(15, 89) deg about a random line of nodes. The view direction is ingoing with
+// impact parameter b.
 __global__ void k_sample_rays(int64_t n, uint32_t key, float* __restrict__ pos3,
                               float* __restrict__ dir3) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -37,8 +38,10 @@ __global__ void k_sample_rays(int64_t n, uint32_t key, float* __restrict__ pos3,
         beta = 1.0f + exp10f(-2.69897f + (tmax + 2.69897f) * urand(key, ii, 2));
     } else if (u < 0.6f) {
         beta = 1.0f - exp10f(-2.69897f + 2.5440680f * urand(key, ii, 2));   // 0.002 .. 0.7
-    } else {
+    } else if (u < 0.95f) {
         beta = 0.3f + (beta_max - 0.3f) * urand(key, ii, 2);
+    } else {
+        beta = 0.3f * urand(key, ii, 2);
     }
     float b = beta * kBcritF;
     if (b > bmax) b = bmax;
@@ -115,7 +118,8 @@ __global__ void k_targets(int64_t n, const float* __restrict__ pos3, const float
     M[5 * i + 2] = mu[1] * m;
     M[5 * i + 3] = !maxed ? m : 0.0f;    // out-of-steps rays: reach unknown
     M[5 * i + 4] = !maxed ? m : 0.0f;
-    if (dom && steps_max) atomicMax(&steps_max[step_bin(g.r, x[1])], (uint32_t)out.steps[i]);
+    if ((dom || plunging(g)) && steps_max)
+        atomicMax(&steps_max[step_bin(g.r, x2_of(g.bm))], (uint32_t)out.steps[i]);
 }
 
 // ------------------------------------------------------------------------------------------------

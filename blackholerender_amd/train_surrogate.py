@@ -32,6 +32,9 @@ def main(argv=None):
     ap.add_argument("--revalidate", default=None, metavar="MODEL",
                     help="no training: load MODEL, rebuild its error map and margins by the "
                          "validation pass (after a change of the decode), save to --out")
+    ap.add_argument("--restep", action="store_true",
+                    help="with --revalidate: also rebuild the step table from --rays fresh "
+                         "integrator rays (after a change of the sampled domain)")
     a = ap.parse_args(argv)
 
     from . import Tracer, march_params
@@ -41,6 +44,8 @@ def main(argv=None):
     tr = Tracer(0)
     if a.revalidate:
         sur = Surrogate.load(tr, a.revalidate)
+        if a.restep:
+            sur.make_dataset(a.rays, seed=a.seed)
         v = sur.validate(a.validate, seed=a.seed + 1000, tol_phi=a.tol)
         print("validation", json.dumps(v), flush=True)
         if a.out:
