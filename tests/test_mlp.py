@@ -218,6 +218,43 @@ def test_trained_model_frame_accuracy(tracer, table):
 
 
 @pytest.mark.gpu
+def test_near_hole_frame_plunging_and_band_rays(tracer):
+    """320x180 frame of the full-form camera (r_obs 15, i 80, FOV 60, 700 steps, disk [6, 30]; the
+    shadow fills much of the view): plunging rays (b < 0.3 b_crit, decided by the exact orbit
+    alone) and near-critical rays of the FP64 band that the model serves render the FP64 class and
+    crossing count, served escape directions stay within 1e-4 rad of FP64, every unserved ray is
+    bit-identical to the plain render."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd.surrogate import Surrogate
+    from helpers import bit_mismatches
+    sur = Surrogate.load(tracer, _model_path("lawS_700_disk6-30.bhsmlp"))
+    sur.set_table(1024, 256)
+    cam = bhs.orbit_camera(15.0, 80.0, 60.0, 320, 180, jitter_hashed=1, seed=21)
+    params = bhs.march_params(max_steps=700, disk_outer=30.0)
+    print("[mlp] near-hole frame", flush=True)
+    S = sur.render(cam, params).numpy()
+    torch.cuda.synchronize()
+    P = tracer.render(cam, params).numpy()
+    R = tracer.render(cam, bhs.march_params(max_steps=700, disk_outer=30.0,
+                                            precision=bhs.PREC_F64)).numpy()
+    sv = (S["flags"] & bhs.FLAG_SURROGATE) > 0
+    beta = P["b"] / (3.0 * np.sqrt(3.0))
+    plunge = beta < 0.3
+    near = np.abs(beta - 1.0) < 0.02
+    assert plunge.sum() > 300 and (sv & plunge).sum() > 0.9 * plunge.sum()
+    assert (sv & near).sum() > 0.5 * near.sum(), ((sv & near).sum(), near.sum())
+    assert np.array_equal(S["cls"][sv], R["cls"][sv])
+    assert np.array_equal(S["n_cross"][sv], R["n_cross"][sv])
+    assert np.array_equal(S["flags"][sv] & 2, R["flags"][sv] & 2)
+    esc = sv & ((S["flags"] & 2) > 0)
+    assert _angle(S["esc"][esc], R["esc"][esc]).max() <= 1e-4
+    mm = bit_mismatches({k: v[~sv] for k, v in S.items()}, {k: v[~sv] for k, v in P.items()})
+    assert all(v == 0 for v in mm.values()), mm
+    sur.close()
+
+
+@pytest.mark.gpu
 def test_batch_equals_single_calls(tracer):
     """bhs_render_surrogate_batch over four cameras (three sizes and poses, the fourth a jittered
     copy of the first's pose, which shares its table) equals the single-camera calls bit for bit
