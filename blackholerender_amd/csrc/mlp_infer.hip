@@ -467,8 +467,9 @@ struct SurCamTable {
 // position are wave-uniform (scalar) and only the pixel inside the tile is per lane.
 struct WaveItem {
     uint32_t ci;
-    int64_t pix;   // -1: padding or past the end
+    int64_t pix;     // -1: padding or past the end
     uint32_t col, row;
+    uint32_t item;   // the pixel's work item in the integrator's tile order (for the lists)
 };
 template <typename Cams>
 __device__ __forceinline__ WaveItem wave_item(const Cams& cams, uint32_t q_wave, uint32_t n_items) {
@@ -484,6 +485,36 @@ __device__ __forceinline__ WaveItem wave_item(const Cams& cams, uint32_t q_wave,
     w.pix = (qw + lane < n_items && w.col < (uint32_t)c.width && w.row < (uint32_t)c.height)
                 ? (int64_t)w.row * c.width + w.col
                 : -1;
+    w.item = qw + lane;
+    return w;
+}
+
+// As wave_item, with the camera's items taken in scanline order (item base + p is pixel p, the
+// items beyond width * height are padding): a wave's 64 rays are 64 consecutive pixels, so its
+// stores to the output planes are contiguous (an 8x8 tile writes eight 8-pixel row pieces).
+// `item` maps the pixel back to the integrator's tile order.
+template <typename Cams>
+__device__ __forceinline__ WaveItem wave_item_scan(const Cams& cams, uint32_t q_wave,
+                                                   uint32_t n_items) {
+    const uint32_t qw = __builtin_amdgcn_readfirstlane(q_wave);
+    const uint32_t lane = threadIdx.x & 63u;
+    WaveItem w;
+    w.ci = cams.find(qw < n_items ? qw : n_items - 1);
+    const MarchConst& c = cams.cam(w.ci);
+    const uint32_t base = cams.base(w.ci);
+    const uint32_t width = (uint32_t)c.width;
+    const uint32_t p0 = qw - base;                 // wave-uniform
+    const uint32_t row0 = p0 / width, col0 = p0 - row0 * width;
+    uint32_t col = col0 + lane, row = row0;
+    while (col >= width) {                         // at most 64 / width turns
+        col -= width;
+        ++row;
+    }
+    w.col = col;
+    w.row = row;
+    w.pix = (qw + lane < n_items && row < (uint32_t)c.height) ? (int64_t)row * width + col : -1;
+    const uint32_t tiles_x = (width + 7u) >> 3;
+    w.item = base + ((row >> 3) * tiles_x + (col >> 3)) * 64u + ((row & 7u) << 3) + (col & 7u);
     return w;
 }
 
@@ -658,8 +689,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
     const uint32_t chunks = (n_items + kBlock - 1) / kBlock;
     for (uint32_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
-        const uint32_t q = ch * kBlock + threadIdx.x;
-        const WaveItem w = wave_item(cams, ch * kBlock + (threadIdx.x & ~63u), n_items);
+        const WaveItem w = wave_item_scan(cams, ch * kBlock + (threadIdx.x & ~63u), n_items);
         const MarchConst& c = cams.cam(w.ci);
         const int64_t pix = w.pix;
         const RayPrep r = prep_ray(c, k, w);
@@ -678,12 +708,13 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
                 back = true;
             }
         }
-        push_back(c, sc, fl, back, r.band, (listed & kListedBand) != 0, q, s_cnt);
+        push_back(c, sc, fl, back, r.band, (listed & kListedBand) != 0, w.item, s_cnt);
         if (!(listed & kListedBand))
-            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
+            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, w.item,
+                       s_cnt);
         if (!(listed & kListedEarly))
-            push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q,
-                          s_cnt);
+            push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly,
+                          w.item, s_cnt);
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
@@ -885,8 +916,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
     const int64_t nodes = (int64_t)n2 * n3 * 4;
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
-        const uint32_t q = q0 + threadIdx.x;
-        const WaveItem w = wave_item(cams, q0 + (threadIdx.x & ~63u), n_items);
+        const WaveItem w = wave_item_scan(cams, q0 + (threadIdx.x & ~63u), n_items);
         const uint32_t ci = w.ci;
         const MarchConst& c = cams.cam(ci);
         const int64_t pix = w.pix;
@@ -917,12 +947,13 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
                 back = true;
             }
         }
-        push_back(c, sc, fl, back, r.band, (listed & kListedBand) != 0, q, s_cnt);
+        push_back(c, sc, fl, back, r.band, (listed & kListedBand) != 0, w.item, s_cnt);
         if (!(listed & kListedBand))
-            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
+            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, w.item,
+                       s_cnt);
         if (!(listed & kListedEarly))
-            push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q,
-                          s_cnt);
+            push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly,
+                          w.item, s_cnt);
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
