@@ -253,22 +253,22 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
         }
     }
     bool in3 = false;
+    float r3 = 1e30f;
     if (o.n_eq == 3u && c.thin) {
         const float u = u_at(2);
-        const float r = u > 0.0f ? __builtin_amdgcn_rcpf(u) : 1e30f;
+        r3 = u > 0.0f ? __builtin_amdgcn_rcpf(u) : 1e30f;
         const float m3 = fmaxf(margin, kThirdMargin);
-        if (fabsf(r - c.disk_inner) < m3 * c.disk_inner ||
-            fabsf(r - c.disk_outer) < m3 * c.disk_outer)
+        if (fabsf(r3 - c.disk_inner) < m3 * c.disk_inner ||
+            fabsf(r3 - c.disk_outer) < m3 * c.disk_outer)
             return o;
-        in3 = r >= c.disk_inner && r <= c.disk_outer;
-        if (in3 && !(in[0] && in[1])) return o;
+        in3 = r3 >= c.disk_inner && r3 <= c.disk_outer;
     }
     o.dr[0] = o.dr[1] = o.dphi[0] = o.dphi[1] = o.dg[0] = o.dg[1] = 0.0f;
     // Azimuth of the first crossing: the direction cos(psi) e1 + sin(psi) e2 with zero y is
     // +-(e2.y e1 - e1.y e2) (psi in [0, pi): the sign that makes sin(psi) >= 0); the second crossing
     // lies opposite.
     float ph0 = 0.0f;
-    if (in[0] || in[1]) {
+    if (in[0] || in[1] || in3) {
         const float sg = g.e1.y > 0.0f ? -1.0f : 1.0f;
         ph0 = atan2_ieee(sg * (g.e2.y * g.e1.z - g.e1.y * g.e2.z),
                          sg * (g.e2.y * g.e1.x - g.e1.y * g.e2.x));
@@ -290,7 +290,14 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
         }
         ++o.n_cross;
     }
-    if (in3) ++o.n_cross;
+    if (in3) {   // recorded if fewer than two crossings are (same azimuth as the first)
+        if (o.n_cross < 2u) {   // constant indices keep the record in registers
+            const float g3 = redshift_fast(r3, b_axis, lapse_g);
+            if (o.n_cross == 0u) { o.dr[0] = r3; o.dphi[0] = ph0; o.dg[0] = g3; }
+            else { o.dr[1] = r3; o.dphi[1] = ph0; o.dg[1] = g3; }
+        }
+        ++o.n_cross;
+    }
     if (o.escaping) {
         // hardware sine / cosine (absolute error ~1e-6 over the swept angles' range)
         float sf, cf;
