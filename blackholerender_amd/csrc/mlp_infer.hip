@@ -1116,6 +1116,17 @@ namespace {
 
 constexpr uint32_t kRoutedItems = 1u << 24;
 
+// Blocks of kBlock threads of a kernel that are resident together on one CU. Grid-stride kernels
+// launch that many per CU: a larger grid would leave its last blocks to run their whole share
+// afterwards, at a fraction of the occupancy.
+uint32_t blocks_per_cu(const void* kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)per_cu;
+}
+
 // The surrogate render of one camera (Cams = SurOneCam) or of a batch (SurCamTable, table and
 // item_base on the device).
 template <typename Cams>
@@ -1127,8 +1138,14 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                       hipEvent_t ev_end) {
     constexpr bool kOne = std::is_same<Cams, SurOneCam>::value;
     const uint32_t gb0 = (n_items + kBlock - 1) / kBlock;
-    const uint32_t cap = (uint32_t)(8 * (num_cus > 0 ? num_cus : 256));
-    const dim3 grid(gb0 < cap ? (gb0 ? gb0 : 1u) : cap);
+    static const uint32_t pre_per_cu =
+        blocks_per_cu(reinterpret_cast<const void*>(&k_sur_pre<Cams>));
+    static const uint32_t tab_per_cu =
+        blocks_per_cu(reinterpret_cast<const void*>(&k_surrogate_tab<Cams>));
+    auto grid_of = [&](uint32_t per_cu) {
+        const uint32_t cap = per_cu * (uint32_t)(num_cus > 0 ? num_cus : 256);
+        return dim3(gb0 < cap ? (gb0 ? gb0 : 1u) : cap);
+    };
     const bool f64 = kOne && c0.precision == BHS_PREC_F64;
     // Latency mode (up to kRoutedItems work items): a pre-pass routes the rays first, so the side
     // marches, whose serial rays are the longest chains of a frame, run beside the network pass;
@@ -1175,7 +1192,8 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     hipError_t e;
     if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
     if (routed) {
-        hipLaunchKernelGGL(k_sur_pre<Cams>, grid, dim3(kBlock), 0, s, cams, k, n_items, sc, fl);
+        hipLaunchKernelGGL(k_sur_pre<Cams>, grid_of(pre_per_cu), dim3(kBlock), 0, s, cams, k,
+                           n_items, sc, fl);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = side()) != hipSuccess) return e;
     }
@@ -1192,8 +1210,8 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                            dim3(kBlock), 0, s, cams, ncam, (const uint8_t*)packed,
                            (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
                            tab->n3, tab->node);
-        hipLaunchKernelGGL(k_surrogate_tab<Cams>, grid, dim3(kBlock), 0, s, cams, k,
-                           (const float*)tab->node, (const float*)tab->phi,
+        hipLaunchKernelGGL(k_surrogate_tab<Cams>, grid_of(tab_per_cu), dim3(kBlock), 0, s, cams,
+                           k, (const float*)tab->node, (const float*)tab->phi,
                            (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
                            tab->n3, out, sc, fl, n_items, listed);
     } else {
