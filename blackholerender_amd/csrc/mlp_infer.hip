@@ -390,27 +390,53 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
-// Appends the items of the lanes with `want` to a list with one atomic per block (wave ballots,
-// per-wave counts in LDS). Every thread of the block must call it; s_cnt: 5 words of LDS.
-__device__ __forceinline__ void push_block(uint32_t* count, uint32_t* ids, uint32_t cap, bool want,
+// Fallback lists of the surrogate kernels: a ray that is not served goes to at most one, the FP32
+// late list, the call's FP64 band list, the early FP32 list or the late band list.
+enum Dest : int { kToNone = -1, kToFb = 0, kToBand = 1, kToEarly = 2, kToLate = 3 };
+constexpr int kDests = 4;
+// LDS words of one push: per list, the four waves' counts and the block's base.
+constexpr int kPushWords = kDests * 4 + kDests;
+
+__device__ __forceinline__ uint32_t* list_count(const Scratch& sc, const SurLists& fl, int j) {
+    return j == kToFb ? fl.fb_count
+                      : (j == kToBand ? sc.band_count
+                                      : (j == kToEarly ? fl.early_count : fl.lb_count));
+}
+__device__ __forceinline__ uint32_t* list_ids(const Scratch& sc, const SurLists& fl, int j) {
+    return j == kToFb ? fl.fb_ids
+                      : (j == kToBand ? sc.band_ids : (j == kToEarly ? fl.early_ids : fl.lb_ids));
+}
+
+// Appends each lane's item to list `dest` (kToNone: none) with one atomic per list and block
+// (wave ballots, per-wave counts in LDS). Every thread of the block calls it, with a 256-thread
+// block. Successive calls alternate between the two halves of s_cnt (2 kPushWords words): a wave
+// writes a half again only after the barriers of the call in between, which every wave reaches
+// after its reads of that half, so two barriers per call suffice.
+__device__ __forceinline__ void push_lists(const Scratch& sc, const SurLists& fl, int dest,
                                            uint32_t item, uint32_t* s_cnt) {
-    const uint64_t m = __ballot(want);
     const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) s_cnt[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-        s_cnt[4] = tot ? atomicAdd(count, tot) : 0u;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kDests; ++j) {
+        const uint64_t m = __ballot(dest == j);
+        if ((threadIdx.x & 63) == 0) s_cnt[j * 4 + wave] = (uint32_t)__popcll(m);
+        if (dest == j) mine = m;
     }
     __syncthreads();
-    uint32_t base = s_cnt[4];
-    for (int w = 0; w < wave; ++w) base += s_cnt[w];
-    if (want) {
-        const uint32_t rank =
-            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        if (base + rank < cap) ids[base + rank] = item;
+    if (threadIdx.x < (unsigned)kDests) {
+        const int j = threadIdx.x;
+        const uint32_t tot = s_cnt[j * 4] + s_cnt[j * 4 + 1] + s_cnt[j * 4 + 2] + s_cnt[j * 4 + 3];
+        s_cnt[kDests * 4 + j] = tot ? atomicAdd(list_count(sc, fl, j), tot) : 0u;
     }
     __syncthreads();
+    if (dest >= 0) {
+        uint32_t base = s_cnt[kDests * 4 + dest];
+        for (int w = 0; w < wave; ++w) base += s_cnt[dest * 4 + w];
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(mine >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
+        const uint32_t cap = dest == kToBand ? sc.band_cap : fl.fb_cap;
+        if (base + rank < cap) list_ids(sc, fl, dest)[base + rank] = item;
+    }
 }
 
 // Adds the block's counters to the launch statistics with one atomic per counter and block.
@@ -637,44 +663,32 @@ __device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst&
     return r;
 }
 
-// Fallback list of a ray that the surrogate does not serve: FP64 precision sends everything to the
-// FP64 band list after the network pass.
-__device__ __forceinline__ void push_fallback(const MarchConst& c, const Scratch& sc,
-                                              uint32_t* count, uint32_t* ids, uint32_t cap,
-                                              bool want, uint32_t q, uint32_t* s_cnt) {
-    if (c.precision == BHS_PREC_F64)
-        push_block(sc.band_count, sc.band_ids, sc.band_cap, want, q, s_cnt);
-    else
-        push_block(count, ids, cap, want, q, s_cnt);
-}
-
-// A network candidate the decode sends back (back): a band ray to the FP64 band (the late band
-// list when the band march already runs, band_running, else the call's band list), any other to
-// the FP32 late list; FP64 precision sends everything to the band list. Every thread of the block
-// calls it.
-__device__ __forceinline__ void push_back(const MarchConst& c, const Scratch& sc,
-                                          const SurLists& fl, bool back, bool band,
-                                          bool band_running, uint32_t q, uint32_t* s_cnt) {
-    const bool to_band = back && (band || c.precision == BHS_PREC_F64);
-    push_block(fl.fb_count, fl.fb_ids, fl.fb_cap, back && !to_band, q, s_cnt);
-    if (band_running)
-        push_block(fl.lb_count, fl.lb_ids, fl.fb_cap, to_band, q, s_cnt);
-    else
-        push_block(sc.band_count, sc.band_ids, sc.band_cap, to_band, q, s_cnt);
+// List of a ray after the surrogate pass (listed: the lists already built). A network candidate
+// the decode sends back (back) goes to the FP64 band if it is a band ray (the late band list when
+// the band march already runs, else the call's band list), else to the FP32 late list; a band ray
+// to the band list and an early fallback ray to the early list, unless those are already built.
+// FP64 precision sends every unserved ray to the band list.
+__device__ __forceinline__ int dest_of(const MarchConst& c, const RayPrep& r, bool back,
+                                       uint32_t listed) {
+    const bool f64 = c.precision == BHS_PREC_F64;
+    if (back) return (r.band || f64) ? ((listed & kListedBand) ? kToLate : kToBand) : kToFb;
+    if (r.route == kBand) return (listed & kListedBand) ? kToNone : kToBand;
+    if (r.route == kEarly) return (listed & kListedEarly) ? kToNone : (f64 ? kToBand : kToEarly);
+    return kToNone;
 }
 
 // Pre-pass: lists the FP64 band rays and the early fallback rays.
 template <typename Cams>
 __global__ __launch_bounds__(kBlock) void k_sur_pre(const Cams cams, const SurConst k,
                                                     uint32_t n_items, Scratch sc, SurLists fl) {
-    __shared__ uint32_t s_cnt[5];
+    __shared__ uint32_t s_cnt[2 * kPushWords];
+    uint32_t half = 0;
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
-        const uint32_t q = q0 + threadIdx.x;
         const WaveItem w = wave_item(cams, q0 + (threadIdx.x & ~63u), n_items);
         const MarchConst& c = cams.cam(w.ci);
         const RayPrep r = prep_ray(c, k, w);
-        push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, q, s_cnt);
-        push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly, q, s_cnt);
+        push_lists(sc, fl, dest_of(c, r, false, 0u), w.item, s_cnt + half);
+        half ^= kPushWords;
     }
 }
 
@@ -688,7 +702,8 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
                                                          uint32_t n_items, uint32_t listed) {
     __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
     __shared__ float s_y[4][64 * kOut];
-    __shared__ uint32_t s_cnt[5];
+    __shared__ uint32_t s_cnt[2 * kPushWords];
+    uint32_t half = 0;
     __shared__ uint32_t s_st[4][4];
     stage_weights(packed, s_w);
     __syncthreads();
@@ -715,13 +730,8 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
                 back = true;
             }
         }
-        push_back(c, sc, fl, back, r.band, (listed & kListedBand) != 0, w.item, s_cnt);
-        if (!(listed & kListedBand))
-            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, w.item,
-                       s_cnt);
-        if (!(listed & kListedEarly))
-            push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly,
-                          w.item, s_cnt);
+        push_lists(sc, fl, dest_of(c, r, back, listed), w.item, s_cnt + half);
+        half ^= kPushWords;
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
@@ -918,7 +928,8 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
                                                           int32_t n2, int32_t n3, const SoA out,
                                                           Scratch sc, SurLists fl,
                                                           uint32_t n_items, uint32_t listed) {
-    __shared__ uint32_t s_cnt[5];
+    __shared__ uint32_t s_cnt[2 * kPushWords];
+    uint32_t half = 0;
     __shared__ uint32_t s_st[4][4];
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
     const int64_t nodes = (int64_t)n2 * n3 * 4;
@@ -954,13 +965,8 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
                 back = true;
             }
         }
-        push_back(c, sc, fl, back, r.band, (listed & kListedBand) != 0, w.item, s_cnt);
-        if (!(listed & kListedBand))
-            push_block(sc.band_count, sc.band_ids, sc.band_cap, r.route == kBand, w.item,
-                       s_cnt);
-        if (!(listed & kListedEarly))
-            push_fallback(c, sc, fl.early_count, fl.early_ids, fl.fb_cap, r.route == kEarly,
-                          w.item, s_cnt);
+        push_lists(sc, fl, dest_of(c, r, back, listed), w.item, s_cnt + half);
+        half ^= kPushWords;
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
