@@ -21,7 +21,7 @@ constexpr int kStepBins = 16, kStepBinsB = 16;
 
 // Log r_obs bin of the radius margins and the step table.
 __device__ __forceinline__ int r_bin(float r) {
-    const int bin = (int)((logf(r) - kLogRLo) * ((float)kStepBins / (kLogRHi - kLogRLo)));
+    const int bin = (int)((log_r(r) - kLogRLo) * ((float)kStepBins / (kLogRHi - kLogRLo)));
     return bin < 0 ? 0 : (bin >= kStepBins ? kStepBins - 1 : bin);
 }
 

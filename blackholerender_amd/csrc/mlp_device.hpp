@@ -191,6 +191,31 @@ struct RayGeom {
     bool planar;          // degenerate orbit plane (radial ray) or camera on the equator
 };
 
+// atan2 with hardware reciprocals (the polynomial of atan2_ieee; error ~3e-7 rad): the angles of
+// the surrogate's features and outputs need ~1e-6, not the integrator's exact rounding.
+__device__ __forceinline__ float atan2_fast(float y, float x) {
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
+    const float mx = ax > ay ? ax : ay;
+    const float mn = ax > ay ? ay : ax;
+    if (mx == 0.0f) return 0.0f;
+    float t = mn * __builtin_amdgcn_rcpf(mx);
+    float base = 0.0f;
+    if (t > 0.41421356f) {
+        t = (t - 1.0f) * __builtin_amdgcn_rcpf(t + 1.0f);
+        base = 0.78539816f;
+    }
+    const float s = t * t;
+    float p = -0.065067887f;
+    p = __builtin_fmaf(p, s, 0.10763717f);
+    p = __builtin_fmaf(p, s, -0.14266309f);
+    p = __builtin_fmaf(p, s, 0.19999637f);
+    p = __builtin_fmaf(p, s, -0.33333331f);
+    float a = base + __builtin_fmaf(t * s, p, t);
+    if (ay > ax) a = 1.5707964f - a;
+    if (x < 0.0f) a = 3.1415927f - a;
+    return y < 0.0f ? -a : a;
+}
+
 // (Hardware square-root and reciprocal approximations: the frame and the view angle feed the
 // network and the decode, which need ~1e-6 relative, not the integrator's exact rounding.)
 __device__ __forceinline__ RayGeom ray_geom(const V3<float>& P, const V3<float>& d,
@@ -213,7 +238,7 @@ __device__ __forceinline__ RayGeom ray_geom(const V3<float>& P, const V3<float>&
     g.bm = s.b * (1.0f / kBcritF) - 1.0f;
     const float f = __builtin_amdgcn_sqrtf(fmaxf(1.0f - 2.0f * ir, 1e-12f));
     g.sin_a = s.b * f * ir;
-    float psi = atan2f(-g.e1.y, g.e2.y);
+    float psi = atan2_fast(-g.e1.y, g.e2.y);
     if (psi < 0.0f) psi += kPi;
     if (psi >= kPi) psi -= kPi;
     g.psi = psi;
@@ -270,9 +295,16 @@ __device__ __forceinline__ float orbit_u(const Orbit& o, float phi) {
 }
 
 
+// Natural log of r_obs from the hardware log2 (r_obs in [1, 1e4]: absolute error ~1e-7).
+__device__ __forceinline__ float log_r(float r) {
+    return __builtin_amdgcn_logf(r) * 0.69314718f;
+}
+
 // Input feature of beta - 1 (scale kX2Scale, normalised by kX2Norm).
 __device__ __forceinline__ float x2_of(float bm) {
-    const float m = log1pf(fabsf(bm) * (1.0f / kX2Scale)) * (1.0f / kX2Norm);
+    // hardware log2: 1 + |beta - 1| / kX2Scale >= 1 (absolute error ~1e-7 in x2)
+    const float m = __builtin_amdgcn_logf(1.0f + fabsf(bm) * (1.0f / kX2Scale)) *
+                    (0.69314718f / kX2Norm);
     return bm < 0.0f ? -m : m;
 }
 
@@ -283,7 +315,7 @@ __device__ __forceinline__ float bm_of_x2(float x2) {
 }
 
 __device__ __forceinline__ void features(const RayGeom& g, float x[3]) {
-    x[0] = (logf(g.r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
+    x[0] = (log_r(g.r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
     x[1] = x2_of(g.bm);
     x[2] = g.psi * (2.0f / kPi) - 1.0f;
 }

@@ -270,7 +270,7 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
     float ph0 = 0.0f;
     if (in[0] || in[1] || in3) {
         const float sg = g.e1.y > 0.0f ? -1.0f : 1.0f;
-        ph0 = atan2_ieee(sg * (g.e2.y * g.e1.z - g.e1.y * g.e2.z),
+        ph0 = atan2_fast(sg * (g.e2.y * g.e1.z - g.e1.y * g.e2.z),
                          sg * (g.e2.y * g.e1.x - g.e1.y * g.e2.x));
     }
 #pragma unroll
@@ -617,7 +617,7 @@ __device__ __forceinline__ void sur_geom(const MarchConst& c, int64_t pix, uint3
     g.sin_a = tn;
     g.b = g.r * tn * __builtin_amdgcn_rcpf(f);
     g.bm = g.b * (1.0f / kBcritF) - 1.0f;
-    float psi = atan2f(-g.e1.y, g.e2.y);
+    float psi = atan2_fast(-g.e1.y, g.e2.y);
     if (psi < 0.0f) psi += kPi;
     if (psi >= kPi) psi -= kPi;
     g.psi = psi;
@@ -851,7 +851,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_table(const Cams cams, uint32_t n
         const int64_t q = (p % per) * kBlock + threadIdx.x;
         const TabGeom t = tab_geom(range + 2 * k, n2, n3);
         const float r = cam_radius(cams.cam(k));
-        const float x1 = (logf(r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
+        const float x1 = (log_r(r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
         const int32_t i = (int32_t)(q % n2), j = (int32_t)(q / n2);
         const float x[3] = {x1, __builtin_fmaf((float)i, t.h2, t.x2_lo),
                             __builtin_fmaf((float)j, t.h3, -1.0f)};
