@@ -479,20 +479,35 @@ struct SurOneCam {
     __device__ __forceinline__ const MarchConst& cam(uint32_t) const { return c; }
     __device__ __forceinline__ uint32_t base(uint32_t) const { return 0u; }
 };
+// The camera table and the prefix sums are read-only during a launch and the surrogate kernels
+// index them with wave-uniform values: read through the constant address space, those loads
+// become scalar loads and the camera's fields live in scalar registers.
 struct SurCamTable {
     const MarchConst* cams;
     const uint32_t* item_base;
     uint32_t ncam;
+    using ConstU32 = const __attribute__((address_space(4))) uint32_t;
     __device__ __forceinline__ uint32_t find(uint32_t q) const {
+        ConstU32* ib = (ConstU32*)item_base;
         uint32_t lo = 0, hi = ncam;
         while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;
-            if (item_base[mid] <= q) lo = mid; else hi = mid;
+            if (ib[mid] <= q) lo = mid; else hi = mid;
         }
         return lo;
     }
-    __device__ __forceinline__ const MarchConst& cam(uint32_t k) const { return cams[k]; }
-    __device__ __forceinline__ uint32_t base(uint32_t k) const { return item_base[k]; }
+    __device__ __forceinline__ MarchConst cam(uint32_t k) const {
+        static_assert(sizeof(MarchConst) % 4 == 0, "dword copy");
+        MarchConst c;
+        ConstU32* src = (ConstU32*)(cams + k);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&c);
+#pragma unroll
+        for (uint32_t i = 0; i < sizeof(MarchConst) / 4; ++i) dst[i] = src[i];
+        return c;
+    }
+    __device__ __forceinline__ uint32_t base(uint32_t k) const {
+        return ((ConstU32*)item_base)[k];
+    }
 };
 
 // Camera and pixel of the calling lane's item in the wave whose first item is q_wave: a wave's 64
