@@ -365,6 +365,24 @@ __device__ __forceinline__ Dec decode_plunge(const MarchConst& c, const RayGeom&
 
 __device__ __forceinline__ void store_dec(const SoA& out, int64_t i, const Dec& o, float b) {
     const int64_t n = out.n;
+    if (out.cls && out.flags && out.n_cross && out.steps && out.esc && out.disk && out.b) {
+        // every plane requested (the usual case): no per-plane tests
+        out.cls[i] = (uint8_t)o.cls;
+        out.flags[i] = (uint8_t)o.flags;
+        out.n_cross[i] = (uint8_t)o.n_cross;
+        out.steps[i] = 0;
+        out.esc[i] = o.esc[0];
+        out.esc[i + n] = o.esc[1];
+        out.esc[i + 2 * n] = o.esc[2];
+        out.disk[i] = o.dr[0];
+        out.disk[i + n] = o.dr[1];
+        out.disk[i + 2 * n] = o.dphi[0];
+        out.disk[i + 3 * n] = o.dphi[1];
+        out.disk[i + 4 * n] = o.dg[0];
+        out.disk[i + 5 * n] = o.dg[1];
+        out.b[i] = b;
+        return;
+    }
     if (out.cls) out.cls[i] = (uint8_t)o.cls;
     if (out.flags) out.flags[i] = (uint8_t)o.flags;
     if (out.n_cross) out.n_cross[i] = (uint8_t)o.n_cross;
