@@ -174,14 +174,18 @@ int table_for(bhs_ctx* ctx, bhs_mlp* m, int32_t ncam, M::SurTable* tab) {
 // Fallback lists for n_items work items: [counters (256 B) | late ids | early ids].
 int lists_for(bhs_ctx* ctx, bhs_mlp* m, uint32_t n_items, M::SurLists* fl) {
     int st;
-    if ((st = grow(ctx, &m->fb, &m->fb_bytes, 256 + (size_t)n_items * 12, "fallback lists")) !=
-        BHS_OK)
+    // statistics partials: 4 words per table-pass block (at most one block per 256 items)
+    const size_t part_words = 4 * (((size_t)n_items + 255) / 256 + 1);
+    if ((st = grow(ctx, &m->fb, &m->fb_bytes, 256 + (size_t)n_items * 12 + part_words * 4,
+                   "fallback lists")) != BHS_OK)
         return st;
-    // [counters: late, early, early work, late band, late band work | late | early | late band]
+    // [counters: late, early, early work, late band, late band work | late | early | late band |
+    //  statistics partials]
     uint32_t* cnt = (uint32_t*)m->fb;
     uint32_t* late = (uint32_t*)((char*)m->fb + 256);
     *fl = M::SurLists{late,        cnt,     late + n_items, cnt + 1, cnt + 2,
-                      late + 2 * (size_t)n_items, cnt + 3, cnt + 4, n_items};
+                      late + 2 * (size_t)n_items, cnt + 3, cnt + 4, n_items,
+                      late + 3 * (size_t)n_items};
     return BHS_OK;
 }
 

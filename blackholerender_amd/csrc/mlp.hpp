@@ -93,6 +93,7 @@ struct SurLists {
     uint32_t* lb_count;       // (latency mode): FP64 march after the network pass
     uint32_t* lb_work;        // work counter of that march
     uint32_t fb_cap;
+    uint32_t* part;           // table pass: per-block statistics partials (4 words per block)
 };
 // Side stream of the early FP32 fallback march.
 struct SurStreams {

@@ -488,6 +488,57 @@ __device__ __forceinline__ void add_block_stats(Stats* st, uint32_t served, uint
 
 constexpr int kBlock = 256;
 
+// As add_block_stats, into the block's four words of `part` (no atomics; k_stats_reduce adds the
+// partials of a launch to the statistics).
+__device__ __forceinline__ void store_block_stats(uint32_t* part, uint32_t served, uint32_t cross,
+                                                  uint32_t esc, uint32_t fb,
+                                                  uint32_t (*s_st)[4]) {
+    const int wave = threadIdx.x >> 6;
+    served = wave_sum(served);
+    cross = wave_sum(cross);
+    esc = wave_sum(esc);
+    fb = wave_sum(fb);
+    if ((threadIdx.x & 63) == 0) {
+        s_st[wave][0] = served;
+        s_st[wave][1] = cross;
+        s_st[wave][2] = esc;
+        s_st[wave][3] = fb;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4u)
+        part[4 * blockIdx.x + threadIdx.x] =
+            s_st[0][threadIdx.x] + s_st[1][threadIdx.x] + s_st[2][threadIdx.x] + s_st[3][threadIdx.x];
+}
+
+// Sums n_blocks blocks' partial statistics (one block of kBlock threads).
+__global__ __launch_bounds__(kBlock) void k_stats_reduce(const uint32_t* __restrict__ part,
+                                                         uint32_t n_blocks, Stats* st) {
+    __shared__ unsigned long long s_v[4][kBlock / 64];
+    unsigned long long v[4] = {0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < n_blocks; b += kBlock)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] += part[4 * b + k];
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+        if ((threadIdx.x & 63) == 0) s_v[k][wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t[4] = {0, 0, 0, 0};
+        for (int w = 0; w < kBlock / 64; ++w)
+            for (int k = 0; k < 4; ++k) t[k] += s_v[k][w];
+        if (t[0]) {
+            atomicAdd(&st->rays, t[0]);
+            atomicAdd(&st->surrogate, t[0]);
+        }
+        if (t[1]) atomicAdd(&st->crossings, t[1]);
+        if (t[2]) atomicAdd(&st->escapes, t[2]);
+        if (t[3]) atomicAdd(&st->fallback, t[3]);
+    }
+}
+
 // Camera sources of the surrogate kernels: one camera by value, or the device camera table of a
 // batch with the prefix sums of the per-camera work items (all multiples of 64, so a wave's 64
 // items always belong to one camera).
@@ -950,6 +1001,15 @@ __device__ __forceinline__ float phi_lookup(const float* __restrict__ phi, const
     return __builtin_fmaf(a, phi[i + 1] - phi[i], phi[i]);
 }
 
+// Chunks of kBlock items per block of the table pass (cpb): its blocks are short-lived, and the
+// hardware dispatcher hands them out as slots free up, so the pass ends evenly although chunks
+// near the critical curve cost several times the others (a persistent grid-stride share does not).
+// cpb is the call's chunk count over four times the resident blocks, clamped to [kTabChunksMin,
+// kTabChunksMax] (fewer, longer blocks cost less per chunk; a single frame needs enough blocks
+// to fill the chip). The block statistics go to per-block partials: one atomic per block and
+// counter would serialise.
+constexpr uint32_t kTabChunksMin = 4, kTabChunksMax = 16;
+
 // Surrogate render in table mode: as k_surrogate with the network replaced by the table lookup.
 template <typename Cams>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
@@ -960,13 +1020,22 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
                                                           const uint32_t* __restrict__ slot,
                                                           int32_t n2, int32_t n3, const SoA out,
                                                           Scratch sc, SurLists fl,
-                                                          uint32_t n_items, uint32_t listed) {
+                                                          uint32_t n_items, uint32_t listed,
+                                                          uint32_t* __restrict__ part,
+                                                          uint32_t cpb) {
     __shared__ uint32_t s_cnt[2 * kPushWords];
     uint32_t half = 0;
     __shared__ uint32_t s_st[4][4];
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
     const int64_t nodes = (int64_t)n2 * n3 * 4;
-    for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
+    // part: cpb chunks per block, statistics partials; else persistent grid-stride
+    const bool own = part != nullptr;
+    const uint64_t q_first = own ? (uint64_t)blockIdx.x * cpb * kBlock
+                                 : (uint64_t)blockIdx.x * kBlock;
+    const uint64_t q_last = own ? q_first + (uint64_t)cpb * kBlock : (uint64_t)n_items;
+    const uint32_t q_end = (uint32_t)(q_last < n_items ? q_last : n_items);
+    const uint32_t q_step = own ? (uint32_t)kBlock : gridDim.x * kBlock;
+    for (uint32_t q0 = (uint32_t)q_first; q0 < q_end; q0 += q_step) {
         const WaveItem w = wave_item_scan(cams, q0 + (threadIdx.x & ~63u), n_items);
         const uint32_t ci = w.ci;
         const MarchConst& c = cams.cam(ci);
@@ -1002,7 +1071,10 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
         half ^= kPushWords;
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
-    add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
+    if (own)
+        store_block_stats(part, n_served, n_cross, n_esc, n_fb, s_st);
+    else
+        add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
 }
 
 // Network outputs of explicit feature rows.
@@ -1249,10 +1321,16 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                            dim3(kBlock), 0, s, cams, ncam, (const uint8_t*)packed,
                            (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
                            tab->n3, tab->node);
-        hipLaunchKernelGGL(k_surrogate_tab<Cams>, grid_of(tab_per_cu), dim3(kBlock), 0, s, cams,
-                           k, (const float*)tab->node, (const float*)tab->phi,
+        const uint32_t resident = tab_per_cu * (uint32_t)(num_cus > 0 ? num_cus : 256);
+        uint32_t cpb = gb0 / (4u * resident);
+        cpb = cpb < kTabChunksMin ? kTabChunksMin : (cpb > kTabChunksMax ? kTabChunksMax : cpb);
+        const uint32_t tab_blocks = (gb0 + cpb - 1) / cpb;
+        hipLaunchKernelGGL(k_surrogate_tab<Cams>, dim3(tab_blocks ? tab_blocks : 1u), dim3(kBlock),
+                           0, s, cams, k, (const float*)tab->node, (const float*)tab->phi,
                            (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
-                           tab->n3, out, sc, fl, n_items, listed);
+                           tab->n3, out, sc, fl, n_items, listed, fl.part, cpb);
+        hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(kBlock), 0, s, (const uint32_t*)fl.part,
+                           tab_blocks, sc.stats);
     } else {
         hipLaunchKernelGGL(k_surrogate<Cams>, dim3(persistent_blocks(gb0, num_cus)), dim3(kBlock),
                            0, s, cams, k, (const uint8_t*)packed, out, sc, fl, n_items,
