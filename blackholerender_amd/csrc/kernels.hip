@@ -342,15 +342,22 @@ void k_band64(const Cams cams, const SoA out, Scratch sc) {
 // alone): every position, by resident waves pulling rays from it (queue counter sc.band_work);
 // finished lanes are refilled as in k_persist, so a wave is not held by its slowest ray. ROOTS: the FP64
 // root refinement of march_f64 for cameras with root64. Exit: the list is finite, every ray ends
-// within max_steps iterations and the counter only grows, so every wave leaves.
-template <typename Cams, bool ROOTS>
-__global__ __launch_bounds__(kBandBlock) __attribute__((amdgpu_waves_per_eu(ROOTS ? 2 : 3)))
-void k_band64_queue(const Cams cams, const SoA out, Scratch sc, uint32_t lanes) {
+// within max_steps iterations and the counter only grows, so every wave leaves. WIDE: register cap
+// for five resident waves per SIMD, for lists longer than wide_n rays, where the list's volume,
+// not its longest ray, sets the time; the other variant keeps the registers that make a single
+// chain fastest and takes lists of at most wide_n rays (wide_n = ~0u: every list).
+constexpr uint32_t kWideBand = 1024u * kBandBlock;
+
+template <typename Cams, bool ROOTS, bool WIDE>
+__global__ __launch_bounds__(kBandBlock)
+__attribute__((amdgpu_waves_per_eu(ROOTS ? 2 : (WIDE ? 5 : 3))))
+void k_band64_queue(const Cams cams, const SoA out, Scratch sc, uint32_t lanes, uint32_t wide_n) {
     constexpr bool kBatch = BandCams<Cams>::kBatch;
     __shared__ BandCams<Cams> s_c;
     __shared__ MarchK<double> s_k[kBatch ? kMaxBatchCams : 1];
     const uint32_t n = *sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap;
-    if (n <= lanes) return;   // uniform: k_band64's list
+    if (n <= lanes) return;                   // uniform: k_band64's list
+    if (WIDE != (n > wide_n)) return;         // uniform: the other variant's list
     uint32_t ncam = 1;
     if constexpr (kBatch) {
         ncam = cams.ncam;
@@ -773,12 +780,24 @@ hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hi
         roots = cams.c.root64 != 0;
     else
         roots = sc.roots != nullptr;   // batches allocate root states iff a camera has root64
-    if (roots)
-        hipLaunchKernelGGL((k_band64_queue<Cams, true>), g, dim3(kBandBlock), 0, aux, cams, out,
-                           sc, lanes);
-    else
-        hipLaunchKernelGGL((k_band64_queue<Cams, false>), g, dim3(kBandBlock), 0, aux, cams, out,
-                           sc, lanes);
+    if (roots) {
+        hipLaunchKernelGGL((k_band64_queue<Cams, true, false>), g, dim3(kBandBlock), 0, aux, cams,
+                           out, sc, lanes, ~0u);
+        return hipGetLastError();
+    }
+    // batches whose list may exceed kWideBand rays (the wide variant's registers fit the batch
+    // form only): the wide variant first (it exits at once on a shorter list), then the other (it
+    // exits at once on a longer one)
+    const bool wide = BandCams<Cams>::kBatch && sc.band_cap > kWideBand;
+    if constexpr (BandCams<Cams>::kBatch) {
+        if (wide) {
+            hipLaunchKernelGGL((k_band64_queue<Cams, false, true>), dim3(1280), dim3(kBandBlock),
+                               0, aux, cams, out, sc, lanes, kWideBand);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+    }
+    hipLaunchKernelGGL((k_band64_queue<Cams, false, false>), g, dim3(kBandBlock), 0, aux, cams,
+                       out, sc, lanes, wide ? kWideBand : ~0u);
     return hipGetLastError();
 }
 
