@@ -510,12 +510,12 @@ __device__ __forceinline__ void store_block_stats(uint32_t* part, uint32_t serve
             s_st[0][threadIdx.x] + s_st[1][threadIdx.x] + s_st[2][threadIdx.x] + s_st[3][threadIdx.x];
 }
 
-// Sums n_blocks blocks' partial statistics (one block of kBlock threads).
+// Sums n_blocks blocks' partial statistics (grid-stride; one atomic per counter and block).
 __global__ __launch_bounds__(kBlock) void k_stats_reduce(const uint32_t* __restrict__ part,
                                                          uint32_t n_blocks, Stats* st) {
     __shared__ unsigned long long s_v[4][kBlock / 64];
     unsigned long long v[4] = {0, 0, 0, 0};
-    for (uint32_t b = threadIdx.x; b < n_blocks; b += kBlock)
+    for (uint32_t b = blockIdx.x * kBlock + threadIdx.x; b < n_blocks; b += gridDim.x * kBlock)
 #pragma unroll
         for (int k = 0; k < 4; ++k) v[k] += part[4 * b + k];
     const int wave = threadIdx.x >> 6;
@@ -1329,8 +1329,9 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                            0, s, cams, k, (const float*)tab->node, (const float*)tab->phi,
                            (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
                            tab->n3, out, sc, fl, n_items, listed, fl.part, cpb);
-        hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(kBlock), 0, s, (const uint32_t*)fl.part,
-                           tab_blocks, sc.stats);
+        const uint32_t red_blocks = (tab_blocks + 4 * kBlock - 1) / (4 * kBlock);
+        hipLaunchKernelGGL(k_stats_reduce, dim3(red_blocks ? red_blocks : 1u), dim3(kBlock), 0, s,
+                           (const uint32_t*)fl.part, tab_blocks, sc.stats);
     } else {
         hipLaunchKernelGGL(k_surrogate<Cams>, dim3(persistent_blocks(gb0, num_cus)), dim3(kBlock),
                            0, s, cams, k, (const uint8_t*)packed, out, sc, fl, n_items,
