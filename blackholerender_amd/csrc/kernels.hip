@@ -765,7 +765,8 @@ constexpr uint32_t kBandQueueItems = 1u << 24;
 // finds the list too long would hold the stream until the other kernels free room for its blocks).
 // The ROOTS variant (crossing states in registers) only for launches with FP64 root refinement.
 template <typename Cams>
-hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hipStream_t aux) {
+hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hipStream_t aux,
+                         bool wide_ok = false) {
     const dim3 g = band_grid(sc.band_cap);
     hipError_t e;
     uint32_t lanes = 0;
@@ -785,10 +786,12 @@ hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hi
                            out, sc, lanes, ~0u);
         return hipGetLastError();
     }
-    // batches whose list may exceed kWideBand rays (the wide variant's registers fit the batch
-    // form only): the wide variant first (it exits at once on a shorter list), then the other (it
-    // exits at once on a longer one)
-    const bool wide = BandCams<Cams>::kBatch && sc.band_cap > kWideBand;
+    // batches of fallback lists (wide_ok) that may exceed kWideBand rays (the wide variant's
+    // registers fit the batch form only): the wide variant first (it exits at once on a shorter
+    // list), then the other (it exits at once on a longer one). Not beside the plain march, whose
+    // persistent blocks take the slots the wide blocks leave for a moment, so that the band's
+    // waves would wait for them.
+    const bool wide = wide_ok && BandCams<Cams>::kBatch && sc.band_cap > kWideBand;
     if constexpr (BandCams<Cams>::kBatch) {
         if (wide) {
             hipLaunchKernelGGL((k_band64_queue<Cams, false, true>), dim3(1280), dim3(kBandBlock),
@@ -936,7 +939,7 @@ hipError_t launch_band_list_batch(const MarchConst* table, const uint32_t* item_
     hipError_t e;
     if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess) return e;
-    if ((e = launch_band64(cams, out, sc, aux)) != hipSuccess) return e;
+    if ((e = launch_band64(cams, out, sc, aux, true)) != hipSuccess) return e;
     return hipEventRecord(ev_join, aux);
 }
 

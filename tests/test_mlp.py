@@ -335,3 +335,30 @@ def test_throughput_mode_batch_equals_single_calls(tracer):
         for name in ("esc", "disk"):
             assert torch.equal(getattr(b, name)[:, off[k]:off[k + 1]], getattr(one, name)), (k, name)
     sur.close()
+
+
+@pytest.mark.gpu
+def test_long_band_fallback_list_equals_plain(tracer):
+    """A surrogate batch whose FP64 band list exceeds 262,144 rays (marched by the five-wave band
+    variant): two 1024x1024 cameras at r_obs = 1200 (outside the served domain, so every ray falls
+    back) with a 1 degree field on the shadow and a 25% band; the output equals the plain batch
+    render (the other band variant) bit for bit."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd.surrogate import Surrogate
+    from helpers import bit_mismatches
+    sur = Surrogate.load(tracer, _model_path(MODEL_HIGH))
+    sur.set_table(256, 64)
+    cams = [bhs.orbit_camera(1200.0, 80.0, 1.0, 1024, 1024, jitter_hashed=1, seed=11, camera_id=k,
+                             azimuth_deg=20.0 * k) for k in range(2)]
+    params = bhs.march_params(band_rel=0.25)
+    print("[mlp] surrogate batch 2 x 1024x1024 at r_obs 1200, 25% band", flush=True)
+    b = sur.render_batch(cams, params)
+    torch.cuda.synchronize()
+    st = tracer.last_stats()
+    assert st["surrogate"] == 0 and st["band64"] > 262144, st
+    p = tracer.render_batch(cams, params)
+    torch.cuda.synchronize()
+    mm = bit_mismatches(b.numpy(), p.numpy())
+    assert all(v == 0 for v in mm.values()), mm
+    sur.close()
