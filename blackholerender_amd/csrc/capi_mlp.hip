@@ -175,9 +175,11 @@ int table_for(bhs_ctx* ctx, bhs_mlp* m, int32_t ncam, M::SurTable* tab) {
 int lists_for(bhs_ctx* ctx, bhs_mlp* m, uint32_t n_items, M::SurLists* fl) {
     int st;
     // statistics partials: 4 words per table-pass block (at most one block per 256 items)
-    const size_t part_words = 4 * (((size_t)n_items + 255) / 256 + 1);
-    if ((st = grow(ctx, &m->fb, &m->fb_bytes, 256 + (size_t)n_items * 12 + part_words * 4,
-                   "fallback lists")) != BHS_OK)
+    const size_t chunks = ((size_t)n_items + 255) / 256 + 1;
+    const size_t part_words = 2 * 4 * chunks;
+    if ((st = grow(ctx, &m->fb, &m->fb_bytes,
+                   256 + (size_t)n_items * 12 + (part_words + chunks) * 4, "fallback lists")) !=
+        BHS_OK)
         return st;
     // [counters: late, early, early work, late band, late band work | late | early | late band |
     //  statistics partials]
@@ -185,7 +187,7 @@ int lists_for(bhs_ctx* ctx, bhs_mlp* m, uint32_t n_items, M::SurLists* fl) {
     uint32_t* late = (uint32_t*)((char*)m->fb + 256);
     *fl = M::SurLists{late,        cnt,     late + n_items, cnt + 1, cnt + 2,
                       late + 2 * (size_t)n_items, cnt + 3, cnt + 4, n_items,
-                      late + 3 * (size_t)n_items};
+                      late + 3 * (size_t)n_items, late + 3 * (size_t)n_items + part_words};
     return BHS_OK;
 }
 

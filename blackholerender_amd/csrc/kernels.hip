@@ -787,11 +787,15 @@ hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hi
         return hipGetLastError();
     }
     // batches of fallback lists (wide_ok) that may exceed kWideBand rays (the wide variant's
-    // registers fit the batch form only): the wide variant first (it exits at once on a shorter
-    // list), then the other (it exits at once on a longer one). Not beside the plain march, whose
-    // persistent blocks take the slots the wide blocks leave for a moment, so that the band's
-    // waves would wait for them.
+    // registers fit the batch form only): the other variant first (it exits at once on a longer
+    // list), then the wide one (it exits at once on a shorter one): a kernel that only exits still
+    // waits for slots beside a running pass, so the short, latency-bound list must not wait behind
+    // it. Not beside the plain march, whose persistent blocks take the slots the wide blocks leave
+    // for a moment, so that the band's waves would wait for them.
     const bool wide = wide_ok && BandCams<Cams>::kBatch && sc.band_cap > kWideBand;
+    hipLaunchKernelGGL((k_band64_queue<Cams, false, false>), g, dim3(kBandBlock), 0, aux, cams,
+                       out, sc, lanes, wide ? kWideBand : ~0u);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if constexpr (BandCams<Cams>::kBatch) {
         if (wide) {
             hipLaunchKernelGGL((k_band64_queue<Cams, false, true>), dim3(1280), dim3(kBandBlock),
@@ -799,9 +803,7 @@ hipError_t launch_band64(const Cams& cams, const SoA& out, const Scratch& sc, hi
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
     }
-    hipLaunchKernelGGL((k_band64_queue<Cams, false, false>), g, dim3(kBandBlock), 0, aux, cams,
-                       out, sc, lanes, wide ? kWideBand : ~0u);
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 // The band fork: scan on s, FP64 march on aux between ev_fork and ev_join.

@@ -94,6 +94,7 @@ struct SurLists {
     uint32_t* lb_work;        // work counter of that march
     uint32_t fb_cap;
     uint32_t* part;           // table pass: per-block statistics partials (4 words per block)
+    uint32_t* near;           // table pass, large calls: near-critical flag per 256-item chunk
 };
 // Side stream of the early FP32 fallback march.
 struct SurStreams {

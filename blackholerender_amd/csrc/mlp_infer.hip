@@ -644,11 +644,6 @@ enum Route : uint32_t {
     kPlunge = 4,  // plunging ray: exact orbit alone (decode_plunge may send it back)
 };
 
-// Lists already built when the network pass runs (bit mask): the FP64 band (by the pre-pass or
-// the integrator's band scan) and the early fallback list (by the pre-pass).
-constexpr uint32_t kListedBand = 1u;
-constexpr uint32_t kListedEarly = 2u;
-
 struct RayPrep {
     RayGeom g;
     float b_axis, lapse_g;   // impact parameter about the spin axis, lapse (redshift)
@@ -747,31 +742,40 @@ __device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst&
     return r;
 }
 
-// List of a ray after the surrogate pass (listed: the lists already built). A network candidate
-// the decode sends back (back) goes to the FP64 band if it is a band ray (the late band list when
-// the band march already runs, else the call's band list), else to the FP32 late list; a band ray
-// to the band list and an early fallback ray to the early list, unless those are already built.
-// FP64 precision sends every unserved ray to the band list.
-__device__ __forceinline__ int dest_of(const MarchConst& c, const RayPrep& r, bool back,
-                                       uint32_t listed) {
-    const bool f64 = c.precision == BHS_PREC_F64;
-    if (back) return (r.band || f64) ? ((listed & kListedBand) ? kToLate : kToBand) : kToFb;
-    if (r.route == kBand) return (listed & kListedBand) ? kToNone : kToBand;
-    if (r.route == kEarly) return (listed & kListedEarly) ? kToNone : (f64 ? kToBand : kToEarly);
-    return kToNone;
+// Lists of the rays a surrogate pass does not serve, by class (4 bits each, kNoList: not listed
+// by this pass): FP64 band rays and early fallback rays (known before the decode), band rays and
+// other rays the decode sends back.
+constexpr uint32_t kNoList = 15u;
+constexpr uint32_t pack_dests(uint32_t band, uint32_t early, uint32_t back_band,
+                              uint32_t back) {
+    return band | (early << 4) | (back_band << 8) | (back << 12);
+}
+
+__device__ __forceinline__ int dest_of(const RayPrep& r, bool back, uint32_t dests) {
+    int cls = -1;
+    if (back)
+        cls = r.band ? 2 : 3;
+    else if (r.route == kBand)
+        cls = 0;
+    else if (r.route == kEarly)
+        cls = 1;
+    if (cls < 0) return kToNone;
+    const uint32_t d = (dests >> (4 * cls)) & 15u;
+    return d == kNoList ? kToNone : (int)d;
 }
 
 // Pre-pass: lists the FP64 band rays and the early fallback rays.
 template <typename Cams>
 __global__ __launch_bounds__(kBlock) void k_sur_pre(const Cams cams, const SurConst k,
-                                                    uint32_t n_items, Scratch sc, SurLists fl) {
+                                                    uint32_t n_items, Scratch sc, SurLists fl,
+                                                    uint32_t dests) {
     __shared__ uint32_t s_cnt[2 * kPushWords];
     uint32_t half = 0;
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
         const WaveItem w = wave_item(cams, q0 + (threadIdx.x & ~63u), n_items);
         const MarchConst& c = cams.cam(w.ci);
         const RayPrep r = prep_ray(c, k, w);
-        push_lists(sc, fl, dest_of(c, r, false, 0u), w.item, s_cnt + half);
+        push_lists(sc, fl, dest_of(r, false, dests), w.item, s_cnt + half);
         half ^= kPushWords;
     }
 }
@@ -783,7 +787,7 @@ template <typename Cams>
 __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const SurConst k,
                                                          const uint8_t* __restrict__ packed,
                                                          const SoA out, Scratch sc, SurLists fl,
-                                                         uint32_t n_items, uint32_t listed) {
+                                                         uint32_t n_items, uint32_t dests) {
     __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
     __shared__ float s_y[4][64 * kOut];
     __shared__ uint32_t s_cnt[2 * kPushWords];
@@ -814,7 +818,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
                 back = true;
             }
         }
-        push_lists(sc, fl, dest_of(c, r, back, listed), w.item, s_cnt + half);
+        push_lists(sc, fl, dest_of(r, back, dests), w.item, s_cnt + half);
         half ^= kPushWords;
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
@@ -1001,6 +1005,59 @@ __device__ __forceinline__ float phi_lookup(const float* __restrict__ phi, const
     return __builtin_fmaf(a, phi[i + 1] - phi[i], phi[i]);
 }
 
+// Near-critical chunks: a 256-item chunk whose rays, sampled at nine pixel centres per 64 items
+// (jitter ignored), include an inward ray with |beta - 1| < kNearBeta. They hold the long
+// integrator rays of a frame (the FP64 band, rays that orbit); large table passes take them first
+// so that their fallback marches run beside the rest of the pass. A scheduling class only.
+constexpr float kNearBeta = 0.5f;
+
+__device__ __forceinline__ bool near_segment(const MarchConst& c, uint32_t p0) {
+    const uint32_t width = (uint32_t)c.width, height = (uint32_t)c.height;
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    const float r = __builtin_amdgcn_sqrtf(dot3(P, P));
+    if (!(r > 2.5f)) return false;
+    const float ir = __builtin_amdgcn_rcpf(r);
+    const V3<float> e1{P.x * ir, P.y * ir, P.z * ir};
+    const float kb = r * __builtin_amdgcn_rsqf(1.0f - 2.0f * ir) * (1.0f / kBcritF);
+    const float ih = __builtin_amdgcn_rcpf((float)c.height);
+    const float sc = 2.0f * c.tan_half_fov * ih;
+    const float x0 = c.tan_half_fov * (float)c.width * ih;
+    const float* B = c.basis;
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const uint32_t p = p0 + (k < 8 ? 8u * k : 63u);
+        const uint32_t row = p / width, col = p - row * width;
+        if (row >= height) continue;
+        const V3<float> d0{((float)col + 0.5f) * sc - x0,
+                           ((float)row + 0.5f) * sc - c.tan_half_fov, -1.0f};
+        const V3<float> d{B[0] * d0.x + B[3] * d0.y + B[6] * d0.z,
+                          B[1] * d0.x + B[4] * d0.y + B[7] * d0.z,
+                          B[2] * d0.x + B[5] * d0.y + B[8] * d0.z};
+        const float nr = dot3(d, e1) * __builtin_amdgcn_rsqf(dot3(d, d));
+        const float beta = kb * __builtin_amdgcn_sqrtf(fmaxf(1.0f - nr * nr, 0.0f));
+        hit = hit || (nr < 0.0f && fabsf(beta - 1.0f) < kNearBeta);
+    }
+    return hit;
+}
+
+// Near-critical flag of each 256-item chunk of a call (one thread per 64-item segment).
+template <typename Cams>
+__global__ __launch_bounds__(kBlock) void k_chunk_class(const Cams cams, uint32_t n_items,
+                                                        uint32_t* __restrict__ near) {
+    const uint32_t seg = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n_seg = n_items >> 6;
+    bool hit = false;
+    if (seg < n_seg) {
+        const uint32_t q = seg << 6;
+        const uint32_t ci = cams.find(q);
+        hit = near_segment(cams.cam(ci), q - cams.base(ci));
+    }
+    const uint64_t m = __ballot(hit);
+    const uint32_t lane = threadIdx.x & 63u;
+    if ((lane & 3u) == 0u && seg < n_seg) near[seg >> 2] = ((m >> lane) & 0xFull) != 0ull;
+}
+
 // Chunks of kBlock items per block of the table pass (cpb): its blocks are short-lived, and the
 // hardware dispatcher hands them out as slots free up, so the pass ends evenly although chunks
 // near the critical curve cost several times the others (a persistent grid-stride share does not).
@@ -1020,9 +1077,10 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
                                                           const uint32_t* __restrict__ slot,
                                                           int32_t n2, int32_t n3, const SoA out,
                                                           Scratch sc, SurLists fl,
-                                                          uint32_t n_items, uint32_t listed,
+                                                          uint32_t n_items, uint32_t dests,
                                                           uint32_t* __restrict__ part,
-                                                          uint32_t cpb) {
+                                                          uint32_t cpb, uint32_t phase,
+                                                          const uint32_t* __restrict__ near) {
     __shared__ uint32_t s_cnt[2 * kPushWords];
     uint32_t half = 0;
     __shared__ uint32_t s_st[4][4];
@@ -1036,6 +1094,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
     const uint32_t q_end = (uint32_t)(q_last < n_items ? q_last : n_items);
     const uint32_t q_step = own ? (uint32_t)kBlock : gridDim.x * kBlock;
     for (uint32_t q0 = (uint32_t)q_first; q0 < q_end; q0 += q_step) {
+        if (phase != 0u && (near[q0 / kBlock] != 0u) != (phase == 1u)) continue;   // uniform
         const WaveItem w = wave_item_scan(cams, q0 + (threadIdx.x & ~63u), n_items);
         const uint32_t ci = w.ci;
         const MarchConst& c = cams.cam(ci);
@@ -1067,7 +1126,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
                 back = true;
             }
         }
-        push_lists(sc, fl, dest_of(c, r, back, listed), w.item, s_cnt + half);
+        push_lists(sc, fl, dest_of(r, back, dests), w.item, s_cnt + half);
         half ^= kPushWords;
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
@@ -1258,15 +1317,20 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
         return dim3(gb0 < cap ? (gb0 ? gb0 : 1u) : cap);
     };
     const bool f64 = kOne && c0.precision == BHS_PREC_F64;
-    // Latency mode (up to kRoutedItems work items): a pre-pass routes the rays first, so the side
-    // marches, whose serial rays are the longest chains of a frame, run beside the network pass;
-    // their lists are short. Throughput mode (larger calls): the network pass routes the rays
-    // itself and the lists, millions of rays long, are marched after it (the FP64 band beside the
-    // FP32 lists). Starting the band before the network pass gains nothing there: both are
-    // VALU-bound, and the pre-pass or band scan is extra work. The FP32 lists run at two resident
-    // waves per SIMD (131k lanes), which leaves room for the network pass or the FP64 band.
+    // The fallback marches' time is set by their longest serial rays, which lie near the critical
+    // impact parameter. Latency mode (up to kRoutedItems work items): a pre-pass routes the rays
+    // first, so the side marches run beside the whole surrogate pass. Larger calls in table mode
+    // (phased): the pass takes the near-critical chunks first, then their FP64 band and FP32
+    // fallback marches run on the side streams beside the pass over the other chunks, whose
+    // fallback rays (short) follow it; the pass's short-lived blocks take the slots the side
+    // marches leave. Larger calls in network mode: the network pass routes the rays itself and
+    // the lists are marched after it. The FP32 lists run at two resident waves per SIMD (131k
+    // lanes), which leaves room for the surrogate pass or the FP64 band.
     const bool routed = !f64 && n_items <= kRoutedItems;
-    const uint32_t listed = routed ? (kListedBand | kListedEarly) : 0u;
+    const bool phased = tab && !f64 && !routed;
+    const uint32_t to_band = pack_dests(kToBand, kToBand, kToBand, kToBand);
+    const uint32_t d_late = pack_dests(kNoList, kNoList, kToLate, kToFb);   // lists built before
+    const uint32_t d_all = pack_dests(kToBand, kToEarly, kToBand, kToFb);
     Tuning tl = t;
     tl.mode = 1;
     tl.block = 256;
@@ -1304,7 +1368,7 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
     if (routed) {
         hipLaunchKernelGGL(k_sur_pre<Cams>, grid_of(pre_per_cu), dim3(kBlock), 0, s, cams, k,
-                           n_items, sc, fl);
+                           n_items, sc, fl, pack_dests(kToBand, kToEarly, kNoList, kNoList));
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = side()) != hipSuccess) return e;
     }
@@ -1325,22 +1389,41 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
         uint32_t cpb = gb0 / (4u * resident);
         cpb = cpb < kTabChunksMin ? kTabChunksMin : (cpb > kTabChunksMax ? kTabChunksMax : cpb);
         const uint32_t tab_blocks = (gb0 + cpb - 1) / cpb;
-        hipLaunchKernelGGL(k_surrogate_tab<Cams>, dim3(tab_blocks ? tab_blocks : 1u), dim3(kBlock),
-                           0, s, cams, k, (const float*)tab->node, (const float*)tab->phi,
-                           (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
-                           tab->n3, out, sc, fl, n_items, listed, fl.part, cpb);
-        const uint32_t red_blocks = (tab_blocks + 4 * kBlock - 1) / (4 * kBlock);
-        hipLaunchKernelGGL(k_stats_reduce, dim3(red_blocks ? red_blocks : 1u), dim3(kBlock), 0, s,
-                           (const uint32_t*)fl.part, tab_blocks, sc.stats);
+        auto pass = [&](uint32_t dests, uint32_t phase) {
+            uint32_t* part = fl.part + (phase == 2u ? 4u * tab_blocks : 0u);
+            hipLaunchKernelGGL(k_surrogate_tab<Cams>, dim3(tab_blocks ? tab_blocks : 1u),
+                               dim3(kBlock), 0, s, cams, k, (const float*)tab->node,
+                               (const float*)tab->phi, (const float*)tab->range,
+                               (const uint32_t*)tab->slot, tab->n2, tab->n3, out, sc, fl, n_items,
+                               dests, part, cpb, phase, (const uint32_t*)fl.near);
+            const uint32_t red_blocks = (tab_blocks + 4 * kBlock - 1) / (4 * kBlock);
+            hipLaunchKernelGGL(k_stats_reduce, dim3(red_blocks ? red_blocks : 1u), dim3(kBlock),
+                               0, s, (const uint32_t*)part, tab_blocks, sc.stats);
+            return hipGetLastError();
+        };
+        if (phased) {
+            // near-critical chunks first, their band and fallback marches beside the rest
+            const uint32_t segs = n_items >> 6;
+            hipLaunchKernelGGL(k_chunk_class<Cams>, dim3((segs + kBlock - 1) / kBlock),
+                               dim3(kBlock), 0, s, cams, n_items, fl.near);
+            if ((e = pass(pack_dests(kToBand, kToEarly, kToBand, kToEarly), 1u)) != hipSuccess)
+                return e;
+            if ((e = side()) != hipSuccess) return e;
+            if ((e = pass(pack_dests(kToLate, kToFb, kToLate, kToFb), 2u)) != hipSuccess)
+                return e;
+        } else if ((e = pass(f64 ? to_band : (routed ? d_late : d_all), 0u)) != hipSuccess) {
+            return e;
+        }
     } else {
         hipLaunchKernelGGL(k_surrogate<Cams>, dim3(persistent_blocks(gb0, num_cus)), dim3(kBlock),
                            0, s, cams, k, (const uint8_t*)packed, out, sc, fl, n_items,
-                           listed);
+                           f64 ? to_band : (routed ? d_late : d_all));
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
-    if (routed && any_band) {
-        // band rays the decode sent back (their band march had started): FP64 march on aux2
+    if ((routed || phased) && any_band) {
+        // band rays listed after the band march started (sent back by the decode, or met in the
+        // second phase): FP64 march on aux2
         Scratch sb = sc;
         sb.band_ids = fl.lb_ids;
         sb.band_count = fl.lb_count;
@@ -1353,7 +1436,7 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                                        ss.ev_fork2, ss.ev_join2);
         if (e != hipSuccess) return e;
     }
-    if (!routed && !f64 && (e = side()) != hipSuccess) return e;
+    if (!routed && !phased && !f64 && (e = side()) != hipSuccess) return e;
     if (f64) {   // every unserved ray is in the band list: FP64 march now
         if ((e = band()) != hipSuccess) return e;
     } else if ((e = march_list(fl.fb_ids, fl.fb_count, sc, s)) != hipSuccess) {
