@@ -1009,7 +1009,7 @@ __device__ __forceinline__ float phi_lookup(const float* __restrict__ phi, const
 // (jitter ignored), include an inward ray with |beta - 1| < kNearBeta. They hold the long
 // integrator rays of a frame (the FP64 band, rays that orbit); large table passes take them first
 // so that their fallback marches run beside the rest of the pass. A scheduling class only.
-constexpr float kNearBeta = 0.5f;
+constexpr float kNearBeta = 0.8f;
 
 __device__ __forceinline__ bool near_segment(const MarchConst& c, uint32_t p0) {
     const uint32_t width = (uint32_t)c.width, height = (uint32_t)c.height;
