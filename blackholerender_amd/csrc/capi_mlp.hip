@@ -157,17 +157,19 @@ float quantile(std::vector<float>& v, double q) {
 
 // Table-mode buffers for ncam cameras (tab->n2 = 0 when the model runs per ray).
 int table_for(bhs_ctx* ctx, bhs_mlp* m, int32_t ncam, M::SurTable* tab) {
-    *tab = M::SurTable{nullptr, nullptr, nullptr, nullptr, 0, 0};
+    *tab = M::SurTable{nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr};
     if (m->tab_n2 <= 0) return BHS_OK;
     const size_t nodes = (size_t)m->tab_n2 * (size_t)m->tab_n3 * (size_t)ncam;
     const size_t phi_bytes = (size_t)ncam * M::kPhiNodes * 4;
     int st;
-    if ((st = grow(ctx, &m->tab, &m->tab_bytes, 1024 + phi_bytes + nodes * 16,
+    // [range (64 x 2 floats) | slot (64) | per-camera setup terms (64 x 16 words) | phi | nodes]
+    constexpr size_t kHead = 512 + 512 + 64 * 64;
+    if ((st = grow(ctx, &m->tab, &m->tab_bytes, kHead + phi_bytes + nodes * 16,
                    "surrogate table")) != BHS_OK)
         return st;
     char* base = (char*)m->tab;
-    *tab = M::SurTable{(float*)(base + 1024 + phi_bytes), (float*)(base + 1024), (float*)base,
-                       (uint32_t*)(base + 512), m->tab_n2, m->tab_n3};
+    *tab = M::SurTable{(float*)(base + kHead + phi_bytes), (float*)(base + kHead), (float*)base,
+                       (uint32_t*)(base + 512), m->tab_n2, m->tab_n3, (float*)(base + 1024)};
     return BHS_OK;
 }
 

@@ -25,12 +25,15 @@ __device__ __forceinline__ int r_bin(float r) {
     return bin < 0 ? 0 : (bin >= kStepBins ? kStepBins - 1 : bin);
 }
 
-// Step-table bin of a ray at r_obs with feature x2.
-__device__ __forceinline__ int step_bin(float r, float x2) {
+// Step-table bin of a ray in r_obs bin rb with feature x2.
+__device__ __forceinline__ int step_bin_rb(int rb, float x2) {
     int b = (int)((x2 + 1.0f) * (0.5f * (float)kStepBinsB));
     b = b < 0 ? 0 : (b >= kStepBinsB ? kStepBinsB - 1 : b);
-    return r_bin(r) * kStepBinsB + b;
+    return rb * kStepBinsB + b;
 }
+
+// Step-table bin of a ray at r_obs with feature x2.
+__device__ __forceinline__ int step_bin(float r, float x2) { return step_bin_rb(r_bin(r), x2); }
 
 // Per-model constants of the surrogate render kernel.
 struct SurConst {
@@ -117,6 +120,7 @@ struct SurTable {
     float* range;     // ncam x 2 floats (x2 bounds)
     uint32_t* slot;   // ncam table slots
     int32_t n2, n3;
+    float* cam;       // ncam x 16 words: the per-camera terms of the ray setup (SurCam)
 };
 // tab null: per-ray network inference. ev_begin / ev_mid / ev_end (may be null) bracket the
 // network part and the FP32 fallback march.

@@ -198,9 +198,8 @@ __device__ __forceinline__ bool cell_served(const SurConst& k, const float x[3])
     return (k.serve[cell >> 5] >> (cell & 31)) & 1u;
 }
 
-__device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k, const RayGeom& g,
-                                         float x2) {
-    return (float)k.steps_max[step_bin(g.r, x2)] * k.steps_slack < (float)c.max_steps;
+__device__ __forceinline__ bool steps_ok(const MarchConst& c, const SurConst& k, int rb, float x2) {
+    return (float)k.steps_max[step_bin_rb(rb, x2)] * k.steps_slack < (float)c.max_steps;
 }
 
 // Relative margin around the annulus edges for a third crossing (exact orbit, no learned residual).
@@ -224,7 +223,7 @@ __device__ __forceinline__ float redshift_fast(float r, float b_axis, float laps
 }
 
 template <typename UAt>
-__device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& k, float b_axis,
+__device__ __forceinline__ Dec decode_with(const MarchConst& c, float margin, float b_axis,
                                            float lapse_g, const RayGeom& g, float Phi,
                                            UAt u_at) {
     Dec o;
@@ -237,7 +236,6 @@ __device__ __forceinline__ Dec decode_with(const MarchConst& c, const SurConst& 
                        : (a <= kPi ? 1u : (a <= 2.0f * kPi ? 2u : (a <= 3.0f * kPi ? 3u : 4u)));
     if (o.n_eq >= 4u && c.thin) return o;
     bool in[2] = {false, false};
-    const float margin = k.radius_margin[r_bin(g.r)];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         o.r_eq[q] = 1e30f;
@@ -321,7 +319,7 @@ __device__ __forceinline__ Dec decode(const MarchConst& c, const SurConst& k, fl
     const Orbit ob = orbit(g);
     // the exact sweep to infinity with the learned residual, or to the horizon test radius
     const float Phi = ob.phi_end + (g.bm > 0.0f ? kResScale * y[0] : 0.0f);
-    return decode_with(c, k, b_axis, lapse_g, g, Phi, [&](int q) {
+    return decode_with(c, k.radius_margin[r_bin(g.r)], b_axis, lapse_g, g, Phi, [&](int q) {
         const float u = orbit_u(ob, g.psi + (float)q * kPi);
         return q < 2 ? u * __expf(kURes * y[1 + q]) : u;
     });
@@ -647,18 +645,64 @@ enum Route : uint32_t {
 struct RayPrep {
     RayGeom g;
     float b_axis, lapse_g;   // impact parameter about the spin axis, lapse (redshift)
+    float margin;            // relative crossing-radius margin of the camera's r_obs bin
     float x[3];
     Route route;
     bool band;   // in the integrator's FP64 band (BAND64 precision)
 };
+
+// The per-camera terms of the surrogate's ray setup (16 words): r_obs, its reciprocal and the
+// radial unit vector, 1/sqrt(1 - 2/r) (impact parameter), the b_axis and lapse factors, the
+// pixel-to-NDC scales, the x1 feature and the log r_obs bin. Computed once per camera (table mode:
+// k_tab_range stores them) instead of once per wave.
+struct SurCam {
+    float r, ir, e1[3], inv_f, bax_k, lapse_g, inv_w, inv_h, asp, x1;
+    int32_t rbin;
+    float pad_[3];
+};
+static_assert(sizeof(SurCam) == 64, "SurCam layout");
+
+__device__ __forceinline__ SurCam sur_cam(const MarchConst& c) {
+    SurCam pc;
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    pc.r = __builtin_amdgcn_sqrtf(dot3(P, P));
+    pc.ir = __builtin_amdgcn_rcpf(pc.r);
+    pc.e1[0] = P.x * pc.ir;
+    pc.e1[1] = P.y * pc.ir;
+    pc.e1[2] = P.z * pc.ir;
+    const float lapse2 = 1.0f - 2.0f * pc.ir;
+    pc.inv_f = __builtin_amdgcn_rcpf(__builtin_amdgcn_sqrtf(fmaxf(lapse2, 1e-12f)));
+    pc.bax_k = __builtin_amdgcn_rsqf(fmaxf(lapse2, 1e-6f));
+    pc.lapse_g = __builtin_amdgcn_sqrtf(fmaxf(lapse2, 1e-4f));
+    const float wf = (float)c.width, hf = (float)c.height;
+    pc.inv_w = __builtin_amdgcn_rcpf(wf);
+    pc.inv_h = __builtin_amdgcn_rcpf(hf);
+    pc.asp = wf * pc.inv_h;
+    pc.x1 = (log_r(pc.r) - kLogRLo) * (2.0f / (kLogRHi - kLogRLo)) - 1.0f;
+    pc.rbin = r_bin(pc.r);
+    pc.pad_[0] = pc.pad_[1] = pc.pad_[2] = 0.0f;
+    return pc;
+}
+
+// Camera k's record from a table written by k_tab_range (read-only during the launch and indexed
+// wave-uniformly: scalar loads).
+__device__ __forceinline__ SurCam load_sur_cam(const float* pre, uint32_t k) {
+    using ConstU32 = const __attribute__((address_space(4))) uint32_t;
+    SurCam pc;
+    ConstU32* src = (ConstU32*)(pre + 16 * (size_t)k);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&pc);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) dst[i] = src[i];
+    return pc;
+}
 
 // Geometry of the camera ray of pixel (col, row) for the surrogate: the view direction as
 // ray_dir (blackhole.frag:68-72) and the orbit-plane frame, impact parameter and redshift inputs
 // of lawS_setup (geodesic.glsl:27-40), from hardware reciprocal / square-root approximations
 // (relative error ~1e-7). The tangential part of the initial coordinate velocity is the view
 // direction's, so the frame needs no velocity; |e1 x d| = |d - (d.e1) e1| is sin(alpha).
-__device__ __forceinline__ void sur_geom(const MarchConst& c, int64_t pix, uint32_t col,
-                                         uint32_t row, RayPrep& r) {
+__device__ __forceinline__ void sur_geom(const MarchConst& c, const SurCam& pc, int64_t pix,
+                                         uint32_t col, uint32_t row, RayPrep& r) {
     float px, py;
     if (c.jitter_hashed) {
         const uint32_t pi2 = 2u * (uint32_t)pix;
@@ -668,11 +712,9 @@ __device__ __forceinline__ void sur_geom(const MarchConst& c, int64_t pix, uint3
         px = ((float)col + 0.5f) + c.jitter[0];
         py = ((float)row + 0.5f) + c.jitter[1];
     }
-    const float wf = (float)c.width, hf = (float)c.height;
-    const float ndx = __builtin_fmaf(px * __builtin_amdgcn_rcpf(wf), 2.0f, -1.0f);
-    const float ndy = __builtin_fmaf(py * __builtin_amdgcn_rcpf(hf), 2.0f, -1.0f);
-    const V3<float> d0{(ndx * c.tan_half_fov) * (wf * __builtin_amdgcn_rcpf(hf)),
-                       ndy * c.tan_half_fov, -1.0f};
+    const float ndx = __builtin_fmaf(px * pc.inv_w, 2.0f, -1.0f);
+    const float ndy = __builtin_fmaf(py * pc.inv_h, 2.0f, -1.0f);
+    const V3<float> d0{(ndx * c.tan_half_fov) * pc.asp, ndy * c.tan_half_fov, -1.0f};
     const float* B = c.basis;
     V3<float> d{__builtin_fmaf(B[6], d0.z, __builtin_fmaf(B[3], d0.y, B[0] * d0.x)),
                 __builtin_fmaf(B[7], d0.z, __builtin_fmaf(B[4], d0.y, B[1] * d0.x)),
@@ -681,9 +723,8 @@ __device__ __forceinline__ void sur_geom(const MarchConst& c, int64_t pix, uint3
     d.x *= id; d.y *= id; d.z *= id;
     RayGeom& g = r.g;
     const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-    g.r = __builtin_amdgcn_sqrtf(dot3(P, P));
-    const float ir = __builtin_amdgcn_rcpf(g.r);
-    g.e1 = V3<float>{P.x * ir, P.y * ir, P.z * ir};
+    g.r = pc.r;
+    g.e1 = V3<float>{pc.e1[0], pc.e1[1], pc.e1[2]};
     const float nr = dot3(d, g.e1);
     const V3<float> t{d.x - nr * g.e1.x, d.y - nr * g.e1.y, d.z - nr * g.e1.z};
     const float tn = __builtin_amdgcn_sqrtf(dot3(t, t));
@@ -691,17 +732,15 @@ __device__ __forceinline__ void sur_geom(const MarchConst& c, int64_t pix, uint3
     const float it = tn > 1e-6f ? __builtin_amdgcn_rcpf(tn) : 0.0f;
     g.e2 = V3<float>{t.x * it, t.y * it, t.z * it};
     g.ingoing = nr < 0.0f;
-    const float lapse2 = 1.0f - 2.0f * ir;
-    const float f = __builtin_amdgcn_sqrtf(fmaxf(lapse2, 1e-12f));
     g.sin_a = tn;
-    g.b = g.r * tn * __builtin_amdgcn_rcpf(f);
+    g.b = g.r * tn * pc.inv_f;
     g.bm = g.b * (1.0f / kBcritF) - 1.0f;
     float psi = atan2_fast(-g.e1.y, g.e2.y);
     if (psi < 0.0f) psi += kPi;
     if (psi >= kPi) psi -= kPi;
     g.psi = psi;
-    r.b_axis = -(P.z * d.x - P.x * d.z) * __builtin_amdgcn_rsqf(fmaxf(lapse2, 1e-6f));
-    r.lapse_g = __builtin_amdgcn_sqrtf(fmaxf(lapse2, 1e-4f));
+    r.b_axis = -(P.z * d.x - P.x * d.z) * pc.bax_k;
+    r.lapse_g = pc.lapse_g;
 }
 
 // Rays with |beta - 1| below this get the integrator's exact setup: there the swept angle moves by
@@ -712,15 +751,16 @@ constexpr float kExactBeta = 0.05f;
 // ray of the FP64 band that the model serves is a network candidate; the other band rays are
 // marched in FP64. Band membership must agree with the integrator's exact setup: near-critical
 // rays (kExactBeta) and rays within 1e-5 b_crit of the band edge are classified by it.
-__device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst& k,
-                                            const WaveItem& w) {
+__device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurCam& pc,
+                                            const SurConst& k, const WaveItem& w) {
     const int64_t pix = w.pix;
     RayPrep r;
     r.x[0] = r.x[1] = r.x[2] = 0.0f;
     r.route = kNone;
     r.band = false;
+    r.margin = k.radius_margin[pc.rbin];
     if (pix < 0) return r;
-    sur_geom(c, pix, w.col, w.row, r);
+    sur_geom(c, pc, pix, w.col, w.row, r);
     const bool band64 = c.precision == BHS_PREC_F32_BAND64;
     if (fabsf(r.g.bm) < kExactBeta ||
         (band64 && fabsf(fabsf(r.g.b - kBcritF) - c.band_abs) < 1e-5f * kBcritF)) {
@@ -734,9 +774,11 @@ __device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurConst&
     r.band = band64 && __builtin_fabsf(r.g.b - kBcritF) < c.band_abs;
     r.route = r.band ? kBand : kEarly;
     if (in_domain(r.g)) {
-        features(r.g, r.x);
-        if (cell_served(k, r.x) && steps_ok(c, k, r.g, r.x[1])) r.route = kNet;
-    } else if (plunging(r.g) && steps_ok(c, k, r.g, x2_of(r.g.bm))) {
+        r.x[0] = pc.x1;   // the features of features(), x1 from the camera record
+        r.x[1] = x2_of(r.g.bm);
+        r.x[2] = r.g.psi * (2.0f / kPi) - 1.0f;
+        if (cell_served(k, r.x) && steps_ok(c, k, pc.rbin, r.x[1])) r.route = kNet;
+    } else if (plunging(r.g) && steps_ok(c, k, pc.rbin, x2_of(r.g.bm))) {
         r.route = kPlunge;
     }
     return r;
@@ -768,13 +810,14 @@ __device__ __forceinline__ int dest_of(const RayPrep& r, bool back, uint32_t des
 template <typename Cams>
 __global__ __launch_bounds__(kBlock) void k_sur_pre(const Cams cams, const SurConst k,
                                                     uint32_t n_items, Scratch sc, SurLists fl,
-                                                    uint32_t dests) {
+                                                    uint32_t dests, const float* cam_pre) {
     __shared__ uint32_t s_cnt[2 * kPushWords];
     uint32_t half = 0;
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
         const WaveItem w = wave_item(cams, q0 + (threadIdx.x & ~63u), n_items);
         const MarchConst& c = cams.cam(w.ci);
-        const RayPrep r = prep_ray(c, k, w);
+        const SurCam pc = cam_pre ? load_sur_cam(cam_pre, w.ci) : sur_cam(c);
+        const RayPrep r = prep_ray(c, pc, k, w);
         push_lists(sc, fl, dest_of(r, false, dests), w.item, s_cnt + half);
         half ^= kPushWords;
     }
@@ -802,7 +845,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
         const WaveItem w = wave_item_scan(cams, ch * kBlock + (threadIdx.x & ~63u), n_items);
         const MarchConst& c = cams.cam(w.ci);
         const int64_t pix = w.pix;
-        const RayPrep r = prep_ray(c, k, w);
+        const RayPrep r = prep_ray(c, sur_cam(c), k, w);
         const bool cand = r.route == kNet;
         float y[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (__any(cand)) mlp_wave(s_w, r.x, s_y[wave], y);
@@ -848,10 +891,13 @@ __device__ __forceinline__ bool same_pose(const MarchConst& a, const MarchConst&
 
 template <typename Cams>
 __global__ void k_tab_range(const Cams cams, uint32_t ncam, float* __restrict__ range,
-                            uint32_t* __restrict__ slot) {
+                            uint32_t* __restrict__ slot, float* __restrict__ cam_pre) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncam) return;
     const MarchConst& c = cams.cam(k);
+    const SurCam pc = sur_cam(c);
+    const uint32_t* pw = reinterpret_cast<const uint32_t*>(&pc);
+    for (int i = 0; i < 16; ++i) reinterpret_cast<uint32_t*>(cam_pre)[16 * k + i] = pw[i];
     uint32_t first = k;
     for (uint32_t j = 0; j < k; ++j)
         if (same_pose(cams.cam(j), c)) {
@@ -1080,7 +1126,8 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
                                                           uint32_t n_items, uint32_t dests,
                                                           uint32_t* __restrict__ part,
                                                           uint32_t cpb, uint32_t phase,
-                                                          const uint32_t* __restrict__ near) {
+                                                          const uint32_t* __restrict__ near,
+                                                          const float* __restrict__ cam_pre) {
     __shared__ uint32_t s_cnt[2 * kPushWords];
     uint32_t half = 0;
     __shared__ uint32_t s_st[4][4];
@@ -1099,7 +1146,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
         const uint32_t ci = w.ci;
         const MarchConst& c = cams.cam(ci);
         const int64_t pix = w.pix;
-        const RayPrep r = prep_ray(c, k, w);
+        const RayPrep r = prep_ray(c, load_sur_cam(cam_pre, ci), k, w);
         bool back = false;
         if (r.route == kNet) {
             const uint32_t ts = slot[ci];
@@ -1107,7 +1154,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
             tab_lookup(node + ts * nodes, tab_geom(range + 2 * ts, n2, n3), n2, n3, r.x, v);
             const float Phi = phi_lookup(phi + (int64_t)ts * kPhiNodes, range + 2 * ts, r.x[1]) +
                               (r.g.bm > 0.0f ? kResScale * v[0] : 0.0f);
-            const Dec o = decode_with(c, k, r.b_axis, r.lapse_g, r.g, Phi,
+            const Dec o = decode_with(c, r.margin, r.b_axis, r.lapse_g, r.g, Phi,
                                       [&](int q) { return v[1 + q]; });
             if (o.serve) {
                 store_dec(out, c.ray_offset + pix, o, r.g.b);
@@ -1366,16 +1413,18 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     };
     hipError_t e;
     if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
+    if (tab)   // x2 ranges, table slots and the per-camera setup terms
+        hipLaunchKernelGGL(k_tab_range<Cams>, dim3((ncam + 63) / 64), dim3(64), 0, s, cams, ncam,
+                           tab->range, tab->slot, tab->cam);
     if (routed) {
         hipLaunchKernelGGL(k_sur_pre<Cams>, grid_of(pre_per_cu), dim3(kBlock), 0, s, cams, k,
-                           n_items, sc, fl, pack_dests(kToBand, kToEarly, kNoList, kNoList));
+                           n_items, sc, fl, pack_dests(kToBand, kToEarly, kNoList, kNoList),
+                           tab ? (const float*)tab->cam : nullptr);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = side()) != hipSuccess) return e;
     }
     if (tab) {
         const int64_t nodes = (int64_t)tab->n2 * tab->n3;
-        hipLaunchKernelGGL(k_tab_range<Cams>, dim3((ncam + 63) / 64), dim3(64), 0, s, cams, ncam,
-                           tab->range, tab->slot);
         hipLaunchKernelGGL(k_tab_phi<Cams>,
                            dim3((uint32_t)(((int64_t)ncam * kPhiNodes + kBlock - 1) / kBlock)),
                            dim3(kBlock), 0, s, cams, ncam, (const float*)tab->range,
@@ -1395,7 +1444,8 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                                dim3(kBlock), 0, s, cams, k, (const float*)tab->node,
                                (const float*)tab->phi, (const float*)tab->range,
                                (const uint32_t*)tab->slot, tab->n2, tab->n3, out, sc, fl, n_items,
-                               dests, part, cpb, phase, (const uint32_t*)fl.near);
+                               dests, part, cpb, phase, (const uint32_t*)fl.near,
+                               (const float*)tab->cam);
             const uint32_t red_blocks = (tab_blocks + 4 * kBlock - 1) / (4 * kBlock);
             hipLaunchKernelGGL(k_stats_reduce, dim3(red_blocks ? red_blocks : 1u), dim3(kBlock),
                                0, s, (const uint32_t*)part, tab_blocks, sc.stats);
