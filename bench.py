@@ -177,7 +177,7 @@ def main(argv=None) -> int:
     ap.add_argument("--plain-only", action="store_true", help="time only the plain integrator")
     ap.add_argument("--per-ray", action="store_true",
                     help="surrogate with per-ray network inference instead of the table mode")
-    ap.add_argument("--table", default="1024x256",
+    ap.add_argument("--table", default="1024x128",
                     help="table-mode grid N2xN3 over (b/b_crit, psi) features")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default)")
     ap.add_argument("--tuning", default=None,

@@ -1114,33 +1114,27 @@ __global__ __launch_bounds__(kBlock) void k_chunk_class(const Cams cams, uint32_
 constexpr uint32_t kTabChunksMin = 4, kTabChunksMax = 16;
 
 // Surrogate render in table mode: as k_surrogate with the network replaced by the table lookup.
+// Block b takes chunks [b cpb, (b + 1) cpb); phase 0 every chunk, phase 1 / 2 only the
+// near-critical / other ones (flags `near`). dests: the lists of the unserved rays; cam_pre: the
+// per-camera setup terms; part: the blocks' statistics partials.
 template <typename Cams>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
-void k_surrogate_tab(const Cams cams, const SurConst k,
-                                                          const float* __restrict__ node,
-                                                          const float* __restrict__ phi,
-                                                          const float* __restrict__ range,
-                                                          const uint32_t* __restrict__ slot,
-                                                          int32_t n2, int32_t n3, const SoA out,
-                                                          Scratch sc, SurLists fl,
-                                                          uint32_t n_items, uint32_t dests,
-                                                          uint32_t* __restrict__ part,
-                                                          uint32_t cpb, uint32_t phase,
-                                                          const uint32_t* __restrict__ near,
-                                                          const float* __restrict__ cam_pre) {
+void k_surrogate_tab(const Cams cams, const SurConst k, const float* __restrict__ node,
+                     const float* __restrict__ phi, const float* __restrict__ range,
+                     const uint32_t* __restrict__ slot, int32_t n2, int32_t n3, const SoA out,
+                     Scratch sc, SurLists fl, uint32_t n_items, uint32_t dests,
+                     uint32_t* __restrict__ part, uint32_t cpb, uint32_t phase,
+                     const uint32_t* __restrict__ near, const float* __restrict__ cam_pre) {
     __shared__ uint32_t s_cnt[2 * kPushWords];
     uint32_t half = 0;
     __shared__ uint32_t s_st[4][4];
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
     const int64_t nodes = (int64_t)n2 * n3 * 4;
-    // part: cpb chunks per block, statistics partials; else persistent grid-stride
-    const bool own = part != nullptr;
-    const uint64_t q_first = own ? (uint64_t)blockIdx.x * cpb * kBlock
-                                 : (uint64_t)blockIdx.x * kBlock;
-    const uint64_t q_last = own ? q_first + (uint64_t)cpb * kBlock : (uint64_t)n_items;
+    // the block's cpb chunks; statistics to the block's partials
+    const uint64_t q_first = (uint64_t)blockIdx.x * cpb * kBlock;
+    const uint64_t q_last = q_first + (uint64_t)cpb * kBlock;
     const uint32_t q_end = (uint32_t)(q_last < n_items ? q_last : n_items);
-    const uint32_t q_step = own ? (uint32_t)kBlock : gridDim.x * kBlock;
-    for (uint32_t q0 = (uint32_t)q_first; q0 < q_end; q0 += q_step) {
+    for (uint32_t q0 = (uint32_t)q_first; q0 < q_end; q0 += kBlock) {
         if (phase != 0u && (near[q0 / kBlock] != 0u) != (phase == 1u)) continue;   // uniform
         const WaveItem w = wave_item_scan(cams, q0 + (threadIdx.x & ~63u), n_items);
         const uint32_t ci = w.ci;
@@ -1177,10 +1171,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k,
         half ^= kPushWords;
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
-    if (own)
-        store_block_stats(part, n_served, n_cross, n_esc, n_fb, s_st);
-    else
-        add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
+    store_block_stats(part, n_served, n_cross, n_esc, n_fb, s_st);
 }
 
 // Network outputs of explicit feature rows.
