@@ -701,6 +701,9 @@ __device__ __forceinline__ SurCam load_sur_cam(const float* pre, uint32_t k) {
 // of lawS_setup (geodesic.glsl:27-40), from hardware reciprocal / square-root approximations
 // (relative error ~1e-7). The tangential part of the initial coordinate velocity is the view
 // direction's, so the frame needs no velocity; |e1 x d| = |d - (d.e1) e1| is sin(alpha).
+__device__ __forceinline__ void sur_geom_dir(const MarchConst& c, const SurCam& pc,
+                                             const V3<float>& d, RayPrep& r);
+
 __device__ __forceinline__ void sur_geom(const MarchConst& c, const SurCam& pc, int64_t pix,
                                          uint32_t col, uint32_t row, RayPrep& r) {
     float px, py;
@@ -721,6 +724,12 @@ __device__ __forceinline__ void sur_geom(const MarchConst& c, const SurCam& pc, 
                 __builtin_fmaf(B[8], d0.z, __builtin_fmaf(B[5], d0.y, B[2] * d0.x))};
     const float id = __builtin_amdgcn_rsqf(dot3(d, d));
     d.x *= id; d.y *= id; d.z *= id;
+    sur_geom_dir(c, pc, d, r);
+}
+
+// The surrogate geometry of a ray of unit view direction d (fast setup).
+__device__ __forceinline__ void sur_geom_dir(const MarchConst& c, const SurCam& pc,
+                                             const V3<float>& d, RayPrep& r) {
     RayGeom& g = r.g;
     const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
     g.r = pc.r;
@@ -743,14 +752,15 @@ __device__ __forceinline__ void sur_geom(const MarchConst& c, const SurCam& pc, 
     r.lapse_g = pc.lapse_g;
 }
 
-// Rays with |beta - 1| below this get the integrator's exact setup: there the swept angle moves by
-// ~|db / b| / |beta - 1| rad, and the approximate setup's few 1e-7 would approach 1e-4 rad.
+// Rays with |beta - 1| below this take the integrator's own view direction (ray_dir, correctly
+// rounded) before the fast geometry: there the swept angle moves by ~|db / b| / |beta - 1| rad, and
+// the approximate direction's few 1e-7 would approach 1e-4 rad.
 constexpr float kExactBeta = 0.05f;
 
 // Classification of a ray; identical in the pre-pass and the network pass (same operations). A
 // ray of the FP64 band that the model serves is a network candidate; the other band rays are
-// marched in FP64. Band membership must agree with the integrator's exact setup: near-critical
-// rays (kExactBeta) and rays within 1e-5 b_crit of the band edge are classified by it.
+// marched in FP64. Band membership must agree with the integrator's exact setup: rays within
+// 1e-5 b_crit of the band edge are classified by it.
 __device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurCam& pc,
                                             const SurConst& k, const WaveItem& w) {
     const int64_t pix = w.pix;
@@ -762,14 +772,18 @@ __device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurCam& p
     if (pix < 0) return r;
     sur_geom(c, pc, pix, w.col, w.row, r);
     const bool band64 = c.precision == BHS_PREC_F32_BAND64;
-    if (fabsf(r.g.bm) < kExactBeta ||
-        (band64 && fabsf(fabsf(r.g.b - kBcritF) - c.band_abs) < 1e-5f * kBcritF)) {
-        const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    const bool edge = band64 && fabsf(fabsf(r.g.b - kBcritF) - c.band_abs) < 1e-5f * kBcritF;
+    if (edge || fabsf(r.g.bm) < kExactBeta) {
         const V3<float> d = ray_dir(c, pix);
-        const Setup<float> s = lawS_setup(P, d);
-        r.g = ray_geom(P, d, s);
-        r.b_axis = s.b_axis;
-        r.lapse_g = s.lapse_g;
+        if (edge) {
+            const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+            const Setup<float> s = lawS_setup(P, d);
+            r.g = ray_geom(P, d, s);
+            r.b_axis = s.b_axis;
+            r.lapse_g = s.lapse_g;
+        } else {
+            sur_geom_dir(c, pc, d, r);
+        }
     }
     r.band = band64 && __builtin_fabsf(r.g.b - kBcritF) < c.band_abs;
     r.route = r.band ? kBand : kEarly;
