@@ -127,8 +127,8 @@ __device__ __forceinline__ float atan2_ieee(float y, float x) {
 // blackhole.frag:68-72: px = fragCoord + jitter; ndc = px/res*2-1; dirCam = normalize(ndc.x tanH
 // aspect, ndc.y tanH, -1); dir = normalize(B dirCam). Row 0 is the bottom row. In hashed mode the
 // per-pixel u in [0,1)^2 (pcg keyed by seed, camera, sample, pixel) replaces the pixel centre.
-__device__ __forceinline__ V3<float> ray_dir(const MarchConst& c, int64_t pixel) {
-    const int32_t col = (int32_t)(pixel % c.width), row = (int32_t)(pixel / c.width);
+__device__ __forceinline__ V3<float> ray_dir_at(const MarchConst& c, int64_t pixel, int32_t col,
+                                                int32_t row) {
     float px, py;
     if (c.jitter_hashed) {
         const uint32_t pi2 = 2u * (uint32_t)pixel;
@@ -152,6 +152,11 @@ __device__ __forceinline__ V3<float> ray_dir(const MarchConst& c, int64_t pixel)
     const float i1 = 1.0f / __builtin_sqrtf(dot3(d, d));
     d.x *= i1; d.y *= i1; d.z *= i1;
     return d;
+}
+
+// As ray_dir_at for pixel = row * width + col.
+__device__ __forceinline__ V3<float> ray_dir(const MarchConst& c, int64_t pixel) {
+    return ray_dir_at(c, pixel, (int32_t)(pixel % c.width), (int32_t)(pixel / c.width));
 }
 
 // ---------------------------------------------------------------------------------- law S setup

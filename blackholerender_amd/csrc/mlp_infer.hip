@@ -774,7 +774,7 @@ __device__ __forceinline__ RayPrep prep_ray(const MarchConst& c, const SurCam& p
     const bool band64 = c.precision == BHS_PREC_F32_BAND64;
     const bool edge = band64 && fabsf(fabsf(r.g.b - kBcritF) - c.band_abs) < 1e-5f * kBcritF;
     if (edge || fabsf(r.g.bm) < kExactBeta) {
-        const V3<float> d = ray_dir(c, pix);
+        const V3<float> d = ray_dir_at(c, pix, (int32_t)w.col, (int32_t)w.row);
         if (edge) {
             const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
             const Setup<float> s = lawS_setup(P, d);
