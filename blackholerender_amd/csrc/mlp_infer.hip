@@ -1337,6 +1337,9 @@ hipError_t launch_validate(int64_t n, const float* pos3, const float* dir3, cons
 namespace {
 
 constexpr uint32_t kRoutedItems = 1u << 24;
+// Table mode: the two-phase pass wins from about four 1080p frames per call; the pre-pass up to
+// two (measured crossover).
+constexpr uint32_t kRoutedTableItems = 1u << 22;
 
 // Blocks of kBlock threads of a kernel that are resident together on one CU. Grid-stride kernels
 // launch that many per CU: a larger grid would leave its last blocks to run their whole share
@@ -1370,15 +1373,15 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     };
     const bool f64 = kOne && c0.precision == BHS_PREC_F64;
     // The fallback marches' time is set by their longest serial rays, which lie near the critical
-    // impact parameter. Latency mode (up to kRoutedItems work items): a pre-pass routes the rays
-    // first, so the side marches run beside the whole surrogate pass. Larger calls in table mode
-    // (phased): the pass takes the near-critical chunks first, then their FP64 band and FP32
+    // impact parameter. Latency mode (up to kRoutedItems work items, kRoutedTableItems in table
+    // mode): a pre-pass routes the rays first, so the side marches run beside the whole surrogate
+    // pass. Larger calls in table mode (phased): the pass takes the near-critical chunks first, then their FP64 band and FP32
     // fallback marches run on the side streams beside the pass over the other chunks, whose
     // fallback rays (short) follow it; the pass's short-lived blocks take the slots the side
     // marches leave. Larger calls in network mode: the network pass routes the rays itself and
     // the lists are marched after it. The FP32 lists run at two resident waves per SIMD (131k
     // lanes), which leaves room for the surrogate pass or the FP64 band.
-    const bool routed = !f64 && n_items <= kRoutedItems;
+    const bool routed = !f64 && n_items <= (tab ? kRoutedTableItems : kRoutedItems);
     const bool phased = tab && !f64 && !routed;
     const uint32_t to_band = pack_dests(kToBand, kToBand, kToBand, kToBand);
     const uint32_t d_late = pack_dests(kNoList, kNoList, kToLate, kToFb);   // lists built before
