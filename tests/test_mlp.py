@@ -362,3 +362,33 @@ def test_long_band_fallback_list_equals_plain(tracer):
     mm = bit_mismatches(b.numpy(), p.numpy())
     assert all(v == 0 for v in mm.values()), mm
     sur.close()
+
+
+@pytest.mark.gpu
+def test_large_single_camera_two_phase_equals_batch(tracer):
+    """A single-camera call above 2^22 work items runs the two-phase pass in its one-camera form;
+    a 2560x2048 camera alone must equal its slice of a batch with a second, small camera (the
+    camera-table form of the same pass) bit for bit."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd.surrogate import Surrogate
+    from helpers import bit_mismatches
+    sur = Surrogate.load(tracer, _model_path(MODEL_HIGH))
+    sur.set_table(1024, 128)
+    big = bhs.orbit_camera(30.0, 85.0, 60.0, 2560, 2048, jitter_hashed=1, seed=7, camera_id=0)
+    small = bhs.orbit_camera(40.0, 60.0, 50.0, 128, 96, jitter_hashed=1, seed=7, camera_id=1)
+    params = bhs.march_params()
+    print("[mlp] one-camera two-phase call 2560x2048", flush=True)
+    one = sur.render(big, params)
+    torch.cuda.synchronize()
+    st = tracer.last_stats()
+    assert st["rays"] == 2560 * 2048 and st["surrogate"] > 0.9 * st["rays"], st
+    O = one.numpy()
+    b = sur.render_batch([big, small], params)
+    torch.cuda.synchronize()
+    B = b.numpy()
+    off = tracer.batch_offsets([big, small])
+    part = {key: v[off[0]:off[1]] for key, v in B.items()}
+    mm = bit_mismatches(part, O)
+    assert all(v == 0 for v in mm.values()), mm
+    sur.close()
