@@ -19,6 +19,10 @@ rays (blackholerender_amd.workload model, from the kernel's own step/crossing/es
 average duration from HIP events recorded on its stream inside the timed region. traffic: HBM
 bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/traffic.json), if measured for the
 workload.
+footprint: the sky footprint post-pass (bhs_sky_footprint_batch: GLSL-quad derivatives of the
+escape direction and makeFootprint, shaders/blackhole.frag:297-306, shaders/sky.glsl:27-51) over
+one surrogate render of the step's frames, all output planes, timed by HIP events on its stream;
+HBM-bound, algorithmic bytes 82 per pixel.
 cpu_baseline: the CPU oracle's FP64 law-S march of whole frames of the same step (about one second
 of wall time) on this host's cores.
 --config c5: the full form instead (3840x2160, 8 spp, 4 poses, 700 steps, disk [6, 30]).
@@ -297,6 +301,7 @@ def main(argv=None) -> int:
                            surrogate=(f"table {tab[0]}x{tab[1]}" if tab else "per-ray network")),
             "roofline": roof, "plain": plain_rec, "surrogate": sur_rec,
         })
+        res["footprint"] = _footprint_leg(tracer, sur, cams, params, args.steps, args.warmup)
         if args.config == "c2" and len(cams) > 1 and not args.no_single:
             # the same workload one frame per call (latency-bound: the longest serial rays of a
             # frame, FP64 band and fallback marches, set its time)
@@ -325,6 +330,49 @@ def main(argv=None) -> int:
     print(json.dumps(res), flush=True)
     tracer.close()
     return 0
+
+
+# Sky footprint post-pass, algorithmic HBM bytes per pixel: reads flags (1 B) and the escape
+# direction (12 B); writes ddx, ddy, e1, e2, rinv (3 planes each), area, mu (17 floats) and the
+# fallback byte.
+FOOTPRINT_BYTES_PER_PIXEL = 13 + 17 * 4 + 1
+
+
+def _footprint_leg(tracer, sur, cams, params, steps, warmup):
+    """Times the sky footprint pass (all planes) over one surrogate render of the cameras: HIP
+    events on the launch stream around `steps` calls (each call also uploads the camera table of
+    a batch)."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd import workload as W
+    n = sum(c.width * c.height for c in cams)
+    stream = torch.cuda.current_stream()
+    rays = bhs.RayBuffers(n, "cuda:0")
+    if len(cams) == 1:
+        sur.render(cams[0], params, rays, stream)
+        fp = lambda: tracer.sky_footprint(cams[0], rays, fout, stream=stream)
+    else:
+        sur.render_batch(cams, params, rays, stream)
+        fp = lambda: tracer.sky_footprint_batch(cams, rays, fout, stream=stream)
+    fout = bhs.FootprintBuffers(n, "cuda:0")
+    for _ in range(max(warmup, 1)):
+        fp()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fp()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    gbs = n * FOOTPRINT_BYTES_PER_PIXEL / (ms * 1e-3) / 1e9
+    fallback = float(fout.fallback.float().mean())
+    del fout, rays
+    return {"kernel": "k_footprint (sky footprint post-pass, all planes)",
+            "call_ms": round(ms, 5), "Gpix_per_s": round(n / (ms * 1e-3) / 1e9, 2),
+            "bytes_per_pixel": FOOTPRINT_BYTES_PER_PIXEL,
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": W.PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": round(gbs / W.PEAK_HBM_GBS, 4)},
+            "fallback_fraction": round(fallback, 5)}
 
 
 def _legs(tracer, sur, cams, params, steps, warmup):

@@ -8,6 +8,9 @@ Reference interfaces mirrored:
   trace(...)        bhs::physics::trace (src/physics/Geodesic.h:107-140), same defaults and
                     TraceResult fields (status Escaped/Captured/MaxSteps, dir, steps, minR,
                     equatorCrossings); FP64 on the GPU.
+  Tracer.sky_footprint
+                    the sky footprint of shaders/blackhole.frag:297-306 and shaders/sky.glsl:27-51
+                    as a post-pass over the ray planes of a render.
   Tracer.render     the thin-disk trace pass of shaders/blackhole.frag:67-253 driven by
                     Renderer::tracePass/setTraceUniforms (src/Renderer.cpp:242-387), returning per-ray
                     geometry (class, steps, escape direction, disk crossings r/phi/g) instead of colour.
@@ -27,8 +30,8 @@ from ._capi import (CLASS_DISK, CLASS_ESCAPE, CLASS_HORIZON, DISK_NONE, DISK_THI
                     TRACE_MAXSTEPS)
 from .camera import PRESETS, Camera, look_basis, orbit_camera
 
-__all__ = ["Tracer", "BhsError", "TraceResult", "RayBuffers", "march_params", "trace",
-           "Camera", "orbit_camera", "look_basis", "PRESETS", "critical_impact_parameter",
+__all__ = ["Tracer", "BhsError", "TraceResult", "RayBuffers", "FootprintBuffers", "march_params",
+           "trace", "Camera", "orbit_camera", "look_basis", "PRESETS", "critical_impact_parameter",
            "CLASS_HORIZON", "CLASS_DISK", "CLASS_ESCAPE", "FLAG_MAXSTEPS", "FLAG_ESCAPED",
            "FLAG_BAND64", "FLAG_ROOT64", "FLAG_SURROGATE", "PREC_F32",
            "PREC_F32_BAND64", "PREC_F64", "DISK_NONE", "DISK_THIN", "TRACE_ESCAPED",
@@ -119,6 +122,41 @@ class RayBuffers:
                     steps=self.steps.cpu().numpy().view(np.uint16).astype(np.int32),
                     esc=self.esc.cpu().numpy().T.copy(), disk_r=d[0:2].T.copy(),
                     disk_phi=d[2:4].T.copy(), disk_g=d[4:6].T.copy(), b=self.b.cpu().numpy())
+
+
+class FootprintBuffers:
+    """Device sky footprint planes (bhs_footprint_soa) held as torch tensors; `planes` names the
+    planes to allocate (the others are passed as NULL and not written)."""
+
+    ALL = ("ddx", "ddy", "e1", "e2", "rinv", "area", "mu", "fallback")
+
+    def __init__(self, n: int, device, planes=ALL):
+        import torch
+        self.n = n
+        for k in self.ALL:
+            t = None
+            if k in planes:
+                if k in ("area", "mu"):
+                    t = torch.empty(n, dtype=torch.float32, device=device)
+                elif k == "fallback":
+                    t = torch.empty(n, dtype=torch.uint8, device=device)
+                else:
+                    t = torch.empty((3, n), dtype=torch.float32, device=device)
+            setattr(self, k, t)
+
+    def soa(self) -> _capi.FootprintSoA:
+        return _capi.FootprintSoA(*[getattr(self, k).data_ptr() if getattr(self, k) is not None
+                                    else None for k in self.ALL])
+
+    def numpy(self) -> dict:
+        """Host copies of the allocated planes (three-component planes as (n, 3))."""
+        out = {}
+        for k in self.ALL:
+            t = getattr(self, k)
+            if t is not None:
+                a = t.cpu().numpy()
+                out[k] = a.T.copy() if a.ndim == 2 else a
+        return out
 
 
 class Tracer:
@@ -222,6 +260,33 @@ class Tracer:
         rc = self._lib.bhs_render_batch(self._ctx, ca, pa, len(cams), C.byref(soa),
                                         C.c_void_p(self._stream_ptr(stream)))
         self._check(rc, "bhs_render_batch")
+        return out
+
+    def sky_footprint(self, cam: Camera, rays: RayBuffers, out: FootprintBuffers | None = None,
+                      coarse: bool = False, stream=None) -> FootprintBuffers:
+        """Sky footprint post-pass over the planes of a render of `cam` (bhs_sky_footprint):
+        derivatives of escDir over 2x2 quads (shaders/blackhole.frag:297-306) and makeFootprint
+        (shaders/sky.glsl:27-51)."""
+        if out is None:
+            out = FootprintBuffers(cam.width * cam.height, f"cuda:{self.device}")
+        cs = _camera_struct(cam)
+        rs, fs = rays.soa(), out.soa()
+        rc = self._lib.bhs_sky_footprint(self._ctx, C.byref(cs), C.byref(rs), int(bool(coarse)),
+                                         C.byref(fs), C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_sky_footprint")
+        return out
+
+    def sky_footprint_batch(self, cams, rays: RayBuffers, out: FootprintBuffers | None = None,
+                            coarse: bool = False, stream=None) -> FootprintBuffers:
+        """The footprint pass over the planes of a batch render (bhs_sky_footprint_batch)."""
+        n = len(cams)
+        if out is None:
+            out = FootprintBuffers(self.batch_offsets(cams)[-1], f"cuda:{self.device}")
+        ca = (_capi.Camera * n)(*[_camera_struct(c) for c in cams])
+        rs, fs = rays.soa(), out.soa()
+        rc = self._lib.bhs_sky_footprint_batch(self._ctx, ca, n, C.byref(rs), int(bool(coarse)),
+                                               C.byref(fs), C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_sky_footprint_batch")
         return out
 
     def graph(self, cams, params, out: RayBuffers | None = None) -> "BatchGraph":

@@ -48,6 +48,12 @@ class RaySoA(C.Structure):
                 ("b_dev", C.c_void_p)]
 
 
+class FootprintSoA(C.Structure):
+    _fields_ = [("ddx_dev", C.c_void_p), ("ddy_dev", C.c_void_p), ("e1_dev", C.c_void_p),
+                ("e2_dev", C.c_void_p), ("rinv_dev", C.c_void_p), ("area_dev", C.c_void_p),
+                ("mu_dev", C.c_void_p), ("fallback_dev", C.c_void_p)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("waves_per_simd", C.c_int32),
                 ("refill_min", C.c_int32)]
@@ -105,6 +111,10 @@ def load():
     L.bhs_set_timing.argtypes = [vp, C.c_int]
     L.bhs_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(i32)]
     L.bhs_render_batch.argtypes = [vp, vp, vp, i32, C.POINTER(RaySoA), vp]
+    L.bhs_sky_footprint.argtypes = [vp, C.POINTER(Camera), C.POINTER(RaySoA), i32,
+                                    C.POINTER(FootprintSoA), vp]
+    L.bhs_sky_footprint_batch.argtypes = [vp, vp, i32, C.POINTER(RaySoA), i32,
+                                          C.POINTER(FootprintSoA), vp]
     L.bhs_graph_create.argtypes = [vp, vp, vp, i32, C.POINTER(RaySoA), C.POINTER(vp)]
     L.bhs_graph_update.argtypes = [vp, vp, vp, vp]
     L.bhs_graph_launch.argtypes = [vp, vp, vp]

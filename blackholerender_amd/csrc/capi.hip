@@ -12,6 +12,7 @@
 
 #include "../../include/bhs_trace.h"
 #include "capi_internal.hpp"
+#include "footprint.hpp"
 #include "kernels.hpp"
 
 // A captured batch launch (bhs_graph_*): owns its camera table and scratch so later calls on the
@@ -480,6 +481,75 @@ int bhs_render_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_param
                                tr ? tr->k1 : nullptr)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_render_batch launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
+    ctx->last_stream = s;
+    return BHS_OK;
+}
+
+int bhs_sky_footprint(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* rays,
+                      int32_t coarse, const bhs_footprint_soa* out, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    int st;
+    if ((st = check_camera(ctx, cam)) != BHS_OK) return st;
+    if (!rays || !rays->flags_dev || !rays->esc_dev || !out)
+        return fail(ctx, BHS_E_INVALID, "bhs_sky_footprint: rays (flags, esc) or out is NULL%s");
+    DeviceGuard g(ctx->device);
+    bhs::MarchConst c;
+    std::memset(&c, 0, sizeof c);
+    fill_camera(c, cam);
+    c.ray_offset = 0;
+    const int64_t n = (int64_t)cam->width * cam->height;
+    const bhs::FootprintPlanes o{out->ddx_dev, out->ddy_dev,  out->e1_dev,       out->e2_dev,
+                                 out->rinv_dev, out->area_dev, out->mu_dev, out->fallback_dev, n};
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if ((e = bhs::launch_footprint(c, bhs::FootprintIn{rays->flags_dev, rays->esc_dev}, o,
+                                   coarse != 0, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_sky_footprint launch");
+    ctx->last_stream = s;
+    return BHS_OK;
+}
+
+int bhs_sky_footprint_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
+                            const bhs_ray_soa* rays, int32_t coarse,
+                            const bhs_footprint_soa* out, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!cams) return fail(ctx, BHS_E_INVALID, "bhs_sky_footprint_batch: cameras is NULL%s");
+    if (ncam < 1 || (uint32_t)ncam > bhs::kMaxBatchCams)
+        return fail(ctx, BHS_E_INVALID, "bhs_sky_footprint_batch: ncam must be in 1..64%s");
+    if (!rays || !rays->flags_dev || !rays->esc_dev || !out)
+        return fail(ctx, BHS_E_INVALID, "bhs_sky_footprint_batch: rays (flags, esc) or out is NULL%s");
+    BatchTable t;
+    t.cams.resize((size_t)ncam);
+    t.item_base.assign((size_t)ncam + 1, 0u);
+    int64_t max_quads = 0;
+    bool widths_even = true;
+    for (int32_t k = 0; k < ncam; ++k) {
+        int st;
+        if ((st = check_camera(ctx, &cams[k])) != BHS_OK) return st;
+        bhs::MarchConst c;
+        std::memset(&c, 0, sizeof c);
+        fill_camera(c, &cams[k]);
+        c.ray_offset = t.n_rays;
+        t.cams[(size_t)k] = c;
+        t.n_rays += (int64_t)cams[k].width * cams[k].height;
+        max_quads = std::max(max_quads, (int64_t)((cams[k].width + 1) / 2) *
+                                            ((cams[k].height + 1) / 2));
+        widths_even = widths_even && (cams[k].width & 1) == 0;
+    }
+    DeviceGuard g(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    void* table_dev = nullptr;
+    int st;
+    if ((st = upload_batch(ctx, t, s, &table_dev)) != BHS_OK) return st;
+    const bhs::FootprintPlanes o{out->ddx_dev, out->ddy_dev,  out->e1_dev,       out->e2_dev,
+                                 out->rinv_dev, out->area_dev, out->mu_dev, out->fallback_dev,
+                                 t.n_rays};
+    hipError_t e;
+    if ((e = bhs::launch_footprint_batch((const bhs::MarchConst*)table_dev, (uint32_t)ncam,
+                                         max_quads, widths_even,
+                                         bhs::FootprintIn{rays->flags_dev, rays->esc_dev}, o,
+                                         coarse != 0, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_sky_footprint_batch launch");
     ctx->last_stream = s;
     return BHS_OK;
 }

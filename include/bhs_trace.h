@@ -25,6 +25,11 @@
  *                               no bodies, no gas, disk opacity 0 (law S). The GLSL colour output is
  *                               replaced by per-ray geometry in structure-of-arrays form.
  *  - bhs_march_rays           : the same law-S march (blackhole.frag:77-253) on explicit rays.
+ *  - bhs_sky_footprint / bhs_sky_footprint_batch :
+ *                               the sky footprint of shaders/blackhole.frag:297-306 (GLSL
+ *                               dFdx/dFdy of escDir over 2x2 quads, the unlensed footprint at
+ *                               status edges) and makeFootprint, shaders/sky.glsl:17-51, as a
+ *                               post-pass over a render's ray planes.
  *  - bhs_mlp_*, bhs_render_surrogate : no counterpart in the reference (learned surrogate of the
  *                               march, falling back to it near b_crit).
  */
@@ -163,6 +168,35 @@ int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* para
  * camera table is uploaded with the launch (stream-ordered). */
 int bhs_render_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* params,
                      int32_t ncam, const bhs_ray_soa* out, void* stream);
+
+/* Sky footprint planes (per pixel, caller-owned device buffers; any may be NULL). Three-component
+ * fields are 3 planes of n floats, like esc_dev. */
+typedef struct bhs_footprint_soa {
+    float* ddx_dev;           /* 3n : dFdx of escDir; of the view direction where the quad falls back */
+    float* ddy_dev;           /* 3n : dFdy, likewise */
+    float* e1_dev;            /* 3n : Footprint.e1, orthonormal footprint basis (sky.glsl:19) */
+    float* e2_dev;            /* 3n : Footprint.e2 */
+    float* rinv_dev;          /* 3n : Footprint r11, r12, r22, the inverse R factor (sky.glsl:20) */
+    float* area_dev;          /* n  : la * lbp = sqrt(Footprint.det), the footprint solid angle */
+    float* mu_dev;            /* n  : Footprint.mu = min(uPixelOmega0 / area, 1e4) (sky.glsl:49) */
+    uint8_t* fallback_dev;    /* n  : 1 where the unlensed derivatives were used */
+} bhs_footprint_soa;
+
+/* Sky footprint post-pass over the ray planes of a bhs_render of `cam` (reads rays->flags_dev and
+ * rays->esc_dev, which must be set; n = width * height). Per pixel: the derivatives of escDir
+ * (its escape direction if BHS_FLAG_ESCAPED, else its view direction) across its 2x2 quad, or those
+ * of the view direction where the quad's pair mixes escaped and non-escaped pixels
+ * (blackhole.frag:297-306), and for escaped pixels makeFootprint(escDir, ddx, ddy, uPixelOmega0)
+ * (sky.glsl:27-51; zeros elsewhere). coarse: 0 fine derivatives (each pixel's own row and column
+ * pair), 1 coarse (the quad's first row and column). Pixels beyond an odd edge count as not
+ * escaped. Stream-ordered. */
+int bhs_sky_footprint(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* rays,
+                      int32_t coarse, const bhs_footprint_soa* out, void* stream);
+/* The same over the planes of a bhs_render_batch / bhs_render_surrogate_batch of ncam <= 64
+ * cameras (camera k at the prefix sum of width * height), in one launch. */
+int bhs_sky_footprint_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
+                            const bhs_ray_soa* rays, int32_t coarse,
+                            const bhs_footprint_soa* out, void* stream);
 
 /* The same batch captured once into a HIP graph and replayed: bhs_graph_create captures (and
  * instantiates) the whole launch sequence for these cameras, settings and output buffers with the

@@ -44,6 +44,11 @@ class LawSRay(C.Structure):
                 ("disk_phi", C.c_float * 2), ("disk_g", C.c_float * 2), ("b", C.c_float)]
 
 
+class Footprint(C.Structure):
+    _fields_ = [("e1", C.c_float * 3), ("e2", C.c_float * 3), ("rinv", C.c_float * 3),
+                ("area", C.c_float), ("mu", C.c_float)]
+
+
 LAWS_RAY_DTYPE = np.dtype([("cls", "<i4"), ("flags", "<i4"), ("steps", "<i4"), ("n_cross", "<i4"),
                            ("esc", "<f4", 3), ("disk_r", "<f4", 2), ("disk_phi", "<f4", 2),
                            ("disk_g", "<f4", 2), ("b", "<f4")])
@@ -98,6 +103,14 @@ def lib():
                                     C.c_int32]
         L.oracle_march_batch.argtypes = [fp, fp, C.c_int64, C.POINTER(LawSParams), C.c_int32,
                                          C.c_void_p, C.c_int32]
+        L.oracle_ray_dir_at.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                        C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
+                                        C.c_int64, fp]
+        L.oracle_make_footprint.argtypes = [fp, fp, fp, C.c_float, C.POINTER(Footprint)]
+        vp = C.c_void_p
+        L.oracle_sky_footprint.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                           C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, vp, vp,
+                                           vp, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_atan2f.argtypes = [C.c_float, C.c_float]
         L.oracle_atan2f.restype = C.c_float
         _lib = L
@@ -261,4 +274,62 @@ def render(cam, params: LawSParams, mode="f32", first=0, count=None, stride=1,
                         int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
                         int(cam.get("camera_id", 0)), int(cam.get("sample", 0)), C.byref(params),
                         _MODES[mode], first, count, stride, out.ctypes.data, threads)
+    return out
+
+
+def ray_dir_at(cam, col: int, row: int):
+    """Float view direction of pixel (col, row) of a camera dict; col/row may lie outside the image."""
+    p, pp = _f(cam["pos"], 3)
+    b, bp = _f(cam["basis"], 9)
+    j, jp = _f(cam.get("jitter", (0.0, 0.0)), 2)
+    o = _F3()
+    lib().oracle_ray_dir_at(pp, bp, cam["tan_half_fov"], cam["width"], cam["height"], jp,
+                            int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
+                            int(cam.get("camera_id", 0)), int(cam.get("sample", 0)),
+                            row * cam["width"] + col, col, row, o)
+    return np.array(o[:], dtype=np.float32)
+
+
+def make_footprint(d, a, b, omega0: float) -> dict:
+    """makeFootprint (shaders/sky.glsl:27-51) of one escape direction and its derivatives."""
+    f = Footprint()
+    _, dp_ = _f(d, 3)
+    _, ap = _f(a, 3)
+    _, bp = _f(b, 3)
+    lib().oracle_make_footprint(dp_, ap, bp, omega0, C.byref(f))
+    return dict(e1=np.array(f.e1[:], np.float32), e2=np.array(f.e2[:], np.float32),
+                rinv=np.array(f.rinv[:], np.float32), area=np.float32(f.area),
+                mu=np.float32(f.mu))
+
+
+def pixel_omega0(cam) -> np.float32:
+    """uPixelOmega0 = (2 tan(fov/2) / h)^2 in float (src/Renderer.cpp:246-247,331)."""
+    pa = np.float32(2.0) * np.float32(cam["tan_half_fov"]) / np.float32(cam["height"])
+    return np.float32(pa * pa)
+
+
+def sky_footprint(cam, flags, esc, coarse=False) -> dict:
+    """Footprint image of one camera (shaders/blackhole.frag:297-306 + shaders/sky.glsl:27-51).
+
+    flags: (n,) uint8 ray flags (bit 2 = escaped); esc: (n, 3) float escape directions. Returns
+    ddx, ddy, e1, e2, rinv as (n, 3) float32, area and mu as (n,) float32, fallback as (n,) uint8.
+    """
+    n = cam["width"] * cam["height"]
+    flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(n)
+    esc_planes = np.ascontiguousarray(np.asarray(esc, dtype=np.float32).reshape(n, 3).T)
+    out = {k: np.zeros((3, n), np.float32) for k in ("ddx", "ddy", "e1", "e2", "rinv")}
+    out["area"] = np.zeros(n, np.float32)
+    out["mu"] = np.zeros(n, np.float32)
+    out["fallback"] = np.zeros(n, np.uint8)
+    p, pp = _f(cam["pos"], 3)
+    b, bp = _f(cam["basis"], 9)
+    j, jp = _f(cam.get("jitter", (0.0, 0.0)), 2)
+    lib().oracle_sky_footprint(pp, bp, cam["tan_half_fov"], cam["width"], cam["height"], jp,
+                               int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
+                               int(cam.get("camera_id", 0)), int(cam.get("sample", 0)),
+                               int(bool(coarse)), flags.ctypes.data, esc_planes.ctypes.data,
+                               *(out[k].ctypes.data for k in ("ddx", "ddy", "e1", "e2", "rinv",
+                                                             "area", "mu", "fallback")))
+    for k in ("ddx", "ddy", "e1", "e2", "rinv"):
+        out[k] = np.ascontiguousarray(out[k].T)
     return out

@@ -354,8 +354,15 @@ static uint32_t jitter_base(uint64_t seed, uint32_t camera_id, uint32_t sample) 
 void oracle_ray_dir(const float pos[3], const float B[9], float tan_half_fov, int32_t width,
                     int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
                     uint32_t camera_id, uint32_t sample, int64_t pixel, float dir_out[3]) {
+    oracle_ray_dir_at(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
+                      sample, pixel, pixel % width, pixel / width, dir_out);
+}
+
+void oracle_ray_dir_at(const float pos[3], const float B[9], float tan_half_fov, int32_t width,
+                       int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                       uint32_t camera_id, uint32_t sample, int64_t pixel, int64_t col, int64_t row,
+                       float dir_out[3]) {
     (void)pos;
-    const int64_t col = pixel % width, row = pixel / width;
     float px, py;
     if (jitter_hashed) {
         const uint32_t base = jitter_base(seed, camera_id, sample);
@@ -377,6 +384,128 @@ void oracle_ray_dir(const float pos[3], const float B[9], float tan_half_fov, in
     for (int k = 0; k < 3; ++k) d[k] = fmaf(B[6 + k], c[2], fmaf(B[3 + k], c[1], B[k] * c[0]));
     const float id = 1.0f / sqrtf(fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0])));
     for (int k = 0; k < 3; ++k) dir_out[k] = d[k] * id;
+}
+
+/* ================================================================================================
+ * Sky footprint: shaders/blackhole.frag:297-306 (screen-space derivatives of the escape direction
+ * over 2x2 pixel quads, the unlensed footprint at status edges) and shaders/sky.glsl:27-51
+ * (makeFootprint: QR factorisation of the screen->sky Jacobian)
+ * ============================================================================================== */
+static float dot3f(const float a[3], const float b[3]) {
+    return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0]));
+}
+
+static void cross3f(const float a[3], const float b[3], float o[3]) {
+    o[0] = fmaf(a[1], b[2], -(a[2] * b[1]));
+    o[1] = fmaf(a[2], b[0], -(a[0] * b[2]));
+    o[2] = fmaf(a[0], b[1], -(a[1] * b[0]));
+}
+
+static void normalize3f(float v[3]) {
+    const float i = 1.0f / sqrtf(dot3f(v, v));
+    for (int k = 0; k < 3; ++k) v[k] *= i;
+}
+
+/* sky.glsl:27-51 for escape direction d and derivatives a (dFdx), b (dFdy). */
+void oracle_make_footprint(const float d[3], const float a[3], const float b[3], float omega0,
+                           oracle_footprint* f) {
+    const float min_axis = 1e-3f * sqrtf(omega0);                          /* :31 */
+    const float la_raw = sqrtf(dot3f(a, a));                               /* :32 */
+    const float la = fmaxf(la_raw, min_axis);                              /* :33 */
+    if (la_raw > 1e-12f) {                                                 /* :34 */
+        for (int k = 0; k < 3; ++k) f->e1[k] = a[k] / la_raw;
+    } else {
+        const float up[3] = {1e-3f, 1.0f + 1e-3f, 1e-3f};
+        cross3f(d, up, f->e1);
+        normalize3f(f->e1);
+    }
+    const float b1 = dot3f(b, f->e1);                                      /* :35 */
+    float bp[3];                                                           /* :36 */
+    for (int k = 0; k < 3; ++k) bp[k] = fmaf(-b1, f->e1[k], b[k]);
+    const float lbp_raw = sqrtf(dot3f(bp, bp));                            /* :37 */
+    const float lbp = fmaxf(lbp_raw, min_axis);                            /* :38 */
+    if (lbp_raw > 1e-3f * min_axis) {                                      /* :39 */
+        for (int k = 0; k < 3; ++k) f->e2[k] = bp[k] / lbp_raw;
+    } else {
+        cross3f(d, f->e1, f->e2);
+        normalize3f(f->e2);
+    }
+    f->rinv[0] = 1.0f / la;                                                /* :40-42 */
+    f->rinv[1] = -b1 / (la * lbp);
+    f->rinv[2] = 1.0f / lbp;
+    f->area = la * lbp;                                                    /* :47 */
+    f->mu = fminf(omega0 / f->area, 1e4f);                                 /* :49 */
+}
+
+/* Effective escDir (blackhole.frag:98: escDir starts as the view direction and is replaced only on
+ * escape), view direction and escape flag of pixel (col, row); pixels outside the image (a quad's
+ * missing lanes) count as not escaped. */
+static void fp_lane(const float pos[3], const float B[9], float tan_half_fov, int32_t width,
+                    int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                    uint32_t camera_id, uint32_t sample, const uint8_t* flags, const float* esc,
+                    int64_t n, int64_t col, int64_t row, float E[3], float D[3], float* s) {
+    const int64_t pixel = row * width + col;
+    oracle_ray_dir_at(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
+                      sample, pixel, col, row, D);
+    const int inside = col < width && row < height;
+    const int escaped = inside && (flags[pixel] & 2);
+    *s = escaped ? 1.0f : 0.0f;
+    for (int k = 0; k < 3; ++k) E[k] = escaped ? esc[(int64_t)k * n + pixel] : D[k];
+}
+
+void oracle_sky_footprint(const float pos[3], const float B[9], float tan_half_fov, int32_t width,
+                          int32_t height, const float jitter[2], int32_t jitter_hashed,
+                          uint64_t seed, uint32_t camera_id, uint32_t sample, int32_t coarse,
+                          const uint8_t* flags, const float* esc, float* ddx, float* ddy,
+                          float* e1, float* e2, float* rinv, float* area, float* mu,
+                          uint8_t* fallback) {
+    const int64_t n = (int64_t)width * height;
+    /* Renderer.cpp:246-247,331: pixelAngle = 2 tan(fov/2) / h, uPixelOmega0 = pixelAngle^2 */
+    const float pixel_angle = 2.0f * tan_half_fov / (float)height;
+    const float omega0 = pixel_angle * pixel_angle;
+    for (int64_t row = 0; row < height; ++row) {
+        for (int64_t col = 0; col < width; ++col) {
+            const int64_t x0 = col & ~(int64_t)1, y0 = row & ~(int64_t)1;
+            /* fine derivatives (GLSL dFdxFine / dFdyFine): the horizontal pair in this pixel's row
+             * and the vertical pair in its column; coarse: the quad's first row and column */
+            const int64_t ry = coarse ? y0 : row, cx = coarse ? x0 : col;
+            float Ea[3], Da[3], sa, Eb[3], Db[3], sb, Ec[3], Dc[3], sc_, Ed[3], Dd[3], sd;
+            fp_lane(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
+                    sample, flags, esc, n, x0, ry, Ea, Da, &sa);
+            fp_lane(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
+                    sample, flags, esc, n, x0 + 1, ry, Eb, Db, &sb);
+            fp_lane(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
+                    sample, flags, esc, n, cx, y0, Ec, Dc, &sc_);
+            fp_lane(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
+                    sample, flags, esc, n, cx, y0 + 1, Ed, Dd, &sd);
+            float dx[3], dy[3];
+            const int edge = (sb - sa) != 0.0f || (sd - sc_) != 0.0f;      /* :303-306 */
+            for (int k = 0; k < 3; ++k) {
+                dx[k] = edge ? Db[k] - Da[k] : Eb[k] - Ea[k];
+                dy[k] = edge ? Dd[k] - Dc[k] : Ed[k] - Ec[k];
+            }
+            const int64_t p = row * width + col;
+            for (int k = 0; k < 3; ++k) {
+                if (ddx) ddx[k * n + p] = dx[k];
+                if (ddy) ddy[k * n + p] = dy[k];
+            }
+            if (fallback) fallback[p] = (uint8_t)edge;
+            oracle_footprint f;
+            memset(&f, 0, sizeof f);
+            if (flags[p] & 2) {   /* :308: the sky (and its footprint) only for escaped rays */
+                float d[3];
+                for (int k = 0; k < 3; ++k) d[k] = esc[k * n + p];
+                oracle_make_footprint(d, dx, dy, omega0, &f);
+            }
+            for (int k = 0; k < 3; ++k) {
+                if (e1) e1[k * n + p] = f.e1[k];
+                if (e2) e2[k * n + p] = f.e2[k];
+                if (rinv) rinv[k * n + p] = f.rinv[k];
+            }
+            if (area) area[p] = f.area;
+            if (mu) mu[p] = f.mu;
+        }
+    }
 }
 
 /* ================================================================================================

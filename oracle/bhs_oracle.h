@@ -85,6 +85,31 @@ void oracle_ray_dir(const float pos[3], const float basis[9], float tan_half_fov
                     int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
                     uint32_t camera_id, uint32_t sample, int64_t pixel, float dir_out[3]);
 uint32_t oracle_pcg(uint32_t v);
+/* As oracle_ray_dir for an explicit column and row (which may lie outside the image; the hashed
+ * jitter then uses pixel = row * width + col). */
+void oracle_ray_dir_at(const float pos[3], const float basis[9], float tan_half_fov, int32_t width,
+                       int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                       uint32_t camera_id, uint32_t sample, int64_t pixel, int64_t col, int64_t row,
+                       float dir_out[3]);
+
+/* ---- sky footprint: shaders/blackhole.frag:297-306, shaders/sky.glsl:17-51 -------------------- */
+typedef struct oracle_footprint {
+    float e1[3], e2[3];   /* orthonormal footprint basis in sky space */
+    float rinv[3];        /* r11, r12, r22 of the inverse R factor */
+    float area;           /* la * lbp = sqrt(det) */
+    float mu;             /* min(omega0 / area, 1e4) */
+} oracle_footprint;
+void oracle_make_footprint(const float d[3], const float a[3], const float b[3], float omega0,
+                           oracle_footprint* f);
+/* Footprint image of one camera's ray planes (flags: n bytes, bit 2 = escaped; esc: 3 planes of n
+ * floats). Outputs are planes of n floats (ddx, ddy, e1, e2, rinv: 3 planes), any may be NULL;
+ * non-escaped pixels get derivatives and a zero footprint. coarse: 0 fine derivatives, 1 coarse. */
+void oracle_sky_footprint(const float pos[3], const float basis[9], float tan_half_fov,
+                          int32_t width, int32_t height, const float jitter[2],
+                          int32_t jitter_hashed, uint64_t seed, uint32_t camera_id,
+                          uint32_t sample, int32_t coarse, const uint8_t* flags, const float* esc,
+                          float* ddx, float* ddy, float* e1, float* e2, float* rinv, float* area,
+                          float* mu, uint8_t* fallback);
 
 /* Law S in float and in double (same operation order). Inputs are the float camera position and
  * float direction; the double march promotes them exactly. */
