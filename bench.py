@@ -301,7 +301,12 @@ def main(argv=None) -> int:
                            surrogate=(f"table {tab[0]}x{tab[1]}" if tab else "per-ray network")),
             "roofline": roof, "plain": plain_rec, "surrogate": sur_rec,
         })
-        res["footprint"] = _footprint_leg(tracer, sur, cams, params, args.steps, args.warmup)
+        res["footprint"] = _footprint_leg(tracer, sur, cams, params, args.steps, args.warmup,
+                                          cpu=not args.no_cpu)
+        fkey = f"footprint_{args.config}x{len(cams)}"
+        if tr and fkey in tr.get("workloads", {}):
+            res["footprint"]["roofline"]["traffic"] = tr["workloads"][fkey]["bytes_per_launch"]
+            res["footprint"]["roofline"]["traffic_source"] = tr["workloads"][fkey]["source"]
         if args.config == "c2" and len(cams) > 1 and not args.no_single:
             # the same workload one frame per call (latency-bound: the longest serial rays of a
             # frame, FP64 band and fallback marches, set its time)
@@ -338,10 +343,10 @@ def main(argv=None) -> int:
 FOOTPRINT_BYTES_PER_PIXEL = 13 + 17 * 4 + 1
 
 
-def _footprint_leg(tracer, sur, cams, params, steps, warmup):
+def _footprint_leg(tracer, sur, cams, params, steps, warmup, cpu=True):
     """Times the sky footprint pass (all planes) over one surrogate render of the cameras: HIP
     events on the launch stream around `steps` calls (each call also uploads the camera table of
-    a batch)."""
+    a batch). cpu: also the CPU oracle's pass over the first camera (one thread)."""
     import torch
     import blackholerender_amd as bhs
     from blackholerender_amd import workload as W
@@ -366,13 +371,25 @@ def _footprint_leg(tracer, sur, cams, params, steps, warmup):
     ms = e0.elapsed_time(e1) / steps
     gbs = n * FOOTPRINT_BYTES_PER_PIXEL / (ms * 1e-3) / 1e9
     fallback = float(fout.fallback.float().mean())
+    rec = {"kernel": "k_footprint (sky footprint post-pass, all planes)",
+           "call_ms": round(ms, 5), "Gpix_per_s": round(n / (ms * 1e-3) / 1e9, 2),
+           "bytes_per_pixel": FOOTPRINT_BYTES_PER_PIXEL,
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": W.PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": round(gbs / W.PEAK_HBM_GBS, 4)},
+           "fallback_fraction": round(fallback, 5)}
+    if cpu:
+        import oracle as O
+        cam0 = cams[0]
+        n0 = cam0.width * cam0.height
+        fl = rays.flags[:n0].cpu().numpy()
+        esc = rays.esc[:, :n0].cpu().numpy().T.copy()
+        t0 = time.perf_counter()
+        O.sky_footprint(cam0.as_dict(), fl, esc)
+        dt = time.perf_counter() - t0
+        rec["cpu_oracle"] = {"value": round(n0 / dt / 1e6, 2), "unit": "Mpix/s", "cores": 1,
+                             "kind": "port", "sample": f"one {cam0.width}x{cam0.height} frame"}
     del fout, rays
-    return {"kernel": "k_footprint (sky footprint post-pass, all planes)",
-            "call_ms": round(ms, 5), "Gpix_per_s": round(n / (ms * 1e-3) / 1e9, 2),
-            "bytes_per_pixel": FOOTPRINT_BYTES_PER_PIXEL,
-            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": W.PEAK_HBM_GBS,
-                         "unit": "GB/s", "frac": round(gbs / W.PEAK_HBM_GBS, 4)},
-            "fallback_fraction": round(fallback, 5)}
+    return rec
 
 
 def _legs(tracer, sur, cams, params, steps, warmup):
