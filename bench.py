@@ -23,6 +23,8 @@ footprint: the sky footprint post-pass (bhs_sky_footprint_batch: GLSL-quad deriv
 escape direction and makeFootprint, shaders/blackhole.frag:297-306, shaders/sky.glsl:27-51) over
 one surrogate render of the step's frames, all output planes, timed by HIP events on its stream;
 HBM-bound, algorithmic bytes 82 per pixel.
+gas_lens: the gas lens table (bhs_gas_lens_table, shaders/gas_lens.comp:11-44) at the reference's
+size, 1024 fan rays x 1024 swept angles, beside the oracle's build of it on one thread.
 cpu_baseline: the CPU oracle's FP64 law-S march of whole frames of the same step (about one second
 of wall time) on this host's cores.
 --config c5: the full form instead (3840x2160, 8 spp, 4 poses, 700 steps, disk [6, 30]).
@@ -303,6 +305,8 @@ def main(argv=None) -> int:
         })
         res["footprint"] = _footprint_leg(tracer, sur, cams, params, args.steps, args.warmup,
                                           cpu=not args.no_cpu)
+        if args.config == "c2":
+            res["gas_lens"] = _gas_lens_leg(tracer, args.steps, cpu=not args.no_cpu)
         fkey = f"footprint_{args.config}x{len(cams)}"
         if tr and fkey in tr.get("workloads", {}):
             res["footprint"]["roofline"]["traffic"] = tr["workloads"][fkey]["bytes_per_launch"]
@@ -389,6 +393,33 @@ def _footprint_leg(tracer, sur, cams, params, steps, warmup, cpu=True):
         rec["cpu_oracle"] = {"value": round(n0 / dt / 1e6, 2), "unit": "Mpix/s", "cores": 1,
                              "kind": "port", "sample": f"one {cam0.width}x{cam0.height} frame"}
     del fout, rays
+    return rec
+
+
+def _gas_lens_leg(tracer, steps, cpu=True, cam_r=30.0):
+    """Times the gas lens table at the reference's size (1024 rays x 1024 swept angles, camera at
+    r = 30 M): HIP events on the launch stream around `steps` builds; cpu: the oracle's build of
+    the same table on one thread."""
+    import torch
+    import blackholerender_amd as bhs
+    stream = torch.cuda.current_stream()
+    ac = bhs.shadow_angular_radius_f32(cam_r)
+    out = tracer.gas_lens_table(cam_r, ac, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        tracer.gas_lens_table(cam_r, ac, out=out, stream=stream)
+    e1.record(stream)
+    e1.synchronize()
+    rec = {"kernel": "k_gas_lens (1024-ray fan, 1024 swept angles, r_cam 30)",
+           "ms_per_table": round(e0.elapsed_time(e1) / steps, 4),
+           "bound": "latency of the longest fan ray (16 waves on 256 CUs)"}
+    if cpu:
+        import oracle as O
+        t0 = time.perf_counter()
+        O.gas_lens_table(cam_r, ac)
+        rec["cpu_oracle_ms_per_table"] = round((time.perf_counter() - t0) * 1e3, 2)
+        rec["cpu_cores"] = 1
     return rec
 
 

@@ -8,6 +8,8 @@ Reference interfaces mirrored:
   trace(...)        bhs::physics::trace (src/physics/Geodesic.h:107-140), same defaults and
                     TraceResult fields (status Escaped/Captured/MaxSteps, dir, steps, minR,
                     equatorCrossings); FP64 on the GPU.
+  Tracer.gas_lens_table
+                    the gas lens table of shaders/gas_lens.comp:11-44 (GasParticles' lens program).
   Tracer.sky_footprint
                     the sky footprint of shaders/blackhole.frag:297-306 and shaders/sky.glsl:27-51
                     as a post-pass over the ray planes of a render.
@@ -45,6 +47,16 @@ kIsco = 6.0
 def critical_impact_parameter() -> float:
     """b_crit = 3 sqrt(3) (src/physics/Geodesic.h:19)."""
     return 3.0 * np.sqrt(3.0)
+
+
+def shadow_angular_radius_f32(r: float) -> float:
+    """The shadow's angular radius for a static observer at r, in float as the host computes
+    uAlphaCrit (src/GasParticles.cpp:40-44)."""
+    f = np.float32
+    r = f(r)
+    s = min(f(1.0), f(f(5.19615242) / r) * f(np.sqrt(max(f(1.0) - f(2.0) / r, f(0.0)))))
+    a = f(np.arcsin(s))
+    return float(a if r >= f(3.0) else f(3.14159265) - a)
 
 
 class BhsError(RuntimeError):
@@ -287,6 +299,21 @@ class Tracer:
         rc = self._lib.bhs_sky_footprint_batch(self._ctx, ca, n, C.byref(rs), int(bool(coarse)),
                                                C.byref(fs), C.c_void_p(self._stream_ptr(stream)))
         self._check(rc, "bhs_sky_footprint_batch")
+        return out
+
+    def gas_lens_table(self, cam_r: float, alpha_crit: float | None = None, n_rays: int = 1024,
+                       n_phis: int = 1024, out=None, stream=None):
+        """Gas lens table (bhs_gas_lens_table; shaders/gas_lens.comp:11-44): an (n_phis, n_rays)
+        float32 tensor of log r per swept angle and fan ray. alpha_crit defaults to the shadow's
+        angular radius at cam_r in float (src/GasParticles.cpp:40-44), as GasParticles passes it."""
+        import torch
+        if alpha_crit is None:
+            alpha_crit = shadow_angular_radius_f32(cam_r)
+        if out is None:
+            out = torch.empty((n_phis, n_rays), dtype=torch.float32, device=f"cuda:{self.device}")
+        rc = self._lib.bhs_gas_lens_table(self._ctx, cam_r, alpha_crit, n_rays, n_phis,
+                                          out.data_ptr(), C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_gas_lens_table")
         return out
 
     def graph(self, cams, params, out: RayBuffers | None = None) -> "BatchGraph":

@@ -13,6 +13,7 @@
 #include "../../include/bhs_trace.h"
 #include "capi_internal.hpp"
 #include "footprint.hpp"
+#include "gas_lens.hpp"
 #include "kernels.hpp"
 
 // A captured batch launch (bhs_graph_*): owns its camera table and scratch so later calls on the
@@ -550,6 +551,26 @@ int bhs_sky_footprint_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
                                          bhs::FootprintIn{rays->flags_dev, rays->esc_dev}, o,
                                          coarse != 0, s)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_sky_footprint_batch launch");
+    ctx->last_stream = s;
+    return BHS_OK;
+}
+
+int bhs_gas_lens_table(bhs_ctx* ctx, float cam_r, float alpha_crit, int32_t n_rays,
+                       int32_t n_phis, float* table_dev, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!table_dev) return fail(ctx, BHS_E_INVALID, "bhs_gas_lens_table: table is NULL%s");
+    if (!std::isfinite(cam_r) || !(cam_r > 2.01f))
+        return fail(ctx, BHS_E_INVALID, "bhs_gas_lens_table: cam_r must be finite and > 2.01%s");
+    if (!std::isfinite(alpha_crit) || !(alpha_crit > 0.0f) || !(alpha_crit < 3.14159265f))
+        return fail(ctx, BHS_E_INVALID, "bhs_gas_lens_table: alpha_crit must lie in (0, pi)%s");
+    if (n_rays < 1 || n_phis < 2 || (int64_t)n_rays * n_phis > (int64_t)1 << 31)
+        return fail(ctx, BHS_E_INVALID, "bhs_gas_lens_table: need n_rays >= 1, n_phis >= 2%s");
+    DeviceGuard g(ctx->device);
+    const bhs::LensArgs a{cam_r, alpha_crit, BHS_LENS_RMAX, n_rays, n_phis, BHS_LENS_MAX_STEPS};
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if ((e = bhs::launch_gas_lens(a, table_dev, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_gas_lens_table launch");
     ctx->last_stream = s;
     return BHS_OK;
 }

@@ -25,6 +25,8 @@
  *                               no bodies, no gas, disk opacity 0 (law S). The GLSL colour output is
  *                               replaced by per-ray geometry in structure-of-arrays form.
  *  - bhs_march_rays           : the same law-S march (blackhole.frag:77-253) on explicit rays.
+ *  - bhs_gas_lens_table       : the gas lens table of shaders/gas_lens.comp:11-44 (GasParticles'
+ *                               lens program, src/GasParticles.cpp:164-166).
  *  - bhs_sky_footprint / bhs_sky_footprint_batch :
  *                               the sky footprint of shaders/blackhole.frag:297-306 (GLSL
  *                               dFdx/dFdy of escDir over 2x2 quads, the unlensed footprint at
@@ -197,6 +199,19 @@ int bhs_sky_footprint(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* ra
 int bhs_sky_footprint_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
                             const bhs_ray_soa* rays, int32_t coarse,
                             const bhs_footprint_soa* out, void* stream);
+
+/* Gas lens table (shaders/gas_lens.comp:11-44 with lensAlpha, shaders/gas_lens.glsl:13-23; built
+ * at src/GasParticles.cpp:164-166): n_rays fan rays (local angle alpha_i = lensAlpha((i + 0.5) /
+ * n_rays, alpha_crit) from the inward radial) leave a static camera at (cam_r, 0, 0) turning
+ * towards +y; each is marched with law S's float setup and RK4 at ds = (r < 8 ? 0.004 : 0.01) r for
+ * at most BHS_LENS_MAX_STEPS steps. table_dev: n_phis rows of n_rays floats; row j holds log r at
+ * swept angle j * 2 pi / (n_phis - 1) (linear in r between steps, at least log 2; log 2 once the
+ * ray fell in, log BHS_LENS_RMAX once it escaped). The reference's sizes are 1024 x 1024 and
+ * alpha_crit = shadowAngularRadius(cam_r). Stream-ordered. */
+#define BHS_LENS_RMAX 2.0e5f
+#define BHS_LENS_MAX_STEPS 20000
+int bhs_gas_lens_table(bhs_ctx* ctx, float cam_r, float alpha_crit, int32_t n_rays,
+                       int32_t n_phis, float* table_dev, void* stream);
 
 /* The same batch captured once into a HIP graph and replayed: bhs_graph_create captures (and
  * instantiates) the whole launch sequence for these cameras, settings and output buffers with the

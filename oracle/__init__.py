@@ -111,6 +111,8 @@ def lib():
         L.oracle_sky_footprint.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
                                            C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, vp, vp,
                                            vp, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_gas_lens_table.argtypes = [C.c_float, C.c_float, C.c_int32, C.c_int32,
+                                            C.c_int32, C.c_float, C.c_void_p]
         L.oracle_atan2f.argtypes = [C.c_float, C.c_float]
         L.oracle_atan2f.restype = C.c_float
         _lib = L
@@ -333,3 +335,12 @@ def sky_footprint(cam, flags, esc, coarse=False) -> dict:
     for k in ("ddx", "ddy", "e1", "e2", "rinv"):
         out[k] = np.ascontiguousarray(out[k].T)
     return out
+
+
+def gas_lens_table(cam_r: float, alpha_crit: float, n_rays=1024, n_phis=1024, max_steps=20000,
+                   r_max=2.0e5) -> np.ndarray:
+    """Gas lens table (shaders/gas_lens.comp:11-44): (n_phis, n_rays) float32 log r."""
+    t = np.zeros((n_phis, n_rays), np.float32)
+    lib().oracle_gas_lens_table(cam_r, alpha_crit, n_rays, n_phis, max_steps, r_max,
+                                t.ctypes.data)
+    return t

@@ -509,6 +509,71 @@ void oracle_sky_footprint(const float pos[3], const float B[9], float tan_half_f
 }
 
 /* ================================================================================================
+ * Gas lens table: shaders/gas_lens.comp:11-44 with lensAlpha (shaders/gas_lens.glsl:13-23). The
+ * march is law S's float setup and RK4 (lawS_setup_f32, rk4_f32 above); the swept angle uses the
+ * IEEE-only atan2 above; log, cos and sin are double-precision evaluations rounded to float; the
+ * fan exponent LENS_P = 3 is the exact cube t*t*t.
+ * ============================================================================================== */
+static float lens_alpha(float u, float ac) {                             /* gas_lens.glsl:19-23 */
+    const float pi = 3.14159265358979f;
+    const float uc = ac / pi;
+    if (u < uc) {
+        const float t = (uc - u) / uc;
+        return ac - ac * (t * t * t);
+    }
+    const float t = (u - uc) / (1.0f - uc);
+    return ac + (pi - ac) * (t * t * t);
+}
+
+static float logf_d(float x) { return (float)log((double)x); }
+
+void oracle_gas_lens_ray(float cam_r, float alpha_crit, int32_t n_rays, int32_t n_phis,
+                         int32_t max_steps, float r_max, int32_t i, float* table) {
+    const float a = lens_alpha(((float)i + 0.5f) / (float)n_rays, alpha_crit);     /* :14 */
+    const float P[3] = {cam_r, 0.0f, 0.0f};                                        /* :16 */
+    const float d[3] = {-(float)cos((double)a), (float)sin((double)a), 0.0f};
+    setup_f32 st;
+    lawS_setup_f32(P, d, &st);                                                     /* :17-19 */
+    float x[3] = {P[0], P[1], P[2]}, v[3] = {st.v[0], st.v[1], st.v[2]};
+    const float dphi_node = 6.28318530717959f / (float)(n_phis - 1);              /* :20 */
+    const float log_h = logf_d(2.0f);                                              /* :21 */
+    table[i] = logf_d(cam_r);                                                      /* :22 */
+    int32_t j = 1;
+    float phi = 0.0f, r = cam_r, r2 = dot3_f32(x, x);
+    for (int32_t s = 0; s < max_steps && j < n_phis; ++s) {                        /* :25 */
+        const float ds = (r < 8.0f ? 0.004f : 0.01f) * r;                          /* :27 */
+        const float x0[3] = {x[0], x[1], x[2]};
+        const float r0 = r;
+        const float vlen = sqrtf(dot3_f32(v, v));
+        rk4_f32(x, v, st.c3, ds / vlen, r2, r);                                    /* :30 */
+        r2 = dot3_f32(x, x);
+        r = sqrtf(r2);                                                             /* :31 */
+        const float cz = fmaf(x0[0], x[1], -(x0[1] * x[0]));
+        const float cd = fmaf(x0[1], x[1], x0[0] * x[0]);
+        const float dphi = oracle_atan2f(cz, cd);                                  /* :32 */
+        const float phi1 = phi + dphi;
+        for (; j < n_phis && (float)j * dphi_node <= phi1; ++j) {                  /* :34-37 */
+            const float den = dphi > 1e-12f ? dphi : 1e-12f;
+            float t = ((float)j * dphi_node - phi) / den;
+            t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+            const float lr = logf_d(fmaf(t, r - r0, r0));
+            table[(int64_t)j * n_rays + i] = lr > log_h ? lr : log_h;
+        }
+        phi = phi1;
+        const float xv = dot3_f32(x, v);
+        if (r < 2.0f + 0.01f || (r > r_max && xv > 0.0f)) break;                   /* :39 */
+    }
+    const float fill = r < 10.0f ? log_h : logf_d(r_max);                          /* :42-43 */
+    for (; j < n_phis; ++j) table[(int64_t)j * n_rays + i] = fill;
+}
+
+void oracle_gas_lens_table(float cam_r, float alpha_crit, int32_t n_rays, int32_t n_phis,
+                           int32_t max_steps, float r_max, float* table) {
+    for (int32_t i = 0; i < n_rays; ++i)
+        oracle_gas_lens_ray(cam_r, alpha_crit, n_rays, n_phis, max_steps, r_max, i, table);
+}
+
+/* ================================================================================================
  * Threaded batch drivers (CPU baseline and test reference)
  * ============================================================================================== */
 typedef struct render_job {

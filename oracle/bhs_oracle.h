@@ -142,6 +142,15 @@ void oracle_march_batch(const float* pos3, const float* dir3, int64_t n, const o
  * in any IEEE-754 float implementation that honours fmaf. */
 float oracle_atan2f(float y, float x);
 
+/* ---- gas lens table: shaders/gas_lens.comp:11-44, shaders/gas_lens.glsl:13-23 -------------------
+ * table: n_phis rows of n_rays floats; row j holds log r of every fan ray at swept angle
+ * j * 2 pi / (n_phis - 1) (log 2 once fallen in, log r_max once escaped). The reference's sizes:
+ * 1024 x 1024, max_steps 20000, r_max 2e5; alpha_crit = shadowAngularRadius(cam_r). */
+void oracle_gas_lens_ray(float cam_r, float alpha_crit, int32_t n_rays, int32_t n_phis,
+                         int32_t max_steps, float r_max, int32_t i, float* table);
+void oracle_gas_lens_table(float cam_r, float alpha_crit, int32_t n_rays, int32_t n_phis,
+                           int32_t max_steps, float r_max, float* table);
+
 #ifdef __cplusplus
 }
 #endif
