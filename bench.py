@@ -232,9 +232,9 @@ def main(argv=None) -> int:
         print(json.dumps({"copy_probe_bytes": x.numel() * 4, "dispatches": 3}), flush=True)
         return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (every ray is "
-              "independent; see DESIGN.md for how the frame would shard). Run with --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (every ray is "
+              "independent, so a frame shards by 8x8 tiles or whole cameras across GPUs, one "
+              "process each). Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import torch
