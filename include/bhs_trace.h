@@ -195,7 +195,10 @@ typedef struct bhs_footprint_soa {
 int bhs_sky_footprint(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* rays,
                       int32_t coarse, const bhs_footprint_soa* out, void* stream);
 /* The same over the planes of a bhs_render_batch / bhs_render_surrogate_batch of ncam <= 64
- * cameras (camera k at the prefix sum of width * height), in one launch. */
+ * cameras (camera k at the prefix sum of width * height), in one launch. Its cameras go through
+ * the context's camera-table buffer, which the batch renders use too: issue it on the stream of
+ * the render it follows (or after that render has completed), never beside a batch render of the
+ * same context on another stream. */
 int bhs_sky_footprint_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
                             const bhs_ray_soa* rays, int32_t coarse,
                             const bhs_footprint_soa* out, void* stream);
