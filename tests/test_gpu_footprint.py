@@ -4,7 +4,8 @@ The HIP pass runs on the ray planes of a HIP render; the oracle (oracle_sky_foot
 restatement of shaders/blackhole.frag:297-306 and shaders/sky.glsl:27-51) runs on host copies of
 the same planes. Bar: bit-exact for every plane (both sides follow one IEEE operation order, with
 correctly rounded divides and square roots). Cases: the 256x256 reference camera, hashed jitter at
-odd sizes (missing quad partners), 1x1 and 2x1 images, fine and coarse derivatives, a batch of
+odd sizes (missing quad partners; an even width with an odd height takes the paired path's
+missing-row branch), 1x1 and 2x1 images, fine and coarse derivatives, a batch of
 cameras of different sizes, NULL planes, and the planes of a surrogate render.
 """
 import numpy as np
@@ -24,7 +25,8 @@ def _check(g, ref, what):
 
 
 @pytest.mark.parametrize("coarse", [False, True])
-@pytest.mark.parametrize("w,h,hashed", [(256, 256, 0), (37, 29, 1), (1, 1, 0), (2, 1, 1)])
+@pytest.mark.parametrize("w,h,hashed", [(256, 256, 0), (37, 29, 1), (38, 29, 1), (1, 1, 0),
+                                         (2, 1, 1)])
 def test_footprint_bit_exact(tracer, coarse, w, h, hashed):
     import torch
     import blackholerender_amd as bhs
