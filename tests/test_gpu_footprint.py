@@ -103,3 +103,24 @@ def test_footprint_rejects_missing_planes(tracer):
     rs = rays.soa()
     assert tracer._lib.bhs_sky_footprint_batch(tracer._ctx, C.byref(cs), 0, C.byref(rs), 0,
                                                C.byref(fs), None) == -1
+
+
+@pytest.mark.parametrize("coarse", [False, True])
+def test_footprint_batch_paired_path_equals_oracle(tracer, coarse):
+    """Every width even: the batch takes the paired-access kernel (the one the 1080p batches use);
+    one camera has an odd height."""
+    import torch
+    import blackholerender_amd as bhs
+    cams = [bhs.orbit_camera(30.0, 85.0, 60.0, 96, 54, jitter_hashed=1, seed=9, camera_id=0),
+            bhs.orbit_camera(15.0, 60.0, 50.0, 34, 17, jitter_hashed=1, seed=9, camera_id=1),
+            bhs.orbit_camera(30.0, 85.0, 60.0, 96, 54, jitter_hashed=1, seed=9, camera_id=2)]
+    print(f"[footprint] paired batch coarse={coarse}", flush=True)
+    rays = tracer.render_batch(cams, bhs.march_params())
+    fp = tracer.sky_footprint_batch(cams, rays, coarse=coarse)
+    torch.cuda.synchronize()
+    r, g = rays.numpy(), fp.numpy()
+    offs = tracer.batch_offsets(cams)
+    for k, cam in enumerate(cams):
+        s = slice(offs[k], offs[k + 1])
+        ref = O.sky_footprint(cam.as_dict(), r["flags"][s], r["esc"][s], coarse=coarse)
+        _check({p: v[s] for p, v in g.items()}, ref, f"camera {k}")
