@@ -21,8 +21,9 @@
 //     (blackhole.frag:308); other pixels get zeros there and their derivatives.
 //
 // Roofline: HBM. Per pixel it reads 13 B (flags, escape direction) and writes up to 69 B (17
-// float planes and the fallback byte); the arithmetic (four view directions and four footprints
-// per quad, ~40 flops a pixel with three divides and three square roots) is far below the ridge.
+// float planes and the fallback byte); the arithmetic per pixel (a view direction: three divides,
+// two square roots; a footprint: eight divides, three square roots; ~60 further operations) stays
+// below the memory time (the measured traffic equals the algorithmic bytes).
 
 #include "footprint.hpp"
 
