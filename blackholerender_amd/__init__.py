@@ -103,6 +103,40 @@ def _camera_struct(cam: Camera) -> _capi.Camera:
     return c
 
 
+def _check_layout(t, dtype: str, shape: tuple, what: str):
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what}: expected a torch tensor, got {type(t).__name__}")
+    if str(t.dtype) != f"torch.{dtype}":
+        raise ValueError(f"{what}: dtype {t.dtype}, expected torch.{dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+
+
+def _check_place(t, device: int, what: str):
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: tensor is not contiguous")
+    if t.device.type != "cuda" or t.device.index != device:
+        raise ValueError(f"{what}: tensor on {t.device}, expected cuda:{device}")
+
+
+def _check_tensor(t, dtype: str, shape: tuple, device: int, what: str):
+    """Raises ValueError unless t is a contiguous tensor of this dtype and shape on cuda:device."""
+    _check_layout(t, dtype, shape, what)
+    _check_place(t, device, what)
+
+
+def _check_rays_like(buf, layout, n: int, device: int, what: str):
+    if getattr(buf, "n", None) != n:
+        raise ValueError(f"{what}: buffers hold {getattr(buf, 'n', None)} rays, the call needs {n}")
+    present = [(name, getattr(buf, name), dtype, planes) for name, dtype, planes in layout
+               if getattr(buf, name) is not None]
+    for name, t, dtype, planes in present:
+        _check_layout(t, dtype, (planes, n) if planes > 1 else (n,), f"{what}.{name}")
+    for name, t, _, _ in present:
+        _check_place(t, device, f"{what}.{name}")
+
+
 class RayBuffers:
     """Device structure-of-arrays output (bhs_ray_soa) held as torch tensors."""
 
@@ -116,6 +150,16 @@ class RayBuffers:
         self.esc = torch.empty((3, n), dtype=torch.float32, device=device)
         self.disk = torch.empty((6, n), dtype=torch.float32, device=device)
         self.b = torch.empty(n, dtype=torch.float32, device=device)
+
+    _LAYOUT = (("cls", "uint8", 1), ("flags", "uint8", 1), ("n_cross", "uint8", 1),
+               ("steps", "int16", 1), ("esc", "float32", 3), ("disk", "float32", 6),
+               ("b", "float32", 1))
+
+    def check(self, n: int, device: int, what: str):
+        """Raises ValueError unless these buffers hold exactly n rays, every plane a contiguous
+        tensor of its dtype on cuda:device (the C side takes the plane stride from the camera
+        sizes, so any other n would read or write out of bounds)."""
+        _check_rays_like(self, self._LAYOUT, n, device, what)
 
     def soa(self) -> _capi.RaySoA:
         return _capi.RaySoA(self.cls.data_ptr(), self.flags.data_ptr(), self.n_cross.data_ptr(),
@@ -155,6 +199,13 @@ class FootprintBuffers:
                 else:
                     t = torch.empty((3, n), dtype=torch.float32, device=device)
             setattr(self, k, t)
+
+    def check(self, n: int, device: int, what: str):
+        """Raises ValueError unless the allocated planes hold exactly n pixels each, contiguous
+        tensors of their dtypes on cuda:device."""
+        layout = [(k, "float32" if k not in ("fallback",) else "uint8",
+                   1 if k in ("area", "mu", "fallback") else 3) for k in self.ALL]
+        _check_rays_like(self, layout, n, device, what)
 
     def soa(self) -> _capi.FootprintSoA:
         return _capi.FootprintSoA(*[getattr(self, k).data_ptr() if getattr(self, k) is not None
@@ -235,6 +286,7 @@ class Tracer:
             params = march_params()
         if out is None:
             out = RayBuffers(cam.width * cam.height, f"cuda:{self.device}")
+        out.check(cam.width * cam.height, self.device, "render: out")
         cs = _camera_struct(cam)
         soa = out.soa()
         rc = self._lib.bhs_render(self._ctx, C.byref(cs), C.byref(params), C.byref(soa),
@@ -268,6 +320,7 @@ class Tracer:
         ca, pa = self._batch_structs(cams, params)
         if out is None:
             out = RayBuffers(self.batch_offsets(cams)[-1], f"cuda:{self.device}")
+        out.check(self.batch_offsets(cams)[-1], self.device, "render_batch: out")
         soa = out.soa()
         rc = self._lib.bhs_render_batch(self._ctx, ca, pa, len(cams), C.byref(soa),
                                         C.c_void_p(self._stream_ptr(stream)))
@@ -279,8 +332,11 @@ class Tracer:
         """Sky footprint post-pass over the planes of a render of `cam` (bhs_sky_footprint):
         derivatives of escDir over 2x2 quads (shaders/blackhole.frag:297-306) and makeFootprint
         (shaders/sky.glsl:27-51)."""
+        n = cam.width * cam.height
+        rays.check(n, self.device, "sky_footprint: rays")
         if out is None:
-            out = FootprintBuffers(cam.width * cam.height, f"cuda:{self.device}")
+            out = FootprintBuffers(n, f"cuda:{self.device}")
+        out.check(n, self.device, "sky_footprint: out")
         cs = _camera_struct(cam)
         rs, fs = rays.soa(), out.soa()
         rc = self._lib.bhs_sky_footprint(self._ctx, C.byref(cs), C.byref(rs), int(bool(coarse)),
@@ -292,8 +348,11 @@ class Tracer:
                             coarse: bool = False, stream=None) -> FootprintBuffers:
         """The footprint pass over the planes of a batch render (bhs_sky_footprint_batch)."""
         n = len(cams)
+        total = self.batch_offsets(cams)[-1]
+        rays.check(total, self.device, "sky_footprint_batch: rays")
         if out is None:
-            out = FootprintBuffers(self.batch_offsets(cams)[-1], f"cuda:{self.device}")
+            out = FootprintBuffers(total, f"cuda:{self.device}")
+        out.check(total, self.device, "sky_footprint_batch: out")
         ca = (_capi.Camera * n)(*[_camera_struct(c) for c in cams])
         rs, fs = rays.soa(), out.soa()
         rc = self._lib.bhs_sky_footprint_batch(self._ctx, ca, n, C.byref(rs), int(bool(coarse)),
@@ -311,6 +370,7 @@ class Tracer:
             alpha_crit = shadow_angular_radius_f32(cam_r)
         if out is None:
             out = torch.empty((n_phis, n_rays), dtype=torch.float32, device=f"cuda:{self.device}")
+        _check_tensor(out, "float32", (n_phis, n_rays), self.device, "gas_lens_table: out")
         rc = self._lib.bhs_gas_lens_table(self._ctx, cam_r, alpha_crit, n_rays, n_phis,
                                           out.data_ptr(), C.c_void_p(self._stream_ptr(stream)))
         self._check(rc, "bhs_gas_lens_table")
@@ -328,11 +388,14 @@ class Tracer:
             params = march_params()
         pos3 = pos3.contiguous()
         dir3 = dir3.contiguous()
-        if pos3.dtype != torch.float32 or dir3.dtype != torch.float32 or pos3.shape != dir3.shape:
-            raise ValueError("pos3/dir3 must be float32 tensors of equal shape (n,3)")
+        if pos3.dim() != 2 or pos3.shape[1] != 3:
+            raise ValueError("march_rays: pos3 must have shape (n, 3)")
         n = pos3.shape[0]
+        _check_tensor(pos3, "float32", (n, 3), self.device, "march_rays: pos3")
+        _check_tensor(dir3, "float32", (n, 3), self.device, "march_rays: dir3")
         if out is None:
             out = RayBuffers(n, pos3.device)
+        out.check(n, self.device, "march_rays: out")
         soa = out.soa()
         rc = self._lib.bhs_march_rays(self._ctx, pos3.data_ptr(), dir3.data_ptr(), n,
                                       C.byref(params), C.byref(soa),
@@ -397,6 +460,7 @@ class BatchGraph:
         ca, pa = tracer._batch_structs(self.cams, params)
         self.out = out if out is not None else RayBuffers(tracer.batch_offsets(self.cams)[-1],
                                                           f"cuda:{tracer.device}")
+        self.out.check(tracer.batch_offsets(self.cams)[-1], tracer.device, "graph: out")
         self._soa = self.out.soa()
         h = C.c_void_p()
         rc = tracer._lib.bhs_graph_create(tracer._ctx, ca, pa, len(self.cams), C.byref(self._soa),

@@ -177,6 +177,15 @@ bhs_ctx::TimingRec* next_timer(bhs_ctx* ctx) {
     return &ctx->timers[ctx->timers_used++];
 }
 
+int order_stream(bhs_ctx* ctx, hipStream_t s) {
+    if (!ctx->has_last || ctx->last_stream == s) return BHS_OK;
+    hipError_t e;
+    if ((e = hipEventRecord(ctx->ev_order, ctx->last_stream)) != hipSuccess ||
+        (e = hipStreamWaitEvent(s, ctx->ev_order, 0)) != hipSuccess)
+        return hip_fail(ctx, e, "stream ordering");
+    return BHS_OK;
+}
+
 // Render scratch for n_rays rays and n_items work items: [Stats | band count | work counter]
 // (kStatsBytes, cleared per launch) [band ids] [root states, 2 per ray].
 size_t scratch_bytes(int64_t n_rays, int64_t n_items, bool roots) {
@@ -317,7 +326,8 @@ bhs_ctx* bhs_create(int device, int* status) {
         hipEventCreateWithFlags(&ctx->ev_join2, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_upload, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_order, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
         bhs_destroy(ctx);
         if (status) *status = BHS_E_DEVICE;
@@ -345,6 +355,7 @@ void bhs_destroy(bhs_ctx* ctx) {
         if (ctx->ev_upload) (void)hipEventDestroy(ctx->ev_upload);
         if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
         if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+        if (ctx->ev_order) (void)hipEventDestroy(ctx->ev_order);
         if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
         if (ctx->ev_fork2) (void)hipEventDestroy(ctx->ev_fork2);
         if (ctx->ev_join2) (void)hipEventDestroy(ctx->ev_join2);
@@ -414,6 +425,7 @@ int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* para
         BHS_OK)
         return st;
     hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
     hipError_t e;
     bhs_ctx::TimingRec* tr = next_timer(ctx);
     if (tr) (void)hipEventRecord(tr->call0, s);
@@ -424,7 +436,7 @@ int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* para
                                 tr ? tr->k1 : nullptr)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_render launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
-    ctx->last_stream = s;
+    note_stream(ctx, s);
     return BHS_OK;
 }
 
@@ -439,10 +451,11 @@ int bhs_march_rays(bhs_ctx* ctx, const float* pos3_dev, const float* dir3_dev, i
     bhs::Scratch sc;
     if ((st = prepare_scratch(ctx, 0, 0, false, &sc)) != BHS_OK) return st;
     hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
     hipError_t e;
     if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_march_rays clear");
-    ctx->last_stream = s;
+    note_stream(ctx, s);
     if (n == 0) return BHS_OK;
     bhs::MarchConst c = make_const(params);
     bhs_ctx::TimingRec* tr = next_timer(ctx);
@@ -465,6 +478,7 @@ int bhs_render_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_param
         return fail(ctx, BHS_E_INVALID, "bhs_render_batch: root64 needs n_cross, flags and disk outputs%s");
     DeviceGuard g(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
     void* table_dev = nullptr;
     if ((st = upload_batch(ctx, t, s, &table_dev)) != BHS_OK) return st;
     bhs::Scratch sc;
@@ -482,7 +496,7 @@ int bhs_render_batch(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_param
                                tr ? tr->k1 : nullptr)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_render_batch launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
-    ctx->last_stream = s;
+    note_stream(ctx, s);
     return BHS_OK;
 }
 
@@ -502,11 +516,12 @@ int bhs_sky_footprint(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* ra
     const bhs::FootprintPlanes o{out->ddx_dev, out->ddy_dev,  out->e1_dev,       out->e2_dev,
                                  out->rinv_dev, out->area_dev, out->mu_dev, out->fallback_dev, n};
     hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
     hipError_t e;
     if ((e = bhs::launch_footprint(c, bhs::FootprintIn{rays->flags_dev, rays->esc_dev}, o,
                                    coarse != 0, s)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_sky_footprint launch");
-    ctx->last_stream = s;
+    note_stream(ctx, s);
     return BHS_OK;
 }
 
@@ -539,6 +554,7 @@ int bhs_sky_footprint_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
     }
     DeviceGuard g(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
     void* table_dev = nullptr;
     int st;
     if ((st = upload_batch(ctx, t, s, &table_dev)) != BHS_OK) return st;
@@ -551,7 +567,7 @@ int bhs_sky_footprint_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
                                          bhs::FootprintIn{rays->flags_dev, rays->esc_dev}, o,
                                          coarse != 0, s)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_sky_footprint_batch launch");
-    ctx->last_stream = s;
+    note_stream(ctx, s);
     return BHS_OK;
 }
 
@@ -568,10 +584,11 @@ int bhs_gas_lens_table(bhs_ctx* ctx, float cam_r, float alpha_crit, int32_t n_ra
     DeviceGuard g(ctx->device);
     const bhs::LensArgs a{cam_r, alpha_crit, BHS_LENS_RMAX, n_rays, n_phis, BHS_LENS_MAX_STEPS};
     hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
     hipError_t e;
     if ((e = bhs::launch_gas_lens(a, table_dev, s)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_gas_lens_table launch");
-    ctx->last_stream = s;
+    note_stream(ctx, s);
     return BHS_OK;
 }
 

@@ -17,7 +17,11 @@ struct bhs_ctx {
     // second side stream (surrogate render: early FP32 fallback march beside the FP64 band)
     hipStream_t aux2 = nullptr;
     hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
+    // Stream of the context's most recent stream-ordered call (has_last: one was made). A call on
+    // another stream first waits for it (ev_order): scratch, camera table and lists are shared.
     hipStream_t last_stream = nullptr;
+    bool has_last = false;
+    hipEvent_t ev_order = nullptr;
     // Render scratch: [Stats | band_count (pad 256) | band_ids | roots]
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
@@ -71,6 +75,14 @@ void fill_camera(bhs::MarchConst& c, const bhs_camera* cam);
 bhs::SoA make_soa(const bhs_ray_soa* o, int64_t n);
 bhs_ctx::TimingRec* next_timer(bhs_ctx* ctx);
 int prepare_scratch(bhs_ctx* ctx, int64_t n_rays, int64_t n_items, bool roots, bhs::Scratch* sc);
+// Orders a call on stream s after the context's previous stream-ordered call when that one was
+// issued on another stream (event on the old stream, wait on s).
+int order_stream(bhs_ctx* ctx, hipStream_t s);
+// Records s as the stream of the context's latest call.
+inline void note_stream(bhs_ctx* ctx, hipStream_t s) {
+    ctx->last_stream = s;
+    ctx->has_last = true;
+}
 
 // Host-side camera table of a batch: constants, output offsets (prefix sums of width*height) and
 // work-item prefix sums (camera_items per camera).

@@ -295,6 +295,7 @@ int bhs_mlp_make_dataset(bhs_ctx* ctx, bhs_mlp* m, int64_t n_rays, uint32_t seed
         return hip_fail(ctx, e, "bhs_mlp_make_dataset copy");
     m->n_data = n_rays;
     ctx->last_stream = nullptr;
+    ctx->has_last = false;
     return BHS_OK;
 }
 
@@ -411,12 +412,13 @@ int bhs_mlp_forward(bhs_ctx* ctx, bhs_mlp* m, const float* x_dev, int64_t n, flo
         return fail(ctx, BHS_E_INVALID, "bhs_mlp_forward: NULL buffer or n < 0%s");
     DeviceGuard g(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
     int st;
     if ((st = pack(ctx, m, s)) != BHS_OK) return st;
     if (n == 0) return BHS_OK;
     hipError_t e = M::launch_forward(x_dev, n, m->packed, y_dev, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "bhs_mlp_forward");
-    ctx->last_stream = s;
+    note_stream(ctx, s);
     return BHS_OK;
 }
 
@@ -540,6 +542,7 @@ int bhs_mlp_validate(bhs_ctx* ctx, bhs_mlp* m, int64_t n_rays, uint32_t seed, fl
     m->tol_phi = tol_phi;
     if (out_host) *out_host = v;
     ctx->last_stream = nullptr;
+    ctx->has_last = false;
     return BHS_OK;
 }
 
@@ -697,6 +700,7 @@ int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
     M::SurTable tab{};
     if ((st = table_for(ctx, m, 1, &tab)) != BHS_OK) return st;
     hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
     if ((st = pack(ctx, m, s)) != BHS_OK) return st;
     hipError_t e;
     bhs_ctx::TimingRec* tr = next_timer(ctx);
@@ -710,7 +714,7 @@ int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
         hipSuccess)
         return hip_fail(ctx, e, "bhs_render_surrogate launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
-    ctx->last_stream = s;
+    note_stream(ctx, s);
     return BHS_OK;
 }
 
@@ -735,6 +739,7 @@ int bhs_render_surrogate_batch(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cams,
                     "bhs_render_surrogate_batch: root64 needs n_cross, flags and disk outputs%s");
     DeviceGuard g(ctx->device);
     hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
     void* table_dev = nullptr;
     if ((st = upload_batch(ctx, t, s, &table_dev)) != BHS_OK) return st;
     bhs::Scratch sc;
@@ -761,7 +766,7 @@ int bhs_render_surrogate_batch(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cams,
                                        tr ? tr->k1 : nullptr, nullptr)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_render_surrogate_batch launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
-    ctx->last_stream = s;
+    note_stream(ctx, s);
     return BHS_OK;
 }
 

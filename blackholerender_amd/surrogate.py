@@ -146,6 +146,7 @@ class Surrogate:
         n = sum(c.width * c.height for c in cams)
         if out is None:
             out = RayBuffers(n, f"cuda:{t.device}")
+        out.check(n, t.device, "surrogate render_batch: out")
         soa = out.soa()
         self._check(self._lib.bhs_render_surrogate_batch(
             t._ctx, self._m, C.addressof(cs), C.addressof(ps), len(cams), C.byref(soa),
@@ -161,6 +162,7 @@ class Surrogate:
             params = march_params()
         if out is None:
             out = RayBuffers(cam.width * cam.height, f"cuda:{t.device}")
+        out.check(cam.width * cam.height, t.device, "surrogate render: out")
         cs = _camera_struct(cam)
         soa = out.soa()
         self._check(self._lib.bhs_render_surrogate(t._ctx, self._m, C.byref(cs), C.byref(params),

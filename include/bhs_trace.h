@@ -13,6 +13,9 @@
  *    struct arguments. The context owns its streams, scratch buffers, graphs and MLP weights.
  *  - bhs_trace_rays is synchronous (like the reference's trace()). Everything taking a `stream`
  *    argument is stream-ordered and asynchronous; `stream` is a hipStream_t (NULL = the null stream).
+ *  - The calls of one context stay in issue order across streams: a call on another stream than
+ *    the context's previous stream-ordered call first makes its stream wait for that call, since
+ *    the context's scratch, camera table and fallback lists are shared by all of them.
  *  - A context may be used by one host thread at a time.
  *
  * Reference interfaces replaced (paths relative to the reference repository):
@@ -195,10 +198,7 @@ typedef struct bhs_footprint_soa {
 int bhs_sky_footprint(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* rays,
                       int32_t coarse, const bhs_footprint_soa* out, void* stream);
 /* The same over the planes of a bhs_render_batch / bhs_render_surrogate_batch of ncam <= 64
- * cameras (camera k at the prefix sum of width * height), in one launch. Its cameras go through
- * the context's camera-table buffer, which the batch renders use too: issue it on the stream of
- * the render it follows (or after that render has completed), never beside a batch render of the
- * same context on another stream. */
+ * cameras (camera k at the prefix sum of width * height), in one launch. */
 int bhs_sky_footprint_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
                             const bhs_ray_soa* rays, int32_t coarse,
                             const bhs_footprint_soa* out, void* stream);

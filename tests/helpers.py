@@ -62,3 +62,26 @@ def oracle_params_for(cfg, root64=0, band_rel=0.02):
     return oracle.lawS_params(max_steps=cfg["max_steps"], step_scale=ss, disk_step=dstep,
                               disk_inner=cfg["disk_inner"], disk_outer=cfg["disk_outer"],
                               disk_thickness=0.035, disk_mode=1, root64=root64, band_rel=band_rel)
+
+
+def oracle_threads() -> int:
+    """Worker threads for the CPU oracle: the CPUs this process may run on, at most 16."""
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def gather_rays(out, idx) -> dict:
+    """Rays idx (a 1-D int64 index array) of device RayBuffers, gathered on the device and copied
+    to the host in the field layout of RayBuffers.numpy()."""
+    import torch
+    i = torch.as_tensor(np.asarray(idx, np.int64), device=out.cls.device)
+    d = out.disk[:, i].cpu().numpy()
+    return dict(cls=out.cls[i].cpu().numpy(), flags=out.flags[i].cpu().numpy(),
+                n_cross=out.n_cross[i].cpu().numpy(),
+                steps=out.steps[i].cpu().numpy().view(np.uint16).astype(np.int32),
+                esc=out.esc[:, i].cpu().numpy().T.copy(), disk_r=d[0:2].T.copy(),
+                disk_phi=d[2:4].T.copy(), disk_g=d[4:6].T.copy(), b=out.b[i].cpu().numpy())

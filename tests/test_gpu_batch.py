@@ -128,6 +128,45 @@ def test_graph_replays_equal_eager(tracer):
         gr.close()
 
 
+def _planes_equal(a, b):
+    """Per plane of two RayBuffers, whether they are equal bit for bit (compared on the device)."""
+    import torch
+    return {k: bool(torch.equal(getattr(a, k).view(torch.uint8) if getattr(a, k).dtype !=
+                                torch.uint8 else getattr(a, k),
+                                getattr(b, k).view(torch.uint8) if getattr(b, k).dtype !=
+                                torch.uint8 else getattr(b, k)))
+            for k in ("cls", "flags", "n_cross", "steps", "esc", "disk", "b")}
+
+
+def test_c3_graph_replays_equal_eager(tracer):
+    """The 64-camera batch itself captured in one HIP graph: three replays, with eager work and
+    synchronisations between them, each equal to the eager batch bit for bit (every ray of every
+    camera), and the replay's counters equal to the eager call's."""
+    import torch
+    from blackholerender_amd import workload as W
+    cams, params = W.batch_c3()
+    print("[batch] 64-camera graph", flush=True)
+    eager = tracer.render_batch(cams, params)
+    torch.cuda.synchronize()
+    est = tracer.last_stats()
+    gr = tracer.graph(cams, params)
+    try:
+        for rep in range(3):
+            gr.out.cls.fill_(7)
+            gr.out.disk.fill_(-1.0)
+            gr.out.steps.fill_(-1)
+            torch.cuda.synchronize()
+            gr.launch()
+            torch.cuda.synchronize()
+            eq = _planes_equal(gr.out, eager)
+            assert all(eq.values()), (rep, eq)
+            gst = gr.stats()
+            for k in ("rays", "steps", "crossings", "escapes", "band64"):
+                assert gst[k] == est[k], (rep, k, gst[k], est[k])
+    finally:
+        gr.close()
+
+
 def test_batch_rejects_bad_arguments(tracer):
     import blackholerender_amd as bhs
     cams, params = _mixed_batch(bhs)

@@ -124,3 +124,27 @@ def test_footprint_batch_paired_path_equals_oracle(tracer, coarse):
         s = slice(offs[k], offs[k + 1])
         ref = O.sky_footprint(cam.as_dict(), r["flags"][s], r["esc"][s], coarse=coarse)
         _check({p: v[s] for p, v in g.items()}, ref, f"camera {k}")
+
+
+def test_footprint_1080p_single_and_batch(tracer):
+    """The benchmarked shape: 1920x1080 frames of the 1080p benchmark (the paired kernel of even
+    widths and its int64 plane offsets), one camera and a two-camera batch, every plane
+    bit-identical to the oracle."""
+    import torch
+    from blackholerender_amd import workload as W
+    cams = [W.camera_for(W.C2, sample=k) for k in range(2)]
+    params = W.params_for(W.C2)
+    print("[footprint] 1920x1080 single + 2-camera batch", flush=True)
+    one = tracer.render(cams[0], params)
+    fp1 = tracer.sky_footprint(cams[0], one)
+    rays = tracer.render_batch(cams, params)
+    fpb = tracer.sky_footprint_batch(cams, rays)
+    torch.cuda.synchronize()
+    r1 = one.numpy()
+    _check(fp1.numpy(), O.sky_footprint(cams[0].as_dict(), r1["flags"], r1["esc"]), "1080p")
+    r, g = rays.numpy(), fpb.numpy()
+    n = 1920 * 1080
+    for k in range(2):
+        ref = O.sky_footprint(cams[k].as_dict(), r["flags"][k * n:(k + 1) * n],
+                              r["esc"][k * n:(k + 1) * n])
+        _check({p: v[k * n:(k + 1) * n] for p, v in g.items()}, ref, f"1080p batch camera {k}")
