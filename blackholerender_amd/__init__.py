@@ -294,6 +294,23 @@ class Tracer:
         self._check(rc, "bhs_render")
         return out
 
+    def render_noalloc(self, cam: Camera, params: _capi.MarchParams | None = None,
+                       out: RayBuffers | None = None, stream=None) -> RayBuffers:
+        """bhs_render_noalloc: render() whose scratch is the library's static device array (no
+        device allocation in the call; renders that need more than BHS_NOALLOC_SCRATCH_BYTES of
+        scratch are rejected). `out` must be given: the call allocates nothing."""
+        if params is None:
+            params = march_params()
+        if out is None:
+            raise ValueError("render_noalloc: out must be given (the call allocates nothing)")
+        out.check(cam.width * cam.height, self.device, "render_noalloc: out")
+        cs = _camera_struct(cam)
+        soa = out.soa()
+        rc = self._lib.bhs_render_noalloc(self._ctx, C.byref(cs), C.byref(params), C.byref(soa),
+                                          C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_render_noalloc")
+        return out
+
     @staticmethod
     def _batch_structs(cams, params):
         n = len(cams)

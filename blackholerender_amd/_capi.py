@@ -106,6 +106,8 @@ def load():
     L.bhs_sync.argtypes = [vp]
     L.bhs_trace_rays.argtypes = [vp, vp, vp, i64, i32, f64, f64, f64, vp]
     L.bhs_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams), C.POINTER(RaySoA), vp]
+    L.bhs_render_noalloc.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams),
+                                     C.POINTER(RaySoA), vp]
     L.bhs_march_rays.argtypes = [vp, vp, vp, i64, C.POINTER(MarchParams), C.POINTER(RaySoA), vp]
     L.bhs_last_stats.argtypes = [vp, C.POINTER(Stats)]
     L.bhs_set_timing.argtypes = [vp, C.c_int]
@@ -151,7 +153,8 @@ def load():
                  "bhs_mlp_get_info", "bhs_mlp_save", "bhs_mlp_set_table", "bhs_render_surrogate",
                  "bhs_render_surrogate_batch"):
         getattr(L, name).restype = C.c_int
-    for name in ("bhs_sync", "bhs_trace_rays", "bhs_render", "bhs_march_rays", "bhs_last_stats",
+    for name in ("bhs_sync", "bhs_trace_rays", "bhs_render", "bhs_render_noalloc",
+                 "bhs_march_rays", "bhs_last_stats",
                  "bhs_set_timing", "bhs_timing", "bhs_set_tuning", "bhs_get_tuning",
                  "bhs_autotune", "bhs_render_batch", "bhs_graph_create", "bhs_graph_update",
                  "bhs_graph_launch", "bhs_graph_stats"):

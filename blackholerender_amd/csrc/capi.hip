@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -211,6 +212,7 @@ int prepare_scratch(bhs_ctx* ctx, int64_t n_rays, int64_t n_items, bool roots, b
                     "render scratch");
     if (st != BHS_OK) return st;
     layout_scratch(ctx->scratch, n_items, roots, sc);
+    ctx->stats_src = nullptr;
     return BHS_OK;
 }
 
@@ -295,6 +297,37 @@ int upload_batch(bhs_ctx* ctx, const BatchTable& t, hipStream_t s, void** table_
 }  // namespace capi
 }  // namespace bhs
 
+// ------------------------------------------------------------------ static scratch (no-alloc render)
+// Scratch of bhs_render_noalloc: stats and counters, the FP64 band list and the root states of one
+// render, in a device array of fixed size instead of memory the context allocates. Each GPU holds
+// its own copy at its own address; every no-alloc call on that GPU shares it and the array keeps
+// the last call's contents (the counters are cleared by a kernel at the start of each call).
+constexpr size_t kNoAllocScratchBytes = BHS_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};   // device address of `scratch`, per device
+
+// Device address of `scratch` on `device` (which must be current), resolved once per device. The
+// array's name is a host-side stand-in: only this address may reach kernels and copies.
+int resolve_static_scratch(int device, void** ptr) {
+    if (device < 0 || device >= kMaxDevices) return BHS_E_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return BHS_E_DEVICE;
+        }
+        g_scratch_dev[device] = p;
+    }
+    *ptr = g_scratch_dev[device];
+    return BHS_OK;
+}
+}  // namespace
+
 using namespace bhs::capi;
 
 extern "C" {
@@ -327,7 +360,8 @@ bhs_ctx* bhs_create(int device, int* status) {
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_upload, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_order, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->ev_order, hipEventDisableTiming) != hipSuccess ||
+        resolve_static_scratch(device, &ctx->static_scratch) != BHS_OK) {
         (void)hipGetLastError();
         bhs_destroy(ctx);
         if (status) *status = BHS_E_DEVICE;
@@ -437,6 +471,48 @@ int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* para
         return hip_fail(ctx, e, "bhs_render launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
     note_stream(ctx, s);
+    return BHS_OK;
+}
+
+int bhs_render_noalloc(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
+                       const bhs_ray_soa* out, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    int st;
+    if ((st = check_camera(ctx, cam)) != BHS_OK) return st;
+    if ((st = check_params(ctx, params)) != BHS_OK) return st;
+    if (!out) return fail(ctx, BHS_E_INVALID, "bhs_render_noalloc: out is NULL%s");
+    if (params->root64 && (!out->n_cross_dev || !out->flags_dev || !out->disk_dev))
+        return fail(ctx, BHS_E_INVALID,
+                    "bhs_render_noalloc: root64 needs n_cross, flags and disk outputs%s");
+    const int64_t n = (int64_t)cam->width * cam->height;
+    const uint32_t n_items = bhs::camera_items(cam->width, cam->height);
+    const bool roots = params->root64 && params->precision != BHS_PREC_F64;
+    const size_t need = scratch_bytes(n, n_items, roots);
+    if (need > kNoAllocScratchBytes) {
+        char what[96];
+        std::snprintf(what, sizeof what, "%zu bytes", need);
+        return fail(ctx, BHS_E_INVALID,
+                    "bhs_render_noalloc: this render needs %s of scratch, more than the 48 MiB "
+                    "static maximum (4 bytes per work item, 56 more per ray with root64)", what);
+    }
+    if (!ctx->static_scratch)
+        return fail(ctx, BHS_E_STATE, "bhs_render_noalloc: static scratch not resolved%s");
+    DeviceGuard g(ctx->device);
+    bhs::MarchConst c = make_const(params);
+    fill_camera(c, cam);
+    c.ray_offset = 0;
+    bhs::Scratch sc;
+    layout_scratch(ctx->static_scratch, n_items, roots, &sc);
+    hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
+    hipError_t e;
+    if ((e = bhs::launch_zero(ctx->static_scratch, kStatsBytes, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_render_noalloc clear");
+    if ((e = bhs::launch_render(c, make_soa(out, n), sc, ctx->tuning, ctx->num_cus, s, ctx->aux,
+                                ctx->ev_fork, ctx->ev_join, nullptr, nullptr)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_render_noalloc launch");
+    note_stream(ctx, s);
+    ctx->stats_src = ctx->static_scratch;
     return BHS_OK;
 }
 
@@ -855,12 +931,13 @@ int bhs_timing(bhs_ctx* ctx, double* march_ms_avg, double* total_ms_avg, int32_t
 int bhs_last_stats(bhs_ctx* ctx, bhs_stats* out_host) {
     if (!ctx || !out_host) return BHS_E_INVALID;
     std::memset(out_host, 0, sizeof *out_host);
-    if (!ctx->scratch) return BHS_OK;
+    const void* src = ctx->stats_src ? ctx->stats_src : ctx->scratch;
+    if (!src) return BHS_OK;
     DeviceGuard g(ctx->device);
     hipError_t e = hipStreamSynchronize(ctx->last_stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "bhs_last_stats sync");
     bhs::Stats s;
-    if ((e = hipMemcpy(&s, ctx->scratch, sizeof s, hipMemcpyDeviceToHost)) != hipSuccess)
+    if ((e = hipMemcpy(&s, src, sizeof s, hipMemcpyDeviceToHost)) != hipSuccess)
         return hip_fail(ctx, e, "bhs_last_stats copy");
     out_host->rays = s.rays;
     out_host->steps = s.steps;

@@ -25,6 +25,10 @@ struct bhs_ctx {
     // Render scratch: [Stats | band_count (pad 256) | band_ids | roots]
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
+    // Device address of the library's static scratch array on this device (bhs_render_noalloc),
+    // and where bhs_last_stats reads the latest call's counters (null: `scratch`).
+    void* static_scratch = nullptr;
+    const void* stats_src = nullptr;
     // Law-T staging (pos3, dir3, results).
     void* tstage = nullptr;
     size_t tstage_bytes = 0;

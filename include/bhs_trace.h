@@ -167,6 +167,21 @@ int bhs_trace_rays(bhs_ctx* ctx, const double* pos3_host, const double* dir3_hos
 int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
                const bhs_ray_soa* out, void* stream);
 
+/* bhs_render that allocates nothing: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and results as bhs_render, but the render's scratch (stats
+ * and counters, the FP64 band list, FP64 root states with root64) lives in a static device array
+ * of the library, BHS_NOALLOC_SCRATCH_BYTES per GPU (held by each GPU the library runs on until
+ * the process ends), instead of memory the context allocates. A render needs 256 bytes plus 4 per
+ * work item (its image padded to whole 8x8 tiles) plus, with root64, 56 per ray; larger renders
+ * are rejected with BHS_E_INVALID (a 3840x2160 frame fits without root64). Every no-alloc call on
+ * a GPU shares that GPU's array: calls that overlap in time on one GPU (other streams or other
+ * contexts) race and give wrong results. bhs_last_stats reads the counters of the latest call.
+ * Stream-ordered; timing (bhs_set_timing) does not record it. */
+#define BHS_NOALLOC_SCRATCH_BYTES (48u << 20)
+int bhs_render_noalloc(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
+                       const bhs_ray_soa* out, void* stream);
+
 /* Law-S render of up to 64 cameras (each with its own size and march settings, FP32 or FP32 with
  * the FP64 band) in one persistent launch: camera k's rays occupy out[offset_k, offset_k + w_k h_k)
  * with offset_k = sum of w_j h_j over j < k, each camera in the pixel order of bhs_render. The
