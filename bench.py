@@ -25,9 +25,19 @@ one surrogate render of the step's frames, all output planes, timed by HIP event
 HBM-bound, algorithmic bytes 82 per pixel.
 gas_lens: the gas lens table (bhs_gas_lens_table, shaders/gas_lens.comp:11-44) at the reference's
 size, 1024 fan rays x 1024 swept angles, beside the oracle's build of it on one thread.
-cpu_baseline: the CPU oracle's FP64 law-S march of whole frames of the same step (about one second
-of wall time) on this host's cores.
---config c5: the full form instead (3840x2160, 8 spp, 4 poses, 700 steps, disk [6, 30]).
+surrogate.roofline: the surrogate pass (table build + two k_surrogate_tab dispatches) against the
+HBM peak (45-byte records written) and the VALU issue peak (wave-instructions per launch from
+rocprofv3, profiles/valu.json).
+per_ray: the surrogate with per-ray bf16 MFMA network inference on one 1080p frame per call, with
+its MFMA fraction.
+full_form: the full form (3840x2160, 8 jittered spp, 4 poses = 32 cameras, 265,420,800 rays per call,
+700 steps, disk [6, 30]) through both legs, with its own accuracy block.
+batch64: the 64-camera batch captured in one HIP graph and replayed per step, beside the eager call
+and the plain march kernel's roofline on it.
+cpu_baseline: the CPU paths built for this host (law T, the reference's trace() at its defaults,
+and law S in FP64) on the 256x256 camera and a 1080p frame, all CPUs of this job and one thread,
+best of 5 runs.
+--config c5: the full form as the headline instead.
 """
 from __future__ import annotations
 
@@ -52,49 +62,120 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def host_threads() -> int:
+def host_cpus() -> dict:
+    """CPUs this process may use: its affinity set, capped by the cgroup CPU quota when one is set
+    (the host's share for this job), and at most 16, the CPU share of one GPU on the benchmark
+    hosts; plus nproc and the CPU model for the record."""
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, int(os.environ.get("BHS_CPU_THREADS", "16"))))
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+            if q != "max":
+                quota = max(1, int(int(q) / int(per)))
+    except (OSError, ValueError):
+        pass
+    use = min(aff, quota) if quota else aff
+    return {"threads": max(1, min(use, 16)), "affinity": aff, "cgroup_cpus": quota,
+            "nproc": os.cpu_count(), "cpu": cpu_model()}
 
 
-def run_cpu_baseline(cfg, cams, target_seconds: float = 1.0, max_seconds: float = 20.0) -> dict:
-    """FP64 law-S march of the workload's rays by the CPU oracle: whole frames of the step, in
-    order, until target_seconds of wall time (about 16 CPU-seconds on 16 threads); a strided
-    sample of the first frame if one frame would take longer than max_seconds."""
+def _best_of(fn, runs):
+    """Warm-up call, then the best and median wall time of `runs` calls."""
+    import statistics
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return min(ts), statistics.median(ts)
+
+
+def _stride_for(n, per_ray_s, target_s):
+    """Pixel stride that makes one timed run over a camera last about target_s."""
+    import math
+    return max(1, math.ceil(n * per_ray_s / target_s))
+
+
+def run_cpu_baseline(runs: int = 5, target_s: float = 0.3) -> dict:
+    """The CPU paths on the 256x256 camera and on one 1920x1080 benchmark frame (camera 0 of the
+    step), each on all CPUs of this process's share and on one thread, best of `runs` timed runs
+    after a warm-up, each run about target_s of wall time (a strided sample of the camera's pixels
+    when the whole camera would take longer):
+      law_T: the reference's CPU path, bhs::physics::trace (src/physics/Geodesic.h:107-140) at its
+             own defaults (maxSteps 200000, stepScale 0.005, escTol 1e-4, rEscMin 30), FP64, on
+             the camera's rays (the oracle's restatement, oracle_trace_T);
+      law_S: the shader's thin-disk march (shaders/blackhole.frag:77-253) in FP64 with the
+             camera's step settings (oracle_lawS_f64).
+    Built for this host by the oracle's Makefile (gcc -O3 -march=native -ffp-contract=off); the
+    prebuilt -O2 x86-64-v3 library if that build fails. value = law_S on the 1080p frame, all
+    CPUs."""
+    import tempfile
     import numpy as np
     import oracle
+    from blackholerender_amd import workload as W
     from blackholerender_amd.camera import PRESETS
-    ss, dstep = PRESETS[cfg["max_steps"]]
-    p = oracle.lawS_params(max_steps=cfg["max_steps"], step_scale=ss, disk_step=dstep,
-                           disk_inner=cfg["disk_inner"], disk_outer=cfg["disk_outer"])
-    n = cfg["width"] * cfg["height"]
-    threads = host_threads()
-    # probe: every 64th ray of the first frame
-    first = cams[0].as_dict()
-    t0 = time.perf_counter()
-    oracle.render(first, p, mode="f64", stride=64, threads=threads)
-    est_frame = (time.perf_counter() - t0) * 64
-    if est_frame > max_seconds:
-        stride = int(np.ceil(est_frame / max_seconds))
-        t0 = time.perf_counter()
-        rays = len(oracle.render(first, p, mode="f64", stride=stride, threads=threads))
-        dt = time.perf_counter() - t0
-        what = f"{rays} of the {n} rays (every {stride}th pixel) of the first frame"
-    else:
-        rays, frames = 0, 0
-        t0 = time.perf_counter()
-        while frames < len(cams) and (frames == 0 or time.perf_counter() - t0 < target_seconds):
-            rays += len(oracle.render(cams[frames].as_dict(), p, mode="f64", threads=threads))
-            frames += 1
-        dt = time.perf_counter() - t0
-        what = f"{frames} whole frame(s) of the step ({rays} rays)"
-    return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads,
-            "kind": "port",
-            "sample": (f"FP64 law-S oracle (oracle/bhs_oracle.c, gcc -O2 x86-64-v3) on {what}, "
-                       f"{threads} threads, {dt:.2f} s, CPU {cpu_model()}")}
+    info = host_cpus()
+    threads = info["threads"]
+    tmp = tempfile.mkdtemp(prefix="bhs_cpu_baseline_")
+    try:
+        path = oracle.build_native(os.path.join(tmp, "libbhs_oracle_native.so"))
+        build = "gcc -O3 -march=native -ffp-contract=off -fno-fast-math (built on this host)"
+    except Exception as e:  # the prebuilt oracle, named in the record
+        path = oracle.build()
+        build = f"prebuilt gcc -O2 -march=x86-64-v3 (native build failed: {type(e).__name__})"
+    table = {}
+    with oracle.using(path):
+        for cfg in (W.C1, W.C2):
+            cam = W.camera_for(cfg, sample=0).as_dict()
+            n = cfg["width"] * cfg["height"]
+            ss, dstep = PRESETS[cfg["max_steps"]]
+            p = oracle.lawS_params(max_steps=cfg["max_steps"], step_scale=ss, disk_step=dstep,
+                                   disk_inner=cfg["disk_inner"], disk_outer=cfg["disk_outer"])
+            pos = np.asarray(cam["pos"], np.float64)
+            for law in ("law_S", "law_T"):
+                for th in (threads, 1):
+                    probe_stride = max(1, n // (512 * th))
+                    if law == "law_S":
+                        run = lambda st: oracle.render(cam, p, mode="f64", stride=st, threads=th)
+                    else:
+                        def run(st, th=th):
+                            d = oracle.camera_dirs(cam, 0, None, st).astype(np.float64)
+                            return oracle.trace_batch(np.broadcast_to(pos, d.shape), d,
+                                                      threads=th)
+                    t0 = time.perf_counter()
+                    k = len(run(probe_stride))
+                    per_ray = (time.perf_counter() - t0) / max(k, 1)
+                    stride = _stride_for(n, per_ray, target_s)
+                    rays = (n + stride - 1) // stride
+                    if law == "law_T":   # the camera's rays are generated outside the timed call
+                        d = oracle.camera_dirs(cam, 0, None, stride).astype(np.float64)
+                        pp = np.ascontiguousarray(np.broadcast_to(pos, d.shape))
+                        fn = lambda: oracle.trace_batch(pp, d, threads=th)
+                    else:
+                        fn = lambda: oracle.render(cam, p, mode="f64", stride=stride, threads=th)
+                    best, med = _best_of(fn, runs)
+                    table.setdefault(law, {}).setdefault(cfg["name"], {})[
+                        "all" if th == threads else "1_thread"] = {
+                        "mrays_per_s": round(rays / best / 1e6, 4), "rays": rays, "stride": stride,
+                        "threads": th, "best_s": round(best, 4), "median_s": round(med, 4)}
+    try:
+        os.remove(path) if path.startswith(tmp) else None
+        os.rmdir(tmp)
+    except OSError:
+        pass
+    head = table["law_S"][W.C2["name"]]["all"]
+    return {"value": head["mrays_per_s"], "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": (f"FP64 law-S march (oracle/bhs_oracle.c) of {head['rays']} rays (every "
+                       f"{head['stride']}th pixel) of the first 1920x1080 frame of the step, "
+                       f"{threads} threads, best of {runs} runs; law T and the 256x256 camera "
+                       f"beside it in `table`"),
+            "build": build, "runs": runs, **{k: v for k, v in info.items() if k != "threads"},
+            "table": table}
 
 
 def _tune_batch(tracer, cams, params, base: dict) -> dict:
@@ -159,6 +240,42 @@ def _radius_err(A, R, sel):
             "p50": float(np.median(e)), "n": int(e.size)}
 
 
+def _phi_g_err(A, R, sel):
+    """Azimuth error (rad, wrapped to [-pi, pi]) and relative redshift error of the recorded
+    crossings of rays `sel` (both records present)."""
+    import numpy as np
+    ep, eg = [], []
+    for k in range(2):
+        m = sel & (A["n_cross"] > k) & (R["n_cross"] > k)
+        if m.any():
+            d = A["disk_phi"][m, k].astype(np.float64) - R["disk_phi"][m, k]
+            ep.append(np.abs((d + np.pi) % (2.0 * np.pi) - np.pi))
+            eg.append(np.abs(A["disk_g"][m, k].astype(np.float64) / R["disk_g"][m, k] - 1.0))
+    out = {}
+    for name, e in (("phi_rad", ep), ("g_rel", eg)):
+        e = np.concatenate(e) if e else np.zeros(1)
+        out[name] = {"max": float(e.max()), "p999": float(np.quantile(e, 0.999)),
+                     "p50": float(np.median(e)), "n": int(e.size)}
+    return out
+
+
+def _plain_vs_fp64(P, R):
+    """The plain integrator's FP32 rays (outside the FP64 band) against the FP64 march of the same
+    rays: class, crossing-count and step-count agreement, escape and crossing errors."""
+    import numpy as np
+    f32 = (P["flags"] & 4) == 0
+    both = f32 & ((P["flags"] & 2) > 0) & ((R["flags"] & 2) > 0)
+    e = _angle(P["esc"][both], R["esc"][both]) if both.any() else np.zeros(1)
+    n = int(f32.sum())
+    return {"rays_fp32": n,
+            "class_mismatch": int((P["cls"][f32] != R["cls"][f32]).sum()),
+            "n_cross_mismatch": int((P["n_cross"][f32] != R["n_cross"][f32]).sum()),
+            "steps_mismatch_rate": round(float((P["steps"][f32] != R["steps"][f32]).mean()), 6),
+            "steps_abs_diff_max": int(np.abs(P["steps"][f32] - R["steps"][f32]).max()),
+            "escape_err_max_rad": float(e.max()), "escape_err_p50_rad": float(np.median(e)),
+            "disk_r_rel_err": _radius_err(P, R, f32), **_phi_g_err(P, R, f32)}
+
+
 def _accuracy(sur_out, plain_out, ref_out, ref_root_out, n):
     """Served-ray escape error, crossing-radius error and record agreement of a surrogate frame
     against FP64 (chord crossings, and refined crossings ref_root_out)."""
@@ -184,6 +301,9 @@ def _accuracy(sur_out, plain_out, ref_out, ref_root_out, n):
         "disk_r_rel_err_served_vs_fp64_refined": _radius_err(S, ref_root_out, sv),
         "disk_r_rel_err_served_vs_fp64_chord": _radius_err(S, R, sv),
         "disk_r_rel_err_plain_vs_fp64_chord": _radius_err(plain_out, R, sv),
+        "disk_phi_g_err_served_vs_fp64_refined": _phi_g_err(S, ref_root_out, sv),
+        "disk_phi_g_err_served_vs_fp64_chord": _phi_g_err(S, R, sv),
+        "plain_fp32_vs_fp64": _plain_vs_fp64(plain_out, R),
         "rays": n,
     }
 
@@ -215,6 +335,9 @@ def main(argv=None) -> int:
     ap.add_argument("--table", default="1024x128",
                     help="table-mode grid N2xN3 over (b/b_crit, psi) features")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default)")
+    ap.add_argument("--no-full-form", action="store_true", help="skip the full-form record")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 64-camera graph record")
+    ap.add_argument("--no-per-ray", action="store_true", help="skip the per-ray network record")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shape 'mode,block,waves_per_simd,refill_min' (skips the "
                          "autotuner)")
@@ -321,7 +444,13 @@ def main(argv=None) -> int:
             res["single_frame"] = {"value": s1["value"], "ms_per_step": s1["ms_per_step"],
                                    "speedup_vs_plain": s1["speedup_vs_plain"],
                                    "plain_value": p1["value"],
-                                   "plain_ms_per_step": p1["ms_per_step"]}
+                                   "plain_ms_per_step": p1["ms_per_step"],
+                                   "network_part_ms": s1["network_part_ms"],
+                                   "surrogate_call_ms": s1["call_ms"],
+                                   "plain_kernel_ms": p1["kernel_ms"],
+                                   "plain_call_ms": p1["call_ms"]}
+            tracer.set_tuning(tuned["mode"], tuned["block"], tuned["waves_per_simd"],
+                              tuned["refill_min"])   # back to the batch's shape
         if not args.no_accuracy:
             cam0 = cams[0]
             n0 = cam0.width * cam0.height
@@ -333,9 +462,20 @@ def main(argv=None) -> int:
             torch.cuda.synchronize()
             res["accuracy"] = _accuracy(s0.numpy(), pl0.numpy(), ref.numpy(), ref_root.numpy(), n0)
             res["accuracy"]["frame"] = "camera 0 of the step vs FP64 law-S march of the same rays"
+        if args.config == "c2":
+            res["surrogate"]["roofline"] = _surrogate_roofline(
+                {"network_part_ms": res["surrogate"]["network_part_ms"]}, n, key)
+            if not args.no_per_ray:
+                res["per_ray"] = _per_ray_leg(tracer, sur, cams[0], params,
+                                              max(2, args.steps // 4), 2)
         sur.close()
+        if args.config == "c2" and not args.no_full_form:
+            res["full_form"] = _full_form_leg(tracer, max(3, args.steps // 4), 2,
+                                              accuracy=not args.no_accuracy)
+    if args.config == "c2" and not args.no_c3:
+        res["batch64"] = _c3_leg(tracer, max(3, args.steps // 4), 2)
     if not args.no_cpu and args.config == "c2":
-        res["cpu_baseline"] = run_cpu_baseline(cfg, cams)
+        res["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(res), flush=True)
     tracer.close()
     return 0
@@ -421,6 +561,167 @@ def _gas_lens_leg(tracer, steps, cpu=True, cam_r=30.0):
         rec["cpu_oracle_ms_per_table"] = round((time.perf_counter() - t0) * 1e3, 2)
         rec["cpu_cores"] = 1
     return rec
+
+
+# Surrogate pass, algorithmic HBM bytes per ray: the record it writes (cls, flags, n_cross: 1 B
+# each; steps 2; escape direction 12; two crossings' r, phi, g 24; b 4). It reads no per-ray input.
+RECORD_BYTES_PER_RAY = 45
+# MFMA work of one per-ray network evaluation, padded to the 16x16x32 tiles (the three hidden
+# layers' 128x128 products with their 64-lane padding).
+MLP_FLOPS_PADDED = 77_824
+# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per SIMD every 4 cycles.
+PEAK_VALU_GINST = 256 * 4 * 2.4 / 4
+
+
+def _pmc(key):
+    path = os.path.join(ROOT, "profiles", "valu.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("workloads", {}).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def _surrogate_roofline(sur_rec, n, key):
+    """Rooflines of the surrogate pass (the table build and the two k_surrogate_tab dispatches,
+    timed together by HIP events on the call's stream): HBM bytes of the records it writes, and
+    VALU wave-instructions per launch from rocprofv3 (profiles/valu.json) over the same time."""
+    ms = sur_rec["network_part_ms"]
+    gbs = n * RECORD_BYTES_PER_RAY / (ms * 1e-3) / 1e9
+    rec = {"kernel": "surrogate pass (k_tab_range, k_tab_phi, k_table, k_chunk_class, 2 x "
+                     "k_surrogate_tab, k_stats_reduce)",
+           "pass_ms": ms,
+           "hbm": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                   "frac": round(gbs / 8000.0, 4), "bytes_per_ray": RECORD_BYTES_PER_RAY}}
+    pm = _pmc(key)
+    if pm:
+        gi = pm["valu_insts_per_launch"] / (ms * 1e-3) / 1e9
+        rec["valu"] = {"bound": "valu", "achieved": round(gi, 2), "peak": PEAK_VALU_GINST,
+                       "unit": "G wave-instructions/s", "frac": round(gi / PEAK_VALU_GINST, 4),
+                       "valu_insts_per_launch": pm["valu_insts_per_launch"],
+                       "source": pm["source"]}
+    return rec
+
+
+def _full_form_leg(tracer, steps, warmup, accuracy=True):
+    """The full form (3840x2160, 8 jittered spp, 4 poses: 32 cameras, 265,420,800 rays per call;
+    700 steps, disk [6, 30]) through both legs with the full-form model (table mode 1024x128), in
+    the launch shape tuned for the 1080p batch; accuracy on camera 0 (8,294,400 rays) against the
+    FP64 march of the same rays."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd import workload as W
+    from blackholerender_amd.surrogate import Surrogate
+    cams = W.cameras_c5()
+    params = W.params_for(W.C5)
+    sur = Surrogate.load(tracer, os.path.join(ROOT, "blackholerender_amd", "models",
+                                              "lawS_700_disk6-30.bhsmlp"))
+    sur.set_table(1024, 128)
+    plain_rec, sur_rec, roof = _legs(tracer, sur, cams, params, steps, warmup)
+    n = sum(c.width * c.height for c in cams)
+    rec = {"workload": ("full form: 3840x2160, 8 jittered spp, 4 poses (32 cameras, "
+                        "265,420,800 rays per step), 700 steps, r_obs=15M, incl 80deg, FOV 60deg, "
+                        "thin disk [6,30]M"),
+           "model": "lawS_700_disk6-30.bhsmlp", "surrogate_mode": "table 1024x128",
+           "value": sur_rec.pop("value"), "unit": "Mrays/s",
+           "ms_per_step": sur_rec.pop("ms_per_step"),
+           "speedup_vs_plain": sur_rec.pop("speedup_vs_plain"), "steps": steps,
+           "warmup": warmup, "plain": plain_rec, "surrogate": sur_rec, "roofline": roof,
+           "surrogate_roofline": _surrogate_roofline(sur_rec, n, f"c5x{len(cams)}")}
+    if accuracy:
+        cam0 = cams[0]
+        ref = tracer.render(cam0, W.params_for(W.C5, precision=bhs.PREC_F64))
+        ref_root = tracer.render(cam0, W.params_for(W.C5, precision=bhs.PREC_F64, root64=1))
+        pl0 = tracer.render(cam0, params)
+        s0 = sur.render(cam0, params)
+        torch.cuda.synchronize()
+        rec["accuracy"] = _accuracy(s0.numpy(), pl0.numpy(), ref.numpy(), ref_root.numpy(),
+                                    cam0.width * cam0.height)
+        rec["accuracy"]["frame"] = "camera 0 of the full form vs FP64 law-S march of the same rays"
+        del ref, ref_root, pl0, s0
+    sur.close()
+    return rec
+
+
+def _c3_leg(tracer, steps, warmup):
+    """The 64-camera batch (resolutions 64..1024 per axis, r_obs 8..1000 M, inclinations 0..90,
+    FOV 20..90, 200-700 steps with their presets; seeded) captured in one HIP graph
+    (bhs_graph_create) and replayed once per step; beside it the eager batch call with the plain
+    march kernel's roofline."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd import workload as W
+    cams, params = W.batch_c3()
+    n = sum(c.width * c.height for c in cams)
+    stream = torch.cuda.current_stream()
+    out = bhs.RayBuffers(n, "cuda:0")
+    eager = lambda: tracer.render_batch(cams, params, out, stream)
+    eager()
+    torch.cuda.synchronize()
+    tracer.set_timing(True)
+    wall_e = _timed(eager, steps, warmup)
+    tm = tracer.timing()
+    tracer.set_timing(False)
+    eager()
+    st = tracer.last_stats()
+    fl = out.flags.to(torch.int32)
+    f32 = (fl & bhs.FLAG_BAND64) == 0
+    stp = out.steps_i32()
+    ncr = out.n_cross.to(torch.int32)
+    flops = W.march_flops(int(f32.sum()), int(stp[f32].sum()),
+                          int(((fl & bhs.FLAG_ESCAPED) != 0)[f32].sum()), int(ncr[f32].sum()),
+                          int(torch.clamp(ncr, max=2)[f32].sum()))
+    tfl = flops / (tm["march_ms"] * 1e-3) / 1e12
+    g = tracer.graph(cams, params, out)
+    try:
+        wall_g = _timed(lambda: g.launch(stream), steps, warmup)
+        gst = g.stats()
+    finally:
+        g.close()
+    del out
+    return {"workload": (f"64 cameras in one call, {n:,} rays (seeded sizes 64..1024 per axis, "
+                         f"r_obs 8..1000 M, incl 0..90 deg, FOV 20..90 deg, 200/300/450/700 steps)"),
+            "graph_value": round(n * steps / wall_g / 1e6, 2), "unit": "Mrays/s",
+            "graph_ms_per_step": round(wall_g / steps * 1e3, 4),
+            "eager_value": round(n * steps / wall_e / 1e6, 2),
+            "eager_ms_per_step": round(wall_e / steps * 1e3, 4),
+            "kernel_ms": round(tm["march_ms"], 5), "band64_rays": st["band64"],
+            "graph_stats_equal_eager": all(gst[k] == st[k] for k in ("rays", "steps", "band64")),
+            "roofline": {"bound": "valu", "achieved": round(tfl, 3), "peak": W.PEAK_FP32_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(tfl / W.PEAK_FP32_TFLOPS, 4),
+                         "kernel": "k_persist (plain FP32 law-S march, batch form)",
+                         "flops_per_launch": flops}}
+
+
+def _per_ray_leg(tracer, sur, cam, params, steps, warmup):
+    """The surrogate with per-ray network inference (every candidate ray through the bf16 MFMA
+    network, 3->128->128->128->5) on one 1080p frame per call; MFMA fraction = padded network
+    flops of the served rays (a lower bound of the evaluations) over the network pass's time
+    (HIP events on the call's stream)."""
+    import torch
+    import blackholerender_amd as bhs
+    n = cam.width * cam.height
+    stream = torch.cuda.current_stream()
+    out = bhs.RayBuffers(n, "cuda:0")
+    sur.set_table(0, 0)
+    fn = lambda: sur.render(cam, params, out, stream)
+    fn()
+    torch.cuda.synchronize()
+    tracer.set_timing(True)
+    wall = _timed(fn, steps, warmup)
+    tm = tracer.timing()
+    tracer.set_timing(False)
+    fn()
+    st = tracer.last_stats()
+    tf = st["surrogate"] * MLP_FLOPS_PADDED / (tm["march_ms"] * 1e-3) / 1e12
+    del out
+    return {"value": round(n * steps / wall / 1e6, 2), "unit": "Mrays/s",
+            "ms_per_step": round(wall / steps * 1e3, 4), "network_pass_ms": round(tm["march_ms"], 5),
+            "served": st["surrogate"],
+            "mfma": {"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(tf / 2500.0, 5),
+                     "flops_per_evaluation": MLP_FLOPS_PADDED,
+                     "evaluations": "served rays (lower bound)"}}
 
 
 def _legs(tracer, sur, cams, params, steps, warmup):

@@ -9,6 +9,7 @@ march of shaders/blackhole.frag:67-253 in float and double with one pinned opera
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -62,61 +63,88 @@ _D3 = C.c_double * 3
 _F3 = C.c_float * 3
 
 
+def build_native(out_path: str) -> str:
+    """The same oracle source at -O3 -march=native (the timed CPU baseline, built on the host that
+    times it) into out_path; returns out_path."""
+    subprocess.run(["make", "-C", _HERE, "-s", "native", f"NATIVE_OUT={out_path}"], check=True)
+    return out_path
+
+
+@contextlib.contextmanager
+def using(path: str):
+    """Every oracle call inside the block goes to the library at `path` (same API)."""
+    global _lib
+    prev = _lib
+    _lib = _bind(path)
+    try:
+        yield
+    finally:
+        _lib = prev
+
+
 def lib():
     global _lib
     if _lib is None:
         build()
-        L = C.CDLL(_LIB_PATH)
-        dp, fp = C.POINTER(C.c_double), C.POINTER(C.c_float)
-        L.oracle_critical_impact_parameter.restype = C.c_double
-        L.oracle_accel.argtypes = [dp, C.c_double, dp]
-        L.oracle_rk4_step.argtypes = [dp, dp, C.c_double, C.c_double]
-        L.oracle_initial_velocity.argtypes = [dp, dp, dp]
-        L.oracle_impact_parameter.argtypes = [dp, dp]
-        L.oracle_impact_parameter.restype = C.c_double
-        L.oracle_impact_parameter_about_axis.argtypes = [dp, dp, dp]
-        L.oracle_impact_parameter_about_axis.restype = C.c_double
-        L.oracle_far_field_direction.argtypes = [dp, dp, C.c_double, dp]
-        L.oracle_shadow_angular_radius.argtypes = [C.c_double]
-        L.oracle_shadow_angular_radius.restype = C.c_double
-        L.oracle_kepler_omega.argtypes = [C.c_double]
-        L.oracle_kepler_omega.restype = C.c_double
-        L.oracle_redshift_factor.argtypes = [C.c_double, C.c_double, C.c_double]
-        L.oracle_redshift_factor.restype = C.c_double
-        L.oracle_trace_T.argtypes = [dp, dp, C.c_int32, C.c_double, C.c_double, C.c_double,
-                                     C.POINTER(TraceResult)]
-        L.oracle_trace_T_batch.argtypes = [dp, dp, C.c_int64, C.c_int32, C.c_double, C.c_double,
-                                           C.c_double, C.c_void_p, C.c_int32]
-        L.oracle_ray_dir.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
-                                     C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, fp]
-        L.oracle_pcg.argtypes = [C.c_uint32]
-        L.oracle_pcg.restype = C.c_uint32
-        for name in ("oracle_lawS_f32", "oracle_lawS_f64", "oracle_lawS_band"):
-            getattr(L, name).argtypes = [fp, fp, C.POINTER(LawSParams), C.POINTER(LawSRay)]
-        L.oracle_lawS_h_f32.argtypes = [fp, fp]
-        L.oracle_lawS_h_f32.restype = C.c_float
-        L.oracle_lawS_b_f32.argtypes = [fp, fp]
-        L.oracle_lawS_b_f32.restype = C.c_float
-        L.oracle_render.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
-                                    C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(LawSParams),
-                                    C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_void_p,
-                                    C.c_int32]
-        L.oracle_march_batch.argtypes = [fp, fp, C.c_int64, C.POINTER(LawSParams), C.c_int32,
-                                         C.c_void_p, C.c_int32]
-        L.oracle_ray_dir_at.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
-                                        C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
-                                        C.c_int64, fp]
-        L.oracle_make_footprint.argtypes = [fp, fp, fp, C.c_float, C.POINTER(Footprint)]
-        vp = C.c_void_p
-        L.oracle_sky_footprint.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
-                                           C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, vp, vp,
-                                           vp, vp, vp, vp, vp, vp, vp, vp]
-        L.oracle_gas_lens_table.argtypes = [C.c_float, C.c_float, C.c_int32, C.c_int32,
-                                            C.c_int32, C.c_float, C.c_void_p]
-        L.oracle_atan2f.argtypes = [C.c_float, C.c_float]
-        L.oracle_atan2f.restype = C.c_float
-        _lib = L
+        _lib = _bind(_LIB_PATH)
     return _lib
+
+
+def _bind(path: str):
+    """ctypes view of the oracle library at `path`, with every function's argument types."""
+    L = C.CDLL(path)
+    dp, fp = C.POINTER(C.c_double), C.POINTER(C.c_float)
+    L.oracle_critical_impact_parameter.restype = C.c_double
+    L.oracle_accel.argtypes = [dp, C.c_double, dp]
+    L.oracle_rk4_step.argtypes = [dp, dp, C.c_double, C.c_double]
+    L.oracle_initial_velocity.argtypes = [dp, dp, dp]
+    L.oracle_impact_parameter.argtypes = [dp, dp]
+    L.oracle_impact_parameter.restype = C.c_double
+    L.oracle_impact_parameter_about_axis.argtypes = [dp, dp, dp]
+    L.oracle_impact_parameter_about_axis.restype = C.c_double
+    L.oracle_far_field_direction.argtypes = [dp, dp, C.c_double, dp]
+    L.oracle_shadow_angular_radius.argtypes = [C.c_double]
+    L.oracle_shadow_angular_radius.restype = C.c_double
+    L.oracle_kepler_omega.argtypes = [C.c_double]
+    L.oracle_kepler_omega.restype = C.c_double
+    L.oracle_redshift_factor.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.oracle_redshift_factor.restype = C.c_double
+    L.oracle_trace_T.argtypes = [dp, dp, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                 C.POINTER(TraceResult)]
+    L.oracle_trace_T_batch.argtypes = [dp, dp, C.c_int64, C.c_int32, C.c_double, C.c_double,
+                                       C.c_double, C.c_void_p, C.c_int32]
+    L.oracle_ray_dir.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                 C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, fp]
+    L.oracle_pcg.argtypes = [C.c_uint32]
+    L.oracle_pcg.restype = C.c_uint32
+    for name in ("oracle_lawS_f32", "oracle_lawS_f64", "oracle_lawS_band"):
+        getattr(L, name).argtypes = [fp, fp, C.POINTER(LawSParams), C.POINTER(LawSRay)]
+    L.oracle_lawS_h_f32.argtypes = [fp, fp]
+    L.oracle_lawS_h_f32.restype = C.c_float
+    L.oracle_lawS_b_f32.argtypes = [fp, fp]
+    L.oracle_lawS_b_f32.restype = C.c_float
+    L.oracle_render.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(LawSParams),
+                                C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_void_p,
+                                C.c_int32]
+    L.oracle_march_batch.argtypes = [fp, fp, C.c_int64, C.POINTER(LawSParams), C.c_int32,
+                                     C.c_void_p, C.c_int32]
+    L.oracle_ray_dir_at.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                    C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
+                                    C.c_int64, fp]
+    L.oracle_make_footprint.argtypes = [fp, fp, fp, C.c_float, C.POINTER(Footprint)]
+    vp = C.c_void_p
+    L.oracle_sky_footprint.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                       C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, vp, vp,
+                                       vp, vp, vp, vp, vp, vp, vp, vp]
+    L.oracle_gas_lens_table.argtypes = [C.c_float, C.c_float, C.c_int32, C.c_int32,
+                                        C.c_int32, C.c_float, C.c_void_p]
+    L.oracle_atan2f.argtypes = [C.c_float, C.c_float]
+    L.oracle_atan2f.restype = C.c_float
+    L.oracle_camera_dirs.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                     C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
+                                     C.c_int64, fp]
+    return L
 
 
 # ---------------------------------------------------------------------------------- law T helpers
@@ -276,6 +304,22 @@ def render(cam, params: LawSParams, mode="f32", first=0, count=None, stride=1,
                         int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
                         int(cam.get("camera_id", 0)), int(cam.get("sample", 0)), C.byref(params),
                         _MODES[mode], first, count, stride, out.ctypes.data, threads)
+    return out
+
+
+def camera_dirs(cam, first=0, count=None, stride=1) -> np.ndarray:
+    """(count, 3) float32 view directions of pixels first, first + stride, ... of a camera dict."""
+    n = cam["width"] * cam["height"]
+    if count is None:
+        count = (n - first + stride - 1) // stride
+    out = np.zeros((count, 3), np.float32)
+    p, pp = _f(cam["pos"], 3)
+    b, bp = _f(cam["basis"], 9)
+    j, jp = _f(cam.get("jitter", (0.0, 0.0)), 2)
+    lib().oracle_camera_dirs(pp, bp, cam["tan_half_fov"], cam["width"], cam["height"], jp,
+                             int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
+                             int(cam.get("camera_id", 0)), int(cam.get("sample", 0)), first, count,
+                             stride, out.ctypes.data_as(C.POINTER(C.c_float)))
     return out
 
 

@@ -386,6 +386,18 @@ void oracle_ray_dir_at(const float pos[3], const float B[9], float tan_half_fov,
     for (int k = 0; k < 3; ++k) dir_out[k] = d[k] * id;
 }
 
+/* View directions (float, xyz interleaved) of pixels first, first + stride, ... of a camera: the
+ * rays of a render, for callers that march them by another law (law T's trace on camera rays). */
+void oracle_camera_dirs(const float pos[3], const float B[9], float tan_half_fov, int32_t width,
+                        int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                        uint32_t camera_id, uint32_t sample, int64_t first, int64_t count,
+                        int64_t stride, float* dir3_out) {
+    static const float zero2[2] = {0.0f, 0.0f};
+    for (int64_t k = 0; k < count; ++k)
+        oracle_ray_dir(pos, B, tan_half_fov, width, height, jitter ? jitter : zero2, jitter_hashed,
+                       seed, camera_id, sample, first + k * stride, dir3_out + 3 * k);
+}
+
 /* ================================================================================================
  * Sky footprint: shaders/blackhole.frag:297-306 (screen-space derivatives of the escape direction
  * over 2x2 pixel quads, the unlensed footprint at status edges) and shaders/sky.glsl:27-51

@@ -154,4 +154,9 @@ void oracle_gas_lens_table(float cam_r, float alpha_crit, int32_t n_rays, int32_
 #ifdef __cplusplus
 }
 #endif
+void oracle_camera_dirs(const float pos[3], const float B[9], float tan_half_fov, int32_t width,
+                        int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                        uint32_t camera_id, uint32_t sample, int64_t first, int64_t count,
+                        int64_t stride, float* dir3_out);
+
 #endif
