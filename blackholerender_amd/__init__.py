@@ -32,7 +32,8 @@ from ._capi import (CLASS_DISK, CLASS_ESCAPE, CLASS_HORIZON, DISK_NONE, DISK_THI
                     TRACE_MAXSTEPS)
 from .camera import PRESETS, Camera, look_basis, orbit_camera
 
-__all__ = ["Tracer", "BhsError", "TraceResult", "RayBuffers", "FootprintBuffers", "march_params",
+__all__ = ["Tracer", "BhsError", "TraceResult", "RayBuffers", "FootprintBuffers", "ShadeBuffers",
+           "disk_shading", "halton_jitter", "march_params",
            "trace", "Camera", "orbit_camera", "look_basis", "PRESETS", "critical_impact_parameter",
            "CLASS_HORIZON", "CLASS_DISK", "CLASS_ESCAPE", "FLAG_MAXSTEPS", "FLAG_ESCAPED",
            "FLAG_BAND64", "FLAG_ROOT64", "FLAG_SURROGATE", "PREC_F32",
@@ -222,6 +223,41 @@ class FootprintBuffers:
         return out
 
 
+def disk_shading(density=1.0, opacity=2.0, peak_temp=6000.0, brightness=5.5, noise_scale=1.0,
+                 noise_strength=0.85, warmth=0.4, doppler=0.0, time=0.0,
+                 inflow=0.0) -> _capi.DiskShading:
+    """bhs_disk_shading; defaults of the reference's Settings (src/Settings.h:32-45)."""
+    return _capi.DiskShading(density, opacity, peak_temp, brightness, noise_scale,
+                             noise_strength, warmth, doppler, time, inflow)
+
+
+def halton_jitter(sample: int):
+    """Sub-pixel jitter of accumulation sample `sample` (jitterFor, src/Renderer.cpp:16-29)."""
+    o = (C.c_float * 2)()
+    _capi.load().bhs_halton_jitter(sample, o)
+    return (o[0], o[1])
+
+
+class ShadeBuffers:
+    """Device shading planes (bhs_shade_soa): rgb (3, n) and sky_weight (n) float32."""
+
+    def __init__(self, n: int, device):
+        import torch
+        self.n = n
+        self.rgb = torch.empty((3, n), dtype=torch.float32, device=device)
+        self.sky_weight = torch.empty(n, dtype=torch.float32, device=device)
+
+    def check(self, n: int, device: int, what: str):
+        _check_rays_like(self, (("rgb", "float32", 3), ("sky_weight", "float32", 1)), n, device,
+                         what)
+
+    def soa(self) -> _capi.ShadeSoA:
+        return _capi.ShadeSoA(self.rgb.data_ptr(), self.sky_weight.data_ptr())
+
+    def numpy(self) -> dict:
+        return dict(rgb=self.rgb.cpu().numpy().T.copy(), sky_weight=self.sky_weight.cpu().numpy())
+
+
 class Tracer:
     """One bhs_ctx on one HIP device."""
 
@@ -392,6 +428,65 @@ class Tracer:
                                           out.data_ptr(), C.c_void_p(self._stream_ptr(stream)))
         self._check(rc, "bhs_gas_lens_table")
         return out
+
+    # ---- thin-disk shading, accumulation --------------------------------------------------------
+    def shade_thin_disk(self, cam: Camera, rays: RayBuffers, params: _capi.MarchParams,
+                        shading: _capi.DiskShading | None = None, out: ShadeBuffers | None = None,
+                        stream=None) -> ShadeBuffers:
+        """Thin-disk colour and transparency (bhs_shade_thin_disk) of a render of `cam` with
+        march parameters `params`: blackhole.frag:209-216,234 with thinDiskSample."""
+        n = cam.width * cam.height
+        rays.check(n, self.device, "shade_thin_disk: rays")
+        if shading is None:
+            shading = disk_shading()
+        if out is None:
+            out = ShadeBuffers(n, f"cuda:{self.device}")
+        out.check(n, self.device, "shade_thin_disk: out")
+        cs = _camera_struct(cam)
+        rs, os_ = rays.soa(), out.soa()
+        rc = self._lib.bhs_shade_thin_disk(self._ctx, C.byref(cs), C.byref(params),
+                                           C.byref(shading), C.byref(rs), C.byref(os_),
+                                           C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_shade_thin_disk")
+        return out
+
+    def shade_thin_disk_batch(self, cams, rays: RayBuffers, params,
+                              shading: _capi.DiskShading | None = None,
+                              out: ShadeBuffers | None = None, stream=None) -> ShadeBuffers:
+        """The shading pass over a batch render (bhs_shade_thin_disk_batch)."""
+        ca, pa = self._batch_structs(cams, params)
+        total = self.batch_offsets(cams)[-1]
+        rays.check(total, self.device, "shade_thin_disk_batch: rays")
+        if shading is None:
+            shading = disk_shading()
+        if out is None:
+            out = ShadeBuffers(total, f"cuda:{self.device}")
+        out.check(total, self.device, "shade_thin_disk_batch: out")
+        rs, os_ = rays.soa(), out.soa()
+        rc = self._lib.bhs_shade_thin_disk_batch(self._ctx, ca, pa, len(cams), C.byref(shading),
+                                                 C.byref(rs), C.byref(os_),
+                                                 C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_shade_thin_disk_batch")
+        return out
+
+    def accumulate(self, cur_rgb, acc_rgb, sample_index: int, cap: int = 256, stream=None):
+        """acc_rgb <- running average with cur_rgb (bhs_accumulate; (3, n) float32 tensors)."""
+        n = cur_rgb.shape[-1]
+        _check_tensor(cur_rgb, "float32", (3, n), self.device, "accumulate: cur_rgb")
+        _check_tensor(acc_rgb, "float32", (3, n), self.device, "accumulate: acc_rgb")
+        rc = self._lib.bhs_accumulate(self._ctx, cur_rgb.data_ptr(), acc_rgb.data_ptr(), n,
+                                      sample_index, cap, C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_accumulate")
+        return acc_rgb
+
+    def disk_textures(self):
+        """Host copies of the context's noise volume ((128,128,128) uint16 half texels, z y x)
+        and blackbody table ((256, 4) float32)."""
+        vol = np.zeros((128, 128, 128), np.uint16)
+        lut = np.zeros((256, 4), np.float32)
+        self._check(self._lib.bhs_disk_textures(self._ctx, vol.ctypes.data, lut.ctypes.data),
+                    "bhs_disk_textures")
+        return vol, lut
 
     def graph(self, cams, params, out: RayBuffers | None = None) -> "BatchGraph":
         """Captures the batch launch into a HIP graph (bhs_graph_create) for replay."""

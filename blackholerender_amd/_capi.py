@@ -54,6 +54,16 @@ class FootprintSoA(C.Structure):
                 ("mu_dev", C.c_void_p), ("fallback_dev", C.c_void_p)]
 
 
+class DiskShading(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ("density", "opacity", "peak_temp", "brightness",
+                                         "noise_scale", "noise_strength", "warmth", "doppler",
+                                         "time", "inflow")]
+
+
+class ShadeSoA(C.Structure):
+    _fields_ = [("rgb_dev", C.c_void_p), ("sky_weight_dev", C.c_void_p)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("waves_per_simd", C.c_int32),
                 ("refill_min", C.c_int32)]
@@ -108,6 +118,20 @@ def load():
     L.bhs_render.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams), C.POINTER(RaySoA), vp]
     L.bhs_render_noalloc.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams),
                                      C.POINTER(RaySoA), vp]
+    L.bhs_shade_thin_disk.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams),
+                                      C.POINTER(DiskShading), C.POINTER(RaySoA),
+                                      C.POINTER(ShadeSoA), vp]
+    L.bhs_shade_thin_disk_batch.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(DiskShading),
+                                            C.POINTER(RaySoA), C.POINTER(ShadeSoA), vp]
+    L.bhs_accumulate.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, vp]
+    L.bhs_accumulate_weight.argtypes = [C.c_int32, C.c_int32]
+    L.bhs_accumulate_weight.restype = C.c_float
+    L.bhs_halton_jitter.argtypes = [C.c_int32, C.POINTER(C.c_float)]
+    L.bhs_halton_jitter.restype = None
+    L.bhs_disk_textures.argtypes = [vp, vp, vp]
+    for name in ("bhs_shade_thin_disk", "bhs_shade_thin_disk_batch", "bhs_accumulate",
+                 "bhs_disk_textures"):
+        getattr(L, name).restype = C.c_int
     L.bhs_march_rays.argtypes = [vp, vp, vp, i64, C.POINTER(MarchParams), C.POINTER(RaySoA), vp]
     L.bhs_last_stats.argtypes = [vp, C.POINTER(Stats)]
     L.bhs_set_timing.argtypes = [vp, C.c_int]

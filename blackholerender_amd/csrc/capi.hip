@@ -385,6 +385,9 @@ void bhs_destroy(bhs_ctx* ctx) {
             (void)hipEventDestroy(r.call1);
         }
         if (ctx->table_dev) (void)hipFree(ctx->table_dev);
+        if (ctx->noise_vol) (void)hipFree(ctx->noise_vol);
+        if (ctx->bb_lut) (void)hipFree(ctx->bb_lut);
+        if (ctx->shade_list) (void)hipFree(ctx->shade_list);
         if (ctx->table_host) (void)hipHostFree(ctx->table_host);
         if (ctx->ev_upload) (void)hipEventDestroy(ctx->ev_upload);
         if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);

@@ -46,6 +46,12 @@ struct bhs_ctx {
     size_t table_host_bytes = 0;
     hipEvent_t ev_upload = nullptr;
     bool upload_pending = false;
+    // Thin-disk shading: the baked noise volume (128^3 half) and blackbody table (256 x RGBA),
+    // made at the first shading call, and the completion list ([count | pad | ray ids]).
+    uint16_t* noise_vol = nullptr;
+    float* bb_lut = nullptr;
+    void* shade_list = nullptr;
+    size_t shade_list_bytes = 0;
 };
 
 

@@ -351,7 +351,9 @@ template <typename R> struct RegSink {
 // x and v are advanced unconditionally: for a ray that finishes in this iteration they are never
 // read again, and a branch around the update would make every divergent merge copy the ray state.
 // Only the counters, the status and the records are predicated.
-template <typename R, typename Sink>
+// ALL: the sink also sees every later in-annulus crossing (slot = its index, 2, 3, ...), as the
+// shading march needs; otherwise only the first two.
+template <typename R, typename Sink, bool ALL = false>
 __device__ __forceinline__ bool march_step(MarchState<R>& st, const Setup<R>& s, const MarchK<R>& k,
                                            Sink& sink, bool live = true) {
     using A = Ar<R>;
@@ -401,7 +403,7 @@ __device__ __forceinline__ bool march_step(MarchState<R>& st, const Setup<R>& s,
         const R hz = A::fma(f, x.z - x0.z, x0.z);
         const R rh = A::sqrt(A::fma(hz, hz, hx * hx));
         if (k.cross_all || (rh >= k.inner && rh <= k.outer)) {            // disk.glsl:114-115
-            if (st.n_cross < 2) {
+            if (ALL || st.n_cross < 2) {
                 const RootState<R> q{{x0.x, x0.y, x0.z}, {v0.x, v0.y, v0.z}, dt};
                 sink.cross(st.n_cross, (float)rh, atan2_ieee((float)hz, (float)hx),
                            (float)redshift(rh, s.b_axis, s.lapse_g), q);

@@ -35,6 +35,12 @@
  *                               dFdx/dFdy of escDir over 2x2 quads, the unlensed footprint at
  *                               status edges) and makeFootprint, shaders/sky.glsl:17-51, as a
  *                               post-pass over a render's ray planes.
+ *  - bhs_shade_thin_disk[_batch], bhs_accumulate, bhs_halton_jitter, bhs_disk_textures :
+ *                               the thin-disk colour and transparency of blackhole.frag:209-216,234
+ *                               (disk.glsl:113-130, blackbody.glsl, noise_bake.comp,
+ *                               DiskTextures.cpp:61-93) as a pass over a render's ray planes; the
+ *                               accumulation pass (Renderer.cpp:389-410, accumulate.frag) and its
+ *                               Halton jitter (Renderer.cpp:16-29).
  *  - bhs_mlp_*, bhs_render_surrogate : no counterpart in the reference (learned surrogate of the
  *                               march, falling back to it near b_crit).
  */
@@ -217,6 +223,71 @@ int bhs_sky_footprint(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* ra
 int bhs_sky_footprint_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
                             const bhs_ray_soa* rays, int32_t coarse,
                             const bhs_footprint_soa* out, void* stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * Thin-disk shading and transparency, accumulation of jittered frames.
+ *
+ * The colour half of the trace pass for the thin disk (shaders/blackhole.frag:209-216,234 with
+ * thinDiskSample, shaders/disk.glsl:113-130; blackbody, shaders/blackbody.glsl; the noise texture of
+ * shaders/noise_bake.comp and the blackbody table of src/DiskTextures.cpp:61-93), over the ray
+ * planes of a bhs_render / bhs_render_batch / bhs_render_surrogate[_batch] of the same cameras and
+ * march parameters. Every in-annulus crossing is composited front to back (colour += T rgb alpha,
+ * T *= 1 - alpha) until T < 0.004, where the reference's march stops (no sky). The recorded first
+ * two crossings are used as recorded; a ray with more crossings that is still transparent after
+ * two is marched again from its pixel (same precision as rendered) for the rest.
+ * ------------------------------------------------------------------------------------------------ */
+
+/* Disk shading uniforms (Settings fields forwarded by Renderer::setTraceUniforms,
+ * src/Renderer.cpp:328-343; defaults of src/Settings.h:38-45: density 1, opacity 2, peak_temp 6000,
+ * brightness 5.5, noise_scale 1, noise_strength 0.85, warmth 0.4, doppler 0, time 0, inflow 0).
+ * The annulus comes from the march parameters. */
+typedef struct bhs_disk_shading {
+    float density;            /* uDiskDensity */
+    float opacity;            /* uDiskOpacity */
+    float peak_temp;          /* uDiskPeakTemp (K at the hottest ring) */
+    float brightness;         /* uDiskBrightness */
+    float noise_scale;        /* uDiskNoiseScale, in [0, 1000] */
+    float noise_strength;     /* uDiskNoiseStrength */
+    float warmth;             /* uDiskWarmth */
+    float doppler;            /* uDopplerStrength (0 film look, 1 physical) */
+    float time;               /* uDiskTime (flow clock in M) */
+    float inflow;             /* uDiskInflow */
+} bhs_disk_shading;
+
+/* Shading output planes (caller-owned device buffers). */
+typedef struct bhs_shade_soa {
+    float* rgb_dev;           /* 3n : linear radiance of the disk crossings, composited */
+    float* sky_weight_dev;    /* n  : transmittance left for the sky (0 unless the ray escaped and
+                                 the disk did not make it opaque): pixel = rgb + sky_weight * sky */
+} bhs_shade_soa;
+
+#define BHS_NOISE_SIZE 128    /* noise volume: 128^3 half-float texels, x fastest */
+#define BHS_BB_LUT_SIZE 256   /* blackbody table: 256 x (r, g, b, luminance / luminance(6500 K)) */
+
+/* Shading pass over the planes of a render of `cam` (needs flags, n_cross and disk planes; n =
+ * width * height). Stream-ordered; the first call of a context bakes its disk tables. */
+int bhs_shade_thin_disk(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
+                        const bhs_disk_shading* shading, const bhs_ray_soa* rays,
+                        const bhs_shade_soa* out, void* stream);
+/* The same over a batch of ncam <= 64 cameras laid out as bhs_render_batch (one annulus). */
+int bhs_shade_thin_disk_batch(bhs_ctx* ctx, const bhs_camera* cams,
+                              const bhs_march_params* params, int32_t ncam,
+                              const bhs_disk_shading* shading, const bhs_ray_soa* rays,
+                              const bhs_shade_soa* out, void* stream);
+
+/* Progressive supersampling (Renderer::accumulatePass, src/Renderer.cpp:389-410, and
+ * shaders/accumulate.frag:8-11): acc = w >= 1 ? cur : acc (1 - w) + cur w over 3n floats (rgb
+ * planes), w = bhs_accumulate_weight(sample_index, cap) = 1 for sample 0, else
+ * 1 / (min(sample_index, cap - 1) + 1). Stream-ordered. */
+float bhs_accumulate_weight(int32_t sample_index, int32_t cap);
+int bhs_accumulate(bhs_ctx* ctx, const float* cur_rgb_dev, float* acc_rgb_dev, int64_t n,
+                   int32_t sample_index, int32_t cap, void* stream);
+/* The sub-pixel jitter of sample `sample` (jitterFor, src/Renderer.cpp:16-29): (halton(s, 2) - 0.5,
+ * halton(s, 3) - 0.5) in float, (0, 0) for sample 0; for bhs_camera.jitter. */
+void bhs_halton_jitter(int32_t sample, float* out2);
+/* Host copies of the context's disk tables (bakes them first if needed); either may be NULL:
+ * noise_host 128^3 uint16 half-float texels, lut_host 256 x 4 floats. Synchronous. */
+int bhs_disk_textures(bhs_ctx* ctx, uint16_t* noise_host, float* lut_host);
 
 /* Gas lens table (shaders/gas_lens.comp:11-44 with lensAlpha, shaders/gas_lens.glsl:13-23; built
  * at src/GasParticles.cpp:164-166): n_rays fan rays (local angle alpha_i = lensAlpha((i + 0.5) /

@@ -50,6 +50,13 @@ class Footprint(C.Structure):
                 ("area", C.c_float), ("mu", C.c_float)]
 
 
+class Shading(C.Structure):
+    """oracle_shading: the disk shading uniforms (src/Renderer.cpp:328-343) and the annulus."""
+    _fields_ = [(k, C.c_float) for k in ("density", "opacity", "peak_temp", "brightness",
+                                         "noise_scale", "noise_strength", "warmth", "doppler",
+                                         "time", "inflow", "inner", "outer")]
+
+
 LAWS_RAY_DTYPE = np.dtype([("cls", "<i4"), ("flags", "<i4"), ("steps", "<i4"), ("n_cross", "<i4"),
                            ("esc", "<f4", 3), ("disk_r", "<f4", 2), ("disk_phi", "<f4", 2),
                            ("disk_g", "<f4", 2), ("b", "<f4")])
@@ -144,6 +151,43 @@ def _bind(path: str):
     L.oracle_camera_dirs.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
                                      C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
                                      C.c_int64, fp]
+    f, vp = C.c_float, C.c_void_p
+    for name in ("oracle_exp2f", "oracle_log2f", "oracle_logf", "oracle_expf"):
+        getattr(L, name).argtypes = [f]
+        getattr(L, name).restype = f
+    L.oracle_powf.argtypes = [f, f]
+    L.oracle_powf.restype = f
+    L.oracle_f2h.argtypes = [f]
+    L.oracle_f2h.restype = C.c_uint16
+    L.oracle_h2f.argtypes = [C.c_uint16]
+    L.oracle_h2f.restype = f
+    L.oracle_grad3.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32, fp]
+    L.oracle_pgnoise3.argtypes = [f, f, f, C.c_int, C.c_uint32]
+    L.oracle_pgnoise3.restype = f
+    L.oracle_noise_texel.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint32]
+    L.oracle_noise_texel.restype = f
+    L.oracle_noise_bake.argtypes = [vp, C.c_uint32, C.c_int32]
+    L.oracle_sample_noise.argtypes = [vp, f, f, f]
+    L.oracle_sample_noise.restype = f
+    L.oracle_blackbody_lut.argtypes = [vp]
+    L.oracle_blackbody.argtypes = [vp, f, fp]
+    sp = C.POINTER(Shading)
+    for name in ("oracle_temp_profile", "oracle_radial_profile"):
+        getattr(L, name).argtypes = [sp, f]
+        getattr(L, name).restype = f
+    L.oracle_disk_noise.argtypes = [sp, vp, f, f, f]
+    L.oracle_disk_noise.restype = f
+    L.oracle_thin_disk_sample.argtypes = [sp, vp, vp, f, f, f, f, fp]
+    L.oracle_shade_rays.argtypes = [fp, fp, f, C.c_int32, C.c_int32, fp, C.c_int32, C.c_uint64,
+                                    C.c_uint32, C.c_uint32, C.POINTER(LawSParams), sp, vp, vp,
+                                    C.c_int64, C.c_int64, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
+    L.oracle_shade_march.argtypes = [fp, fp, f, C.c_int32, C.c_int32, fp, C.c_int32, C.c_uint64,
+                                     C.c_uint32, C.c_uint32, C.POINTER(LawSParams), sp, vp, vp,
+                                     C.c_int64, C.c_int64, C.c_int64, C.c_int32, vp, vp]
+    L.oracle_accumulate_weight.argtypes = [C.c_int32, C.c_int32]
+    L.oracle_accumulate_weight.restype = f
+    L.oracle_accumulate.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32]
+    L.oracle_halton_jitter.argtypes = [C.c_int32, fp]
     return L
 
 
@@ -388,3 +432,119 @@ def gas_lens_table(cam_r: float, alpha_crit: float, n_rays=1024, n_phis=1024, ma
     lib().oracle_gas_lens_table(cam_r, alpha_crit, n_rays, n_phis, max_steps, r_max,
                                 t.ctypes.data)
     return t
+
+
+# ---------------------------------------------------------------------------------- shading
+def shading(density=1.0, opacity=2.0, peak_temp=6000.0, brightness=5.5, noise_scale=1.0,
+            noise_strength=0.85, warmth=0.4, doppler=0.0, time=0.0, inflow=0.0, inner=6.0,
+            outer=20.0) -> Shading:
+    """Shading uniforms; defaults of src/Settings.h:32-45 (thin disk [6, 20])."""
+    return Shading(density, opacity, peak_temp, brightness, noise_scale, noise_strength, warmth,
+                   doppler, time, inflow, inner, outer)
+
+
+_tables = {}
+
+
+def noise_volume(seed=7, threads=8) -> np.ndarray:
+    """The 128^3 R16F noise volume (uint16 texels, index (z, y, x)) of noise_bake.comp (cached)."""
+    key = ("noise", seed)
+    if key not in _tables:
+        v = np.zeros((128, 128, 128), np.uint16)
+        lib().oracle_noise_bake(v.ctypes.data, seed, threads)
+        _tables[key] = v
+    return _tables[key]
+
+
+def blackbody_lut() -> np.ndarray:
+    """(256, 4) float32 blackbody table of DiskTextures::bakeBlackbodyLut."""
+    if "lut" not in _tables:
+        t = np.zeros((256, 4), np.float32)
+        lib().oracle_blackbody_lut(t.ctypes.data)
+        _tables["lut"] = t
+    return _tables["lut"]
+
+
+def blackbody(T, lut=None) -> np.ndarray:
+    lut = blackbody_lut() if lut is None else np.ascontiguousarray(lut, np.float32)
+    o = (C.c_float * 4)()
+    lib().oracle_blackbody(lut.ctypes.data, T, o)
+    return np.array(o[:], np.float32)
+
+
+def thin_disk_sample(sh: Shading, r, phi, g, r_cam, vol=None, lut=None) -> np.ndarray:
+    """thinDiskSample of one crossing: (r, g, b, alpha) float32."""
+    vol = noise_volume() if vol is None else vol
+    lut = blackbody_lut() if lut is None else lut
+    l2 = np.float32(1.0) - np.float32(2.0) / np.float32(r_cam)
+    lapse = np.float32(np.sqrt(max(l2, np.float32(1e-4)), dtype=np.float32))
+    o = (C.c_float * 4)()
+    lib().oracle_thin_disk_sample(C.byref(sh), vol.ctypes.data, lut.ctypes.data, float(lapse),
+                                  r, phi, g, o)
+    return np.array(o[:], np.float32)
+
+
+def shade_rays(cam, params: LawSParams, sh: Shading, rec: dict, first=0, stride=1, vol=None,
+               lut=None):
+    """Thin-disk colour of a camera's rays from their march records (rec: the RayBuffers.numpy()
+    layout of pixels first, first + stride, ...): (rgb (n, 3), sky_weight (n,)) float32."""
+    vol = noise_volume() if vol is None else vol
+    lut = blackbody_lut() if lut is None else lut
+    n = len(rec["flags"])
+    fl = np.ascontiguousarray(rec["flags"], np.uint8)
+    nc = np.ascontiguousarray(np.minimum(rec["n_cross"], 255), np.uint8)
+    dr = np.ascontiguousarray(rec["disk_r"], np.float32)
+    dp = np.ascontiguousarray(rec["disk_phi"], np.float32)
+    dg = np.ascontiguousarray(rec["disk_g"], np.float32)
+    rgb = np.zeros((3, n), np.float32)
+    sky = np.zeros(n, np.float32)
+    p, pp = _f(cam["pos"], 3)
+    b, bp = _f(cam["basis"], 9)
+    j, jp = _f(cam.get("jitter", (0.0, 0.0)), 2)
+    lib().oracle_shade_rays(pp, bp, cam["tan_half_fov"], cam["width"], cam["height"], jp,
+                            int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
+                            int(cam.get("camera_id", 0)), int(cam.get("sample", 0)),
+                            C.byref(params), C.byref(sh), vol.ctypes.data, lut.ctypes.data, first,
+                            n, stride, fl.ctypes.data, nc.ctypes.data, dr.ctypes.data,
+                            dp.ctypes.data, dg.ctypes.data, rgb.ctypes.data, sky.ctypes.data)
+    return rgb.T.copy(), sky
+
+
+def shade_march(cam, params: LawSParams, sh: Shading, mode="band", first=0, count=None, stride=1,
+                vol=None, lut=None):
+    """The same colour marched from scratch with every crossing shaded in the loop (the
+    reference's structure): (rgb (n, 3), sky_weight (n,))."""
+    vol = noise_volume() if vol is None else vol
+    lut = blackbody_lut() if lut is None else lut
+    n = cam["width"] * cam["height"]
+    if count is None:
+        count = (n - first + stride - 1) // stride
+    rgb = np.zeros((3, count), np.float32)
+    sky = np.zeros(count, np.float32)
+    p, pp = _f(cam["pos"], 3)
+    b, bp = _f(cam["basis"], 9)
+    j, jp = _f(cam.get("jitter", (0.0, 0.0)), 2)
+    lib().oracle_shade_march(pp, bp, cam["tan_half_fov"], cam["width"], cam["height"], jp,
+                             int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
+                             int(cam.get("camera_id", 0)), int(cam.get("sample", 0)),
+                             C.byref(params), C.byref(sh), vol.ctypes.data, lut.ctypes.data,
+                             first, count, stride, _MODES[mode], rgb.ctypes.data, sky.ctypes.data)
+    return rgb.T.copy(), sky
+
+
+def accumulate(cur, acc, sample_index: int, cap: int) -> np.ndarray:
+    """acc after accumulate.frag with Renderer::accumulatePass's weight (new array)."""
+    cur = np.ascontiguousarray(cur, np.float32)
+    out = np.ascontiguousarray(acc, np.float32).copy()
+    lib().oracle_accumulate(cur.ctypes.data, out.ctypes.data, cur.size, sample_index, cap)
+    return out
+
+
+def accumulate_weight(sample_index: int, cap: int) -> float:
+    return lib().oracle_accumulate_weight(sample_index, cap)
+
+
+def halton_jitter(sample: int) -> np.ndarray:
+    o = (C.c_float * 2)()
+    lib().oracle_halton_jitter(sample, o)
+    return np.array(o[:], np.float32)

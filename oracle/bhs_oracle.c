@@ -198,6 +198,10 @@ float oracle_atan2f(float y, float x) {
 /* ================================================================================================
  * law S in float and in double
  * ============================================================================================== */
+/* Observer of every in-annulus crossing of a march (index in march order, r, phi, g); a non-zero
+ * return ends the march. */
+typedef int (*oracle_cross_fn)(void* ctx, int index, float r, float phi, float g);
+
 #define REAL float
 #define FMA fmaf
 #define SQRT sqrtf
@@ -307,7 +311,7 @@ void oracle_lawS_f32(const float pos[3], const float dir[3], const oracle_lawS_p
     double st[16];
     memset(st, 0, sizeof st);
     lawS_setup_f32(pos, dir, &s);
-    lawS_march_f32(pos, &s, p, out, p->root64 ? st : NULL);
+    lawS_march_f32(pos, &s, p, out, p->root64 ? st : NULL, NULL, NULL);
     if (p->root64 && out->n_cross > 0)
         apply_root64(p, st, (double)s.c3, (double)s.b_axis, (double)s.lapse_g, out);
 }
@@ -320,7 +324,7 @@ void oracle_lawS_f64(const float pos[3], const float dir[3], const oracle_lawS_p
     double st[16];
     memset(st, 0, sizeof st);
     lawS_setup_f64(P, d, &s);
-    lawS_march_f64(P, &s, p, out, p->root64 ? st : NULL);
+    lawS_march_f64(P, &s, p, out, p->root64 ? st : NULL, NULL, NULL);
     if (p->root64 && out->n_cross > 0) apply_root64(p, st, s.c3, s.b_axis, s.lapse_g, out);
     out->flags |= 4;
 }
@@ -719,3 +723,8 @@ void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int
     for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, trace_worker, &jobs[t]);
     for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
 }
+
+/* ================================================================================================
+ * Thin-disk shading, disk tables, accumulation
+ * ============================================================================================== */
+#include "shade.inc"

@@ -159,4 +159,47 @@ void oracle_camera_dirs(const float pos[3], const float B[9], float tan_half_fov
                         uint32_t camera_id, uint32_t sample, int64_t first, int64_t count,
                         int64_t stride, float* dir3_out);
 
+/* Thin-disk shading (shade.inc). */
+typedef struct oracle_shading {
+    float density, opacity, peak_temp, brightness, noise_scale, noise_strength, warmth, doppler,
+        time, inflow;
+    float inner, outer;
+} oracle_shading;
+float oracle_exp2f(float x);
+float oracle_log2f(float x);
+float oracle_logf(float x);
+float oracle_expf(float x);
+float oracle_powf(float x, float y);
+uint16_t oracle_f2h(float x);
+float oracle_h2f(uint16_t h);
+void oracle_grad3(int cx, int cy, int cz, uint32_t salt, float g[3]);
+float oracle_pgnoise3(float px, float py, float pz, int period, uint32_t salt);
+float oracle_noise_texel(int x, int y, int z, uint32_t seed);
+void oracle_noise_bake(uint16_t* vol, uint32_t seed, int32_t threads);
+float oracle_sample_noise(const uint16_t* vol, float s, float t, float r);
+void oracle_blackbody_lut(float* lut);
+void oracle_blackbody(const float* lut, float T, float o[4]);
+float oracle_temp_profile(const oracle_shading* c, float r);
+float oracle_radial_profile(const oracle_shading* c, float r);
+float oracle_disk_noise(const oracle_shading* c, const uint16_t* vol, float r, float phi,
+                        float y_over_h);
+void oracle_thin_disk_sample(const oracle_shading* c, const uint16_t* vol, const float* lut,
+                             float lapse_cam, float r, float phi, float gphys, float out[4]);
+void oracle_shade_rays(const float pos[3], const float basis[9], float tan_half_fov, int32_t width,
+                       int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                       uint32_t camera_id, uint32_t sample, const oracle_lawS_params* p,
+                       const oracle_shading* c, const uint16_t* vol, const float* lut,
+                       int64_t first, int64_t count, int64_t stride, const uint8_t* flags,
+                       const uint8_t* n_cross, const float* disk_r, const float* disk_phi,
+                       const float* disk_g, float* rgb_out, float* sky_out);
+void oracle_shade_march(const float pos[3], const float basis[9], float tan_half_fov, int32_t width,
+                        int32_t height, const float jitter[2], int32_t jitter_hashed,
+                        uint64_t seed, uint32_t camera_id, uint32_t sample,
+                        const oracle_lawS_params* p, const oracle_shading* c,
+                        const uint16_t* vol, const float* lut, int64_t first, int64_t count,
+                        int64_t stride, int32_t mode, float* rgb_out, float* sky_out);
+float oracle_accumulate_weight(int32_t sample_index, int32_t cap);
+void oracle_accumulate(const float* cur, float* acc, int64_t n, int32_t sample_index, int32_t cap);
+void oracle_halton_jitter(int32_t sample, float out[2]);
+
 #endif
