@@ -23,6 +23,9 @@ footprint: the sky footprint post-pass (bhs_sky_footprint_batch: GLSL-quad deriv
 escape direction and makeFootprint, shaders/blackhole.frag:297-306, shaders/sky.glsl:27-51) over
 one surrogate render of the step's frames, all output planes, timed by HIP events on its stream;
 HBM-bound, algorithmic bytes 82 per pixel.
+shade: the thin-disk colour and transparency pass (bhs_shade_thin_disk_batch, default disk settings)
+over one surrogate render of the step's frames, and the accumulation of its colour (bhs_accumulate);
+HBM-bound, algorithmic bytes 18 per ray plus 12 per recorded crossing (36 per pixel to accumulate).
 gas_lens: the gas lens table (bhs_gas_lens_table, shaders/gas_lens.comp:11-44) at the reference's
 size, 1024 fan rays x 1024 swept angles, beside the oracle's build of it on one thread.
 surrogate.roofline: the surrogate pass (table build + two k_surrogate_tab dispatches) against the
@@ -428,6 +431,8 @@ def main(argv=None) -> int:
         })
         res["footprint"] = _footprint_leg(tracer, sur, cams, params, args.steps, args.warmup,
                                           cpu=not args.no_cpu)
+        res["shade"] = _shade_leg(tracer, sur, cams, params, args.steps, args.warmup,
+                                  cpu=not args.no_cpu)
         if args.config == "c2":
             res["gas_lens"] = _gas_lens_leg(tracer, args.steps, cpu=not args.no_cpu)
         fkey = f"footprint_{args.config}x{len(cams)}"
@@ -533,6 +538,85 @@ def _footprint_leg(tracer, sur, cams, params, steps, warmup, cpu=True):
         rec["cpu_oracle"] = {"value": round(n0 / dt / 1e6, 2), "unit": "Mpix/s", "cores": 1,
                              "kind": "port", "sample": f"one {cam0.width}x{cam0.height} frame"}
     del fout, rays
+    return rec
+
+
+# Shading pass, algorithmic HBM bytes: per ray the flags and crossing count (2 B) read and the
+# colour and sky weight (16 B) written; per recorded crossing its r, phi, g (12 B) read.
+SHADE_BYTES_PER_RAY = 18
+SHADE_BYTES_PER_CROSSING = 12
+ACCUM_BYTES_PER_PIXEL = 36    # current and running rgb read, running rgb written
+
+
+def _shade_leg(tracer, sur, cams, params, steps, warmup, cpu=True):
+    """Times the thin-disk shading pass (bhs_shade_thin_disk_batch at the reference's default disk
+    settings) over one surrogate render of the step's frames, and the accumulation of its colour
+    into a running average: HIP events on the launch stream. cpu: the oracle's shading of the
+    first camera's records on one thread."""
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd import workload as W
+    n = sum(c.width * c.height for c in cams)
+    stream = torch.cuda.current_stream()
+    rays = bhs.RayBuffers(n, "cuda:0")
+    sur.render_batch(cams, params, rays, stream)
+    sh = bhs.disk_shading()
+    out = bhs.ShadeBuffers(n, "cuda:0")
+    fn = lambda: tracer.shade_thin_disk_batch(cams, rays, params, sh, out, stream)
+    for _ in range(max(warmup, 1)):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    rec_cross = int(torch.clamp(rays.n_cross.to(torch.int32), max=2).sum())
+    many = int((rays.n_cross.to(torch.int32) > 2).sum())
+    nbytes = n * SHADE_BYTES_PER_RAY + rec_cross * SHADE_BYTES_PER_CROSSING
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    acc = torch.zeros_like(out.rgb)
+    tracer.accumulate(out.rgb, acc, 0, stream=stream)
+    a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a0.record(stream)
+    for k in range(steps):
+        tracer.accumulate(out.rgb, acc, k + 1, stream=stream)
+    a1.record(stream)
+    a1.synchronize()
+    ams = a0.elapsed_time(a1) / steps
+    ags = n * ACCUM_BYTES_PER_PIXEL / (ams * 1e-3) / 1e9
+    rec = {"kernel": "k_shade_planes + k_shade_complete (thin-disk colour and transparency, "
+                     "default disk settings)",
+           "call_ms": round(ms, 5), "Gpix_per_s": round(n / (ms * 1e-3) / 1e9, 2),
+           "recorded_crossings": rec_cross, "rays_remarched_for_more_crossings": many,
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": W.PEAK_HBM_GBS,
+                        "unit": "GB/s", "frac": round(gbs / W.PEAK_HBM_GBS, 4),
+                        "bytes_per_launch": nbytes},
+           "accumulate": {"kernel": "k_accumulate (running average of the rgb planes)",
+                          "call_ms": round(ams, 5),
+                          "roofline": {"bound": "hbm", "achieved": round(ags, 1),
+                                       "peak": W.PEAK_HBM_GBS, "unit": "GB/s",
+                                       "frac": round(ags / W.PEAK_HBM_GBS, 4)}}}
+    if cpu:
+        import oracle as O
+        cam0 = cams[0]
+        n0 = cam0.width * cam0.height
+        d = rays.disk[:, :n0].cpu().numpy()
+        r0 = dict(flags=rays.flags[:n0].cpu().numpy(), n_cross=rays.n_cross[:n0].cpu().numpy(),
+                  disk_r=d[0:2].T.copy(), disk_phi=d[2:4].T.copy(), disk_g=d[4:6].T.copy())
+        op = O.lawS_params(max_steps=params.max_steps, step_scale=params.step_scale,
+                           disk_step=params.disk_step, disk_inner=params.disk_inner,
+                           disk_outer=params.disk_outer)
+        osh = O.shading(inner=params.disk_inner, outer=params.disk_outer)
+        O.noise_volume()
+        O.blackbody_lut()
+        t0 = time.perf_counter()
+        O.shade_rays(cam0.as_dict(), op, osh, r0)
+        dt = time.perf_counter() - t0
+        rec["cpu_oracle"] = {"value": round(n0 / dt / 1e6, 2), "unit": "Mrays/s", "cores": 1,
+                             "kind": "port", "sample": f"one {cam0.width}x{cam0.height} frame"}
+    del out, rays, acc
     return rec
 
 
