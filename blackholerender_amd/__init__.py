@@ -26,7 +26,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _capi
-from ._capi import (CLASS_DISK, CLASS_ESCAPE, CLASS_HORIZON, DISK_NONE, DISK_THIN, FLAG_BAND64,
+from ._capi import (CLASS_DISK, CLASS_ESCAPE, CLASS_HORIZON, DISK_NONE, DISK_THIN,
+                    DISK_VOLUMETRIC, FLAG_BAND64,
                     FLAG_ESCAPED, FLAG_MAXSTEPS, FLAG_ROOT64, FLAG_SURROGATE,
                     PREC_F32, PREC_F32_BAND64, PREC_F64, TRACE_CAPTURED, TRACE_ESCAPED,
                     TRACE_MAXSTEPS)
@@ -37,7 +38,8 @@ __all__ = ["Tracer", "BhsError", "TraceResult", "RayBuffers", "FootprintBuffers"
            "trace", "Camera", "orbit_camera", "look_basis", "PRESETS", "critical_impact_parameter",
            "CLASS_HORIZON", "CLASS_DISK", "CLASS_ESCAPE", "FLAG_MAXSTEPS", "FLAG_ESCAPED",
            "FLAG_BAND64", "FLAG_ROOT64", "FLAG_SURROGATE", "PREC_F32",
-           "PREC_F32_BAND64", "PREC_F64", "DISK_NONE", "DISK_THIN", "TRACE_ESCAPED",
+           "PREC_F32_BAND64", "PREC_F64", "DISK_NONE", "DISK_THIN", "DISK_VOLUMETRIC",
+           "TRACE_ESCAPED",
            "TRACE_CAPTURED", "TRACE_MAXSTEPS"]
 
 kHorizon = 2.0
@@ -468,6 +470,53 @@ class Tracer:
                                                  C.c_void_p(self._stream_ptr(stream)))
         self._check(rc, "bhs_shade_thin_disk_batch")
         return out
+
+    def render_volumetric(self, cam: Camera, params: _capi.MarchParams | None = None,
+                          shading: _capi.DiskShading | None = None,
+                          rays: RayBuffers | None = None, out: ShadeBuffers | None = None,
+                          stream=None):
+        """Volumetric disk render (bhs_render_volumetric; blackhole.frag DISK_VOLUMETRIC,
+        disk.glsl:81-110). params default to the High preset with disk_mode DISK_VOLUMETRIC.
+        Returns (ShadeBuffers, RayBuffers or None)."""
+        if params is None:
+            params = march_params(disk_mode=DISK_VOLUMETRIC)
+        if shading is None:
+            shading = disk_shading()
+        n = cam.width * cam.height
+        if out is None:
+            out = ShadeBuffers(n, f"cuda:{self.device}")
+        out.check(n, self.device, "render_volumetric: out")
+        if rays is not None:
+            rays.check(n, self.device, "render_volumetric: rays")
+        cs = _camera_struct(cam)
+        os_ = out.soa()
+        rs = C.byref(rays.soa()) if rays is not None else None
+        rc = self._lib.bhs_render_volumetric(self._ctx, C.byref(cs), C.byref(params),
+                                             C.byref(shading), rs, C.byref(os_),
+                                             C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_render_volumetric")
+        return out, rays
+
+    def render_volumetric_batch(self, cams, params, shading: _capi.DiskShading | None = None,
+                                rays: RayBuffers | None = None, out: ShadeBuffers | None = None,
+                                stream=None):
+        """The volumetric render of up to 64 cameras in one call (bhs_render_volumetric_batch)."""
+        ca, pa = self._batch_structs(cams, params)
+        if shading is None:
+            shading = disk_shading()
+        total = self.batch_offsets(cams)[-1]
+        if out is None:
+            out = ShadeBuffers(total, f"cuda:{self.device}")
+        out.check(total, self.device, "render_volumetric_batch: out")
+        if rays is not None:
+            rays.check(total, self.device, "render_volumetric_batch: rays")
+        os_ = out.soa()
+        rs = C.byref(rays.soa()) if rays is not None else None
+        rc = self._lib.bhs_render_volumetric_batch(self._ctx, ca, pa, len(cams), C.byref(shading),
+                                                   rs, C.byref(os_),
+                                                   C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_render_volumetric_batch")
+        return out, rays
 
     def accumulate(self, cur_rgb, acc_rgb, sample_index: int, cap: int = 256, stream=None):
         """acc_rgb <- running average with cur_rgb (bhs_accumulate; (3, n) float32 tensors)."""

@@ -19,7 +19,7 @@ TRACE_ESCAPED, TRACE_CAPTURED, TRACE_MAXSTEPS = 0, 1, 2
 CLASS_HORIZON, CLASS_DISK, CLASS_ESCAPE = 0, 1, 2
 FLAG_MAXSTEPS, FLAG_ESCAPED, FLAG_BAND64, FLAG_ROOT64, FLAG_SURROGATE = 1, 2, 4, 8, 16
 PREC_F32, PREC_F32_BAND64, PREC_F64 = 0, 1, 2
-DISK_NONE, DISK_THIN = 0, 1
+DISK_NONE, DISK_THIN, DISK_VOLUMETRIC = 0, 1, 2
 
 
 class TraceResult(C.Structure):
@@ -123,6 +123,12 @@ def load():
                                       C.POINTER(ShadeSoA), vp]
     L.bhs_shade_thin_disk_batch.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(DiskShading),
                                             C.POINTER(RaySoA), C.POINTER(ShadeSoA), vp]
+    L.bhs_render_volumetric.argtypes = [vp, C.POINTER(Camera), C.POINTER(MarchParams),
+                                        C.POINTER(DiskShading), vp, C.POINTER(ShadeSoA), vp]
+    L.bhs_render_volumetric_batch.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(DiskShading), vp,
+                                              C.POINTER(ShadeSoA), vp]
+    L.bhs_render_volumetric.restype = C.c_int
+    L.bhs_render_volumetric_batch.restype = C.c_int
     L.bhs_accumulate.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, vp]
     L.bhs_accumulate_weight.argtypes = [C.c_int32, C.c_int32]
     L.bhs_accumulate_weight.restype = C.c_float

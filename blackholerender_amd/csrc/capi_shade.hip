@@ -9,6 +9,7 @@
 #include "../../include/bhs_trace.h"
 #include "capi_internal.hpp"
 #include "shade.hpp"
+#include "volume.hpp"
 
 using namespace bhs::capi;
 
@@ -131,9 +132,84 @@ int shade_common(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* p
     return BHS_OK;
 }
 
+int volumetric_common(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_params* params,
+                      int32_t ncam, const bhs_disk_shading* sh, const bhs_ray_soa* rays,
+                      const bhs_shade_soa* out, void* stream, const char* what) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!cams || !params) return fail(ctx, BHS_E_INVALID, "%s: cameras or params is NULL", what);
+    if (ncam < 1 || (uint32_t)ncam > bhs::kMaxBatchCams)
+        return fail(ctx, BHS_E_INVALID, "%s: ncam must be in 1..64", what);
+    if (!out || !out->rgb_dev || !out->sky_weight_dev)
+        return fail(ctx, BHS_E_INVALID, "%s: out needs rgb and sky_weight planes", what);
+    int st;
+    if ((st = check_shading(ctx, sh)) != BHS_OK) return st;
+    // The batch table takes the thin-disk form of the parameters (same step control fields); a
+    // single FP64 camera marches every ray in FP64.
+    std::vector<bhs_march_params> pp(params, params + ncam);
+    bool all_f64 = false;
+    for (int32_t k = 0; k < ncam; ++k) {
+        if (pp[k].disk_mode != BHS_DISK_VOLUMETRIC)
+            return fail(ctx, BHS_E_INVALID, "%s: disk_mode must be BHS_DISK_VOLUMETRIC", what);
+        if (pp[k].root64 || pp[k].cross_all)
+            return fail(ctx, BHS_E_INVALID, "%s: root64 and cross_all do not apply", what);
+        pp[k].disk_mode = BHS_DISK_THIN;
+        if (pp[k].precision == BHS_PREC_F64) {
+            if (ncam > 1)
+                return fail(ctx, BHS_E_INVALID, "%s: BHS_PREC_F64 is for single cameras", what);
+            all_f64 = true;
+            pp[k].precision = BHS_PREC_F32;
+        }
+    }
+    BatchTable t;
+    if ((st = build_batch(ctx, cams, pp.data(), ncam, &t)) != BHS_OK) return st;
+    DeviceGuard g(ctx->device);
+    if ((st = ensure_disk_tables(ctx)) != BHS_OK) return st;
+    bhs::Scratch sc;
+    if ((st = prepare_scratch(ctx, t.n_rays, t.n_items, false, &sc)) != BHS_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
+    void* table_dev = nullptr;
+    if ((st = upload_batch(ctx, t, s, &table_dev)) != BHS_OK) return st;
+    const bhs::shade::ShadeConst sc_sh{sh->density,     sh->opacity, sh->peak_temp, sh->brightness,
+                                       sh->noise_scale, sh->noise_strength, sh->warmth,
+                                       sh->doppler,     sh->time,    sh->inflow, 0.0f, 0.0f};
+    bhs_ray_soa none{};
+    const bhs::SoA ro = make_soa(rays ? rays : &none, t.n_rays);
+    const bhs::ShadeOut o{out->rgb_dev, out->sky_weight_dev, t.n_rays};
+    hipError_t e;
+    bhs_ctx::TimingRec* tr = next_timer(ctx);
+    if (tr) (void)hipEventRecord(tr->call0, s);
+    if ((e = bhs::launch_zero(ctx->scratch, kStatsBytes, s)) != hipSuccess)
+        return hip_fail(ctx, e, "volumetric clear");
+    const bhs::MarchConst* table = (const bhs::MarchConst*)table_dev;
+    const uint32_t* base = (const uint32_t*)((char*)table_dev + table_cams_bytes(ncam));
+    if ((e = bhs::launch_volumetric(table, base, (uint32_t)ncam, t.n_items, all_f64, sc_sh, ro, o,
+                                    ctx->noise_vol, ctx->bb_lut, sc, ctx->num_cus, s,
+                                    tr ? tr->k0 : nullptr, tr ? tr->k1 : nullptr)) != hipSuccess)
+        return hip_fail(ctx, e, "volumetric launch");
+    if (tr) (void)hipEventRecord(tr->call1, s);
+    note_stream(ctx, s);
+    return BHS_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int bhs_render_volumetric(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
+                          const bhs_disk_shading* shading, const bhs_ray_soa* rays,
+                          const bhs_shade_soa* out, void* stream) {
+    return volumetric_common(ctx, cam, params, 1, shading, rays, out, stream,
+                             "bhs_render_volumetric");
+}
+
+int bhs_render_volumetric_batch(bhs_ctx* ctx, const bhs_camera* cams,
+                                const bhs_march_params* params, int32_t ncam,
+                                const bhs_disk_shading* shading, const bhs_ray_soa* rays,
+                                const bhs_shade_soa* out, void* stream) {
+    return volumetric_common(ctx, cams, params, ncam, shading, rays, out, stream,
+                             "bhs_render_volumetric_batch");
+}
 
 int bhs_shade_thin_disk(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
                         const bhs_disk_shading* shading, const bhs_ray_soa* rays,

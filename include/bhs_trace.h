@@ -35,6 +35,8 @@
  *                               dFdx/dFdy of escDir over 2x2 quads, the unlensed footprint at
  *                               status edges) and makeFootprint, shaders/sky.glsl:17-51, as a
  *                               post-pass over a render's ray planes.
+ *  - bhs_render_volumetric[_batch] : the trace pass with the volumetric disk
+ *                               (blackhole.frag:142-253 DISK_VOLUMETRIC, disk.glsl:81-110).
  *  - bhs_shade_thin_disk[_batch], bhs_accumulate, bhs_halton_jitter, bhs_disk_textures :
  *                               the thin-disk colour and transparency of blackhole.frag:209-216,234
  *                               (disk.glsl:113-130, blackbody.glsl, noise_bake.comp,
@@ -85,7 +87,8 @@ enum {
 };
 
 enum { BHS_PREC_F32 = 0, BHS_PREC_F32_BAND64 = 1, BHS_PREC_F64 = 2 };
-enum { BHS_DISK_NONE = 0, BHS_DISK_THIN = 1 };
+enum { BHS_DISK_NONE = 0, BHS_DISK_THIN = 1, BHS_DISK_VOLUMETRIC = 2 };   /* VOLUMETRIC: only
+                                 bhs_render_volumetric[_batch] */
 
 /* TraceResult (src/physics/Geodesic.h:97-103). */
 typedef struct bhs_trace_result {
@@ -274,6 +277,24 @@ int bhs_shade_thin_disk_batch(bhs_ctx* ctx, const bhs_camera* cams,
                               const bhs_march_params* params, int32_t ncam,
                               const bhs_disk_shading* shading, const bhs_ray_soa* rays,
                               const bhs_shade_soa* out, void* stream);
+
+/* Volumetric disk render (shaders/blackhole.frag:142-253 with uDiskMode = DISK_VOLUMETRIC and
+ * shaders/disk.glsl:81-110, the High and Ultra presets' disk): law S with the volumetric slab rule
+ * (steps inside the slab capped at 6 disk_step), emission and absorption (volumetricDiskStep,
+ * diskDensity with the vertical profile, noise and puffing) sampled at ceil(seg / disk_step)
+ * points of the cubic Hermite interpolant of every RK4 step that starts inside the slab, and the
+ * march stopped where the transmittance falls below 0.004. params[k].disk_mode must be
+ * BHS_DISK_VOLUMETRIC (root64 and cross_all 0; FP64 band rule as in bhs_render; BHS_PREC_F64 for
+ * a single camera only). out receives the colour and sky weight; rays (may be NULL, any plane
+ * NULL) receives class (horizon / disk = absorbed / escape), flags, steps, escape direction and
+ * b, with n_cross and the disk planes 0. Stream-ordered. */
+int bhs_render_volumetric(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
+                          const bhs_disk_shading* shading, const bhs_ray_soa* rays,
+                          const bhs_shade_soa* out, void* stream);
+int bhs_render_volumetric_batch(bhs_ctx* ctx, const bhs_camera* cams,
+                                const bhs_march_params* params, int32_t ncam,
+                                const bhs_disk_shading* shading, const bhs_ray_soa* rays,
+                                const bhs_shade_soa* out, void* stream);
 
 /* Progressive supersampling (Renderer::accumulatePass, src/Renderer.cpp:389-410, and
  * shaders/accumulate.frag:8-11): acc = w >= 1 ? cur : acc (1 - w) + cur w over 3n floats (rgb

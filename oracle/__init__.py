@@ -188,6 +188,14 @@ def _bind(path: str):
     L.oracle_accumulate_weight.restype = f
     L.oracle_accumulate.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32]
     L.oracle_halton_jitter.argtypes = [C.c_int32, fp]
+    L.oracle_volumetric.argtypes = [fp, fp, f, C.c_int32, C.c_int32, fp, C.c_int32, C.c_uint64,
+                                    C.c_uint32, C.c_uint32, C.POINTER(LawSParams), sp, vp, vp,
+                                    C.c_int64, C.c_int64, C.c_int64, C.c_int32, vp, vp, vp, vp,
+                                    vp, vp, C.c_int32]
+    L.oracle_vol_ray.argtypes = [fp, fp, C.POINTER(LawSParams), sp, vp, vp, C.c_int32, fp]
+    L.oracle_vol_ray.restype = C.c_int32
+    L.oracle_vol_sample.argtypes = [sp, f, f, vp, vp, f, f, f, f, f, f, fp, fp]
+    L.oracle_hermite_f64.argtypes = [dp, dp, dp, dp, C.c_double, dp]
     return L
 
 
@@ -548,3 +556,60 @@ def halton_jitter(sample: int) -> np.ndarray:
     o = (C.c_float * 2)()
     lib().oracle_halton_jitter(sample, o)
     return np.array(o[:], np.float32)
+
+
+def vol_ray(pos, d, params: LawSParams, sh: Shading, mode="f32", vol=None, lut=None) -> dict:
+    """One volumetric ray: status, rgb, sky_weight, steps, esc."""
+    vol = noise_volume() if vol is None else vol
+    lut = blackbody_lut() if lut is None else lut
+    o = (C.c_float * 8)()
+    p_, pp = _f(pos, 3)
+    d_, dp_ = _f(d, 3)
+    st = lib().oracle_vol_ray(pp, dp_, C.byref(params), C.byref(sh), vol.ctypes.data,
+                              lut.ctypes.data, 1 if mode == "f64" else 0, o)
+    return dict(status=st, rgb=np.array(o[0:3], np.float32), sky_weight=np.float32(o[3]),
+                steps=int(o[4]), esc=np.array(o[5:8], np.float32))
+
+
+def vol_sample(sh: Shading, thickness, lapse, p, r, ds, b_axis, rgb_t, vol=None, lut=None):
+    """One volumetricDiskStep onto the composite rgb_t = (r, g, b, T): returns (r, g, b, T)."""
+    vol = noise_volume() if vol is None else vol
+    lut = blackbody_lut() if lut is None else lut
+    i = (C.c_float * 4)(*[float(v) for v in rgb_t])
+    o = (C.c_float * 4)()
+    lib().oracle_vol_sample(C.byref(sh), thickness, lapse, vol.ctypes.data, lut.ctypes.data,
+                            float(p[0]), float(p[1]), float(p[2]), r, ds, b_axis, i, o)
+    return np.array(o[:], np.float32)
+
+
+def hermite(x0, m0, x1, m1, t) -> np.ndarray:
+    o = _D3()
+    lib().oracle_hermite_f64(_d3(x0), _d3(m0), _d3(x1), _d3(m1), t, o)
+    return np.array(o[:])
+
+
+def volumetric(cam, params: LawSParams, sh: Shading, mode="band", first=0, count=None, stride=1,
+               vol=None, lut=None, threads=8) -> dict:
+    """Volumetric disk render (blackhole.frag DISK_VOLUMETRIC, disk.glsl:81-110) of pixels first,
+    first + stride, ... : rgb (n, 3), sky_weight, cls, flags, steps, esc (n, 3)."""
+    vol = noise_volume() if vol is None else vol
+    lut = blackbody_lut() if lut is None else lut
+    n = cam["width"] * cam["height"]
+    if count is None:
+        count = (n - first + stride - 1) // stride
+    o = dict(rgb=np.zeros((3, count), np.float32), sky_weight=np.zeros(count, np.float32),
+             cls=np.zeros(count, np.uint8), flags=np.zeros(count, np.uint8),
+             steps=np.zeros(count, np.int32), esc=np.zeros((3, count), np.float32))
+    p, pp = _f(cam["pos"], 3)
+    b, bp = _f(cam["basis"], 9)
+    j, jp = _f(cam.get("jitter", (0.0, 0.0)), 2)
+    lib().oracle_volumetric(pp, bp, cam["tan_half_fov"], cam["width"], cam["height"], jp,
+                            int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
+                            int(cam.get("camera_id", 0)), int(cam.get("sample", 0)),
+                            C.byref(params), C.byref(sh), vol.ctypes.data, lut.ctypes.data, first,
+                            count, stride, _MODES[mode], *(o[k].ctypes.data for k in
+                                                          ("rgb", "sky_weight", "cls", "flags",
+                                                           "steps", "esc")), threads)
+    o["rgb"] = o["rgb"].T.copy()
+    o["esc"] = o["esc"].T.copy()
+    return o

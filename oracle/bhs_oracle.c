@@ -728,3 +728,159 @@ void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int
  * Thin-disk shading, disk tables, accumulation
  * ============================================================================================== */
 #include "shade.inc"
+
+/* ================================================================================================
+ * Volumetric disk march (float and double)
+ * ============================================================================================== */
+#define REAL float
+#define FMA fmaf
+#define SQRT sqrtf
+#define FABS fabsf
+#define CEIL ceilf
+#define L(x) ((float)(x))
+#define FN(n) n##_f32
+#include "vol_body.inc"
+#undef REAL
+#undef FMA
+#undef SQRT
+#undef FABS
+#undef CEIL
+#undef L
+#undef FN
+
+#define REAL double
+#define FMA fma
+#define SQRT sqrt
+#define FABS fabs
+#define CEIL ceil
+#define L(x) ((double)(x))
+#define FN(n) n##_f64
+#include "vol_body.inc"
+#undef REAL
+#undef FMA
+#undef SQRT
+#undef FABS
+#undef CEIL
+#undef L
+#undef FN
+
+/* One volumetric ray (mode 0 FP32, 1 FP64) from a float position and direction; returns the
+ * status (0 out of steps, 1 escaped, 2 horizon, 3 opaque). out: rgb (3), sky weight, steps,
+ * escape direction (3). */
+int32_t oracle_vol_ray(const float pos[3], const float dir[3], const oracle_lawS_params* p,
+                       const oracle_shading* c, const uint16_t* vol, const float* lut, int32_t mode,
+                       float out[8]) {
+    sh_comp cp = {{0.0f, 0.0f, 0.0f}, 1.0f, 0, c, vol, lut, sh_cam_lapse(pos), 0};
+    int steps = 0, status;
+    float esc[3];
+    if (mode == 1) {
+        const double P[3] = {pos[0], pos[1], pos[2]};
+        const double d[3] = {dir[0], dir[1], dir[2]};
+        setup_f64 s;
+        lawS_setup_f64(P, d, &s);
+        status = vol_march_f64(P, &s, p, c, cp.lapse, vol, lut, &cp, &steps, esc);
+    } else {
+        setup_f32 s;
+        lawS_setup_f32(pos, dir, &s);
+        status = vol_march_f32(pos, &s, p, c, cp.lapse, vol, lut, &cp, &steps, esc);
+    }
+    for (int k = 0; k < 3; ++k) out[k] = cp.rgb[k];
+    out[3] = status == 1 ? cp.T : 0.0f;
+    out[4] = (float)steps;
+    for (int k = 0; k < 3; ++k) out[5 + k] = esc[k];
+    return status;
+}
+
+/* One volumetricDiskStep at p (|p.xz| = r) onto a running composite (rgb, T): out = rgb', T'. */
+void oracle_vol_sample(const oracle_shading* c, float thickness, float lapse, const uint16_t* vol,
+                       const float* lut, float px, float py, float pz, float r, float ds,
+                       float b_axis, const float rgb_t[4], float out[4]) {
+    sh_comp cp = {{rgb_t[0], rgb_t[1], rgb_t[2]}, rgb_t[3], 0, c, vol, lut, lapse, 0};
+    vol_sample_f32(c, thickness, lapse, vol, lut, px, py, pz, r, ds, b_axis, &cp);
+    out[0] = cp.rgb[0];
+    out[1] = cp.rgb[1];
+    out[2] = cp.rgb[2];
+    out[3] = cp.T;
+}
+
+void oracle_hermite_f64(const double x0[3], const double m0[3], const double x1[3],
+                        const double m1[3], double t, double out[3]) {
+    hermite_f64(x0, m0, x1, m1, t, out);
+}
+
+typedef struct vol_job {
+    const float *pos, *basis;
+    float tan_half_fov;
+    int32_t width, height;
+    const float* jitter;
+    int32_t jitter_hashed;
+    uint64_t seed;
+    uint32_t camera_id, sample;
+    const oracle_lawS_params* p;
+    const oracle_shading* c;
+    const uint16_t* vol;
+    const float* lut;
+    int64_t first, count, stride;
+    int32_t mode;
+    float *rgb, *sky;
+    uint8_t *cls, *flags;
+    int32_t* steps;
+    float* esc;
+    int tid, nthreads;
+} vol_job;
+
+static void* vol_worker(void* arg) {
+    vol_job* j = (vol_job*)arg;
+    const int64_t count = j->count;
+    for (int64_t k = j->tid; k < count; k += j->nthreads) {
+        float d[3], o[8];
+        oracle_ray_dir(j->pos, j->basis, j->tan_half_fov, j->width, j->height, j->jitter,
+                       j->jitter_hashed, j->seed, j->camera_id, j->sample, j->first + k * j->stride,
+                       d);
+        int f64 = j->mode == 1;
+        if (j->mode == 2)
+            f64 = fabsf(oracle_lawS_b_f32(j->pos, d) - kBcritF) < j->p->band_rel * kBcritF;
+        const int status = oracle_vol_ray(j->pos, d, j->p, j->c, j->vol, j->lut, f64, o);
+        int flags = f64 ? 4 : 0;
+        if (status == 0) flags |= 1;
+        if (status == 1) flags |= 2;
+        for (int q = 0; q < 3; ++q) j->rgb[k + q * count] = o[q];
+        j->sky[k] = o[3];
+        j->cls[k] = (uint8_t)(status == 1 ? 2 : (status == 3 ? 1 : 0));
+        j->flags[k] = (uint8_t)flags;
+        j->steps[k] = (int32_t)o[4];
+        for (int q = 0; q < 3; ++q) j->esc[k + q * count] = o[5 + q];
+    }
+    return NULL;
+}
+
+/* Volumetric render of count pixels (first, first + stride, ...) of one camera on `threads`
+ * threads: mode 0 FP32, 1 FP64, 2 FP32 with the FP64 band. Outputs per ray: rgb (3 planes of
+ * count), sky weight, class (0 horizon, 1 absorbed, 2 escape), flags (1 out of steps, 2 escaped,
+ * 4 FP64), steps, escape direction (3 planes of count). */
+void oracle_volumetric(const float pos[3], const float basis[9], float tan_half_fov, int32_t width,
+                       int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                       uint32_t camera_id, uint32_t sample, const oracle_lawS_params* p,
+                       const oracle_shading* c, const uint16_t* vol, const float* lut,
+                       int64_t first, int64_t count, int64_t stride, int32_t mode, float* rgb_out,
+                       float* sky_out, uint8_t* cls_out, uint8_t* flags_out, int32_t* steps_out,
+                       float* esc_out, int32_t threads) {
+    static const float zero2[2] = {0.0f, 0.0f};
+    if (threads < 1) threads = 1;
+    if (threads > 64) threads = 64;
+    vol_job jobs[64];
+    pthread_t th[64];
+    for (int t = 0; t < threads; ++t) {
+        vol_job j = {pos, basis, tan_half_fov, width, height, jitter ? jitter : zero2,
+                     jitter_hashed, seed, camera_id, sample, p, c, vol, lut, first, count,
+                     stride, mode, rgb_out, sky_out, cls_out, flags_out, steps_out, esc_out, t,
+                     threads};
+        jobs[t] = j;
+    }
+    if (threads == 1) {
+        vol_worker(&jobs[0]);
+        return;
+    }
+    for (int t = 0; t < threads; ++t) pthread_create(&th[t], NULL, vol_worker, &jobs[t]);
+    for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
+}

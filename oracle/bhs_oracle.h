@@ -202,4 +202,20 @@ float oracle_accumulate_weight(int32_t sample_index, int32_t cap);
 void oracle_accumulate(const float* cur, float* acc, int64_t n, int32_t sample_index, int32_t cap);
 void oracle_halton_jitter(int32_t sample, float out[2]);
 
+void oracle_volumetric(const float pos[3], const float basis[9], float tan_half_fov, int32_t width,
+                       int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                       uint32_t camera_id, uint32_t sample, const oracle_lawS_params* p,
+                       const oracle_shading* c, const uint16_t* vol, const float* lut,
+                       int64_t first, int64_t count, int64_t stride, int32_t mode, float* rgb_out,
+                       float* sky_out, uint8_t* cls_out, uint8_t* flags_out, int32_t* steps_out,
+                       float* esc_out, int32_t threads);
+int32_t oracle_vol_ray(const float pos[3], const float dir[3], const oracle_lawS_params* p,
+                       const oracle_shading* c, const uint16_t* vol, const float* lut, int32_t mode,
+                       float out[8]);
+void oracle_vol_sample(const oracle_shading* c, float thickness, float lapse, const uint16_t* vol,
+                       const float* lut, float px, float py, float pz, float r, float ds,
+                       float b_axis, const float rgb_t[4], float out[4]);
+void oracle_hermite_f64(const double x0[3], const double m0[3], const double x1[3],
+                        const double m1[3], double t, double out[3]);
+
 #endif
