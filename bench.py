@@ -26,6 +26,8 @@ HBM-bound, algorithmic bytes 82 per pixel.
 shade: the thin-disk colour and transparency pass (bhs_shade_thin_disk_batch, default disk settings)
 over one surrogate render of the step's frames, and the accumulation of its colour (bhs_accumulate);
 HBM-bound, algorithmic bytes 18 per ray plus 12 per recorded crossing (36 per pixel to accumulate).
+volumetric: the volumetric disk (bhs_render_volumetric_batch) at the High preset on 8 jittered 1080p
+frames per call, with the VALU rate of its FP32 march and the oracle's march of a sample beside it.
 gas_lens: the gas lens table (bhs_gas_lens_table, shaders/gas_lens.comp:11-44) at the reference's
 size, 1024 fan rays x 1024 swept angles, beside the oracle's build of it on one thread.
 surrogate.roofline: the surrogate pass (table build + two k_surrogate_tab dispatches) against the
@@ -341,6 +343,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-full-form", action="store_true", help="skip the full-form record")
     ap.add_argument("--no-c3", action="store_true", help="skip the 64-camera graph record")
     ap.add_argument("--no-per-ray", action="store_true", help="skip the per-ray network record")
+    ap.add_argument("--no-volumetric", action="store_true", help="skip the volumetric disk record")
     ap.add_argument("--tuning", default=None,
                     help="fixed launch shape 'mode,block,waves_per_simd,refill_min' (skips the "
                          "autotuner)")
@@ -433,6 +436,9 @@ def main(argv=None) -> int:
                                           cpu=not args.no_cpu)
         res["shade"] = _shade_leg(tracer, sur, cams, params, args.steps, args.warmup,
                                   cpu=not args.no_cpu)
+        if args.config == "c2" and not args.no_volumetric:
+            res["volumetric"] = _volumetric_leg(tracer, 8, max(3, args.steps // 4), 2,
+                                                cpu=not args.no_cpu)
         if args.config == "c2":
             res["gas_lens"] = _gas_lens_leg(tracer, args.steps, cpu=not args.no_cpu)
         fkey = f"footprint_{args.config}x{len(cams)}"
@@ -617,6 +623,71 @@ def _shade_leg(tracer, sur, cams, params, steps, warmup, cpu=True):
         rec["cpu_oracle"] = {"value": round(n0 / dt / 1e6, 2), "unit": "Mrays/s", "cores": 1,
                              "kind": "port", "sample": f"one {cam0.width}x{cam0.height} frame"}
     del out, rays, acc
+    return rec
+
+
+def _volumetric_leg(tracer, frames, steps, warmup, cpu=True):
+    """The volumetric disk at the High preset (450 steps, disk step 0.18, disk [6, 20], default
+    disk settings) on `frames` jittered 1920x1080 frames of the benchmark camera per call
+    (bhs_render_volumetric_batch; FP32 with the FP64 band): wall time per call and the FP32 march
+    kernel's time (HIP events on its stream); VALU instructions per launch from rocprofv3
+    (profiles/valu.json) over that time. cpu: the oracle's volumetric march of a strided sample of
+    the first frame on this host's CPUs."""
+    import numpy as np
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd import workload as W
+    cams = [W.camera_for(W.C2, sample=k) for k in range(frames)]
+    p = bhs.march_params(disk_mode=bhs.DISK_VOLUMETRIC)
+    sh = bhs.disk_shading()
+    n = sum(c.width * c.height for c in cams)
+    stream = torch.cuda.current_stream()
+    rays = bhs.RayBuffers(n, "cuda:0")
+    out = bhs.ShadeBuffers(n, "cuda:0")
+    fn = lambda: tracer.render_volumetric_batch(cams, p, sh, rays=rays, out=out, stream=stream)
+    fn()
+    torch.cuda.synchronize()
+    tracer.set_timing(True)
+    wall = _timed(fn, steps, warmup)
+    tm = tracer.timing()
+    tracer.set_timing(False)
+    fn()
+    st = tracer.last_stats()
+    cls = rays.cls.to(torch.int32)
+    rec = {"workload": (f"{frames} jittered 1920x1080 frame(s) per call ({n:,} rays), volumetric "
+                        f"disk [6, 20], High preset (450 steps, disk step 0.18)"),
+           "value": round(n * steps / wall / 1e6, 2), "unit": "Mrays/s",
+           "ms_per_step": round(wall / steps * 1e3, 4), "steps": steps, "warmup": warmup,
+           "kernel_ms": round(tm["march_ms"], 5), "call_ms": round(tm["total_ms"], 5),
+           "mean_steps": round(st["steps"] / n, 3), "band64_rays": st["band64"],
+           "absorbed_fraction": round(float((cls == bhs.CLASS_DISK).float().mean()), 5),
+           "escaped_fraction": round(float((cls == bhs.CLASS_ESCAPE).float().mean()), 5)}
+    pm = _pmc(f"volumetric_c2x{frames}")
+    if pm and tm["march_ms"] > 0:
+        gi = pm["valu_insts_per_launch"] / (tm["march_ms"] * 1e-3) / 1e9
+        rec["roofline"] = {"bound": "valu", "achieved": round(gi, 2), "peak": PEAK_VALU_GINST,
+                           "unit": "G wave-instructions/s", "frac": round(gi / PEAK_VALU_GINST, 4),
+                           "kernel": "k_vol_f32", "valu_insts_per_launch":
+                           pm["valu_insts_per_launch"], "source": pm["source"]}
+    if cpu:
+        import oracle as O
+        cam0 = cams[0]
+        th = host_cpus()["threads"]
+        op = O.lawS_params(disk_mode=2)
+        osh = O.shading()
+        O.noise_volume()
+        O.blackbody_lut()
+        stride = 509
+        t0 = time.perf_counter()
+        ro = O.volumetric(cam0.as_dict(), op, osh, mode="band", stride=stride, threads=th)
+        dt = time.perf_counter() - t0
+        rec["cpu_oracle"] = {"value": round(len(ro["cls"]) / dt / 1e6, 4), "unit": "Mrays/s",
+                             "cores": th, "kind": "port",
+                             "sample": f"every {stride}th pixel of the first frame "
+                                       f"({len(ro['cls'])} rays), {dt:.2f} s"}
+        g0 = out.rgb[:, :cam0.width * cam0.height:stride].cpu().numpy().T
+        rec["cpu_oracle"]["bit_exact_on_sample"] = bool(np.array_equal(g0, ro["rgb"]))
+    del rays, out
     return rec
 
 

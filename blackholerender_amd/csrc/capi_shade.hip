@@ -183,9 +183,10 @@ int volumetric_common(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_para
         return hip_fail(ctx, e, "volumetric clear");
     const bhs::MarchConst* table = (const bhs::MarchConst*)table_dev;
     const uint32_t* base = (const uint32_t*)((char*)table_dev + table_cams_bytes(ncam));
-    if ((e = bhs::launch_volumetric(table, base, (uint32_t)ncam, t.n_items, all_f64, sc_sh, ro, o,
-                                    ctx->noise_vol, ctx->bb_lut, sc, ctx->num_cus, s,
-                                    tr ? tr->k0 : nullptr, tr ? tr->k1 : nullptr)) != hipSuccess)
+    if ((e = bhs::launch_volumetric(table, base, (uint32_t)ncam, t.n_items, all_f64, t.any_band,
+                                    sc_sh, ro, o, ctx->noise_vol, ctx->bb_lut, sc, ctx->num_cus, s,
+                                    ctx->aux, ctx->ev_fork, ctx->ev_join, tr ? tr->k0 : nullptr,
+                                    tr ? tr->k1 : nullptr)) != hipSuccess)
         return hip_fail(ctx, e, "volumetric launch");
     if (tr) (void)hipEventRecord(tr->call1, s);
     note_stream(ctx, s);

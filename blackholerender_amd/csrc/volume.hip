@@ -1,8 +1,9 @@
 // volume.hip — the volumetric disk render (shaders/blackhole.frag:142-253 with DISK_VOLUMETRIC and
 // shaders/disk.glsl:81-110): one lane per ray marches law S with the volumetric slab rule and
 // composites the disk's emission and absorption along the Hermite sub-samples of every step inside
-// the slab, until the ray escapes, falls in, runs out of steps or turns opaque. FP32 rays first (the
-// FP64 band rays are listed by the same pass), then the band list in FP64.
+// the slab, until the ray escapes, falls in, runs out of steps or turns opaque. A scan lists the FP64
+// band rays (|b - b_crit| < band), whose FP64 march runs on a side stream beside the FP32 march of
+// all the others.
 #include "volume.hpp"
 
 #include "../../include/bhs_trace.h"
@@ -111,39 +112,50 @@ __device__ __forceinline__ uint32_t vol_ray(const MarchConst& c, const shade::Sh
     return (uint32_t)st.i;
 }
 
-// FP32 pass over every work item; band rays (|b - b_crit| < band) go to the band list instead.
+__device__ __forceinline__ bool vol_in_band(const MarchConst& c, int64_t pix) {
+    if (c.precision != BHS_PREC_F32_BAND64) return false;
+    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
+    return __builtin_fabsf(lawS_setup(P, ray_dir(c, pix)).b - kBcritF) < c.band_abs;
+}
+
+// Lists the work items of the FP64 band (|b - b_crit| < band): one atomic per wave.
+__global__ __launch_bounds__(kBlock) void k_vol_scan(VolCams cams, Scratch sc) {
+    for (uint32_t q0 = blockIdx.x * kBlock; q0 < cams.n_items; q0 += gridDim.x * kBlock) {
+        const uint32_t q = q0 + threadIdx.x;
+        bool band = false;
+        if (q < cams.n_items) {
+            const uint32_t ci = cams.find(q);
+            const MarchConst& c = cams.cams[ci];
+            const int64_t pix = vol_item_pixel(c, q - cams.item_base[ci]);
+            band = pix >= 0 && vol_in_band(c, pix);
+        }
+        const uint64_t m = __ballot(band);
+        if (m == 0) continue;
+        uint32_t base = 0;
+        if ((threadIdx.x & 63u) == 0) base = atomicAdd(sc.band_count, (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, 0);
+        if (band) {
+            const uint32_t j = base + __builtin_amdgcn_mbcnt_hi(
+                                          (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (j < sc.band_cap) sc.band_ids[j] = q;
+        }
+    }
+}
+
+// FP32 pass over every work item outside the FP64 band.
 __global__ __launch_bounds__(kBlock) void k_vol_f32(VolCams cams, shade::ShadeConst sh, SoA out,
                                                     ShadeOut so, const uint16_t* __restrict__ vol,
                                                     const float* __restrict__ lut, Scratch sc) {
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < cams.n_items; q0 += gridDim.x * kBlock) {
         const uint32_t q = q0 + threadIdx.x;
-        bool band = false;
         uint32_t ray = 0, steps = 0, esc = 0;
         if (q < cams.n_items) {
             const uint32_t ci = cams.find(q);
             const MarchConst& c = cams.cams[ci];
             const int64_t pix = vol_item_pixel(c, q - cams.item_base[ci]);
-            if (pix >= 0) {
-                if (c.precision == BHS_PREC_F32_BAND64) {
-                    const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
-                    band = __builtin_fabsf(lawS_setup(P, ray_dir(c, pix)).b - kBcritF) < c.band_abs;
-                }
-                if (!band) {
-                    ray = 1;
-                    steps = vol_ray<float>(c, sh, c.ray_offset + pix, pix, out, so, vol, lut, 0u, esc);
-                }
-            }
-        }
-        const uint64_t m = __ballot(band);
-        if (m) {
-            uint32_t base = 0;
-            if ((threadIdx.x & 63u) == 0) base = atomicAdd(sc.band_count, (uint32_t)__popcll(m));
-            base = (uint32_t)__shfl((int)base, 0);
-            if (band) {
-                const uint32_t j = base + __builtin_amdgcn_mbcnt_hi(
-                                              (uint32_t)(m >> 32),
-                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                if (j < sc.band_cap) sc.band_ids[j] = q;
+            if (pix >= 0 && !vol_in_band(c, pix)) {
+                ray = 1;
+                steps = vol_ray<float>(c, sh, c.ray_offset + pix, pix, out, so, vol, lut, 0u, esc);
             }
         }
         ray = wsum(ray);
@@ -193,25 +205,42 @@ __global__ __launch_bounds__(kBlock) void k_vol_f64(VolCams cams, shade::ShadeCo
 }  // namespace
 
 hipError_t launch_volumetric(const MarchConst* table, const uint32_t* item_base, uint32_t ncam,
-                             uint32_t n_items, bool all_f64, const shade::ShadeConst& sh,
-                             const SoA& out, const ShadeOut& so, const uint16_t* vol,
-                             const float* lut, const Scratch& sc, int num_cus, hipStream_t s,
-                             hipEvent_t ev_begin, hipEvent_t ev_end) {
+                             uint32_t n_items, bool all_f64, bool any_band,
+                             const shade::ShadeConst& sh, const SoA& out, const ShadeOut& so,
+                             const uint16_t* vol, const float* lut, const Scratch& sc,
+                             int num_cus, hipStream_t s, hipStream_t aux, hipEvent_t ev_fork,
+                             hipEvent_t ev_join, hipEvent_t ev_begin, hipEvent_t ev_end) {
     const VolCams cams{table, item_base, ncam, n_items};
     const uint32_t blocks = (n_items + kBlock - 1) / kBlock;
+    const uint32_t grid = blocks ? blocks : 1u;
     hipError_t e;
-    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
-    if (!all_f64) {
-        hipLaunchKernelGGL(k_vol_f32, dim3(blocks ? blocks : 1u), dim3(kBlock), 0, s, cams, sh, out,
-                           so, vol, lut, sc);
+    if (all_f64) {
+        if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_vol_f64, dim3(grid), dim3(kBlock), 0, s, cams, sh, out, so, vol, lut,
+                           sc, true);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
+        return hipSuccess;
     }
+    if (any_band) {
+        // The band rays (few, long FP64 chains) march on the side stream beside the FP32 pass.
+        const uint32_t cap = (uint32_t)(num_cus > 0 ? num_cus : 256) * 16u;
+        hipLaunchKernelGGL(k_vol_scan, dim3(grid < cap ? grid : cap), dim3(kBlock), 0, s, cams, sc);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev_fork, s)) != hipSuccess ||
+            (e = hipStreamWaitEvent(aux, ev_fork, 0)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(k_vol_f64, dim3((uint32_t)(num_cus > 0 ? num_cus : 256) * 4u),
+                           dim3(kBlock), 0, aux, cams, sh, out, so, vol, lut, sc, false);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipEventRecord(ev_join, aux)) != hipSuccess) return e;
+    }
+    if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_vol_f32, dim3(grid), dim3(kBlock), 0, s, cams, sh, out, so, vol, lut, sc);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_end && (e = hipEventRecord(ev_end, s)) != hipSuccess) return e;
-    // the band list is short: a grid of (num_cus x 4) blocks covers any list with its loop
-    const uint32_t bb = all_f64 ? (blocks ? blocks : 1u) : (uint32_t)(num_cus > 0 ? num_cus : 256) * 4u;
-    hipLaunchKernelGGL(k_vol_f64, dim3(bb), dim3(kBlock), 0, s, cams, sh, out, so, vol, lut, sc,
-                       all_f64);
-    return hipGetLastError();
+    if (any_band && (e = hipStreamWaitEvent(s, ev_join, 0)) != hipSuccess) return e;
+    return hipSuccess;
 }
 
 }  // namespace bhs
