@@ -97,8 +97,23 @@ __device__ __forceinline__ uint32_t pcg(uint32_t v) {  // shaders/common.glsl:28
 }
 
 // ---------------------------------------------------------------------------------- atan2
-// IEEE-only atan2: t = min/max; t > tan(pi/8) -> pi/4 + atan((t-1)/(t+1)); atan u = u + u^3 P(u^2)
-// with a degree-4 least-squares P (error 1.8e-9 before rounding); quadrant fix-up of atan2.
+// atan u ~ u + u^3 (p0 + p1 u^2 + ... + p4 u^8) on |u| <= tan(pi/8): least squares of the absolute
+// error at 1000 Chebyshev nodes, max error 1.6e-10 before rounding (the recipe and its refit test:
+// blackholerender_amd/fit_constants.py, tests/test_fitted_constants.py).
+constexpr float kAtanP[5] = {-3.333330452e-01f, 1.999792159e-01f, -1.423489004e-01f, 1.053845584e-01f, -5.953448266e-02f};
+
+// u + u^3 P(s), s = u^2, Horner from the highest coefficient.
+__device__ __forceinline__ float atan_poly(float u, float s) {
+    float p = kAtanP[4];
+    p = __builtin_fmaf(p, s, kAtanP[3]);
+    p = __builtin_fmaf(p, s, kAtanP[2]);
+    p = __builtin_fmaf(p, s, kAtanP[1]);
+    p = __builtin_fmaf(p, s, kAtanP[0]);
+    return __builtin_fmaf(u * s, p, u);
+}
+
+// IEEE-only atan2: t = min/max; t > tan(pi/8) -> pi/4 + atan((t-1)/(t+1)); atan_poly; quadrant
+// fix-up of atan2.
 __device__ __forceinline__ float atan2_ieee(float y, float x) {
     const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y);
     const float mx = ax > ay ? ax : ay;
@@ -110,13 +125,7 @@ __device__ __forceinline__ float atan2_ieee(float y, float x) {
         t = (t - 1.0f) / (t + 1.0f);
         base = 0.78539816f;
     }
-    const float s = t * t;
-    float p = -0.065067887f;
-    p = __builtin_fmaf(p, s, 0.10763717f);
-    p = __builtin_fmaf(p, s, -0.14266309f);
-    p = __builtin_fmaf(p, s, 0.19999637f);
-    p = __builtin_fmaf(p, s, -0.33333331f);
-    float a = base + __builtin_fmaf(t * s, p, t);
+    float a = base + atan_poly(t, t * t);
     if (ay > ax) a = 1.5707964f - a;
     if (x < 0.0f) a = 3.1415927f - a;
     if (y < 0.0f) a = -a;

@@ -62,16 +62,19 @@ constexpr float kSinMax = 0.95f;           // domain: sin(view angle) <= 0.95
 constexpr float kBetaMin = 0.3f;           // domain: beta >= 0.3 (near-radial rays: u(psi) is too
                                            // steep in beta for the network's resolution)
 
-// Chebyshev coefficients of ahat(s), s = log(beta - 1) in [log 1e-3, log 200].
+// Chebyshev coefficients of ahat(s), s = log(beta - 1) in [log 1e-3, log 200]: the total swept angle
+// of a null geodesic from infinity to infinity minus pi, least squares at 200 Chebyshev nodes of
+// Darwin's integral (max error 4.7e-7 rad; recipe and refit test: blackholerender_amd/
+// fit_constants.py, tests/test_fitted_constants.py).
 constexpr int kAhatTerms = 25;
 constexpr float kAhatC[kAhatTerms] = {
     2.252328873e+00f, -3.297978401e+00f, 1.115420222e+00f, 6.010319665e-02f,
-    -1.329363883e-01f, -2.116844989e-02f, 2.762471139e-02f, 7.664505392e-03f,
-    -6.423670799e-03f, -2.731988439e-03f, 1.508211251e-03f, 9.516093414e-04f,
-    -3.349816834e-04f, -3.234522010e-04f, 6.337290688e-05f, 1.071531224e-04f,
-    -6.627894436e-06f, -3.451645898e-05f, -2.213727157e-06f, 1.076196713e-05f,
-    2.081819275e-06f, -3.225621867e-06f, -1.096105734e-06f, 9.152259963e-07f,
-    4.800450029e-07f};
+    -1.329363883e-01f, -2.116844803e-02f, 2.762471139e-02f, 7.664506324e-03f,
+    -6.423670333e-03f, -2.731987508e-03f, 1.508211833e-03f, 9.516102145e-04f,
+    -3.349810431e-04f, -3.234512697e-04f, 6.337356172e-05f, 1.071540391e-04f,
+    -6.627239145e-06f, -3.451554221e-05f, -2.213071866e-06f, 1.076288481e-05f,
+    2.082474793e-06f, -3.224703960e-06f, -1.095450330e-06f, 9.161439607e-07f,
+    4.807004075e-07f};
 constexpr float kAhatS0 = -6.9077553f, kAhatS1 = 5.2983174f;
 
 // bm = beta - 1 > 0.
@@ -204,13 +207,7 @@ __device__ __forceinline__ float atan2_fast(float y, float x) {
         t = (t - 1.0f) * __builtin_amdgcn_rcpf(t + 1.0f);
         base = 0.78539816f;
     }
-    const float s = t * t;
-    float p = -0.065067887f;
-    p = __builtin_fmaf(p, s, 0.10763717f);
-    p = __builtin_fmaf(p, s, -0.14266309f);
-    p = __builtin_fmaf(p, s, 0.19999637f);
-    p = __builtin_fmaf(p, s, -0.33333331f);
-    float a = base + __builtin_fmaf(t * s, p, t);
+    float a = base + atan_poly(t, t * t);
     if (ay > ax) a = 1.5707964f - a;
     if (x < 0.0f) a = 3.1415927f - a;
     return y < 0.0f ? -a : a;

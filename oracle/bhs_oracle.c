@@ -168,9 +168,12 @@ void oracle_trace_T(const double cam[3], const double dir[3], int32_t max_steps,
 
 /* ================================================================================================
  * Atan2 from IEEE operations only. Reduction: t = min/max in [0,1]; for t > tan(pi/8),
- * atan t = pi/4 + atan((t-1)/(t+1)); then atan u = u + u^3 P(u^2) with P a degree-4 least-squares
- * fit on [0, tan^2(pi/8)] (max error 1.8e-9 before float rounding). Quadrant fix-up as atan2.
+ * atan t = pi/4 + atan((t-1)/(t+1)); then atan u = u + u^3 P(u^2) with P of degree 4, least
+ * squares of the absolute error at 1000 Chebyshev nodes of [0, tan(pi/8)] (max error 1.6e-10 before
+ * float rounding; recipe blackholerender_amd/fit_constants.py). Quadrant fix-up as atan2.
  * ============================================================================================== */
+static const float kAtanP[5] = {-3.333330452e-01f, 1.999792159e-01f, -1.423489004e-01f, 1.053845584e-01f, -5.953448266e-02f};
+
 float oracle_atan2f(float y, float x) {
     const float ax = fabsf(x), ay = fabsf(y);
     const float mx = ax > ay ? ax : ay;
@@ -183,11 +186,11 @@ float oracle_atan2f(float y, float x) {
         base = 0.78539816f;
     }
     const float s = t * t;
-    float pl = -0.065067887f;
-    pl = fmaf(pl, s, 0.10763717f);
-    pl = fmaf(pl, s, -0.14266309f);
-    pl = fmaf(pl, s, 0.19999637f);
-    pl = fmaf(pl, s, -0.33333331f);
+    float pl = kAtanP[4];
+    pl = fmaf(pl, s, kAtanP[3]);
+    pl = fmaf(pl, s, kAtanP[2]);
+    pl = fmaf(pl, s, kAtanP[1]);
+    pl = fmaf(pl, s, kAtanP[0]);
     float a = base + fmaf(t * s, pl, t);
     if (ay > ax) a = 1.5707964f - a;
     if (x < 0.0f) a = 3.1415927f - a;
