@@ -570,6 +570,20 @@ class Tracer:
         return dict(rays=s.rays, steps=s.steps, crossings=s.crossings, escapes=s.escapes,
                     band64=s.band64, fallback=s.fallback, surrogate=s.surrogate)
 
+    def rsqrt_rn(self, x, out=None, stream=None):
+        """The float march's reciprocal square root (bhs_rsqrt_rn) of a float32 device tensor:
+        1/sqrt(x) rounded once to float."""
+        n = x.numel()
+        _check_tensor(x, "float32", (n,), self.device, "rsqrt_rn: x")
+        if out is None:
+            import torch
+            out = torch.empty_like(x)
+        _check_tensor(out, "float32", (n,), self.device, "rsqrt_rn: out")
+        self._check(self._lib.bhs_rsqrt_rn(self._ctx, C.c_void_p(x.data_ptr()),
+                                           C.c_void_p(out.data_ptr()), n,
+                                           C.c_void_p(self._stream_ptr(stream))), "bhs_rsqrt_rn")
+        return out
+
     def set_tuning(self, mode: int, block: int = 256, waves_per_simd: int = 8,
                    refill_min: int = 32):
         t = _capi.Tuning(mode, block, waves_per_simd, refill_min)

@@ -931,6 +931,21 @@ int bhs_timing(bhs_ctx* ctx, double* march_ms_avg, double* total_ms_avg, int32_t
     return BHS_OK;
 }
 
+int bhs_rsqrt_rn(bhs_ctx* ctx, const float* x_dev, float* y_dev, int64_t n, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    if (n < 0 || (n > 0 && (!x_dev || !y_dev)))
+        return fail(ctx, BHS_E_INVALID, "bhs_rsqrt_rn: NULL buffer or n < 0%s");
+    if (n == 0) return BHS_OK;
+    DeviceGuard g(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
+    hipError_t e;
+    if ((e = bhs::launch_rsqrt_rn(x_dev, y_dev, n, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_rsqrt_rn launch");
+    note_stream(ctx, s);
+    return BHS_OK;
+}
+
 int bhs_last_stats(bhs_ctx* ctx, bhs_stats* out_host) {
     if (!ctx || !out_host) return BHS_E_INVALID;
     std::memset(out_host, 0, sizeof *out_host);

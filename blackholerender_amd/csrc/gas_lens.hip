@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kLensBlock) void k_gas_lens(const LensArgs a,
         const V3<float> x0 = x;
         const float r0 = r;
         const float vlen = __builtin_sqrtf(dot3(v, v));
-        rk4(x, v, st.c3, ds / vlen, r2, r);                                            // :30
+        rk4(x, v, st.c3, ds / vlen, rsqrt_rn_core(r2));                                // :30
         r2 = dot3(x, x);
         r = __builtin_sqrtf(r2);                                                       // :31
         const float cz = __builtin_fmaf(x0.x, x.y, -(x0.y * x.x));

@@ -966,6 +966,21 @@ hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, co
     return hipSuccess;
 }
 
+// The march's reciprocal square root over an array (grid-stride).
+__global__ __launch_bounds__(kBlock) void k_rsqrt_rn(const float* __restrict__ x,
+                                                     float* __restrict__ y, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        y[i] = rsqrt_rn(x[i]);
+}
+
+hipError_t launch_rsqrt_rn(const float* x, float* y, int64_t n, hipStream_t s) {
+    const int64_t want = (n + kBlock - 1) / kBlock;
+    const unsigned blocks = (unsigned)(want < 65536 ? want : 65536);
+    hipLaunchKernelGGL(k_rsqrt_rn, dim3(blocks), dim3(kBlock), 0, s, x, y, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_trace_T(const double* pos3, const double* dir3, int64_t n, int32_t max_steps,
                           double step_scale, double esc_tol, double r_esc_min, void* out,
                           hipStream_t s) {

@@ -93,6 +93,8 @@ hipError_t launch_root_refine_batch(const MarchConst* table, const uint32_t* ite
 hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, const MarchConst& c,
                              const SoA& out, const Scratch& sc, hipStream_t s, hipEvent_t ev_begin,
                              hipEvent_t ev_end);
+// y[i] = rsqrt_rn(x[i]) (the march's correctly rounded reciprocal square root), n >= 1.
+hipError_t launch_rsqrt_rn(const float* x, float* y, int64_t n, hipStream_t s);
 hipError_t launch_trace_T(const double* pos3, const double* dir3, int64_t n, int32_t max_steps,
                           double step_scale, double esc_tol, double r_esc_min, void* out,
                           hipStream_t s);

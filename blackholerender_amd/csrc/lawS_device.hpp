@@ -4,7 +4,8 @@
 //
 // Arithmetic contract (the whole library is compiled with -ffp-contract=off and correctly rounded
 // FP32 division/square root): every multiply-add below that is fused is written as an explicit
-// fma, division and square root are IEEE, nothing is reassociated. The operation order is fixed:
+// fma, division and square root are IEEE, the march loop's reciprocal square root is correctly
+// rounded (rsqrt_rn), nothing is reassociated. The operation order is fixed:
 //   dot(a,b) = fma(a.z,b.z, fma(a.y,b.y, a.x*b.x)),  normalize(a) = a * (1/sqrt(dot(a,a))),
 //   cross(a,b).x = fma(a.y,b.z, -(a.z*b.y)) (cyclic),
 // so one ray's result does not depend on the GPU, the launch shape or the order rays are processed
@@ -36,18 +37,55 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return tiny ? r * 0x1p-16f : r;
 }
 
+// 1/sqrt(x) correctly rounded to float, for x > 0 finite (normal or denormal), from an estimate y0
+// within 2 ulps of it (v_rsq_f32 is documented within 1). One refinement in double: with
+// e = 1 - x y0^2 (x y0 is exact in double, e takes one rounding), 1/sqrt(x) = y0 (1 - e)^(-1/2)
+// = y0 (1 + e/2 + 3e^2/8 + O(e^3)) and |e| < 2^-21, so y0 + (y0 e)(1/2 + 3e/8) is within 2^-60
+// relative of 1/sqrt(x) before its last two roundings (to double, then to float). The 1/sqrt of
+// every float lies at least 1.4 double ulps from the nearest float midpoint, so the result is the
+// correctly rounded one. An exhaustive CPU check of this sequence (every x in [1, 4), every positive
+// denormal, every estimate within 2 ulps of the true value) covers all positive finite x, since
+// each step is exact under scaling by 4^k.
+__device__ __forceinline__ float rsqrt_refine(float x, float y0) {
+    const double xd = (double)x, yd = (double)y0;
+    const double t = xd * yd;
+    const double e = __builtin_fma(-t, yd, 1.0);
+    const double c = __builtin_fma(e, 0.375, 0.5);
+    const double w = yd * e;
+    return (float)__builtin_fma(w, c, yd);
+}
+
+// The march's reciprocal square root: correctly rounded for every positive finite operand. Its
+// operands are squared lengths of positions and velocities; one equal to 0 or +inf (a ray through
+// the exact centre, or beyond 1e19) gives NaN here and a NaN march in the oracle alike.
+__device__ __forceinline__ float rsqrt_rn_core(float x) {
+    return rsqrt_refine(x, __builtin_amdgcn_rsqf(x));
+}
+
+// Correctly rounded 1/sqrt(x) for every float x: +-0 -> +-inf and +inf -> +0 come exactly from the
+// estimate, negative numbers and NaN give NaN.
+__device__ __forceinline__ float rsqrt_rn(float x) {
+    const float y0 = __builtin_amdgcn_rsqf(x);
+    const float y = rsqrt_refine(x, y0);
+    return __builtin_amdgcn_classf(x, 0x260) ? y0 : y;     // class mask: -0, +0, +inf
+}
+
 template <typename R> struct Ar;
 template <> struct Ar<float> {
     static __device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
     static __device__ __forceinline__ float sqrt(float a) { return __builtin_sqrtf(a); }
     // Square root of the march loop's operands (see sqrt_rn): same result as sqrt().
     static __device__ __forceinline__ float sqrt_loop(float a) { return sqrt_rn(a); }
+    // 1/sqrt(a) rounded once (the march loop's reciprocal square root, operands > 0 finite).
+    static __device__ __forceinline__ float rsqrt(float a) { return rsqrt_rn_core(a); }
     static __device__ __forceinline__ float abs(float a) { return __builtin_fabsf(a); }
 };
 template <> struct Ar<double> {
     static __device__ __forceinline__ double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
     static __device__ __forceinline__ double sqrt(double a) { return __builtin_sqrt(a); }
     static __device__ __forceinline__ double sqrt_loop(double a) { return __builtin_sqrt(a); }
+    // In double the reciprocal square root is two IEEE operations: 1 / sqrt(a).
+    static __device__ __forceinline__ double rsqrt(double a) { return 1.0 / __builtin_sqrt(a); }
     static __device__ __forceinline__ double abs(double a) { return __builtin_fabs(a); }
 };
 
@@ -216,28 +254,33 @@ __device__ __forceinline__ Setup<R> lawS_setup(const V3<R>& P, const V3<R>& d) {
 }
 
 // ---------------------------------------------------------------------------------- RK4
-// rk4Step (geodesic.glsl:16-23) with geoAccel (:10-14): a = x * (c3 / (r2 r2 r)), c3 = -3 h2.
-// r2, r of x come from the top of the march iteration and serve the first stage.
+// geoAccel (geodesic.glsl:10-14): a = x * (-3 h2 / r^5) = x * (c3 w^5), c3 = -3 h2 per ray,
+// w = rsqrt(r^2) of the stage position, w^5 = ((w w)(w w)) w.
 template <typename R>
-__device__ __forceinline__ void rk4(V3<R>& x, V3<R>& v, R c3, R dt, R r2, R r) {
+__device__ __forceinline__ R accel_scale(R c3, R w) {
+    const R w2 = w * w;
+    return c3 * ((w2 * w2) * w);
+}
+
+// rk4Step (geodesic.glsl:16-23). w1 = rsqrt(|x|^2) comes from the top of the march iteration and
+// serves the first stage.
+template <typename R>
+__device__ __forceinline__ void rk4(V3<R>& x, V3<R>& v, R c3, R dt, R w1) {
     using A = Ar<R>;
     const R hdt = R(0.5) * dt;
-    const R s1 = c3 / ((r2 * r2) * r);
+    const R s1 = accel_scale(c3, w1);
     const V3<R> a1{x.x * s1, x.y * s1, x.z * s1};
     const V3<R> x2{A::fma(hdt, v.x, x.x), A::fma(hdt, v.y, x.y), A::fma(hdt, v.z, x.z)};
     const V3<R> v2{A::fma(hdt, a1.x, v.x), A::fma(hdt, a1.y, v.y), A::fma(hdt, a1.z, v.z)};
-    const R q2 = dot3(x2, x2);
-    const R s2 = c3 / ((q2 * q2) * A::sqrt_loop(q2));
+    const R s2 = accel_scale(c3, A::rsqrt(dot3(x2, x2)));
     const V3<R> a2{x2.x * s2, x2.y * s2, x2.z * s2};
     const V3<R> x3{A::fma(hdt, v2.x, x.x), A::fma(hdt, v2.y, x.y), A::fma(hdt, v2.z, x.z)};
     const V3<R> v3{A::fma(hdt, a2.x, v.x), A::fma(hdt, a2.y, v.y), A::fma(hdt, a2.z, v.z)};
-    const R q3 = dot3(x3, x3);
-    const R s3 = c3 / ((q3 * q3) * A::sqrt_loop(q3));
+    const R s3 = accel_scale(c3, A::rsqrt(dot3(x3, x3)));
     const V3<R> a3{x3.x * s3, x3.y * s3, x3.z * s3};
     const V3<R> x4{A::fma(dt, v3.x, x.x), A::fma(dt, v3.y, x.y), A::fma(dt, v3.z, x.z)};
     const V3<R> v4{A::fma(dt, a3.x, v.x), A::fma(dt, a3.y, v.y), A::fma(dt, a3.z, v.z)};
-    const R q4 = dot3(x4, x4);
-    const R s4 = c3 / ((q4 * q4) * A::sqrt_loop(q4));
+    const R s4 = accel_scale(c3, A::rsqrt(dot3(x4, x4)));
     const V3<R> a4{x4.x * s4, x4.y * s4, x4.z * s4};
     const R dt6 = dt * R(0.16666666666666666);
     x.x = A::fma(dt6, A::fma(R(2), v3.x, A::fma(R(2), v2.x, v.x)) + v4.x, x.x);
@@ -370,13 +413,15 @@ __device__ __forceinline__ bool march_step(MarchState<R>& st, const Setup<R>& s,
     V3<R>& v = st.v;
     const bool over = !live || st.i >= k.max_steps;
     const R r2 = dot3(x, x);
-    const R r = A::sqrt_loop(r2);
+    const R ir = A::rsqrt(r2);                                           // 1/r
+    const R r = r2 * ir;
     const bool horizon = r < R(2.02);                                    // :144
-    const R iv = R(1) / A::sqrt_loop(dot3(v, v));
+    const R iv = A::rsqrt(dot3(v, v));
     bool escape = false;
     if (!over && !horizon && dot3(x, v) > R(0) && r > k.r_esc_min) {     // :151-155
         const R b = s.h * iv;
-        const R q = (((R(0.75) * b) * b) * b) / (r2 * r2);
+        const R ir2 = ir * ir;
+        const R q = (((R(0.75) * b) * b) * b) * (ir2 * ir2);
         if (q < R(0.01)) {
             const V3<R> e = far_field(x, v, r, iv, s.h2);
             sink.escape((float)e.x, (float)e.y, (float)e.z);
@@ -403,7 +448,7 @@ __device__ __forceinline__ bool march_step(MarchState<R>& st, const Setup<R>& s,
     const R dt = ds * iv;                                                 // :192
     const V3<R> x0 = x;
     const V3<R> v0 = v;
-    rk4(x, v, s.c3, dt, r2, r);                                           // :194
+    rk4(x, v, s.c3, dt, ir);                                              // :194
     if (done) return true;
     st.i = st.i + 1;                                                      // :196
     if (k.thin && ((x.y > R(0)) != (x0.y > R(0)))) {                      // :209-216
@@ -462,7 +507,7 @@ __device__ __forceinline__ bool root64(const double X0[3], const double V0[3], d
     {
         V3<double> x{X0[0], X0[1], X0[2]}, v{V0[0], V0[1], V0[2]};
         const double q = dot3(x, x);
-        rk4(x, v, c3, b, q, __builtin_sqrt(q));
+        rk4(x, v, c3, b, 1.0 / __builtin_sqrt(q));
         fb = x.y;
     }
     if (fa == 0.0) { hx = X0[0]; hz = X0[2]; return true; }
@@ -473,7 +518,7 @@ __device__ __forceinline__ bool root64(const double X0[3], const double V0[3], d
         const double c = (a * fb - b * fa) / (fb - fa);
         V3<double> x{X0[0], X0[1], X0[2]}, v{V0[0], V0[1], V0[2]};
         const double q = dot3(x, x);
-        rk4(x, v, c3, c, q, __builtin_sqrt(q));
+        rk4(x, v, c3, c, 1.0 / __builtin_sqrt(q));
         const double fc = x.y;
         cx = x.x; cz = x.z;
         if (fc == 0.0) break;

@@ -78,15 +78,17 @@ __device__ __forceinline__ bool vol_step(MarchState<R>& st, const Setup<R>& s, c
     V3<R>& v = st.v;
     if (st.i >= k.max_steps) return true;                               // :142, :252
     const R r2 = dot3(x, x);
-    const R r = A::sqrt_loop(r2);
+    const R ir = A::rsqrt(r2);                                          // 1/r
+    const R r = r2 * ir;
     if (r < R(2.02)) {                                                  // :144
         st.status = 2;
         return true;
     }
-    const R iv = R(1) / A::sqrt_loop(dot3(v, v));
+    const R iv = A::rsqrt(dot3(v, v));
     if (dot3(x, v) > R(0) && r > k.r_esc_min) {                         // :151-155
         const R b = s.h * iv;
-        const R q = (((R(0.75) * b) * b) * b) / (r2 * r2);
+        const R ir2 = ir * ir;
+        const R q = (((R(0.75) * b) * b) * b) * (ir2 * ir2);
         if (q < R(0.01)) {
             const V3<R> e = far_field(x, v, r, iv, s.h2);
             esc[0] = (float)e.x;
@@ -117,7 +119,7 @@ __device__ __forceinline__ bool vol_step(MarchState<R>& st, const Setup<R>& s, c
     const R dt = ds * iv;                                               // :192
     const V3<R> x0 = x;
     const V3<R> v0 = v;
-    rk4(x, v, s.c3, dt, r2, r);                                         // :194
+    rk4(x, v, s.c3, dt, ir);                                            // :194
     st.i = st.i + 1;                                                    // :196
     if (near) {                                                         // :217-233
         const V3<R> d{x.x - x0.x, x.y - x0.y, x.z - x0.z};

@@ -347,6 +347,13 @@ int bhs_march_rays(bhs_ctx* ctx, const float* pos3_dev, const float* dir3_dev, i
  * (synchronises the context's last stream). */
 int bhs_last_stats(bhs_ctx* ctx, bhs_stats* out_host);
 
+/* The float march's reciprocal square root over n device floats: y_dev[i] = 1/sqrt(x_dev[i])
+ * rounded once to float (IEEE special values: +-0 -> +-inf, +inf -> +0, negative -> NaN). No
+ * counterpart in the reference (the GLSL march's inversesqrt has no exact rounding): it exposes the
+ * arithmetic primitive the march's results depend on, so a caller or a test can check it against
+ * any correctly rounded implementation. Stream-ordered. */
+int bhs_rsqrt_rn(bhs_ctx* ctx, const float* x_dev, float* y_dev, int64_t n, void* stream);
+
 /* Launch shape of the FP32 march. mode 0: one lane per pixel on a static grid of 8x8-pixel wave
  * tiles; mode 1: resident waves pull rays from a global work queue and refill finished lanes
  * (wave ballot + prefix count, one atomic per refill). Chosen per context by bhs_autotune or set by

@@ -148,6 +148,8 @@ def _bind(path: str):
                                         C.c_int32, C.c_float, C.c_void_p]
     L.oracle_atan2f.argtypes = [C.c_float, C.c_float]
     L.oracle_atan2f.restype = C.c_float
+    L.oracle_rsqrtf.argtypes = [C.c_float]
+    L.oracle_rsqrtf.restype = C.c_float
     L.oracle_camera_dirs.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
                                      C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
                                      C.c_int64, fp]

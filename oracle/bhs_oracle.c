@@ -205,9 +205,17 @@ float oracle_atan2f(float y, float x) {
  * return ends the march. */
 typedef int (*oracle_cross_fn)(void* ctx, int index, float r, float phi, float g);
 
+/* 1/sqrt(x) rounded once to float (the march's reciprocal square root in float). The double
+ * quotient 1.0 / sqrt((double)x) is within one double ulp of 1/sqrt(x), and 1/sqrt of every float
+ * lies at least 1.4 double ulps from the nearest float midpoint (checked exhaustively over
+ * [1, 4), which covers every float by exact scaling by 4^k), so rounding it to float gives the
+ * correctly rounded value: tests/cpp/rsqrt_rn_check.c. */
+float oracle_rsqrtf(float x) { return (float)(1.0 / sqrt((double)x)); }
+
 #define REAL float
 #define FMA fmaf
 #define SQRT sqrtf
+#define RSQRT oracle_rsqrtf
 #define FABS fabsf
 #define L(x) ((float)(x))
 #define FN(n) n##_f32
@@ -215,6 +223,7 @@ typedef int (*oracle_cross_fn)(void* ctx, int index, float r, float phi, float g
 #undef REAL
 #undef FMA
 #undef SQRT
+#undef RSQRT
 #undef FABS
 #undef L
 #undef FN
@@ -222,6 +231,7 @@ typedef int (*oracle_cross_fn)(void* ctx, int index, float r, float phi, float g
 #define REAL double
 #define FMA fma
 #define SQRT sqrt
+#define RSQRT(a) (1.0 / sqrt(a))
 #define FABS fabs
 #define L(x) ((double)(x))
 #define FN(n) n##_f64
@@ -229,6 +239,7 @@ typedef int (*oracle_cross_fn)(void* ctx, int index, float r, float phi, float g
 #undef REAL
 #undef FMA
 #undef SQRT
+#undef RSQRT
 #undef FABS
 #undef L
 #undef FN
@@ -247,7 +258,7 @@ static int root64(const double st[8], double c3, double* hx, double* hz) {
     for (int k = 0; k < 3; ++k) { xa[k] = st[k]; va[k] = st[3 + k]; }
     {
         const double q = dot3_f64(xa, xa);
-        rk4_f64(xa, va, c3, b, q, sqrt(q));
+        rk4_f64(xa, va, c3, b, 1.0 / sqrt(q));
         fb = xa[1];
     }
     if (fa == 0.0) { *hx = st[0]; *hz = st[2]; return 1; }
@@ -259,7 +270,7 @@ static int root64(const double st[8], double c3, double* hx, double* hz) {
         double xs[3], vs[3];
         for (int k = 0; k < 3; ++k) { xs[k] = st[k]; vs[k] = st[3 + k]; }
         const double q = dot3_f64(xs, xs);
-        rk4_f64(xs, vs, c3, c, q, sqrt(q));
+        rk4_f64(xs, vs, c3, c, 1.0 / sqrt(q));
         const double fc = xs[1];
         for (int k = 0; k < 3; ++k) xc[k] = xs[k];
         if (fc == 0.0) break;
@@ -564,7 +575,7 @@ void oracle_gas_lens_ray(float cam_r, float alpha_crit, int32_t n_rays, int32_t 
         const float x0[3] = {x[0], x[1], x[2]};
         const float r0 = r;
         const float vlen = sqrtf(dot3_f32(v, v));
-        rk4_f32(x, v, st.c3, ds / vlen, r2, r);                                    /* :30 */
+        rk4_f32(x, v, st.c3, ds / vlen, oracle_rsqrtf(r2));                        /* :30 */
         r2 = dot3_f32(x, x);
         r = sqrtf(r2);                                                             /* :31 */
         const float cz = fmaf(x0[0], x[1], -(x0[1] * x[0]));
@@ -738,6 +749,7 @@ void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int
 #define REAL float
 #define FMA fmaf
 #define SQRT sqrtf
+#define RSQRT oracle_rsqrtf
 #define FABS fabsf
 #define CEIL ceilf
 #define L(x) ((float)(x))
@@ -746,6 +758,7 @@ void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int
 #undef REAL
 #undef FMA
 #undef SQRT
+#undef RSQRT
 #undef FABS
 #undef CEIL
 #undef L
@@ -754,6 +767,7 @@ void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int
 #define REAL double
 #define FMA fma
 #define SQRT sqrt
+#define RSQRT(a) (1.0 / sqrt(a))
 #define FABS fabs
 #define CEIL ceil
 #define L(x) ((double)(x))
@@ -762,6 +776,7 @@ void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int
 #undef REAL
 #undef FMA
 #undef SQRT
+#undef RSQRT
 #undef FABS
 #undef CEIL
 #undef L

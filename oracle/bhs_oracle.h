@@ -138,6 +138,9 @@ void oracle_render(const float pos[3], const float basis[9], float tan_half_fov,
 void oracle_march_batch(const float* pos3, const float* dir3, int64_t n, const oracle_lawS_params* p,
                         int32_t mode, oracle_lawS_ray* out, int32_t threads);
 
+/* 1/sqrt(x) correctly rounded to float: the reciprocal square root of the float march. */
+float oracle_rsqrtf(float x);
+
 /* Atan2 built from IEEE operations only (range reduction + degree-4 polynomial in t^2); identical
  * in any IEEE-754 float implementation that honours fmaf. */
 float oracle_atan2f(float y, float x);
