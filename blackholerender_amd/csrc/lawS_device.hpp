@@ -37,15 +37,15 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return tiny ? r * 0x1p-16f : r;
 }
 
-// 1/sqrt(x) correctly rounded to float, for x > 0 finite (normal or denormal), from an estimate y0
-// within 2 ulps of it (v_rsq_f32 is documented within 1). One refinement in double: with
-// e = 1 - x y0^2 (x y0 is exact in double, e takes one rounding), 1/sqrt(x) = y0 (1 - e)^(-1/2)
-// = y0 (1 + e/2 + 3e^2/8 + O(e^3)) and |e| < 2^-21, so y0 + (y0 e)(1/2 + 3e/8) is within 2^-60
-// relative of 1/sqrt(x) before its last two roundings (to double, then to float). The 1/sqrt of
-// every float lies at least 1.4 double ulps from the nearest float midpoint, so the result is the
-// correctly rounded one. An exhaustive CPU check of this sequence (every x in [1, 4), every positive
-// denormal, every estimate within 2 ulps of the true value) covers all positive finite x, since
-// each step is exact under scaling by 4^k.
+// 1/sqrt(x) correctly rounded to float, for x a positive normal float, from an estimate y0 within
+// 2 ulps of it (v_rsq_f32 is documented within 1 ulp; it flushes denormal inputs, which therefore
+// go through rsqrt_rn's scaling). One refinement in double: with e = 1 - x y0^2 (x y0 is exact in
+// double, e takes one rounding), 1/sqrt(x) = y0 (1 - e)^(-1/2) = y0 (1 + e/2 + 3e^2/8 + O(e^3)) and
+// |e| < 2^-21, so y0 + (y0 e)(1/2 + 3e/8) is within 2^-60 relative of 1/sqrt(x) before its last
+// two roundings (to double, then to float). The 1/sqrt of every float lies at least 1.4 double
+// ulps from the nearest float midpoint, so the result is the correctly rounded one. An exhaustive
+// CPU check of this sequence (every x in [1, 4), every estimate within 2 ulps of the true value)
+// covers all positive normal x, since each step is exact under scaling by 4^k.
 __device__ __forceinline__ float rsqrt_refine(float x, float y0) {
     const double xd = (double)x, yd = (double)y0;
     const double t = xd * yd;
@@ -55,19 +55,24 @@ __device__ __forceinline__ float rsqrt_refine(float x, float y0) {
     return (float)__builtin_fma(w, c, yd);
 }
 
-// The march's reciprocal square root: correctly rounded for every positive finite operand. Its
-// operands are squared lengths of positions and velocities; one equal to 0 or +inf (a ray through
-// the exact centre, or beyond 1e19) gives NaN here and a NaN march in the oracle alike.
+// The march's reciprocal square root: correctly rounded for every positive normal operand. Its
+// operands are squared lengths of positions and velocities, normal for any ray between 1e-19 and
+// 1e19 from the centre (the march stops at the horizon, r = 2.02).
 __device__ __forceinline__ float rsqrt_rn_core(float x) {
     return rsqrt_refine(x, __builtin_amdgcn_rsqf(x));
 }
 
-// Correctly rounded 1/sqrt(x) for every float x: +-0 -> +-inf and +inf -> +0 come exactly from the
-// estimate, negative numbers and NaN give NaN.
+// Correctly rounded 1/sqrt(x) for every float x: positive normals by rsqrt_rn_core; +-0 -> +-inf,
+// +inf -> +0; positive denormals through an exact scaling by 2^64 (result scaled by 2^32);
+// negative numbers and NaN give NaN.
 __device__ __forceinline__ float rsqrt_rn(float x) {
-    const float y0 = __builtin_amdgcn_rsqf(x);
-    const float y = rsqrt_refine(x, y0);
-    return __builtin_amdgcn_classf(x, 0x260) ? y0 : y;     // class mask: -0, +0, +inf
+    float y = rsqrt_rn_core(x);
+    if (!__builtin_amdgcn_classf(x, 0x100)) {                       // not a positive normal
+        if (x == 0.0f) y = __builtin_copysignf(__builtin_inff(), x);
+        else if (x == __builtin_inff()) y = 0.0f;
+        else if (x > 0.0f) y = rsqrt_rn_core(x * 0x1p+64f) * 0x1p+32f;
+    }
+    return y;
 }
 
 template <typename R> struct Ar;
@@ -76,7 +81,7 @@ template <> struct Ar<float> {
     static __device__ __forceinline__ float sqrt(float a) { return __builtin_sqrtf(a); }
     // Square root of the march loop's operands (see sqrt_rn): same result as sqrt().
     static __device__ __forceinline__ float sqrt_loop(float a) { return sqrt_rn(a); }
-    // 1/sqrt(a) rounded once (the march loop's reciprocal square root, operands > 0 finite).
+    // 1/sqrt(a) rounded once (the march loop's reciprocal square root, positive normal operands).
     static __device__ __forceinline__ float rsqrt(float a) { return rsqrt_rn_core(a); }
     static __device__ __forceinline__ float abs(float a) { return __builtin_fabsf(a); }
 };

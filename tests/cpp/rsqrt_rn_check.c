@@ -11,11 +11,11 @@
  *   2. oracle_rsqrtf's formula equals c(x);
  *   3. the device refinement, restated below operation for operation, returns c(x) from every
  *      estimate y0 within 2 ulps of c(x) (the five floats c - 2 ulp .. c + 2 ulp);
- * and, for every positive denormal x, that the refinement returns c(x) from every estimate within 2
- * ulps (x is exact in double, so nothing in the refinement sees a denormal); plus the values the
- * device passes through from the estimate (+-0 -> +-inf, +inf -> +0) and the ends of the normal
- * range. It also reports the smallest distance of 1/sqrt(x) from a float midpoint in double ulps
- * (long double arithmetic).
+ * and, for every positive denormal x, that the device's scaled path (refinement of x 2^64 from
+ * every estimate within 2 ulps of its value, result times 2^32; the hardware estimate flushes
+ * denormal inputs) returns c(x); plus the special values and the ends of the normal range. It also
+ * reports the smallest distance of 1/sqrt(x) from a float midpoint in double ulps (long double
+ * arithmetic).
  * Exit status 0 when every check holds.
  */
 #include <math.h>
@@ -91,17 +91,18 @@ int main(void) {
         if (m < worst) { worst = m; worst_x = x; }
         ++n;
     }
-    /* positive denormals, refined directly; the oracle's formula is checked against the exact
-     * decision on x 2^64 (a normal float with the same significand; c(x) = c(x 2^64) 2^32) */
+    /* positive denormals: the device refines x 2^64 (exact, normal) and scales the result by
+     * 2^32 (exact); the oracle's formula is checked against the exact decision on x 2^64 */
     for (uint32_t b = 1; b < 0x800000u; ++b) {
         const float x = fb(b);
         const float c = oracle_formula(x);
         const float xs = x * 0x1p+64f;
-        if (!is_cr(xs, c * 0x1p-32f)) ++bad_denorm;
+        const float cs = oracle_formula(xs);
+        if (!is_cr(xs, cs) || c != cs * 0x1p+32f) ++bad_denorm;
         for (int k = -2; k <= 2; ++k)
-            if (device_core(x, fb(bf(c) + k)) != c) ++bad_denorm;
+            if (device_core(xs, fb(bf(cs) + k)) * 0x1p+32f != c) ++bad_denorm;
     }
-    /* special values, which the device takes from the estimate: +-0 -> +-inf, +inf -> +0 */
+    /* special values, set by the device's branch: +-0 -> +-inf, +inf -> +0 */
     if (oracle_formula(0.0f) != INFINITY || oracle_formula(-0.0f) != -INFINITY) ++bad_special;
     if (oracle_formula(INFINITY) != 0.0f || signbit(oracle_formula(INFINITY))) ++bad_special;
     if (!isnan(oracle_formula(-1.0f)) || !isnan(oracle_formula(NAN))) ++bad_special;
