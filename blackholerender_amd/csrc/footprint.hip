@@ -16,7 +16,8 @@
 //     and column for all four pixels;
 //   * pixels of a quad that lie outside the image (odd widths or heights) count as not escaped,
 //     with the view direction of their position; the reference marches such helper pixels, this
-//     pass does not, so their quads take the unlensed footprint wherever the neighbour escaped;
+//     pass does not, so their quads take the unlensed footprint wherever the neighbour escaped,
+//     and the fallback plane tells these pixels (2) from those at a status edge (1);
 //   * the footprint (e1, e2, inverse R factor, area, mu) exists for escaped pixels only
 //     (blackhole.frag:308); other pixels get zeros there and their derivatives.
 //
@@ -82,10 +83,10 @@ __device__ __forceinline__ Footprint make_footprint(const V3<float>& d, const V3
     return f;
 }
 
-// One pixel of a quad: effective escDir E, view direction D, escape flag.
+// One pixel of a quad: effective escDir E, view direction D, escape flag, outside the image.
 struct Lane {
     V3<float> E, D;
-    bool s;
+    bool s, h;
 };
 
 // Pixels (x0, row) and (x0 + 1, row) of a quad row. VEC: both inside the image, their planes
@@ -104,29 +105,35 @@ __device__ __forceinline__ void load_pair(const MarchConst& c, const FootprintIn
         const float2 ez = *(const float2*)(in.esc + 2 * n + g);
         l0.s = (f & kEscapedBit) != 0u;
         l1.s = ((f >> 8) & kEscapedBit) != 0u;
+        l0.h = l1.h = false;
         l0.E = l0.s ? V3<float>{ex.x, ey.x, ez.x} : l0.D;
         l1.E = l1.s ? V3<float>{ex.y, ey.y, ez.y} : l1.D;
     } else {
         const bool in_row = row < c.height;
-        l0.s = in_row && (in.flags[g] & kEscapedBit) != 0u;
-        l1.s = in_row && x0 + 1 < c.width && (in.flags[g + 1] & kEscapedBit) != 0u;
+        l0.h = !in_row;
+        l1.h = !in_row || x0 + 1 >= c.width;
+        l0.s = !l0.h && (in.flags[g] & kEscapedBit) != 0u;
+        l1.s = !l1.h && (in.flags[g + 1] & kEscapedBit) != 0u;
         l0.E = l0.s ? V3<float>{in.esc[g], in.esc[n + g], in.esc[2 * n + g]} : l0.D;
         l1.E = l1.s ? V3<float>{in.esc[g + 1], in.esc[n + g + 1], in.esc[2 * n + g + 1]} : l1.D;
     }
 }
 
-// Outputs of one pixel.
+// Outputs of one pixel. fb: 0, 1 at a status edge, 2 where the edge runs through a helper pixel.
 struct PixOut {
     V3<float> dx, dy;
     Footprint f;
     bool edge;
+    uint32_t fb;
 };
 
 // Pixel `self` with horizontal pair (h0, h1) and vertical pair (v0, v1).
 __device__ __forceinline__ PixOut pix_out(float omega0, const Lane& self, const Lane& h0,
                                           const Lane& h1, const Lane& v0, const Lane& v1) {
     PixOut p;
-    p.edge = h0.s != h1.s || v0.s != v1.s;   // blackhole.frag:303-306
+    const bool ex = h0.s != h1.s, ey = v0.s != v1.s;
+    p.edge = ex || ey;   // blackhole.frag:303-306
+    p.fb = p.edge ? (((ex && (h0.h || h1.h)) || (ey && (v0.h || v1.h))) ? 2u : 1u) : 0u;
     p.dx = p.edge ? sub3(h1.D, h0.D) : sub3(h1.E, h0.E);
     p.dy = p.edge ? sub3(v1.D, v0.D) : sub3(v1.E, v0.E);
     // blackhole.frag:308: the footprint of escaped pixels only (zeros elsewhere); computed for
@@ -178,10 +185,10 @@ __device__ __forceinline__ void store_pair(const FootprintPlanes& o, int64_t g, 
     if (o.mu) put2<VEC>(o.mu, g, p.f.mu, q.f.mu, second);
     if (o.fallback) {
         if (VEC) {
-            *(uint16_t*)(o.fallback + g) = (uint16_t)((p.edge ? 1u : 0u) | (q.edge ? 256u : 0u));
+            *(uint16_t*)(o.fallback + g) = (uint16_t)(p.fb | (q.fb << 8));
         } else {
-            o.fallback[g] = p.edge ? 1u : 0u;
-            if (second) o.fallback[g + 1] = q.edge ? 1u : 0u;
+            o.fallback[g] = (uint8_t)p.fb;
+            if (second) o.fallback[g + 1] = (uint8_t)q.fb;
         }
     }
 }
@@ -212,6 +219,7 @@ __global__ __launch_bounds__(kFpBlock) void k_footprint(const MarchConst one,
         d.E = d.D;
         e.E = e.D;
         d.s = e.s = false;
+        d.h = e.h = true;
     } else {
         load_pair<VEC>(c, in, n, x0, y0 + 1, d, e);
     }

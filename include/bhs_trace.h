@@ -208,7 +208,10 @@ typedef struct bhs_footprint_soa {
     float* rinv_dev;          /* 3n : Footprint r11, r12, r22, the inverse R factor (sky.glsl:20) */
     float* area_dev;          /* n  : la * lbp = sqrt(Footprint.det), the footprint solid angle */
     float* mu_dev;            /* n  : Footprint.mu = min(uPixelOmega0 / area, 1e4) (sky.glsl:49) */
-    uint8_t* fallback_dev;    /* n  : 1 where the unlensed derivatives were used */
+    uint8_t* fallback_dev;    /* n  : 1 where the unlensed derivatives were used at a status edge,
+                                 2 where that edge runs through a helper pixel beyond an odd image
+                                 edge (the reference marches helper pixels; this pass counts them
+                                 as not escaped), 0 elsewhere */
 } bhs_footprint_soa;
 
 /* Sky footprint post-pass over the ray planes of a bhs_render of `cam` (reads rays->flags_dev and
@@ -218,7 +221,7 @@ typedef struct bhs_footprint_soa {
  * (blackhole.frag:297-306), and for escaped pixels makeFootprint(escDir, ddx, ddy, uPixelOmega0)
  * (sky.glsl:27-51; zeros elsewhere). coarse: 0 fine derivatives (each pixel's own row and column
  * pair), 1 coarse (the quad's first row and column). Pixels beyond an odd edge count as not
- * escaped. Stream-ordered. */
+ * escaped (fallback 2 where that decides the fallback). Stream-ordered. */
 int bhs_sky_footprint(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* rays,
                       int32_t coarse, const bhs_footprint_soa* out, void* stream);
 /* The same over the planes of a bhs_render_batch / bhs_render_surrogate_batch of ncam <= 64

@@ -469,17 +469,19 @@ void oracle_make_footprint(const float d[3], const float a[3], const float b[3],
 
 /* Effective escDir (blackhole.frag:98: escDir starts as the view direction and is replaced only on
  * escape), view direction and escape flag of pixel (col, row); pixels outside the image (a quad's
- * missing lanes) count as not escaped. */
+ * missing lanes, the reference's helper pixels) count as not escaped and are reported in *helper. */
 static void fp_lane(const float pos[3], const float B[9], float tan_half_fov, int32_t width,
                     int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
                     uint32_t camera_id, uint32_t sample, const uint8_t* flags, const float* esc,
-                    int64_t n, int64_t col, int64_t row, float E[3], float D[3], float* s) {
+                    int64_t n, int64_t col, int64_t row, float E[3], float D[3], float* s,
+                    int* helper) {
     const int64_t pixel = row * width + col;
     oracle_ray_dir_at(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
                       sample, pixel, col, row, D);
     const int inside = col < width && row < height;
     const int escaped = inside && (flags[pixel] & 2);
     *s = escaped ? 1.0f : 0.0f;
+    *helper = !inside;
     for (int k = 0; k < 3; ++k) E[k] = escaped ? esc[(int64_t)k * n + pixel] : D[k];
 }
 
@@ -500,16 +502,21 @@ void oracle_sky_footprint(const float pos[3], const float B[9], float tan_half_f
              * and the vertical pair in its column; coarse: the quad's first row and column */
             const int64_t ry = coarse ? y0 : row, cx = coarse ? x0 : col;
             float Ea[3], Da[3], sa, Eb[3], Db[3], sb, Ec[3], Dc[3], sc_, Ed[3], Dd[3], sd;
+            int ha, hb, hc, hd;
             fp_lane(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
-                    sample, flags, esc, n, x0, ry, Ea, Da, &sa);
+                    sample, flags, esc, n, x0, ry, Ea, Da, &sa, &ha);
             fp_lane(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
-                    sample, flags, esc, n, x0 + 1, ry, Eb, Db, &sb);
+                    sample, flags, esc, n, x0 + 1, ry, Eb, Db, &sb, &hb);
             fp_lane(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
-                    sample, flags, esc, n, cx, y0, Ec, Dc, &sc_);
+                    sample, flags, esc, n, cx, y0, Ec, Dc, &sc_, &hc);
             fp_lane(pos, B, tan_half_fov, width, height, jitter, jitter_hashed, seed, camera_id,
-                    sample, flags, esc, n, cx, y0 + 1, Ed, Dd, &sd);
+                    sample, flags, esc, n, cx, y0 + 1, Ed, Dd, &sd, &hd);
             float dx[3], dy[3];
-            const int edge = (sb - sa) != 0.0f || (sd - sc_) != 0.0f;      /* :303-306 */
+            const int edge_x = (sb - sa) != 0.0f, edge_y = (sd - sc_) != 0.0f;
+            const int edge = edge_x || edge_y;                               /* :303-306 */
+            /* 2: the edge runs through a helper pixel beyond an odd image edge, whose status the
+             * reference knows (it marches the helper) and this pass does not */
+            const int helper_edge = (edge_x && (ha || hb)) || (edge_y && (hc || hd));
             for (int k = 0; k < 3; ++k) {
                 dx[k] = edge ? Db[k] - Da[k] : Eb[k] - Ea[k];
                 dy[k] = edge ? Dd[k] - Dc[k] : Ed[k] - Ec[k];
@@ -519,7 +526,7 @@ void oracle_sky_footprint(const float pos[3], const float B[9], float tan_half_f
                 if (ddx) ddx[k * n + p] = dx[k];
                 if (ddy) ddy[k * n + p] = dy[k];
             }
-            if (fallback) fallback[p] = (uint8_t)edge;
+            if (fallback) fallback[p] = (uint8_t)(edge ? (helper_edge ? 2 : 1) : 0);
             oracle_footprint f;
             memset(&f, 0, sizeof f);
             if (flags[p] & 2) {   /* :308: the sky (and its footprint) only for escaped rays */

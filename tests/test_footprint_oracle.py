@@ -125,6 +125,7 @@ def test_status_edges_fall_back_to_the_unlensed_derivatives(coarse):
     esc[1, 8] = True                  # lone pixel at (8, 1): quad (8..9, 0..1)
     D, E, flags = _synthetic(cam, esc)
     fp = O.sky_footprint(cam, flags, E, coarse=coarse)
+    assert set(np.unique(fp["fallback"])) <= {0, 1}   # even sizes: no helper pixels
     fb = fp["fallback"].reshape(h, w).astype(bool)
     want = np.zeros((h, w), bool)
     want[:4, 4:6] = True
@@ -155,11 +156,20 @@ def test_odd_edges_count_as_not_escaped():
     esc = np.ones((h, w), bool)
     D, E, flags = _synthetic(cam, esc)
     fp = O.sky_footprint(cam, flags, E)
-    fb = fp["fallback"].reshape(h, w).astype(bool)
-    want = np.zeros((h, w), bool)
-    want[:, 6] = True
-    want[4, :] = True
+    fb = fp["fallback"].reshape(h, w)
+    # the last column's row pairs and the last row's column pairs run through a helper pixel
+    # beyond the edge: fallback 2, told apart from status edges (1)
+    want = np.zeros((h, w), np.uint8)
+    want[:, 6] = 2
+    want[4, :] = 2
     np.testing.assert_array_equal(fb, want)
+    # a status edge inside the image next to the odd column stays 1: pixel (5, 1) not escaped
+    esc2 = esc.copy()
+    esc2[1, 5] = False
+    D2, E2, flags2 = _synthetic(cam, esc2)
+    fb2 = O.sky_footprint(cam, flags2, E2)["fallback"].reshape(h, w)
+    assert fb2[1, 4] == 1 and fb2[1, 5] == 1 and fb2[0, 5] == 1   # row pair (4,5), column pair
+    assert fb2[1, 6] == 2 and fb2[0, 4] == 0
     # the missing partner's view direction is the ray generation continued past the edge
     ddx = fp["ddx"].reshape(h, w, 3)
     np.testing.assert_array_equal(ddx[2, 6], O.ray_dir_at(cam, 7, 2) - O.ray_dir_at(cam, 6, 2))
