@@ -34,7 +34,7 @@ from ._capi import (CLASS_DISK, CLASS_ESCAPE, CLASS_HORIZON, DISK_NONE, DISK_THI
 from .camera import PRESETS, Camera, look_basis, orbit_camera
 
 __all__ = ["Tracer", "BhsError", "TraceResult", "RayBuffers", "FootprintBuffers", "ShadeBuffers",
-           "disk_shading", "halton_jitter", "march_params",
+           "disk_shading", "halton_jitter", "march_params", "sky_params", "SkyCube",
            "trace", "Camera", "orbit_camera", "look_basis", "PRESETS", "critical_impact_parameter",
            "CLASS_HORIZON", "CLASS_DISK", "CLASS_ESCAPE", "FLAG_MAXSTEPS", "FLAG_ESCAPED",
            "FLAG_BAND64", "FLAG_ROOT64", "FLAG_SURROGATE", "PREC_F32",
@@ -231,6 +231,61 @@ def disk_shading(density=1.0, opacity=2.0, peak_temp=6000.0, brightness=5.5, noi
     """bhs_disk_shading; defaults of the reference's Settings (src/Settings.h:32-45)."""
     return _capi.DiskShading(density, opacity, peak_temp, brightness, noise_scale,
                              noise_strength, warmth, doppler, time, inflow)
+
+
+def sky_params(sky_brightness=1.0, star_density=1.0, star_brightness=1.0,
+               doppler=0.0) -> _capi.SkyParams:
+    """bhs_sky_params; defaults of the reference's Settings (src/Settings.h:53-55) and the film
+    look (uDopplerStrength 0)."""
+    return _capi.SkyParams(sky_brightness, star_density, star_brightness, doppler)
+
+
+class SkyCube:
+    """Backdrop cube map for bhs_sample_sky: `levels` is a list of (6, S_l, S_l, 3) arrays (faces
+    +X -X +Y -Y +Z -Z, row j at t = (j + 0.5) / S_l, S_l = max(1, res >> l)), packed as float4 texels
+    in one device tensor in the library's layout."""
+
+    def __init__(self, levels, device):
+        import numpy as np
+        import torch
+        res = int(np.asarray(levels[0]).shape[1])
+        if not 1 <= len(levels) <= 16:
+            raise ValueError("SkyCube: 1..16 levels")
+        flat = []
+        for l, a in enumerate(levels):
+            a = np.asarray(a, np.float32)
+            S = max(1, res >> l)
+            if a.shape != (6, S, S, 3):
+                raise ValueError(f"SkyCube: level {l} must be (6, {S}, {S}, 3), got {a.shape}")
+            t = np.zeros((6, S, S, 4), np.float32)
+            t[..., :3] = a
+            flat.append(t.reshape(-1, 4))
+        self.res, self.levels = res, len(levels)
+        self.texels = torch.from_numpy(np.concatenate(flat)).to(device)
+
+    @classmethod
+    def from_faces(cls, faces, device, levels: int | None = None):
+        """Cube map from level-0 faces (6, res, res, 3) with box-filtered mip levels (as
+        glGenerateMipmap; res a power of two)."""
+        import numpy as np
+        a = np.asarray(faces, np.float32)
+        res = a.shape[1]
+        n = levels if levels is not None else res.bit_length()
+        out = [a]
+        for _ in range(1, n):
+            S = a.shape[1]
+            if S == 1:
+                break
+            a = 0.25 * (a[:, 0::2, 0::2] + a[:, 1::2, 0::2] + a[:, 0::2, 1::2] + a[:, 1::2, 1::2])
+            out.append(a.astype(np.float32))
+        return cls(out, device)
+
+    def struct(self) -> _capi.SkyCube:
+        return _capi.SkyCube(self.texels.data_ptr(), self.res, self.levels)
+
+    def numpy(self):
+        """(texels (m, 4) float32, res, levels) on the host, the oracle's cube form."""
+        return self.texels.cpu().numpy(), self.res, self.levels
 
 
 def halton_jitter(sample: int):
@@ -430,6 +485,47 @@ class Tracer:
                                           out.data_ptr(), C.c_void_p(self._stream_ptr(stream)))
         self._check(rc, "bhs_gas_lens_table")
         return out
+
+    # ---- sky ----------------------------------------------------------------------------------
+    def _sky_call(self, fn, first_args, n, rays, params, cube, weight, rgb, coarse, stream, what):
+        import torch
+        rays.check(n, self.device, f"{what}: rays")
+        if rays.flags is None or rays.esc is None:
+            raise ValueError(f"{what}: the rays need their flags and escape direction planes")
+        if rgb is None:
+            rgb = torch.zeros((3, n), dtype=torch.float32, device=f"cuda:{self.device}")
+        _check_tensor(rgb, "float32", (3, n), self.device, f"{what}: rgb")
+        if weight is not None:
+            _check_tensor(weight, "float32", (n,), self.device, f"{what}: weight")
+        if cube is not None and cube.texels.device != rgb.device:
+            raise ValueError(f"{what}: the cube map lives on {cube.texels.device}")
+        params = sky_params() if params is None else params
+        cs = cube.struct() if cube is not None else None
+        rs = rays.soa()
+        rc = fn(self._ctx, *first_args, C.byref(rs), int(bool(coarse)), C.byref(params),
+                C.byref(cs) if cs is not None else None,
+                C.c_void_p(weight.data_ptr()) if weight is not None else None,
+                C.c_void_p(rgb.data_ptr()), C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, what)
+        return rgb
+
+    def sample_sky(self, cam: Camera, rays: RayBuffers, params: _capi.SkyParams | None = None,
+                   cube: SkyCube | None = None, weight=None, rgb=None, coarse: bool = False,
+                   stream=None):
+        """The sky of the escaped rays of a render of `cam` (bhs_sample_sky; sky.glsl:57-119):
+        rgb (3, n) += weight * sampleSky; weight (n,) is the shading's sky weight (None: 1)."""
+        return self._sky_call(self._lib.bhs_sample_sky, (C.byref(_camera_struct(cam)),),
+                              cam.width * cam.height, rays, params, cube, weight, rgb, coarse,
+                              stream, "bhs_sample_sky")
+
+    def sample_sky_batch(self, cams, rays: RayBuffers, params: _capi.SkyParams | None = None,
+                         cube: SkyCube | None = None, weight=None, rgb=None, coarse: bool = False,
+                         stream=None):
+        """The sky pass over the planes of a batch render (bhs_sample_sky_batch)."""
+        ca = (_capi.Camera * len(cams))(*[_camera_struct(c) for c in cams])
+        return self._sky_call(self._lib.bhs_sample_sky_batch, (ca, len(cams)),
+                              self.batch_offsets(cams)[-1], rays, params, cube, weight, rgb,
+                              coarse, stream, "bhs_sample_sky_batch")
 
     # ---- thin-disk shading, accumulation --------------------------------------------------------
     def shade_thin_disk(self, cam: Camera, rays: RayBuffers, params: _capi.MarchParams,

@@ -64,6 +64,15 @@ class ShadeSoA(C.Structure):
     _fields_ = [("rgb_dev", C.c_void_p), ("sky_weight_dev", C.c_void_p)]
 
 
+class SkyParams(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ("sky_brightness", "star_density", "star_brightness",
+                                         "doppler")]
+
+
+class SkyCube(C.Structure):
+    _fields_ = [("texels_dev", C.c_void_p), ("res", C.c_int32), ("levels", C.c_int32)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("mode", C.c_int32), ("block", C.c_int32), ("waves_per_simd", C.c_int32),
                 ("refill_min", C.c_int32)]
@@ -135,6 +144,12 @@ def load():
     L.bhs_halton_jitter.argtypes = [C.c_int32, C.POINTER(C.c_float)]
     L.bhs_halton_jitter.restype = None
     L.bhs_disk_textures.argtypes = [vp, vp, vp]
+    L.bhs_sample_sky.argtypes = [vp, C.POINTER(Camera), C.POINTER(RaySoA), i32,
+                                 C.POINTER(SkyParams), vp, vp, vp, vp]
+    L.bhs_sample_sky_batch.argtypes = [vp, vp, i32, C.POINTER(RaySoA), i32, C.POINTER(SkyParams),
+                                       vp, vp, vp, vp]
+    L.bhs_sample_sky.restype = C.c_int
+    L.bhs_sample_sky_batch.restype = C.c_int
     for name in ("bhs_shade_thin_disk", "bhs_shade_thin_disk_batch", "bhs_accumulate",
                  "bhs_disk_textures"):
         getattr(L, name).restype = C.c_int

@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbhs_hip.so")
-SOURCES = ["kernels.hip", "footprint.hip", "gas_lens.hip", "shade.hip", "volume.hip", "capi.hip", "capi_shade.hip", "mlp_train.hip", "mlp_infer.hip", "capi_mlp.hip"]
+SOURCES = ["kernels.hip", "footprint.hip", "sky.hip", "gas_lens.hip", "shade.hip", "volume.hip", "capi.hip", "capi_shade.hip", "mlp_train.hip", "mlp_infer.hip", "capi_mlp.hip"]
 ARCH = os.environ.get("BHS_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

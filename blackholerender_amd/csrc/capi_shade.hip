@@ -9,6 +9,7 @@
 #include "../../include/bhs_trace.h"
 #include "capi_internal.hpp"
 #include "shade.hpp"
+#include "sky.hpp"
 #include "volume.hpp"
 
 using namespace bhs::capi;
@@ -195,6 +196,82 @@ int volumetric_common(bhs_ctx* ctx, const bhs_camera* cams, const bhs_march_para
 
 }  // namespace
 
+namespace {
+
+// The sky pass of one camera (batch false) or a batch of ncam cameras.
+int sky_common(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam, bool batch,
+               const bhs_ray_soa* rays, int32_t coarse, const bhs_sky_params* params,
+               const bhs_sky_cube* cube, const float* sky_weight_dev, float* rgb_dev,
+               void* stream) {
+    const char* what = batch ? "bhs_sample_sky_batch" : "bhs_sample_sky";
+    if (!ctx) return BHS_E_INVALID;
+    if (!cams) return fail(ctx, BHS_E_INVALID, "sky: cameras is NULL%s");
+    if (ncam < 1 || (uint32_t)ncam > bhs::kMaxBatchCams)
+        return fail(ctx, BHS_E_INVALID, "sky: ncam must be in 1..64%s");
+    if (!rays || !rays->flags_dev || !rays->esc_dev || !rgb_dev || !params)
+        return fail(ctx, BHS_E_INVALID, "sky: rays (flags, esc), rgb or params is NULL%s");
+    const float pv[4] = {params->sky_brightness, params->star_density, params->star_brightness,
+                         params->doppler};
+    for (float v : pv)
+        if (!std::isfinite(v) || v < 0.0f)
+            return fail(ctx, BHS_E_INVALID, "sky: parameters must be finite and >= 0%s");
+    bhs::sky::Cube cb;
+    std::memset(&cb, 0, sizeof cb);
+    if (cube) {
+        if (!cube->texels_dev || cube->res < 1 || cube->levels < 1 ||
+            cube->levels > bhs::sky::kMaxLevels)
+            return fail(ctx, BHS_E_INVALID,
+                        "sky: the cube map needs texels, res >= 1 and 1..16 levels%s");
+        cb.texels = (const float4*)cube->texels_dev;
+        cb.res = cube->res;
+        cb.levels = cube->levels;
+        int64_t off = 0;
+        for (int l = 0; l < cube->levels; ++l) {
+            cb.off[l] = off;
+            const int64_t S = std::max<int64_t>(1, (int64_t)cube->res >> l);
+            off += 6 * S * S;
+        }
+    }
+    BatchTable t;
+    t.cams.resize((size_t)ncam);
+    t.item_base.assign((size_t)ncam + 1, 0u);
+    int64_t max_pix = 0;
+    for (int32_t k = 0; k < ncam; ++k) {
+        int st;
+        if ((st = check_camera(ctx, &cams[k])) != BHS_OK) return st;
+        bhs::MarchConst c;
+        std::memset(&c, 0, sizeof c);
+        fill_camera(c, &cams[k]);
+        c.ray_offset = t.n_rays;
+        t.cams[(size_t)k] = c;
+        const int64_t np = (int64_t)cams[k].width * cams[k].height;
+        t.n_rays += np;
+        max_pix = std::max(max_pix, np);
+    }
+    DeviceGuard g(ctx->device);
+    int st;
+    if ((st = ensure_disk_tables(ctx)) != BHS_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
+    const void* table_dev = nullptr;
+    if (batch) {
+        void* td = nullptr;
+        if ((st = upload_batch(ctx, t, s, &td)) != BHS_OK) return st;
+        table_dev = td;
+    }
+    const bhs::sky::SkyConst k{params->sky_brightness, params->star_density,
+                               params->star_brightness, params->doppler};
+    hipError_t e;
+    if ((e = bhs::launch_sky(t.cams[0], (const bhs::MarchConst*)table_dev, (uint32_t)ncam, max_pix,
+                             bhs::FootprintIn{rays->flags_dev, rays->esc_dev}, t.n_rays, k, cb,
+                             ctx->bb_lut, sky_weight_dev, rgb_dev, coarse != 0, s)) != hipSuccess)
+        return hip_fail(ctx, e, what);
+    note_stream(ctx, s);
+    return BHS_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int bhs_render_volumetric(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* params,
@@ -264,6 +341,21 @@ void bhs_halton_jitter(int32_t sample, float* out2) {
         }
         out2[k] = r - 0.5f;
     }
+}
+
+int bhs_sample_sky(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* rays, int32_t coarse,
+                   const bhs_sky_params* params, const bhs_sky_cube* cube,
+                   const float* sky_weight_dev, float* rgb_dev, void* stream) {
+    return sky_common(ctx, cam, 1, false, rays, coarse, params, cube, sky_weight_dev, rgb_dev,
+                      stream);
+}
+
+int bhs_sample_sky_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
+                         const bhs_ray_soa* rays, int32_t coarse, const bhs_sky_params* params,
+                         const bhs_sky_cube* cube, const float* sky_weight_dev, float* rgb_dev,
+                         void* stream) {
+    return sky_common(ctx, cams, ncam, true, rays, coarse, params, cube, sky_weight_dev, rgb_dev,
+                      stream);
 }
 
 int bhs_disk_textures(bhs_ctx* ctx, uint16_t* noise_host, float* lut_host) {

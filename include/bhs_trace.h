@@ -37,6 +37,8 @@
  *                               post-pass over a render's ray planes.
  *  - bhs_render_volumetric[_batch] : the trace pass with the volumetric disk
  *                               (blackhole.frag:142-253 DISK_VOLUMETRIC, disk.glsl:81-110).
+ *  - bhs_sample_sky[_batch]     : the sky of escaped rays (sky.glsl:57-119, blackhole.frag:297-310)
+ *                               added to the colour planes.
  *  - bhs_shade_thin_disk[_batch], bhs_accumulate, bhs_halton_jitter, bhs_disk_textures :
  *                               the thin-disk colour and transparency of blackhole.frag:209-216,234
  *                               (disk.glsl:113-130, blackbody.glsl, noise_bake.comp,
@@ -298,6 +300,47 @@ int bhs_render_volumetric_batch(bhs_ctx* ctx, const bhs_camera* cams,
                                 const bhs_march_params* params, int32_t ncam,
                                 const bhs_disk_shading* shading, const bhs_ray_soa* rays,
                                 const bhs_shade_soa* out, void* stream);
+
+/* Sky (shaders/sky.glsl:57-119, sampleSky and starLayer; added at shaders/blackhole.frag:297-310):
+ * the uniforms of the sky and the backdrop cube map (src/SkyBaker.cpp:83-91: a mip-mapped cube map,
+ * GL_LINEAR_MIPMAP_LINEAR, seamless, src/App.cpp:216). */
+typedef struct bhs_sky_params {
+    float sky_brightness;     /* uSkyBrightness (Settings.h: 1) */
+    float star_density;       /* uStarDensity (1) */
+    float star_brightness;    /* uStarBrightness (1) */
+    float doppler;            /* uDopplerStrength (0 film look, 1 physical) */
+} bhs_sky_params;
+
+#define BHS_SKY_MAX_LEVELS 16
+/* Backdrop cube map (device memory, read only): level l in 0..levels-1 has faces of
+ * S_l = max(1, res >> l) texels; texel (i, j) of face f (order +X, -X, +Y, -Y, +Z, -Z; s = (i +
+ * 0.5) / S_l, t = (j + 0.5) / S_l as OpenGL's glTexImage2D rows) at float4 index
+ * off_l + (f * S_l + j) * S_l + i, off_l = 6 (S_0^2 + ... + S_(l-1)^2); each float4 holds r, g, b
+ * and an unused fourth value. The reference stores R11G11B10F texels; convert them to float. */
+typedef struct bhs_sky_cube {
+    const float* texels_dev;
+    int32_t res;              /* face size of level 0, >= 1 */
+    int32_t levels;           /* 1..BHS_SKY_MAX_LEVELS */
+} bhs_sky_cube;
+
+/* Sky pass over the planes of a render of `cam` (reads rays->flags_dev and rays->esc_dev; n =
+ * width * height): for every escaped pixel with weight w != 0 (w = sky_weight_dev[i], the shading
+ * pass's transmittance, or 1 where sky_weight_dev is NULL),
+ *   rgb_dev[i + k n] += w * sampleSky(escDir, dFdx, dFdy, uPixelOmega0, rcam)[k]   (k = r, g, b)
+ * with the derivatives and footprint of bhs_sky_footprint (coarse as there) and rcam = |cam->pos|.
+ * cube may be NULL (no backdrop: the star layers only). The cube lookup is the OpenGL
+ * specification's exact textureGrad (level of detail log2 of the larger screen-axis footprint in
+ * level-0 texels, trilinear, seamless edges and corners). Other pixels are left unchanged.
+ * Stream-ordered; the first call of a context bakes its blackbody table. */
+int bhs_sample_sky(bhs_ctx* ctx, const bhs_camera* cam, const bhs_ray_soa* rays, int32_t coarse,
+                   const bhs_sky_params* params, const bhs_sky_cube* cube,
+                   const float* sky_weight_dev, float* rgb_dev, void* stream);
+/* The same over the planes of a batch render of ncam <= 64 cameras (camera k at the prefix sum of
+ * width * height), in one launch. */
+int bhs_sample_sky_batch(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam,
+                         const bhs_ray_soa* rays, int32_t coarse, const bhs_sky_params* params,
+                         const bhs_sky_cube* cube, const float* sky_weight_dev, float* rgb_dev,
+                         void* stream);
 
 /* Progressive supersampling (Renderer::accumulatePass, src/Renderer.cpp:389-410, and
  * shaders/accumulate.frag:8-11): acc = w >= 1 ? cur : acc (1 - w) + cur w over 3n floats (rgb

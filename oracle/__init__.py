@@ -50,6 +50,12 @@ class Footprint(C.Structure):
                 ("area", C.c_float), ("mu", C.c_float)]
 
 
+class SkyParams(C.Structure):
+    """oracle_sky_params (uSkyBrightness, uStarDensity, uStarBrightness, uDopplerStrength)."""
+    _fields_ = [("sky_brightness", C.c_float), ("star_density", C.c_float),
+                ("star_brightness", C.c_float), ("doppler", C.c_float)]
+
+
 class Shading(C.Structure):
     """oracle_shading: the disk shading uniforms (src/Renderer.cpp:328-343) and the annulus."""
     _fields_ = [(k, C.c_float) for k in ("density", "opacity", "peak_temp", "brightness",
@@ -150,6 +156,19 @@ def _bind(path: str):
     L.oracle_atan2f.restype = C.c_float
     L.oracle_rsqrtf.argtypes = [C.c_float]
     L.oracle_rsqrtf.restype = C.c_float
+    L.oracle_cube_face.argtypes = [fp, fp]
+    L.oracle_cube_face.restype = C.c_int
+    L.oracle_cube_sample.argtypes = [C.c_void_p, C.c_int32, C.c_int32, fp, fp, fp, fp, fp]
+    L.oracle_star_layer.argtypes = [C.c_void_p, fp, C.POINTER(Footprint), C.c_float, C.c_float,
+                                    C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_float, fp]
+    L.oracle_sample_sky.argtypes = [C.POINTER(SkyParams), C.c_void_p, C.c_int32, C.c_int32,
+                                    C.c_void_p, C.c_float, fp, fp, fp, C.POINTER(Footprint),
+                                    C.c_float, fp]
+    L.oracle_sky_pass.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
+                                  C.c_uint64, C.c_uint32, C.c_uint32, C.c_int32, C.c_void_p,
+                                  C.c_void_p, C.POINTER(SkyParams), C.c_void_p, C.c_int32,
+                                  C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                  C.c_int64, C.c_int32]
     L.oracle_camera_dirs.argtypes = [fp, fp, C.c_float, C.c_int32, C.c_int32, fp, C.c_int32,
                                      C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
                                      C.c_int64, fp]
@@ -615,3 +634,102 @@ def volumetric(cam, params: LawSParams, sh: Shading, mode="band", first=0, count
     o["rgb"] = o["rgb"].T.copy()
     o["esc"] = o["esc"].T.copy()
     return o
+
+
+# ---------------------------------------------------------------------------------------- sky
+def sky_params(sky_brightness=1.0, star_density=1.0, star_brightness=1.0, doppler=0.0) -> SkyParams:
+    """The sky's uniforms (src/Settings.h:53-55 defaults; uDopplerStrength 0 = film look)."""
+    return SkyParams(sky_brightness, star_density, star_brightness, doppler)
+
+
+def pack_cube(levels) -> tuple:
+    """A cube map for the sky: levels = [array (6, S_l, S_l, 3) per level] -> (texels (m, 4)
+    float32, res, n_levels) in the library's layout (level-major, face, row j, column i)."""
+    flat = []
+    for a in levels:
+        a = np.asarray(a, np.float32)
+        t = np.zeros(a.shape[:3] + (4,), np.float32)
+        t[..., :3] = a
+        flat.append(t.reshape(-1, 4))
+    return np.ascontiguousarray(np.concatenate(flat)), int(np.asarray(levels[0]).shape[1]), len(levels)
+
+
+def cube_face(d):
+    """Face index and (s, t) of direction d (OpenGL 4.6 Table 8.19)."""
+    st = (C.c_float * 2)()
+    _, dp_ = _f(d, 3)
+    f = lib().oracle_cube_face(dp_, st)
+    return f, float(st[0]), float(st[1])
+
+
+def cube_sample(cube, d, dx, dy):
+    """textureGrad on the cube (texels, res, levels) at d with derivatives dx, dy: (rgb, lod)."""
+    texels, res, levels = cube
+    o = (C.c_float * 3)()
+    lod = C.c_float()
+    _, dp_ = _f(d, 3)
+    _, xp = _f(dx, 3)
+    _, yp = _f(dy, 3)
+    lib().oracle_cube_sample(texels.ctypes.data, res, levels, dp_, xp, yp, o, C.byref(lod))
+    return np.array(o[:], np.float32), float(lod.value)
+
+
+def _footprint_struct(fp: dict) -> Footprint:
+    f = Footprint()
+    for k in range(3):
+        f.e1[k] = float(fp["e1"][k])
+        f.e2[k] = float(fp["e2"][k])
+        f.rinv[k] = float(fp["rinv"][k])
+    f.area = float(fp["area"])
+    f.mu = float(fp["mu"])
+    return f
+
+
+def star_layer(d, fp: dict, N, prob, flux, sig_star, omega0, salt, g_shift, lut=None):
+    """starLayer (shaders/sky.glsl:57-106) of direction d with footprint fp (make_footprint)."""
+    lut = blackbody_lut() if lut is None else lut
+    o = (C.c_float * 3)()
+    _, dp_ = _f(d, 3)
+    f = _footprint_struct(fp)
+    lib().oracle_star_layer(lut.ctypes.data, dp_, C.byref(f), N, prob, flux, sig_star, omega0,
+                            salt, g_shift, o)
+    return np.array(o[:], np.float32)
+
+
+def sample_sky(k: SkyParams, d, dx, dy, fp: dict, omega0, rcam, cube=None, lut=None):
+    """sampleSky (shaders/sky.glsl:108-119) of one escaped direction."""
+    lut = blackbody_lut() if lut is None else lut
+    texels, res, levels = cube if cube is not None else (None, 0, 0)
+    o = (C.c_float * 3)()
+    _, dp_ = _f(d, 3)
+    _, xp = _f(dx, 3)
+    _, yp = _f(dy, 3)
+    f = _footprint_struct(fp)
+    lib().oracle_sample_sky(C.byref(k), texels.ctypes.data if texels is not None else None, res,
+                            levels, lut.ctypes.data, rcam, dp_, xp, yp, C.byref(f), omega0, o)
+    return np.array(o[:], np.float32)
+
+
+def sky_pass(cam, flags, esc, k: SkyParams, cube=None, weight=None, rgb=None, coarse=False,
+             first=0, stride=1, threads=8, lut=None) -> np.ndarray:
+    """The sky pass over one camera's planes: rgb (3, n) += weight * sampleSky for escaped pixels
+    (every stride-th pixel from first); returns the new (3, n) float32 planes."""
+    n = cam["width"] * cam["height"]
+    lut = blackbody_lut() if lut is None else lut
+    flags = np.ascontiguousarray(flags, dtype=np.uint8).reshape(n)
+    esc_planes = np.ascontiguousarray(np.asarray(esc, dtype=np.float32).reshape(n, 3).T)
+    out = (np.zeros((3, n), np.float32) if rgb is None
+           else np.ascontiguousarray(np.array(rgb, dtype=np.float32).reshape(3, n)))
+    w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float32).reshape(n)
+    texels, res, levels = cube if cube is not None else (None, 0, 0)
+    p, pp = _f(cam["pos"], 3)
+    b, bp = _f(cam["basis"], 9)
+    j, jp = _f(cam.get("jitter", (0.0, 0.0)), 2)
+    lib().oracle_sky_pass(pp, bp, cam["tan_half_fov"], cam["width"], cam["height"], jp,
+                          int(cam.get("jitter_hashed", 0)), int(cam.get("seed", 0)),
+                          int(cam.get("camera_id", 0)), int(cam.get("sample", 0)), int(bool(coarse)),
+                          flags.ctypes.data, esc_planes.ctypes.data, C.byref(k),
+                          texels.ctypes.data if texels is not None else None, res, levels,
+                          lut.ctypes.data, w.ctypes.data if w is not None else None,
+                          out.ctypes.data, first, stride, threads)
+    return out

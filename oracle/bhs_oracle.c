@@ -749,6 +749,7 @@ void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int
  * Thin-disk shading, disk tables, accumulation
  * ============================================================================================== */
 #include "shade.inc"
+#include "sky.inc"
 
 /* ================================================================================================
  * Volumetric disk march (float and double)

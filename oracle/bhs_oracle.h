@@ -205,6 +205,30 @@ float oracle_accumulate_weight(int32_t sample_index, int32_t cap);
 void oracle_accumulate(const float* cur, float* acc, int64_t n, int32_t sample_index, int32_t cap);
 void oracle_halton_jitter(int32_t sample, float out[2]);
 
+/* Sky of escaped rays (sky.inc: sky.glsl:57-119 with the backdrop cube map lookup). Cube maps:
+ * float4 texels, levels x 6 faces (+X -X +Y -Y +Z -Z) x S_l x S_l, S_l = max(1, res >> l). */
+typedef struct oracle_sky_params {
+    float sky_brightness, star_density, star_brightness, doppler;
+} oracle_sky_params;
+int oracle_cube_face(const float d[3], float out_st[2]);
+void oracle_cube_sample(const float* texels, int32_t res, int32_t levels, const float d[3],
+                        const float dx[3], const float dy[3], float o[3], float* lod);
+void oracle_star_layer(const float* lut, const float d[3], const oracle_footprint* fp, float N,
+                       float prob, float flux, float sig_star, float omega0, uint32_t salt,
+                       float g_shift, float o[3]);
+void oracle_sample_sky(const oracle_sky_params* k, const float* texels, int32_t res, int32_t levels,
+                       const float* lut, float rcam, const float d[3], const float dx[3],
+                       const float dy[3], const oracle_footprint* fp, float omega0, float o[3]);
+/* The sky pass over a frame: for every escaped pixel (flags bit 2) of the strided sample with
+ * weight w != 0 (weight NULL: 1), rgb[p + k n] += w * sampleSky(...)[k]; the derivatives and the
+ * footprint as oracle_sky_footprint (coarse as there). texels may be NULL. */
+void oracle_sky_pass(const float pos[3], const float B[9], float tan_half_fov, int32_t width,
+                     int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
+                     uint32_t camera_id, uint32_t sample, int32_t coarse, const uint8_t* flags,
+                     const float* esc, const oracle_sky_params* k, const float* texels,
+                     int32_t res, int32_t levels, const float* lut, const float* weight,
+                     float* rgb, int64_t first, int64_t stride, int32_t threads);
+
 void oracle_volumetric(const float pos[3], const float basis[9], float tan_half_fov, int32_t width,
                        int32_t height, const float jitter[2], int32_t jitter_hashed, uint64_t seed,
                        uint32_t camera_id, uint32_t sample, const oracle_lawS_params* p,
