@@ -26,6 +26,9 @@ HBM-bound, algorithmic bytes 82 per pixel.
 shade: the thin-disk colour and transparency pass (bhs_shade_thin_disk_batch, default disk settings)
 over one surrogate render of the step's frames, and the accumulation of its colour (bhs_accumulate);
 HBM-bound, algorithmic bytes 18 per ray plus 12 per recorded crossing (36 per pixel to accumulate).
+sky: the sky pass (bhs_sample_sky_batch: the two star layers and a mip-mapped cube map backdrop,
+sky.glsl:57-119) over the same render weighted by the shading's transmittance, with its VALU rate
+and the oracle's pass over a sample of the first frame, checked bit for bit.
 volumetric: the volumetric disk (bhs_render_volumetric_batch) at the High preset on 8 jittered 1080p
 frames per call, with the VALU rate of its FP32 march and the oracle's march of a sample beside it.
 gas_lens: the gas lens table (bhs_gas_lens_table, shaders/gas_lens.comp:11-44) at the reference's
@@ -350,7 +353,26 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 256 MiB device-to-device copy (calibrates the HBM counters "
                          "FETCH_SIZE / WRITE_SIZE against a known byte count)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only measure the VALU issue rate per instruction class (bhs_valu_probe)")
     args = ap.parse_args(argv)
+    if args.valu_probe:
+        import torch
+        import blackholerender_amd as bhs
+        tr = bhs.Tracer(0)
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        names = ["v_fma_f32", "v_mul_f32", "v_fma_f64", "v_xor_b32 + v_add_u32", "v_rsq_f32"]
+        res = {}
+        for kind, name in enumerate(names):
+            for wps in (4, 8):
+                r = tr.valu_probe(kind, wps, 4096)
+                g = r["wave_instr"] / (r["ms"] * 1e-3) / 1e9
+                res[f"{name} @ {wps} waves/SIMD"] = {
+                    "G_wave_instr_per_s": round(g, 1),
+                    "cycles_per_wave_instr_per_simd_at_2.4GHz": round(
+                        cus * 4 * 2.4e9 / (g * 1e9), 3)}
+        print(json.dumps({"valu_issue_probe": res, "cus": cus}), flush=True)
+        return 0
     if args.copy_probe:
         import torch
         x = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
@@ -435,6 +457,9 @@ def main(argv=None) -> int:
         res["footprint"] = _footprint_leg(tracer, sur, cams, params, args.steps, args.warmup,
                                           cpu=not args.no_cpu)
         res["shade"] = _shade_leg(tracer, sur, cams, params, args.steps, args.warmup,
+                                  cpu=not args.no_cpu)
+        if args.config == "c2":
+            res["sky"] = _sky_leg(tracer, sur, cams, params, args.steps, args.warmup,
                                   cpu=not args.no_cpu)
         if args.config == "c2" and not args.no_volumetric:
             res["volumetric"] = _volumetric_leg(tracer, 8, max(3, args.steps // 4), 2,
@@ -624,6 +649,125 @@ def _shade_leg(tracer, sur, cams, params, steps, warmup, cpu=True):
                              "kind": "port", "sample": f"one {cam0.width}x{cam0.height} frame"}
     del out, rays, acc
     return rec
+
+
+# Sky pass, algorithmic HBM bytes per escaped pixel: flags (1) and escape direction (12) of the
+# pixel (its quad's other lanes come from the same cache lines), the weight (4), the colour read
+# and written (24).
+SKY_BYTES_PER_PIXEL = 41
+
+
+def synthetic_cube_faces(res=512, seed=11):
+    """A backdrop for the sky pass: a bright band about a tilted great circle (a galaxy's disk)
+    with dust-like modulation and a faint floor, on the six faces of a res^2 cube map."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    c = (np.arange(res) + 0.5) / res * 2 - 1
+    U, V = np.meshgrid(c, c)
+    dirs = [(np.ones_like(U), -V, -U), (-np.ones_like(U), -V, U), (U, np.ones_like(U), V),
+            (U, -np.ones_like(U), -V), (U, -V, np.ones_like(U)), (-U, -V, -np.ones_like(U))]
+    north = np.array([0.2, 0.9, 0.38]); north /= np.linalg.norm(north)
+    phase = rng.uniform(0, 6.3, 3)
+    faces = np.zeros((6, res, res, 3), np.float32)
+    for f, (x, y, z) in enumerate(dirs):
+        n = np.sqrt(x * x + y * y + z * z)
+        x, y, z = x / n, y / n, z / n
+        lat = x * north[0] + y * north[1] + z * north[2]
+        band = np.exp(-(lat / 0.12) ** 2)
+        dust = 0.6 + 0.4 * np.sin(9 * x + phase[0]) * np.sin(7 * y + phase[1]) * np.sin(8 * z + phase[2])
+        faces[f, :, :, 0] = 0.004 + 0.05 * band * dust
+        faces[f, :, :, 1] = 0.004 + 0.045 * band * dust
+        faces[f, :, :, 2] = 0.006 + 0.04 * band
+    return faces
+
+
+def _sky_leg(tracer, sur, cams, params, steps, warmup, cpu=True):
+    """Times the sky pass (bhs_sample_sky_batch: the two star layers and the backdrop cube map,
+    512^2 faces with 10 box-filtered levels) over a surrogate render of the step's frames, weighted
+    by the thin-disk shading's transmittance: HIP events on the launch stream; VALU
+    wave-instructions per launch from rocprofv3 (profiles/valu.json). cpu: the oracle's sky pass
+    over a strided sample of the first frame, checked bit for bit against the GPU."""
+    import numpy as np
+    import torch
+    import blackholerender_amd as bhs
+    from blackholerender_amd import workload as W
+    n = sum(c.width * c.height for c in cams)
+    stream = torch.cuda.current_stream()
+    rays = bhs.RayBuffers(n, "cuda:0")
+    sur.render_batch(cams, params, rays, stream)
+    sh = tracer.shade_thin_disk_batch(cams, rays, params, bhs.disk_shading(), stream=stream)
+    cube = bhs.SkyCube.from_faces(synthetic_cube_faces(), "cuda:0")
+    k = bhs.sky_params()
+    rgb = sh.rgb.clone()
+    fn = lambda: tracer.sample_sky_batch(cams, rays, k, cube=cube, weight=sh.sky_weight, rgb=rgb,
+                                         stream=stream)
+    for _ in range(max(warmup, 1)):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    lit = int(((rays.flags & 2) != 0).logical_and(sh.sky_weight != 0).sum())
+    gbs = lit * SKY_BYTES_PER_PIXEL / (ms * 1e-3) / 1e9
+    rec = {"kernel": "k_sky (quad derivatives, footprint, two star layers, cube map backdrop)",
+           "call_ms": round(ms, 5), "Gpix_per_s": round(n / (ms * 1e-3) / 1e9, 2),
+           "sky_pixels": lit, "cube": "512^2 x 6 faces, 10 levels",
+           "hbm": {"bound": "hbm", "achieved": round(gbs, 1), "peak": W.PEAK_HBM_GBS,
+                   "unit": "GB/s", "frac": round(gbs / W.PEAK_HBM_GBS, 4),
+                   "bytes_per_sky_pixel": SKY_BYTES_PER_PIXEL}}
+    pm = _pmc(f"sky_c2x{len(cams)}")
+    if pm:
+        gi = pm["valu_insts_per_launch"] / (ms * 1e-3) / 1e9
+        rec["roofline"] = {"bound": "valu", "achieved": round(gi, 2), "peak": PEAK_VALU_GINST,
+                           "unit": "G wave-instructions/s", "frac": round(gi / PEAK_VALU_GINST, 4),
+                           "kernel": "k_sky", "valu_insts_per_launch": pm["valu_insts_per_launch"],
+                           "source": pm["source"]}
+    if cpu:
+        import oracle as O
+        cam0 = cams[0]
+        n0 = cam0.width * cam0.height
+        th = host_cpus()["threads"]
+        O.blackbody_lut()
+        base = sh.rgb[:, :n0].cpu().numpy()
+        got = tracer.sample_sky(cam0, _first_camera(rays, n0), k, cube=cube,
+                                weight=sh.sky_weight[:n0].contiguous(),
+                                rgb=sh.rgb[:, :n0].contiguous(), stream=stream)
+        torch.cuda.synchronize()
+        stride = 97
+        flags0 = rays.flags[:n0].cpu().numpy()
+        esc0 = rays.esc[:, :n0].cpu().numpy().T.copy()
+        t0 = time.perf_counter()
+        want = O.sky_pass(cam0.as_dict(), flags0, esc0, O.sky_params(), cube=cube.numpy(),
+                          weight=sh.sky_weight[:n0].cpu().numpy(), rgb=base, stride=stride,
+                          threads=th)
+        dt = time.perf_counter() - t0
+        m = np.count_nonzero(flags0[::stride] & 2)
+        rec["cpu_oracle"] = {"value": round(m / dt / 1e6, 4), "unit": "Msky-pixels/s",
+                             "cores": th, "kind": "port",
+                             "sample": f"every {stride}th pixel of the first frame ({m} sky "
+                                       f"pixels; the footprint of the whole frame first), "
+                                       f"{dt:.2f} s",
+                             "bit_exact_on_sample": bool(np.array_equal(
+                                 got.cpu().numpy()[:, ::stride], want[:, ::stride]))}
+    del rays, sh, rgb, cube
+    return rec
+
+
+def _first_camera(rays, n0):
+    """RayBuffers view of the first n0 rays of a batch (contiguous copies of the planes)."""
+    import blackholerender_amd as bhs
+    r = bhs.RayBuffers.__new__(bhs.RayBuffers)
+    r.n = n0
+    for name in ("cls", "flags", "n_cross", "steps", "b"):
+        t = getattr(rays, name)
+        setattr(r, name, t[:n0].contiguous() if t is not None else None)
+    for name in ("esc", "disk"):
+        t = getattr(rays, name)
+        setattr(r, name, t[:, :n0].contiguous() if t is not None else None)
+    return r
 
 
 def _volumetric_leg(tracer, frames, steps, warmup, cpu=True):

@@ -680,6 +680,13 @@ class Tracer:
                                            C.c_void_p(self._stream_ptr(stream))), "bhs_rsqrt_rn")
         return out
 
+    def valu_probe(self, kind: int, waves_per_simd: int = 8, iters: int = 4096) -> dict:
+        """VALU issue-rate probe (bhs_valu_probe): kernel ms and wave64 instructions issued."""
+        ms, wi = C.c_double(), C.c_double()
+        self._check(self._lib.bhs_valu_probe(self._ctx, kind, waves_per_simd, iters, C.byref(ms),
+                                             C.byref(wi)), "bhs_valu_probe")
+        return {"ms": ms.value, "wave_instr": wi.value}
+
     def set_tuning(self, mode: int, block: int = 256, waves_per_simd: int = 8,
                    refill_min: int = 32):
         t = _capi.Tuning(mode, block, waves_per_simd, refill_min)

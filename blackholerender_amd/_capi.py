@@ -156,6 +156,7 @@ def load():
     L.bhs_march_rays.argtypes = [vp, vp, vp, i64, C.POINTER(MarchParams), C.POINTER(RaySoA), vp]
     L.bhs_last_stats.argtypes = [vp, C.POINTER(Stats)]
     L.bhs_rsqrt_rn.argtypes = [vp, vp, vp, i64, vp]
+    L.bhs_valu_probe.argtypes = [vp, i32, i32, i32, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.bhs_set_timing.argtypes = [vp, C.c_int]
     L.bhs_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(i32)]
     L.bhs_render_batch.argtypes = [vp, vp, vp, i32, C.POINTER(RaySoA), vp]
@@ -200,7 +201,7 @@ def load():
                  "bhs_render_surrogate_batch"):
         getattr(L, name).restype = C.c_int
     for name in ("bhs_sync", "bhs_trace_rays", "bhs_render", "bhs_render_noalloc",
-                 "bhs_march_rays", "bhs_last_stats", "bhs_rsqrt_rn",
+                 "bhs_march_rays", "bhs_last_stats", "bhs_rsqrt_rn", "bhs_valu_probe",
                  "bhs_set_timing", "bhs_timing", "bhs_set_tuning", "bhs_get_tuning",
                  "bhs_autotune", "bhs_render_batch", "bhs_graph_create", "bhs_graph_update",
                  "bhs_graph_launch", "bhs_graph_stats"):

@@ -946,6 +946,36 @@ int bhs_rsqrt_rn(bhs_ctx* ctx, const float* x_dev, float* y_dev, int64_t n, void
     return BHS_OK;
 }
 
+int bhs_valu_probe(bhs_ctx* ctx, int32_t kind, int32_t waves_per_simd, int32_t iters, double* ms,
+                   double* wave_instr) {
+    if (!ctx || !ms || !wave_instr) return BHS_E_INVALID;
+    if (kind < 0 || kind > 4 || waves_per_simd < 1 || waves_per_simd > 8 || iters < 1)
+        return fail(ctx, BHS_E_INVALID, "bhs_valu_probe: kind 0..4, waves_per_simd 1..8, iters >= 1%s");
+    DeviceGuard g(ctx->device);
+    const uint32_t blocks = (uint32_t)ctx->num_cus * (uint32_t)waves_per_simd;   // 4 waves a block
+    float* out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e;
+    if ((e = hipMalloc((void**)&out, (size_t)blocks * 256 * sizeof(float))) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_valu_probe alloc");
+    float t = 0.0f;
+    e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = bhs::launch_valu_probe(kind, blocks, iters, out, nullptr);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = bhs::launch_valu_probe(kind, blocks, iters, out, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    if (e != hipSuccess) return hip_fail(ctx, e, "bhs_valu_probe");
+    *ms = t;
+    *wave_instr = (double)blocks * 4.0 * (double)iters * 16.0 * (kind == 3 ? 2.0 : 1.0);
+    return BHS_OK;
+}
+
 int bhs_last_stats(bhs_ctx* ctx, bhs_stats* out_host) {
     if (!ctx || !out_host) return BHS_E_INVALID;
     std::memset(out_host, 0, sizeof *out_host);

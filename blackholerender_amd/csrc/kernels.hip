@@ -981,6 +981,47 @@ hipError_t launch_rsqrt_rn(const float* x, float* y, int64_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+// VALU issue-rate probe: 16 independent chains per lane of one instruction class (0 v_fma_f32,
+// 1 v_mul_f32, 2 v_fma_f64, 3 v_xor_b32 + v_add_u32, 4 v_rsq_f32), `iters` rounds.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_valu_probe(float* __restrict__ out, int32_t iters, float a,
+                                                    float b) {
+    constexpr int K = 16;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    float x[K];
+    double d[K];
+    uint32_t u[K];
+    for (int c = 0; c < K; ++c) {
+        x[c] = 1.0f + 1e-3f * (float)(i & 1023u) + (float)c;
+        d[c] = x[c];
+        u[c] = i * 2654435761u + (uint32_t)c;
+    }
+    for (int32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            if (KIND == 0) x[c] = __builtin_fmaf(x[c], a, b);
+            if (KIND == 1) x[c] = x[c] * a;
+            if (KIND == 2) d[c] = __builtin_fma(d[c], (double)a, (double)b);
+            if (KIND == 3) u[c] = (u[c] ^ 0x5bd1e995u) + 0x27d4eb2du;
+            if (KIND == 4) x[c] = __builtin_amdgcn_rsqf(x[c]);
+        }
+    }
+    float s = 0.0f;
+    for (int c = 0; c < K; ++c) s += x[c] + (float)d[c] + (float)(u[c] & 255u);
+    out[i] = s;
+}
+
+hipError_t launch_valu_probe(int kind, uint32_t blocks, int32_t iters, float* out, hipStream_t s) {
+    switch (kind) {
+        case 0: hipLaunchKernelGGL(k_valu_probe<0>, dim3(blocks), dim3(256), 0, s, out, iters, 0.999f, 1e-3f); break;
+        case 1: hipLaunchKernelGGL(k_valu_probe<1>, dim3(blocks), dim3(256), 0, s, out, iters, 0.999f, 1e-3f); break;
+        case 2: hipLaunchKernelGGL(k_valu_probe<2>, dim3(blocks), dim3(256), 0, s, out, iters, 0.999f, 1e-3f); break;
+        case 3: hipLaunchKernelGGL(k_valu_probe<3>, dim3(blocks), dim3(256), 0, s, out, iters, 0.999f, 1e-3f); break;
+        default: hipLaunchKernelGGL(k_valu_probe<4>, dim3(blocks), dim3(256), 0, s, out, iters, 0.999f, 1e-3f); break;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_trace_T(const double* pos3, const double* dir3, int64_t n, int32_t max_steps,
                           double step_scale, double esc_tol, double r_esc_min, void* out,
                           hipStream_t s) {

@@ -95,6 +95,8 @@ hipError_t launch_march_rays(const float* pos3, const float* dir3, int64_t n, co
                              hipEvent_t ev_end);
 // y[i] = rsqrt_rn(x[i]) (the march's correctly rounded reciprocal square root), n >= 1.
 hipError_t launch_rsqrt_rn(const float* x, float* y, int64_t n, hipStream_t s);
+// VALU issue-rate probe (k_valu_probe): blocks x 256 lanes, 16 chains each, `iters` rounds.
+hipError_t launch_valu_probe(int kind, uint32_t blocks, int32_t iters, float* out, hipStream_t s);
 hipError_t launch_trace_T(const double* pos3, const double* dir3, int64_t n, int32_t max_steps,
                           double step_scale, double esc_tol, double r_esc_min, void* out,
                           hipStream_t s);

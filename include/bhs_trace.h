@@ -400,6 +400,14 @@ int bhs_last_stats(bhs_ctx* ctx, bhs_stats* out_host);
  * any correctly rounded implementation. Stream-ordered. */
 int bhs_rsqrt_rn(bhs_ctx* ctx, const float* x_dev, float* y_dev, int64_t n, void* stream);
 
+/* VALU issue-rate probe (no counterpart in the reference; the measurement behind the VALU
+ * rooflines): waves_per_simd resident waves on every SIMD each run `iters` rounds of 16
+ * independent instructions of one class (kind 0 v_fma_f32, 1 v_mul_f32, 2 v_fma_f64, 3 v_xor_b32 +
+ * v_add_u32, 4 v_rsq_f32). Synchronous; *ms the kernel's time (HIP events), *wave_instr the
+ * wave64 instructions of the class it issued. */
+int bhs_valu_probe(bhs_ctx* ctx, int32_t kind, int32_t waves_per_simd, int32_t iters, double* ms,
+                   double* wave_instr);
+
 /* Launch shape of the FP32 march. mode 0: one lane per pixel on a static grid of 8x8-pixel wave
  * tiles; mode 1: resident waves pull rays from a global work queue and refill finished lanes
  * (wave ballot + prefix count, one atomic per refill). Chosen per context by bhs_autotune or set by
