@@ -118,9 +118,12 @@ def run_cpu_baseline(runs: int = 5, target_s: float = 0.3) -> dict:
              own defaults (maxSteps 200000, stepScale 0.005, escTol 1e-4, rEscMin 30), FP64, on
              the camera's rays (the oracle's restatement, oracle_trace_T);
       law_S: the shader's thin-disk march (shaders/blackhole.frag:77-253) in FP64 with the
-             camera's step settings (oracle_lawS_f64).
-    Built for this host by the oracle's Makefile (gcc -O3 -march=native -ffp-contract=off); the
-    prebuilt -O2 x86-64-v3 library if that build fails. value = law_S on the 1080p frame, all
+             camera's step settings (oracle_lawS_f64), its 1/sqrt as an IEEE square root and
+             division, the fastest IEEE form on x86 (the oracle's exact form, the device's
+             refined float value, takes about 1.4x as long on the CPU).
+    Built for this host by the oracle's Makefile (make native: gcc -O3 -march=native
+    -ffp-contract=off -DORACLE_F64_RSQRT_DIV); the prebuilt -O2 x86-64-v3 library, with the exact
+    form, if that build fails. value = law_S on the 1080p frame, all
     CPUs."""
     import tempfile
     import numpy as np
@@ -132,7 +135,8 @@ def run_cpu_baseline(runs: int = 5, target_s: float = 0.3) -> dict:
     tmp = tempfile.mkdtemp(prefix="bhs_cpu_baseline_")
     try:
         path = oracle.build_native(os.path.join(tmp, "libbhs_oracle_native.so"))
-        build = "gcc -O3 -march=native -ffp-contract=off -fno-fast-math (built on this host)"
+        build = ("gcc -O3 -march=native -ffp-contract=off -fno-fast-math "
+                 "-DORACLE_F64_RSQRT_DIV (built on this host)")
     except Exception as e:  # the prebuilt oracle, named in the record
         path = oracle.build()
         build = f"prebuilt gcc -O2 -march=x86-64-v3 (native build failed: {type(e).__name__})"
@@ -178,7 +182,8 @@ def run_cpu_baseline(runs: int = 5, target_s: float = 0.3) -> dict:
         pass
     head = table["law_S"][W.C2["name"]]["all"]
     return {"value": head["mrays_per_s"], "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": (f"FP64 law-S march (oracle/bhs_oracle.c) of {head['rays']} rays (every "
+            "sample": (f"FP64 law-S march (oracle/bhs_oracle.c, 1/sqrt as IEEE sqrt and "
+                       f"division) of {head['rays']} rays (every "
                        f"{head['stride']}th pixel) of the first 1920x1080 frame of the step, "
                        f"{threads} threads, best of {runs} runs; law T and the 256x256 camera "
                        f"beside it in `table`"),

@@ -5,7 +5,8 @@
 // Arithmetic contract (the whole library is compiled with -ffp-contract=off and correctly rounded
 // FP32 division/square root): every multiply-add below that is fused is written as an explicit
 // fma, division and square root are IEEE, the march loop's reciprocal square root is correctly
-// rounded (rsqrt_rn), nothing is reassociated. The operation order is fixed:
+// rounded in float (rsqrt_rn) and a fixed refinement of it in double (rsqrt_d), nothing is
+// reassociated. The operation order is fixed:
 //   dot(a,b) = fma(a.z,b.z, fma(a.y,b.y, a.x*b.x)),  normalize(a) = a * (1/sqrt(dot(a,a))),
 //   cross(a,b).x = fma(a.y,b.z, -(a.z*b.y)) (cyclic),
 // so one ray's result does not depend on the GPU, the launch shape or the order rays are processed
@@ -75,6 +76,20 @@ __device__ __forceinline__ float rsqrt_rn(float x) {
     return y;
 }
 
+// 1/sqrt(x) for the FP64 march, x a positive double within the normal float range: the correctly
+// rounded float 1/sqrt of (float)x (rsqrt_rn, relative error < 1.5 * 2^-24 against 1/sqrt(x)),
+// refined in double by one second-order step: with y^2 exact in double and e = 1 - x y^2 rounded
+// once, y (1 + e/2 + 3e^2/8) is within 2^-66 relative of 1/sqrt(x) before its last roundings, so
+// the result is within about half a double ulp. Every operation is an IEEE one on a value fixed
+// by x, so a CPU reproduces the result bit for bit.
+__device__ __forceinline__ double rsqrt_d(double x) {
+    const double y = (double)rsqrt_rn((float)x);
+    const double s = y * y;
+    const double e = __builtin_fma(-x, s, 1.0);
+    const double c = __builtin_fma(e, 0.375, 0.5);
+    return __builtin_fma(y * e, c, y);
+}
+
 template <typename R> struct Ar;
 template <> struct Ar<float> {
     static __device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -89,8 +104,8 @@ template <> struct Ar<double> {
     static __device__ __forceinline__ double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
     static __device__ __forceinline__ double sqrt(double a) { return __builtin_sqrt(a); }
     static __device__ __forceinline__ double sqrt_loop(double a) { return __builtin_sqrt(a); }
-    // In double the reciprocal square root is two IEEE operations: 1 / sqrt(a).
-    static __device__ __forceinline__ double rsqrt(double a) { return 1.0 / __builtin_sqrt(a); }
+    // The FP64 march's reciprocal square root (rsqrt_d).
+    static __device__ __forceinline__ double rsqrt(double a) { return rsqrt_d(a); }
     static __device__ __forceinline__ double abs(double a) { return __builtin_fabs(a); }
 };
 
@@ -512,7 +527,7 @@ __device__ __forceinline__ bool root64(const double X0[3], const double V0[3], d
     {
         V3<double> x{X0[0], X0[1], X0[2]}, v{V0[0], V0[1], V0[2]};
         const double q = dot3(x, x);
-        rk4(x, v, c3, b, 1.0 / __builtin_sqrt(q));
+        rk4(x, v, c3, b, rsqrt_d(q));
         fb = x.y;
     }
     if (fa == 0.0) { hx = X0[0]; hz = X0[2]; return true; }
@@ -523,7 +538,7 @@ __device__ __forceinline__ bool root64(const double X0[3], const double V0[3], d
         const double c = (a * fb - b * fa) / (fb - fa);
         V3<double> x{X0[0], X0[1], X0[2]}, v{V0[0], V0[1], V0[2]};
         const double q = dot3(x, x);
-        rk4(x, v, c3, c, 1.0 / __builtin_sqrt(q));
+        rk4(x, v, c3, c, rsqrt_d(q));
         const double fc = x.y;
         cx = x.x; cz = x.z;
         if (fc == 0.0) break;

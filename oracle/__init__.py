@@ -156,6 +156,8 @@ def _bind(path: str):
     L.oracle_atan2f.restype = C.c_float
     L.oracle_rsqrtf.argtypes = [C.c_float]
     L.oracle_rsqrtf.restype = C.c_float
+    L.oracle_rsqrt_d.argtypes = [C.c_double]
+    L.oracle_rsqrt_d.restype = C.c_double
     L.oracle_cube_face.argtypes = [fp, fp]
     L.oracle_cube_face.restype = C.c_int
     L.oracle_cube_sample.argtypes = [C.c_void_p, C.c_int32, C.c_int32, fp, fp, fp, fp, fp]

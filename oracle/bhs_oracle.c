@@ -212,6 +212,16 @@ typedef int (*oracle_cross_fn)(void* ctx, int index, float r, float phi, float g
  * correctly rounded value: tests/cpp/rsqrt_rn_check.c. */
 float oracle_rsqrtf(float x) { return (float)(1.0 / sqrt((double)x)); }
 
+/* 1/sqrt(x) of the double march: the float value above at (float)x, refined by one second-order
+ * step in double, y (1 + e/2 + 3e^2/8) with e = fma(-x, y^2, 1) (y^2 exact). */
+double oracle_rsqrt_d(double x) {
+    const double y = (double)oracle_rsqrtf((float)x);
+    const double s = y * y;
+    const double e = fma(-x, s, 1.0);
+    const double c = fma(e, 0.375, 0.5);
+    return fma(y * e, c, y);
+}
+
 #define REAL float
 #define FMA fmaf
 #define SQRT sqrtf
@@ -231,7 +241,14 @@ float oracle_rsqrtf(float x) { return (float)(1.0 / sqrt((double)x)); }
 #define REAL double
 #define FMA fma
 #define SQRT sqrt
+#ifdef ORACLE_F64_RSQRT_DIV
+/* The timed CPU baseline's build: 1/sqrt as an IEEE square root and division, the cheaper form on
+ * x86 (about half the latency of the float-then-refine form); its results can differ from the
+ * device's double march in the last bits, and the baseline build is used for timing only. */
 #define RSQRT(a) (1.0 / sqrt(a))
+#else
+#define RSQRT oracle_rsqrt_d
+#endif
 #define FABS fabs
 #define L(x) ((double)(x))
 #define FN(n) n##_f64
@@ -258,7 +275,7 @@ static int root64(const double st[8], double c3, double* hx, double* hz) {
     for (int k = 0; k < 3; ++k) { xa[k] = st[k]; va[k] = st[3 + k]; }
     {
         const double q = dot3_f64(xa, xa);
-        rk4_f64(xa, va, c3, b, 1.0 / sqrt(q));
+        rk4_f64(xa, va, c3, b, oracle_rsqrt_d(q));
         fb = xa[1];
     }
     if (fa == 0.0) { *hx = st[0]; *hz = st[2]; return 1; }
@@ -270,7 +287,7 @@ static int root64(const double st[8], double c3, double* hx, double* hz) {
         double xs[3], vs[3];
         for (int k = 0; k < 3; ++k) { xs[k] = st[k]; vs[k] = st[3 + k]; }
         const double q = dot3_f64(xs, xs);
-        rk4_f64(xs, vs, c3, c, 1.0 / sqrt(q));
+        rk4_f64(xs, vs, c3, c, oracle_rsqrt_d(q));
         const double fc = xs[1];
         for (int k = 0; k < 3; ++k) xc[k] = xs[k];
         if (fc == 0.0) break;
@@ -775,7 +792,7 @@ void oracle_trace_T_batch(const double* pos3, const double* dir3, int64_t n, int
 #define REAL double
 #define FMA fma
 #define SQRT sqrt
-#define RSQRT(a) (1.0 / sqrt(a))
+#define RSQRT oracle_rsqrt_d
 #define FABS fabs
 #define CEIL ceil
 #define L(x) ((double)(x))

@@ -140,6 +140,8 @@ void oracle_march_batch(const float* pos3, const float* dir3, int64_t n, const o
 
 /* 1/sqrt(x) correctly rounded to float: the reciprocal square root of the float march. */
 float oracle_rsqrtf(float x);
+/* 1/sqrt(x) of the double march: oracle_rsqrtf((float)x) refined by one step in double. */
+double oracle_rsqrt_d(double x);
 
 /* Atan2 built from IEEE operations only (range reduction + degree-4 polynomial in t^2); identical
  * in any IEEE-754 float implementation that honours fmaf. */
