@@ -76,14 +76,14 @@ __device__ __forceinline__ float rsqrt_rn(float x) {
     return y;
 }
 
-// 1/sqrt(x) for the FP64 march, x a positive double within the normal float range: the correctly
-// rounded float 1/sqrt of (float)x (rsqrt_rn, relative error < 1.5 * 2^-24 against 1/sqrt(x)),
-// refined in double by one second-order step: with y^2 exact in double and e = 1 - x y^2 rounded
-// once, y (1 + e/2 + 3e^2/8) is within 2^-66 relative of 1/sqrt(x) before its last roundings, so
-// the result is within about half a double ulp. Every operation is an IEEE one on a value fixed
-// by x, so a CPU reproduces the result bit for bit.
+// 1/sqrt(x) for the FP64 march, x a positive double within the normal float range (the march's
+// operands, as for rsqrt_rn_core): the correctly rounded float 1/sqrt of (float)x (relative error
+// < 1.5 * 2^-24 against 1/sqrt(x)), refined in double by one second-order step: with y^2 exact in
+// double and e = 1 - x y^2 rounded once, y (1 + e/2 + 3e^2/8) is within 2^-66 relative of
+// 1/sqrt(x) before its last roundings, so the result is within about half a double ulp. Every
+// operation is an IEEE one on a value fixed by x, so a CPU reproduces the result bit for bit.
 __device__ __forceinline__ double rsqrt_d(double x) {
-    const double y = (double)rsqrt_rn((float)x);
+    const double y = (double)rsqrt_rn_core((float)x);
     const double s = y * y;
     const double e = __builtin_fma(-x, s, 1.0);
     const double c = __builtin_fma(e, 0.375, 0.5);
