@@ -55,10 +55,20 @@ __global__ __launch_bounds__(kSkyBlock) void k_sky(const MarchConst one,
     if (w == 0.0f) return;
     const int32_t row = (int32_t)(pix / c.width), col = (int32_t)(pix - (int64_t)row * c.width);
     const int32_t x0 = col & ~1, y0 = row & ~1;
-    const int32_t ry = COARSE ? y0 : row, cx = COARSE ? x0 : col;
-    const Lane h0 = load_lane(c, in, n, x0, ry), h1 = load_lane(c, in, n, x0 + 1, ry);
-    const Lane v0 = load_lane(c, in, n, cx, y0), v1 = load_lane(c, in, n, cx, y0 + 1);
-    const Lane self = load_lane(c, in, n, col, row);
+    const bool odd_x = (col & 1) != 0, odd_y = (row & 1) != 0;
+    Lane h0, h1, v0, v1, self;
+    if constexpr (COARSE) {   // pairs (x0, y0)-(x0 + 1, y0) and (x0, y0)-(x0, y0 + 1)
+        const Lane q00 = load_lane(c, in, n, x0, y0), q10 = load_lane(c, in, n, x0 + 1, y0);
+        const Lane q01 = load_lane(c, in, n, x0, y0 + 1),
+                   q11 = load_lane(c, in, n, x0 + 1, y0 + 1);
+        h0 = q00; h1 = q10; v0 = q00; v1 = q01;
+        self = odd_y ? (odd_x ? q11 : q01) : (odd_x ? q10 : q00);
+    } else {                  // the pixel's own row and column pairs: three distinct lanes
+        self = load_lane(c, in, n, col, row);
+        const Lane hn = load_lane(c, in, n, col ^ 1, row), vn = load_lane(c, in, n, col, row ^ 1);
+        h0 = odd_x ? hn : self; h1 = odd_x ? self : hn;
+        v0 = odd_y ? vn : self; v1 = odd_y ? self : vn;
+    }
     // Renderer.cpp:246-247,331: pixelAngle = 2 tan(fov/2) / h, uPixelOmega0 = pixelAngle^2
     const float pixel_angle = 2.0f * c.tan_half_fov / (float)c.height;
     const float omega0 = pixel_angle * pixel_angle;

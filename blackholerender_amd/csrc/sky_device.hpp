@@ -169,26 +169,102 @@ __device__ __forceinline__ void cube_sample(const Cube& c, const V3<float>& d, c
     }
     const float lam = rho > 0.0f ? log2_ieee(rho) : -1.0f;
     const int q = c.levels - 1;
-    if (!(lam > 0.0f)) {
-        bilinear(c, 0, f, s, t, o);
-    } else if (lam >= (float)q) {
-        bilinear(c, q, f, s, t, o);
-    } else {
+    // level 0 when magnifying, the last level at or past it, else floor(lam) and the next one
+    int l0 = 0, nl = 1;
+    float w = 0.0f;
+    if (lam >= (float)q) {
+        l0 = q;
+    } else if (lam > 0.0f) {
         const float fl = __builtin_floorf(lam);
-        const int d1 = (int)fl;
-        float o1[3], o2[3];
-        bilinear(c, d1, f, s, t, o1);
-        bilinear(c, d1 + 1, f, s, t, o2);
-        const float w = lam - fl;
-        for (int k = 0; k < 3; ++k) o[k] = mixf(o1[k], o2[k], w);
+        l0 = (int)fl;
+        nl = 2;
+        w = lam - fl;
     }
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, b0 = 0.0f, b1 = 0.0f, b2 = 0.0f;
+#pragma unroll 1
+    for (int i = 0; i < nl; ++i) {   // one call site: the seamless paths are large
+        float r[3];
+        bilinear(c, l0 + i, f, s, t, r);
+        if (i == 0) { a0 = r[0]; a1 = r[1]; a2 = r[2]; }
+        else { b0 = r[0]; b1 = r[1]; b2 = r[2]; }
+    }
+    o[0] = nl == 2 ? mixf(a0, b0, w) : a0;
+    o[1] = nl == 2 ? mixf(a1, b1, w) : a1;
+    o[2] = nl == 2 ? mixf(a2, b2, w) : a2;
+}
+
+// The rounds of pcg3d (common.glsl:21-27) after its first, v = 1664525 v + 1013904223, which the
+// caller has applied: the 27 cells of a star layer share the first round's three values per axis.
+__device__ __forceinline__ U3 pcg3d_rest(U3 v) {
+    v.x += v.y * v.z; v.y += v.z * v.x; v.z += v.x * v.y;
+    v.x ^= v.x >> 16u; v.y ^= v.y >> 16u; v.z ^= v.z >> 16u;
+    v.x += v.y * v.z; v.y += v.z * v.x; v.z += v.x * v.y;
+    return v;
+}
+// Its x component alone (the density test reads only h.x).
+__device__ __forceinline__ uint32_t pcg3d_rest_x(U3 v) {
+    v.x += v.y * v.z; v.y += v.z * v.x; v.z += v.x * v.y;
+    v.x ^= v.x >> 16u; v.y ^= v.y >> 16u; v.z ^= v.z >> 16u;
+    return v.x + v.y * v.z;
+}
+
+__device__ __forceinline__ uint32_t sel3u(uint32_t a, uint32_t b, uint32_t c, int i) {
+    return i == 0 ? a : (i == 1 ? b : c);
+}
+
+// a / NL for the dividends of a star layer, a = c + k / 65536 with |c| <= NL + 2 (the cells around
+// a finite unit direction) and k a 16-bit integer: the product by RN(1 / NL) corrected once by its
+// exact remainder. It equals the IEEE quotient for every such a, NL = 200 and NL = 48
+// (tests/cpp/star_div_check.c checks all 33 M of them).
+template <int NL>
+__device__ __forceinline__ float div_lattice(float a) {
+    static_assert(NL == 200 || NL == 48, "checked for the two layers' lattice sizes only");
+    constexpr float N = (float)NL, rN = 1.0f / (float)NL;
+    const float q0 = a * rN;
+    return __builtin_fmaf(__builtin_fmaf(-q0, N, a), rN, q0);
+}
+
+// One cell of a star layer (sky.glsl:80-96): the star's uniforms, position and screen-space
+// exponent e; false when the shell test drops it (the density test is the caller's).
+struct StarCell {
+    float uy, uz, e;
+};
+template <int NL>
+__device__ __forceinline__ bool star_cell(int cx, int cy, int cz, U3 first, const V3<float>& d,
+                                          const fp::Footprint& fp, float s_in2, float s_out2,
+                                          float isaa, float isab, float isbb, StarCell& o) {
+    const float inv16 = 1.0f / 65536.0f;
+    const U3 hh = pcg3d_rest(first);
+    const float u2x = (float)(hh.x & 0xffffu) * inv16;
+    const float u2y = (float)(hh.y >> 16u) * inv16;
+    const float u2z = (float)(hh.y & 0xffffu) * inv16;
+    o.uy = (float)(hh.z >> 16u) * inv16;
+    o.uz = (float)(hh.z & 0xffffu) * inv16;
+    const V3<float> q{div_lattice<NL>((float)cx + u2x), div_lattice<NL>((float)cy + u2y),
+                      div_lattice<NL>((float)cz + u2z)};
+    const float ql2 = dot3(q, q);
+    if (ql2 < s_in2 || ql2 > s_out2) return false;                     // :90
+    const float iq = rsqrt_rn_core(ql2);   // ql2 within the shell: a positive normal
+    const V3<float> delta{q.x * iq - d.x, q.y * iq - d.y, q.z * iq - d.z};
+    const float d1 = dot3(fp.e1, delta), d2 = dot3(fp.e2, delta);
+    const float sp = fp.r11 * d1 + fp.r12 * d2;
+    const float sq = fp.r22 * d2;
+    o.e = ((isaa * sp) * sp + ((2.0f * isab) * sp) * sq) + (isbb * sq) * sq;
+    return true;
 }
 
 // starLayer (sky.glsl:57-106) for escaped direction d with footprint fp; lut: blackbody table.
+// The 27 cells of the 3x3x3 neighbourhood are bit k = 9 (dz + 1) + 3 (dy + 1) + (dx + 1) of two
+// masks, so that a lane works only on its own candidate cells and the wave's lanes stay in step:
+// the density test for all 27 (the hash's x alone), then the shell and footprint tests over the
+// cells that hold a star, then the star's light over the few that pass, in ascending k, the
+// shader's loop order, so the sum is the same sequence of additions.
+template <int NL>
 __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const V3<float>& d,
-                                           const fp::Footprint& fp, float N, float prob,
-                                           float flux, float sig_star, float omega0,
-                                           uint32_t salt, float g_shift, float o[3]) {
+                                           const fp::Footprint& fp, float prob, float flux,
+                                           float sig_star, float omega0, uint32_t salt,
+                                           float g_shift, float o[3]) {
+    constexpr float N = (float)NL;
     const float det = fp.area * fp.area;                                 // Footprint.det
     const float cells = pow_ieee(det, 0.25f) * N;                        // :61
     const float w = smoothstepf(0.8f, 2.5f, cells);                      // :62
@@ -212,40 +288,54 @@ __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const 
                   c0z = (int)__builtin_floorf(pz);
         const float h = 0.5f / N;
         const float s_in2 = (1.0f - h) * (1.0f - h), s_out2 = (1.0f + h) * (1.0f + h);
-        const float inv16 = 1.0f / 65536.0f;
+        // pcg3d's first round of (c + (salt, 7 salt + 1, 13 salt + 2)) for c0 - 1, c0, c0 + 1
+        uint32_t lx[3], ly[3], lz[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            lx[t] = ((uint32_t)(c0x + t - 1) + salt) * 1664525u + 1013904223u;
+            ly[t] = ((uint32_t)(c0y + t - 1) + (salt * 7u + 1u)) * 1664525u + 1013904223u;
+            lz[t] = ((uint32_t)(c0z + t - 1) + (salt * 13u + 2u)) * 1664525u + 1013904223u;
+        }
+        // density (:85): ux > prob  <=>  (h.x >> 16) > prob 2^16, both sides scaled exactly
+        const float pk = prob * 65536.0f;
+        uint32_t cand = 0;
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+            const uint32_t hx = pcg3d_rest_x(U3{lx[k % 3], ly[(k / 3) % 3], lz[k / 9]});
+            cand |= (float)(hx >> 16u) > pk ? 0u : (1u << k);
+        }
+        // shell (:90) and footprint (:96)
+        uint32_t lit = 0;
+        for (uint32_t m = cand; m != 0u; m &= m - 1u) {
+            const int k = __builtin_ctz(m);
+            const int l = (k >= 9) + (k >= 18), r = k - 9 * l;
+            const int j = (r >= 3) + (r >= 6), i = r - 3 * j;
+            StarCell sc;
+            if (star_cell<NL>(c0x + i - 1, c0y + j - 1, c0z + l - 1,
+                              U3{sel3u(lx[0], lx[1], lx[2], i), sel3u(ly[0], ly[1], ly[2], j),
+                                 sel3u(lz[0], lz[1], lz[2], l)},
+                              d, fp, s_in2, s_out2, isaa, isab, isbb, sc) &&
+                sc.e < 30.0f)
+                lit |= 1u << k;
+        }
+        // the light of the stars that pass (:97-100), in ascending k
         const float lt0 = log_ieee(3000.0f), lt1 = log_ieee(22000.0f);
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int cx = c0x + dx, cy = c0y + dy, cz = c0z + dz;
-                    const U3 hh = pcg3d(U3{(uint32_t)cx + salt, (uint32_t)cy + (salt * 7u + 1u),
-                                           (uint32_t)cz + (salt * 13u + 2u)});
-                    const float ux = (float)(hh.x >> 16u) * inv16;
-                    if (ux > prob) continue;                               // :85
-                    const float u2x = (float)(hh.x & 0xffffu) * inv16;
-                    const float u2y = (float)(hh.y >> 16u) * inv16;
-                    const float u2z = (float)(hh.y & 0xffffu) * inv16;
-                    const float uy = (float)(hh.z >> 16u) * inv16;
-                    const float uz = (float)(hh.z & 0xffffu) * inv16;
-                    const V3<float> q{((float)cx + u2x) / N, ((float)cy + u2y) / N,
-                                      ((float)cz + u2z) / N};
-                    const float ql2 = dot3(q, q);
-                    if (ql2 < s_in2 || ql2 > s_out2) continue;              // :90
-                    const float iq = rsqrt_rn(ql2);
-                    const V3<float> delta{q.x * iq - d.x, q.y * iq - d.y, q.z * iq - d.z};
-                    const float d1 = dot3(fp.e1, delta), d2 = dot3(fp.e2, delta);
-                    const float sp = fp.r11 * d1 + fp.r12 * d2;
-                    const float sq = fp.r22 * d2;
-                    const float e = ((isaa * sp) * sp + ((2.0f * isab) * sp) * sq) +
-                                    (isbb * sq) * sq;
-                    if (!(e < 30.0f)) continue;                             // :96
-                    const float F = flux * (1.0f / (uy > 2e-4f ? uy : 2e-4f));
-                    const float T = exp_ieee(mixf(lt0, lt1, pow_ieee(uz, 1.7f)));
-                    float bs[4];
-                    blackbody(lut, T * g_shift, bs);
-                    const float amp = (F * norm) * exp_ieee(-0.5f * e);
-                    for (int k = 0; k < 3; ++k) sum[k] += mixf(1.0f, bs[k], 0.7f) * amp;
-                }
+        for (uint32_t m = lit; m != 0u; m &= m - 1u) {
+            const int k = __builtin_ctz(m);
+            const int l = (k >= 9) + (k >= 18), r = k - 9 * l;
+            const int j = (r >= 3) + (r >= 6), i = r - 3 * j;
+            StarCell sc;
+            star_cell<NL>(c0x + i - 1, c0y + j - 1, c0z + l - 1,
+                          U3{sel3u(lx[0], lx[1], lx[2], i), sel3u(ly[0], ly[1], ly[2], j),
+                             sel3u(lz[0], lz[1], lz[2], l)},
+                          d, fp, s_in2, s_out2, isaa, isab, isbb, sc);
+            const float F = flux * (1.0f / (sc.uy > 2e-4f ? sc.uy : 2e-4f));
+            const float T = exp_ieee(mixf(lt0, lt1, pow_ieee(sc.uz, 1.7f)));
+            float bs[4];
+            blackbody(lut, T * g_shift, bs);
+            const float amp = (F * norm) * exp_ieee(-0.5f * sc.e);
+            for (int c = 0; c < 3; ++c) sum[c] += mixf(1.0f, bs[c], 0.7f) * amp;
+        }
     }
     bool bad = false;                                                        // :103-104
     for (int k = 0; k < 3; ++k) bad = bad || __builtin_isnan(sum[k]) || __builtin_isinf(sum[k]);
@@ -274,9 +364,9 @@ __device__ __forceinline__ void sample_sky(const SkyConst& k, const Cube& cube,
     }
     const float stb = k.star_brightness * g4;
     float s[3];
-    star_layer(lut, d, fp, 200.0f, 0.40f * k.star_density, 0.0045f * stb, 0.0004f, omega0, 11u, g, s);
+    star_layer<200>(lut, d, fp, 0.40f * k.star_density, 0.0045f * stb, 0.0004f, omega0, 11u, g, s);
     for (int j = 0; j < 3; ++j) col[j] += s[j];
-    star_layer(lut, d, fp, 48.0f, 0.25f * k.star_density, 0.06f * stb, 0.0011f, omega0, 23u, g, s);
+    star_layer<48>(lut, d, fp, 0.25f * k.star_density, 0.06f * stb, 0.0011f, omega0, 23u, g, s);
     for (int j = 0; j < 3; ++j) col[j] += s[j];
     for (int j = 0; j < 3; ++j) o[j] = col[j];
 }

@@ -326,3 +326,18 @@ def test_sky_pass_weights_and_escaped_only():
     inc1, inc_half = one[:, m] - base[:, m], half[:, m] - base[:, m]
     assert (inc1 > 0).any()
     np.testing.assert_allclose(inc_half, 0.5 * inc1, rtol=1e-5, atol=1e-7)
+
+
+def test_star_division_sequence_is_exact():
+    """The sky kernel divides a star's lattice coordinate by N with a product and one fma
+    correction; tests/cpp/star_div_check.c compares it with IEEE division for every dividend a
+    layer can form (N = 200 and 48, cells within N + 2, 16-bit offsets) and finds no difference
+    (the uncorrected product differs in ~29% of them)."""
+    import os
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp")
+    subprocess.run(["make", "-s", "-C", here, "star_div_check"], check=True)
+    r = subprocess.run([os.path.join(here, "star_div_check")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "dividends checked: 33161216" in r.stdout and "mismatches: 0" in r.stdout
