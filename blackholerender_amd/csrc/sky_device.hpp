@@ -14,7 +14,10 @@
 //     reference's own images differ from GPU to GPU in the last bits; this is the exact form.
 //   * the star layers follow sky.glsl:57-106 operation for operation, with GLSL's inversesqrt as
 //     the correctly rounded reciprocal square root, pow(x, -1) as 1 / x, exp, log and pow as the
-//     IEEE-only functions of the shading.
+//     IEEE-only functions of the shading. The cells are visited through bit masks (each test
+//     once per cell, the stars' light summed in the shader's cell order), and the division by the
+//     lattice size is a sequence equal to IEEE division on every dividend it meets, so the values
+//     are those of the loop as written.
 #pragma once
 
 #include <hip/hip_runtime.h>
