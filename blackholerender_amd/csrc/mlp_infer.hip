@@ -545,6 +545,7 @@ struct SurOneCam {
     __device__ __forceinline__ uint32_t find(uint32_t) const { return 0u; }
     __device__ __forceinline__ const MarchConst& cam(uint32_t) const { return c; }
     __device__ __forceinline__ uint32_t base(uint32_t) const { return 0u; }
+    __device__ __forceinline__ uint32_t end(uint32_t) const { return ~0u; }
 };
 // The camera table and the prefix sums are read-only during a launch and the surrogate kernels
 // index them with wave-uniform values: read through the constant address space, those loads
@@ -575,6 +576,10 @@ struct SurCamTable {
     __device__ __forceinline__ uint32_t base(uint32_t k) const {
         return ((ConstU32*)item_base)[k];
     }
+    // one past camera k's last item (the prefix sums hold ncam + 1 entries)
+    __device__ __forceinline__ uint32_t end(uint32_t k) const {
+        return ((ConstU32*)item_base)[k + 1];
+    }
 };
 
 // Camera and pixel of the calling lane's item in the wave whose first item is q_wave: a wave's 64
@@ -604,22 +609,54 @@ __device__ __forceinline__ WaveItem wave_item(const Cams& cams, uint32_t q_wave,
     return w;
 }
 
+// The camera and scanline position of a wave's previous item, carried through a block's chunk
+// loop (wave-uniform, in scalar registers): a block's consecutive chunks mostly belong to one
+// camera, so the camera search and the division of the pixel index by the width are skipped
+// while the wave stays in it and moves on by less than a row.
+struct ScanCursor {
+    uint32_t ci = 0u, lo = 1u, hi = 0u;    // camera ci owns items [lo, hi) (empty at first)
+    uint32_t p = 0u, row = 0u, col = 0u;   // pixel index of the previous item, its row and column
+    bool valid = false;
+};
+
 // As wave_item, with the camera's items taken in scanline order (item base + p is pixel p, the
 // items beyond width * height are padding): a wave's 64 rays are 64 consecutive pixels, so its
 // stores to the output planes are contiguous (an 8x8 tile writes eight 8-pixel row pieces).
 // `item` maps the pixel back to the integrator's tile order.
 template <typename Cams>
 __device__ __forceinline__ WaveItem wave_item_scan(const Cams& cams, uint32_t q_wave,
-                                                   uint32_t n_items) {
+                                                   uint32_t n_items, ScanCursor& cur) {
     const uint32_t qw = __builtin_amdgcn_readfirstlane(q_wave);
     const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t qc = qw < n_items ? qw : n_items - 1;
+    if (qc < cur.lo || qc >= cur.hi) {             // another camera (uniform)
+        cur.ci = cams.find(qc);
+        cur.lo = cams.base(cur.ci);
+        cur.hi = cams.end(cur.ci);
+        cur.valid = false;
+    }
     WaveItem w;
-    w.ci = cams.find(qw < n_items ? qw : n_items - 1);
+    w.ci = cur.ci;
     const MarchConst& c = cams.cam(w.ci);
-    const uint32_t base = cams.base(w.ci);
+    const uint32_t base = cur.lo;
     const uint32_t width = (uint32_t)c.width;
     const uint32_t p0 = qw - base;                 // wave-uniform
-    const uint32_t row0 = p0 / width, col0 = p0 - row0 * width;
+    uint32_t row0, col0;
+    if (cur.valid && p0 >= cur.p && p0 - cur.p < width) {
+        col0 = cur.col + (p0 - cur.p);
+        row0 = cur.row;
+        if (col0 >= width) {
+            col0 -= width;
+            ++row0;
+        }
+    } else {
+        row0 = p0 / width;
+        col0 = p0 - row0 * width;
+    }
+    cur.p = p0;
+    cur.row = row0;
+    cur.col = col0;
+    cur.valid = true;
     uint32_t col = col0 + lane, row = row0;
     while (col >= width) {                         // at most 64 / width turns
         col -= width;
@@ -855,8 +892,9 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
     const int wave = threadIdx.x >> 6;
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
     const uint32_t chunks = (n_items + kBlock - 1) / kBlock;
+    ScanCursor cur;
     for (uint32_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
-        const WaveItem w = wave_item_scan(cams, ch * kBlock + (threadIdx.x & ~63u), n_items);
+        const WaveItem w = wave_item_scan(cams, ch * kBlock + (threadIdx.x & ~63u), n_items, cur);
         const MarchConst& c = cams.cam(w.ci);
         const int64_t pix = w.pix;
         const RayPrep r = prep_ray(c, sur_cam(c), k, w);
@@ -1148,9 +1186,10 @@ void k_surrogate_tab(const Cams cams, const SurConst k, const float* __restrict_
     const uint64_t q_first = (uint64_t)blockIdx.x * cpb * kBlock;
     const uint64_t q_last = q_first + (uint64_t)cpb * kBlock;
     const uint32_t q_end = (uint32_t)(q_last < n_items ? q_last : n_items);
+    ScanCursor cur;
     for (uint32_t q0 = (uint32_t)q_first; q0 < q_end; q0 += kBlock) {
         if (phase != 0u && (near[q0 / kBlock] != 0u) != (phase == 1u)) continue;   // uniform
-        const WaveItem w = wave_item_scan(cams, q0 + (threadIdx.x & ~63u), n_items);
+        const WaveItem w = wave_item_scan(cams, q0 + (threadIdx.x & ~63u), n_items, cur);
         const uint32_t ci = w.ci;
         const MarchConst& c = cams.cam(ci);
         const int64_t pix = w.pix;
