@@ -32,7 +32,8 @@ and the oracle's pass over a sample of the first frame, checked bit for bit.
 volumetric: the volumetric disk (bhs_render_volumetric_batch) at the High preset on 8 jittered 1080p
 frames per call, with the VALU rate of its FP32 march and the oracle's march of a sample beside it.
 gas_lens: the gas lens table (bhs_gas_lens_table, shaders/gas_lens.comp:11-44) at the reference's
-size, 1024 fan rays x 1024 swept angles, beside the oracle's build of it on one thread.
+size, 1024 fan rays x 1024 swept angles, beside the oracle's build of it on one thread; `batch`:
+16 such tables for radii 8..1000 M in one launch (bhs_gas_lens_tables).
 surrogate.roofline: the surrogate pass (table build + two k_surrogate_tab dispatches) against the
 HBM peak (45-byte records written) and the VALU issue peak (wave-instructions per launch from
 rocprofv3, profiles/valu.json).
@@ -858,6 +859,20 @@ def _gas_lens_leg(tracer, steps, cpu=True, cam_r=30.0):
     rec = {"kernel": "k_gas_lens (1024-ray fan, 1024 swept angles, r_cam 30)",
            "ms_per_table": round(e0.elapsed_time(e1) / steps, 4),
            "bound": "latency of the longest fan ray (16 waves on 256 CUs)"}
+    # 16 tables of the same size in one launch (bhs_gas_lens_tables): radii log-spaced over
+    # 8..1000 M, each with its own shadow angle
+    radii = [float(8.0 * (1000.0 / 8.0) ** (k / 15.0)) for k in range(16)]
+    acs = [bhs.shadow_angular_radius_f32(r) for r in radii]
+    tabs = tracer.gas_lens_tables(radii, acs, stream=stream)
+    e0.record(stream)
+    for _ in range(steps):
+        tracer.gas_lens_tables(radii, acs, out=tabs, stream=stream)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    rec["batch"] = {"kernel": "k_gas_lens_batch (16 tables of 1024 x 1024, r_cam 8..1000 M)",
+                    "ms_per_call": round(ms, 4), "ms_per_table": round(ms / len(radii), 4)}
+    del tabs
     if cpu:
         import oracle as O
         t0 = time.perf_counter()

@@ -486,6 +486,29 @@ class Tracer:
         self._check(rc, "bhs_gas_lens_table")
         return out
 
+    def gas_lens_tables(self, cam_rs, alpha_crits=None, n_rays: int = 1024, n_phis: int = 1024,
+                        out=None, stream=None):
+        """Several gas lens tables in one launch (bhs_gas_lens_tables): a (len(cam_rs), n_phis,
+        n_rays) float32 tensor, table k equal to gas_lens_table(cam_rs[k], alpha_crits[k])."""
+        import torch
+        k = len(cam_rs)
+        if not 1 <= k <= 64:
+            raise ValueError("gas_lens_tables: 1..64 camera radii")
+        if alpha_crits is None:
+            alpha_crits = [shadow_angular_radius_f32(r) for r in cam_rs]
+        if len(alpha_crits) != k:
+            raise ValueError("gas_lens_tables: one alpha_crit per camera radius")
+        if out is None:
+            out = torch.empty((k, n_phis, n_rays), dtype=torch.float32,
+                              device=f"cuda:{self.device}")
+        _check_tensor(out, "float32", (k, n_phis, n_rays), self.device, "gas_lens_tables: out")
+        ra = (C.c_float * k)(*[float(r) for r in cam_rs])
+        aa = (C.c_float * k)(*[float(a) for a in alpha_crits])
+        rc = self._lib.bhs_gas_lens_tables(self._ctx, k, ra, aa, n_rays, n_phis, out.data_ptr(),
+                                           C.c_void_p(self._stream_ptr(stream)))
+        self._check(rc, "bhs_gas_lens_tables")
+        return out
+
     # ---- sky ----------------------------------------------------------------------------------
     def _sky_call(self, fn, first_args, n, rays, params, cube, weight, rgb, coarse, stream, what):
         import torch

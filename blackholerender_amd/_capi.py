@@ -165,6 +165,8 @@ def load():
     L.bhs_sky_footprint_batch.argtypes = [vp, vp, i32, C.POINTER(RaySoA), i32,
                                           C.POINTER(FootprintSoA), vp]
     L.bhs_gas_lens_table.argtypes = [vp, C.c_float, C.c_float, i32, i32, vp, vp]
+    L.bhs_gas_lens_tables.argtypes = [vp, i32, C.POINTER(C.c_float), C.POINTER(C.c_float), i32,
+                                      i32, vp, vp]
     L.bhs_graph_create.argtypes = [vp, vp, vp, i32, C.POINTER(RaySoA), C.POINTER(vp)]
     L.bhs_graph_update.argtypes = [vp, vp, vp, vp]
     L.bhs_graph_launch.argtypes = [vp, vp, vp]

@@ -671,6 +671,42 @@ int bhs_gas_lens_table(bhs_ctx* ctx, float cam_r, float alpha_crit, int32_t n_ra
     return BHS_OK;
 }
 
+int bhs_gas_lens_tables(bhs_ctx* ctx, int32_t n_tables, const float* cam_r,
+                        const float* alpha_crit, int32_t n_rays, int32_t n_phis,
+                        float* tables_dev, void* stream) {
+    if (!ctx) return BHS_E_INVALID;
+    if (!tables_dev || !cam_r || !alpha_crit)
+        return fail(ctx, BHS_E_INVALID, "bhs_gas_lens_tables: NULL argument%s");
+    if (n_tables < 1 || n_tables > bhs::kLensBatchMax)
+        return fail(ctx, BHS_E_INVALID, "bhs_gas_lens_tables: need 1 <= n_tables <= 64%s");
+    if (n_rays < 1 || n_phis < 2 || (int64_t)n_rays * n_phis * n_tables > (int64_t)1 << 31)
+        return fail(ctx, BHS_E_INVALID,
+                    "bhs_gas_lens_tables: need n_rays >= 1, n_phis >= 2, at most 2^31 entries%s");
+    bhs::LensBatch b{};
+    for (int32_t k = 0; k < n_tables; ++k) {
+        if (!std::isfinite(cam_r[k]) || !(cam_r[k] > 2.01f))
+            return fail(ctx, BHS_E_INVALID, "bhs_gas_lens_tables: cam_r must be finite and > 2.01%s");
+        if (!std::isfinite(alpha_crit[k]) || !(alpha_crit[k] > 0.0f) ||
+            !(alpha_crit[k] < 3.14159265f))
+            return fail(ctx, BHS_E_INVALID, "bhs_gas_lens_tables: alpha_crit must lie in (0, pi)%s");
+        b.cam_r[k] = cam_r[k];
+        b.alpha_crit[k] = alpha_crit[k];
+    }
+    b.r_max = BHS_LENS_RMAX;
+    b.n_rays = n_rays;
+    b.n_phis = n_phis;
+    b.max_steps = BHS_LENS_MAX_STEPS;
+    b.n_tables = n_tables;
+    DeviceGuard g(ctx->device);
+    hipStream_t s = (hipStream_t)stream;
+    if (int os = order_stream(ctx, s)) return os;
+    hipError_t e;
+    if ((e = bhs::launch_gas_lens_batch(b, tables_dev, s)) != hipSuccess)
+        return hip_fail(ctx, e, "bhs_gas_lens_tables launch");
+    note_stream(ctx, s);
+    return BHS_OK;
+}
+
 void bhs_graph_destroy(bhs_graph* g) {
     if (!g) return;
     {

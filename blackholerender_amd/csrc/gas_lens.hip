@@ -30,10 +30,8 @@ __device__ __forceinline__ float lens_alpha(float u, float ac) {   // gas_lens.g
 
 __device__ __forceinline__ float log_d(float x) { return (float)::log((double)x); }
 
-__global__ __launch_bounds__(kLensBlock) void k_gas_lens(const LensArgs a,
-                                                         float* __restrict__ table) {
-    const int32_t i = (int32_t)(blockIdx.x * kLensBlock + threadIdx.x);
-    if (i >= a.n_rays) return;
+// Fan ray i of the table of a (every row j of column i).
+__device__ __forceinline__ void lens_ray(const LensArgs& a, int32_t i, float* __restrict__ table) {
     const int64_t n = a.n_rays;
     const float al = lens_alpha(((float)i + 0.5f) / (float)a.n_rays, a.alpha_crit);   // :14
     const V3<float> P{a.cam_r, 0.0f, 0.0f};                                            // :16
@@ -71,12 +69,38 @@ __global__ __launch_bounds__(kLensBlock) void k_gas_lens(const LensArgs a,
     for (; j < a.n_phis; ++j) table[(int64_t)j * n + i] = fill;
 }
 
+__global__ __launch_bounds__(kLensBlock) void k_gas_lens(const LensArgs a,
+                                                         float* __restrict__ table) {
+    const int32_t i = (int32_t)(blockIdx.x * kLensBlock + threadIdx.x);
+    if (i < a.n_rays) lens_ray(a, i, table);
+}
+
+// Several tables of the same size in one launch (blockIdx.y: the table), so that their fans fill
+// more of the chip than one table's n_rays / 64 waves.
+__global__ __launch_bounds__(kLensBlock) void k_gas_lens_batch(const LensBatch b,
+                                                               float* __restrict__ tables) {
+    const int32_t i = (int32_t)(blockIdx.x * kLensBlock + threadIdx.x);
+    const uint32_t k = blockIdx.y;
+    if (i >= b.n_rays) return;
+    const LensArgs a{b.cam_r[k], b.alpha_crit[k], b.r_max, b.n_rays, b.n_phis, b.max_steps};
+    lens_ray(a, i, tables + (int64_t)k * b.n_rays * b.n_phis);
+}
+
 }  // namespace
 
 hipError_t launch_gas_lens(const LensArgs& a, float* table, hipStream_t s) {
     if (a.n_rays <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gas_lens, dim3((uint32_t)((a.n_rays + kLensBlock - 1) / kLensBlock)),
                        dim3(kLensBlock), 0, s, a, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_gas_lens_batch(const LensBatch& b, float* tables, hipStream_t s) {
+    if (b.n_rays <= 0 || b.n_tables <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gas_lens_batch,
+                       dim3((uint32_t)((b.n_rays + kLensBlock - 1) / kLensBlock),
+                            (uint32_t)b.n_tables),
+                       dim3(kLensBlock), 0, s, b, tables);
     return hipGetLastError();
 }
 

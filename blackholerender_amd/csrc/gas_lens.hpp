@@ -15,4 +15,15 @@ struct LensArgs {
 
 hipError_t launch_gas_lens(const LensArgs& a, float* table, hipStream_t s);
 
+// Tables of up to kLensBatchMax camera radii (same n_rays, n_phis), table k at
+// tables + k * n_rays * n_phis, in one launch.
+constexpr int kLensBatchMax = 64;
+struct LensBatch {
+    float cam_r[kLensBatchMax], alpha_crit[kLensBatchMax];
+    float r_max;
+    int32_t n_rays, n_phis, max_steps, n_tables;
+};
+
+hipError_t launch_gas_lens_batch(const LensBatch& b, float* tables, hipStream_t s);
+
 }  // namespace bhs

@@ -29,7 +29,8 @@
  *                               replaced by per-ray geometry in structure-of-arrays form.
  *  - bhs_march_rays           : the same law-S march (blackhole.frag:77-253) on explicit rays.
  *  - bhs_gas_lens_table       : the gas lens table of shaders/gas_lens.comp:11-44 (GasParticles'
- *                               lens program, src/GasParticles.cpp:164-166).
+ *                               lens program, src/GasParticles.cpp:164-166); bhs_gas_lens_tables:
+ *                               several camera radii in one launch.
  *  - bhs_sky_footprint / bhs_sky_footprint_batch :
  *                               the sky footprint of shaders/blackhole.frag:297-306 (GLSL
  *                               dFdx/dFdy of escDir over 2x2 quads, the unlensed footprint at
@@ -368,6 +369,14 @@ int bhs_disk_textures(bhs_ctx* ctx, uint16_t* noise_host, float* lut_host);
 #define BHS_LENS_MAX_STEPS 20000
 int bhs_gas_lens_table(bhs_ctx* ctx, float cam_r, float alpha_crit, int32_t n_rays,
                        int32_t n_phis, float* table_dev, void* stream);
+/* n_tables <= 64 such tables of the same size in one launch (no counterpart in the reference,
+ * which builds one table per camera radius, src/GasParticles.cpp:164-166; for callers with several
+ * cameras, e.g. the poses of a batch): table k, for cam_r[k] and alpha_crit[k] (host arrays), at
+ * tables_dev + k * n_phis * n_rays, equal to bhs_gas_lens_table's. One table's fan is n_rays / 64
+ * waves, a sixteenth of the chip at 1024 rays; the batch fills it. Stream-ordered. */
+int bhs_gas_lens_tables(bhs_ctx* ctx, int32_t n_tables, const float* cam_r,
+                        const float* alpha_crit, int32_t n_rays, int32_t n_phis,
+                        float* tables_dev, void* stream);
 
 /* The same batch captured once into a HIP graph and replayed: bhs_graph_create captures (and
  * instantiates) the whole launch sequence for these cameras, settings and output buffers with the

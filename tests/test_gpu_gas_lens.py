@@ -38,3 +38,34 @@ def test_gas_lens_table_rejects_bad_arguments(tracer):
     assert L.bhs_gas_lens_table(ctx, 30.0, 0.2, 8, 1, t.data_ptr(), None) == -1    # n_phis < 2
     assert L.bhs_gas_lens_table(ctx, 30.0, 0.2, 8, 8, None, None) == -1
     assert L.bhs_gas_lens_table(ctx, float("nan"), 0.2, 8, 8, t.data_ptr(), None) == -1
+
+
+def test_gas_lens_tables_batch_equals_single_tables(tracer):
+    """bhs_gas_lens_tables: four radii (far, the reference's 30, close, inside the photon sphere)
+    in one launch equal the single-table calls bit for bit, and the oracle on the smallest."""
+    import torch
+    import blackholerender_amd as bhs
+    radii = [1000.0, 30.0, 4.0, 2.5]
+    print(f"[gas lens batch] radii={radii} 257x129", flush=True)
+    tabs = tracer.gas_lens_tables(radii, None, 257, 129)
+    torch.cuda.synchronize()
+    for k, r in enumerate(radii):
+        one = tracer.gas_lens_table(r, bhs.shadow_angular_radius_f32(r), 257, 129)
+        torch.cuda.synchronize()
+        assert torch.equal(tabs[k], one), r
+    ref = O.gas_lens_table(2.5, bhs.shadow_angular_radius_f32(2.5), 257, 129)
+    np.testing.assert_array_equal(tabs[3].cpu().numpy(), ref)
+
+
+def test_gas_lens_tables_rejects_bad_arguments(tracer):
+    import ctypes as C
+    import torch
+    t = torch.empty(4 * 64, dtype=torch.float32, device=f"cuda:{tracer.device}")
+    L, ctx = tracer._lib, tracer._ctx
+    r = (C.c_float * 2)(30.0, 2.0)
+    a = (C.c_float * 2)(0.2, 0.2)
+    assert L.bhs_gas_lens_tables(ctx, 2, r, a, 8, 8, t.data_ptr(), None) == -1     # at the horizon
+    assert L.bhs_gas_lens_tables(ctx, 0, r, a, 8, 8, t.data_ptr(), None) == -1     # no table
+    assert L.bhs_gas_lens_tables(ctx, 65, r, a, 8, 8, t.data_ptr(), None) == -1    # over 64
+    assert L.bhs_gas_lens_tables(ctx, 1, r, a, 8, 8, None, None) == -1
+    assert L.bhs_gas_lens_tables(ctx, 1, None, a, 8, 8, t.data_ptr(), None) == -1

@@ -109,6 +109,18 @@ def test_gas_lens_rejects_bad_table():
     assert t._lib.calls == []
 
 
+def test_gas_lens_tables_rejects_bad_tables():
+    t = _tracer()
+    with pytest.raises(ValueError, match="shape"):
+        t.gas_lens_tables([30.0, 8.0], [0.2, 0.5], 64, 65,
+                          out=torch.empty((1, 65, 64), dtype=torch.float32))
+    with pytest.raises(ValueError, match="1..64"):
+        t.gas_lens_tables([], [], 64, 65)
+    with pytest.raises(ValueError, match="one alpha_crit"):
+        t.gas_lens_tables([30.0, 8.0], [0.2], 64, 65)
+    assert t._lib.calls == []
+
+
 def test_march_rays_rejects_host_rays():
     t = _tracer()
     p = torch.zeros((10, 3), dtype=torch.float32)
