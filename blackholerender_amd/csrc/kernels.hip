@@ -547,7 +547,10 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
     }
     MarchState<float> st;
     Setup<float> s;
-    uint32_t pix = 0, ci = 0;
+    // The lane's ray: its index in the output planes and (batches) its camera's step constants,
+    // both set at refill, so a step reads them without recomputing the camera's addresses.
+    int64_t gi = 0;
+    const MarchK<float>* kp = BATCH ? &s_k[0] : nullptr;
     bool active = false;
     bool more = true;   // wave-uniform: the queue may still hold items
     uint32_t n_rays = 0, n_steps = 0, n_cross = 0, n_esc = 0;
@@ -555,7 +558,7 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
         uint32_t q = 0;
         if (claim_items(sc.work, n_items, refill_min, active, more, q)) {
             const uint32_t item = list ? list[q] : q;
-            uint32_t local = item;
+            uint32_t local = item, ci = 0;
             if (BATCH) {
                 ci = find_cam(s_base, ncam, item);
                 local = item - s_base[ci];
@@ -563,7 +566,8 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
             const MarchConst& c = BATCH ? s_cam[ci] : c1;
             const int64_t p = item_pixel(c, local);
             if (p >= 0) {
-                pix = (uint32_t)p;
+                gi = c.ray_offset + p;
+                if (BATCH) kp = &s_k[ci];
                 const V3<float> P{c.pos[0], c.pos[1], c.pos[2]};
                 s = lawS_setup(P, ray_dir(c, p));
                 if (!(c.precision == BHS_PREC_F32_BAND64 && in_band(c, s.b))) {
@@ -578,9 +582,8 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
         }
         // Every lane steps (idle lanes on stale state, writing nothing), so the ray state stays in
         // place across the loop instead of being copied at a divergent merge.
-        const int64_t gi = (BATCH ? s_cam[ci].ray_offset : c1.ray_offset) + pix;
         GlobalSink sink{out, gi, ROOTS ? sc.roots + 2 * gi : (RootState<float>*)nullptr};
-        const bool fin = march_step(st, s, BATCH ? s_k[ci] : k1, sink, active);
+        const bool fin = march_step(st, s, BATCH ? *kp : k1, sink, active);
         if (active && fin) {
             store_finish(out, gi, st, s.b);
             n_rays += 1;
