@@ -827,7 +827,7 @@ def _volumetric_leg(tracer, frames, steps, warmup, cpu=True):
         osh = O.shading()
         O.noise_volume()
         O.blackbody_lut()
-        stride = 509
+        stride = 37
         t0 = time.perf_counter()
         ro = O.volumetric(cam0.as_dict(), op, osh, mode="band", stride=stride, threads=th)
         dt = time.perf_counter() - t0
