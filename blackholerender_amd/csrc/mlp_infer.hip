@@ -1444,19 +1444,30 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
             return launch_band_list_batch(table, item_base, ncam, n_items, out, sc, s, ss.aux,
                                           ss.ev_fork, ss.ev_join);
     };
-    // Side marches: the early FP32 fallback on aux2 and the FP64 band on aux.
-    auto side = [&]() -> hipError_t {
+    // Side marches: the FP64 band on aux and the early FP32 fallback on aux2, both after the
+    // point of `s` where ev_fork2 was recorded. side_band records that point and launches the
+    // band; side_early makes aux2 wait for it and launches the list, at once (side) or after the
+    // caller has enqueued other work on `s` (the event keeps the point of its recording).
+    auto side_band = [&]() -> hipError_t {
         hipError_t e2;
-        if ((e2 = hipEventRecord(ss.ev_fork2, s)) != hipSuccess ||
-            (e2 = hipStreamWaitEvent(ss.aux2, ss.ev_fork2, 0)) != hipSuccess)
-            return e2;
+        if ((e2 = hipEventRecord(ss.ev_fork2, s)) != hipSuccess) return e2;
         // the band first: its waves (long FP64 chains) take their slots before the lists'
         if (any_band && (e2 = band()) != hipSuccess) return e2;
+        return hipSuccess;
+    };
+    auto side_early = [&]() -> hipError_t {
+        hipError_t e2;
+        if ((e2 = hipStreamWaitEvent(ss.aux2, ss.ev_fork2, 0)) != hipSuccess) return e2;
         Scratch se = sc;
         se.work = fl.early_work;
         if ((e2 = march_list(fl.early_ids, fl.early_count, se, ss.aux2)) != hipSuccess) return e2;
         if ((e2 = hipEventRecord(ss.ev_join2, ss.aux2)) != hipSuccess) return e2;
         return hipSuccess;
+    };
+    auto side = [&]() -> hipError_t {
+        hipError_t e2;
+        if ((e2 = side_band()) != hipSuccess) return e2;
+        return side_early();
     };
     hipError_t e;
     if (ev_begin && (e = hipEventRecord(ev_begin, s)) != hipSuccess) return e;
@@ -1468,7 +1479,9 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
                            n_items, sc, fl, pack_dests(kToBand, kToEarly, kNoList, kNoList),
                            tab ? (const float*)tab->cam : nullptr);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = side()) != hipSuccess) return e;
+        // the band at once (the call's longest chains); the early list, short, is enqueued after
+        // the pass, so that the host's launches of the pass come first
+        if ((e = side_band()) != hipSuccess) return e;
     }
     if (tab) {
         const int64_t nodes = (int64_t)tab->n2 * tab->n3;
@@ -1518,6 +1531,7 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev_mid && (e = hipEventRecord(ev_mid, s)) != hipSuccess) return e;
+    if (routed && (e = side_early()) != hipSuccess) return e;
     if ((routed || phased) && any_band) {
         // band rays listed after the band march started (sent back by the decode, or met in the
         // second phase): FP64 march on aux2
