@@ -246,7 +246,9 @@ int sky_common(bhs_ctx* ctx, const bhs_camera* cams, int32_t ncam, bool batch,
         t.cams[(size_t)k] = c;
         const int64_t np = (int64_t)cams[k].width * cams[k].height;
         t.n_rays += np;
-        max_pix = std::max(max_pix, np);
+        // one wave per 8x8-pixel tile
+        const int64_t tiles = (int64_t)((cams[k].width + 7) / 8) * ((cams[k].height + 7) / 8);
+        max_pix = std::max(max_pix, 64 * tiles);
     }
     DeviceGuard g(ctx->device);
     int st;

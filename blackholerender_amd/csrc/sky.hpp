@@ -13,7 +13,7 @@ namespace bhs {
 // Per escaped pixel g with weight w = weight[g] (1 if weight is null) and w != 0:
 // rgb[g + k n] += w * sampleSky(escDir, dFdx, dFdy, uPixelOmega0, rcam)[k], the derivatives and the
 // footprint as the footprint pass computes them. One camera (table == nullptr, `one`) or ncam
-// cameras of a device table; max_pixels: the largest camera's width * height.
+// cameras of a device table; max_pixels: 64 x the largest camera's count of 8x8-pixel tiles.
 hipError_t launch_sky(const MarchConst& one, const MarchConst* table, uint32_t ncam,
                       int64_t max_pixels, const FootprintIn& in, int64_t n,
                       const sky::SkyConst& k, const sky::Cube& cube, const float* lut,

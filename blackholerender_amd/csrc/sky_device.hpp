@@ -227,17 +227,17 @@ __device__ __forceinline__ float div_lattice(float a) {
     return __builtin_fmaf(__builtin_fmaf(-q0, N, a), rN, q0);
 }
 
-// One cell of a star layer (sky.glsl:80-96): the star's uniforms, position and screen-space
-// exponent e; false when the shell test drops it (the density test is the caller's).
+// One cell of a star layer (sky.glsl:80-96) from its hash hh = pcg3d(c + salts): the star's
+// uniforms, position and screen-space exponent e; false when the shell test drops it (the density
+// test is the caller's).
 struct StarCell {
     float uy, uz, e;
 };
 template <int NL>
-__device__ __forceinline__ bool star_cell(int cx, int cy, int cz, U3 first, const V3<float>& d,
+__device__ __forceinline__ bool star_cell(int cx, int cy, int cz, const U3& hh, const V3<float>& d,
                                           const fp::Footprint& fp, float s_in2, float s_out2,
                                           float isaa, float isab, float isbb, StarCell& o) {
     const float inv16 = 1.0f / 65536.0f;
-    const U3 hh = pcg3d_rest(first);
     const float u2x = (float)(hh.x & 0xffffu) * inv16;
     const float u2y = (float)(hh.y >> 16u) * inv16;
     const float u2z = (float)(hh.y & 0xffffu) * inv16;
@@ -256,18 +256,51 @@ __device__ __forceinline__ bool star_cell(int cx, int cy, int cz, U3 first, cons
     return true;
 }
 
+// Cells of a wave's shared hash table: the box of 3x3x3 neighbourhoods of its pixels (an 8x8 tile
+// spans 36-100 cells on the benchmark view).
+constexpr int kSkyBoxMax = 256;
+
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const int u = __shfl_xor(v, o, 64);
+        v = u < v ? u : v;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const int u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// pcg3d's first round for cell (cx, cy, cz) of a layer with this salt: c + (salt, 7 salt + 1,
+// 13 salt + 2), then v = 1664525 v + 1013904223.
+__device__ __forceinline__ U3 cell_first(int cx, int cy, int cz, uint32_t salt) {
+    return U3{((uint32_t)cx + salt) * 1664525u + 1013904223u,
+              ((uint32_t)cy + (salt * 7u + 1u)) * 1664525u + 1013904223u,
+              ((uint32_t)cz + (salt * 13u + 2u)) * 1664525u + 1013904223u};
+}
+
 // starLayer (sky.glsl:57-106) for escaped direction d with footprint fp; lut: blackbody table.
-// The 27 cells of the 3x3x3 neighbourhood are bit k = 9 (dz + 1) + 3 (dy + 1) + (dx + 1) of two
-// masks, so that a lane works only on its own candidate cells and the wave's lanes stay in step:
-// the density test for all 27 (the hash's x alone), then the shell and footprint tests over the
-// cells that hold a star, then the star's light over the few that pass, in ascending k, the
-// shader's loop order, so the sum is the same sequence of additions.
+// Called by all 64 lanes of a wave; `on`: the lane's pixel takes the sky. The 27 cells of a lane's
+// 3x3x3 neighbourhood are bit k = 9 (dz + 1) + 3 (dy + 1) + (dx + 1) of two masks, so that a lane
+// works only on its own candidate cells and the wave's lanes stay in step: the density test for
+// all 27 (the hash's x), then the shell and footprint tests over the cells that hold a star, then
+// the star's light over the few that pass, in ascending k, the shader's loop order, so the sum is
+// the same sequence of additions. The hashes come from a table the wave fills once in LDS
+// (s_cells, 3 kSkyBoxMax words) for the box spanning its pixels' neighbourhoods, each cell hashed
+// by one lane, when that box holds at most kSkyBoxMax cells; else each lane hashes its own cells,
+// sharing pcg3d's first round along each axis.
 template <int NL>
 __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const V3<float>& d,
                                            const fp::Footprint& fp, float prob, float flux,
                                            float sig_star, float omega0, uint32_t salt,
-                                           float g_shift, float o[3]) {
+                                           float g_shift, bool on, uint32_t* __restrict__ s_cells,
+                                           float o[3]) {
     constexpr float N = (float)NL;
+    const int lane = (int)(threadIdx.x & 63u);
     const float det = fp.area * fp.area;                                 // Footprint.det
     const float cells = pow_ieee(det, 0.25f) * N;                        // :61
     const float w = smoothstepf(0.8f, 2.5f, cells);                      // :62
@@ -275,38 +308,80 @@ __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const 
     blackbody(lut, 5600.0f * g_shift, bb);
     const float ka = ((((prob * N) * N) * 9.517f) * flux) * omega0;      // :63-64, MEAN_FLUX
     float sum[3] = {0.0f, 0.0f, 0.0f};
-    if (w < 1.0f) {
-        const float s2 = sig_star * sig_star, px2 = 0.45f;
-        const float gi11 = fp.r11 * fp.r11 + fp.r12 * fp.r12;            // Footprint.Gi11
-        const float gi12 = fp.r12 * fp.r22, gi22 = fp.r22 * fp.r22;
-        const float rr = fp.r11 * fp.r22;
-        const float det_gi = rr * rr;
-        const float saa = px2 + s2 * gi11, sab = s2 * gi12, sbb = px2 + s2 * gi22;   // :70
-        const float det_s = (px2 * px2 + (px2 * s2) * (gi11 + gi22)) + (s2 * s2) * det_gi;
-        const float inv_det_s = 1.0f / det_s;
-        const float isaa = sbb * inv_det_s, isab = -sab * inv_det_s, isbb = saa * inv_det_s;
-        const float norm = fp.mu / (kTwoPi * __builtin_sqrtf(det_s));   // :74
-        const float px = d.x * N, py = d.y * N, pz = d.z * N;
-        const int c0x = (int)__builtin_floorf(px), c0y = (int)__builtin_floorf(py),
-                  c0z = (int)__builtin_floorf(pz);
-        const float h = 0.5f / N;
-        const float s_in2 = (1.0f - h) * (1.0f - h), s_out2 = (1.0f + h) * (1.0f + h);
-        // pcg3d's first round of (c + (salt, 7 salt + 1, 13 salt + 2)) for c0 - 1, c0, c0 + 1
+    const bool part = on && w < 1.0f;                                    // :66
+    const float s2 = sig_star * sig_star, px2 = 0.45f;
+    const float gi11 = fp.r11 * fp.r11 + fp.r12 * fp.r12;                // Footprint.Gi11
+    const float gi12 = fp.r12 * fp.r22, gi22 = fp.r22 * fp.r22;
+    const float rr = fp.r11 * fp.r22;
+    const float det_gi = rr * rr;
+    const float saa = px2 + s2 * gi11, sab = s2 * gi12, sbb = px2 + s2 * gi22;       // :70
+    const float det_s = (px2 * px2 + (px2 * s2) * (gi11 + gi22)) + (s2 * s2) * det_gi;
+    const float inv_det_s = 1.0f / det_s;
+    const float isaa = sbb * inv_det_s, isab = -sab * inv_det_s, isbb = saa * inv_det_s;
+    const float norm = fp.mu / (kTwoPi * __builtin_sqrtf(det_s));       // :74
+    const int c0x = (int)__builtin_floorf(d.x * N), c0y = (int)__builtin_floorf(d.y * N),
+              c0z = (int)__builtin_floorf(d.z * N);
+    const float h = 0.5f / N;
+    const float s_in2 = (1.0f - h) * (1.0f - h), s_out2 = (1.0f + h) * (1.0f + h);
+    // the box of the participating lanes' neighbourhoods: [b0 - 1, b1 + 1] per axis (uniform)
+    constexpr int kBig = 1 << 30;
+    const int b0x = wave_min_i(part ? c0x : kBig), b0y = wave_min_i(part ? c0y : kBig),
+              b0z = wave_min_i(part ? c0z : kBig);
+    const int b1x = wave_max_i(part ? c0x : -kBig), b1y = wave_max_i(part ? c0y : -kBig),
+              b1z = wave_max_i(part ? c0z : -kBig);
+    if (b0x != kBig) {   // some lane takes part (uniform)
+        const int nx = b1x - b0x + 3, ny = b1y - b0y + 3, nz = b1z - b0z + 3;
+        const bool shared = nx <= kSkyBoxMax && ny <= kSkyBoxMax && nz <= kSkyBoxMax &&
+                            nx * ny * nz <= kSkyBoxMax;                    // uniform
+        const int nxy = nx * ny;
+        // table slot of the lane's cell c0 - (1, 1, 1); cell k adds i + nx j + nxy l
+        const int base = (c0x - b0x) + nx * ((c0y - b0y) + ny * (c0z - b0z));
+        // per-lane form: pcg3d's first round for c0 - 1, c0, c0 + 1 along each axis
         uint32_t lx[3], ly[3], lz[3];
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            lx[t] = ((uint32_t)(c0x + t - 1) + salt) * 1664525u + 1013904223u;
-            ly[t] = ((uint32_t)(c0y + t - 1) + (salt * 7u + 1u)) * 1664525u + 1013904223u;
-            lz[t] = ((uint32_t)(c0z + t - 1) + (salt * 13u + 2u)) * 1664525u + 1013904223u;
-        }
-        // density (:85): ux > prob  <=>  (h.x >> 16) > prob 2^16, both sides scaled exactly
-        const float pk = prob * 65536.0f;
+        const float pk = prob * 65536.0f;   // ux > prob  <=>  (h.x >> 16) > prob 2^16 (exact)
         uint32_t cand = 0;
+        if (shared) {
+            const int vol = nx * ny * nz;
+            for (int sl = lane; sl < vol; sl += 64) {
+                const int iz = sl / nxy, rem = sl - iz * nxy;
+                const int iy = rem / nx, ix = rem - iy * nx;
+                const U3 hh = pcg3d_rest(cell_first(b0x - 1 + ix, b0y - 1 + iy, b0z - 1 + iz, salt));
+                s_cells[3 * sl] = hh.x;
+                s_cells[3 * sl + 1] = hh.y;
+                s_cells[3 * sl + 2] = hh.z;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (part) {
 #pragma unroll
-        for (int k = 0; k < 27; ++k) {
-            const uint32_t hx = pcg3d_rest_x(U3{lx[k % 3], ly[(k / 3) % 3], lz[k / 9]});
-            cand |= (float)(hx >> 16u) > pk ? 0u : (1u << k);
+                for (int k = 0; k < 27; ++k) {                             // :85
+                    const uint32_t hx = s_cells[3 * (base + (k % 3) + nx * ((k / 3) % 3) + nxy * (k / 9))];
+                    cand |= (float)(hx >> 16u) > pk ? 0u : (1u << k);
+                }
+            }
+        } else if (part) {
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const U3 f = cell_first(c0x + t - 1, c0y + t - 1, c0z + t - 1, salt);
+                lx[t] = f.x;
+                ly[t] = f.y;
+                lz[t] = f.z;
+            }
+#pragma unroll
+            for (int k = 0; k < 27; ++k) {                                 // :85
+                const uint32_t hx = pcg3d_rest_x(U3{lx[k % 3], ly[(k / 3) % 3], lz[k / 9]});
+                cand |= (float)(hx >> 16u) > pk ? 0u : (1u << k);
+            }
         }
+        auto hash_of = [&](int i, int j, int l) -> U3 {
+            if (shared) {
+                const int sl = base + i + nx * j + nxy * l;
+                return U3{s_cells[3 * sl], s_cells[3 * sl + 1], s_cells[3 * sl + 2]};
+            }
+            return pcg3d_rest(U3{sel3u(lx[0], lx[1], lx[2], i), sel3u(ly[0], ly[1], ly[2], j),
+                                 sel3u(lz[0], lz[1], lz[2], l)});
+        };
         // shell (:90) and footprint (:96)
         uint32_t lit = 0;
         for (uint32_t m = cand; m != 0u; m &= m - 1u) {
@@ -314,10 +389,8 @@ __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const 
             const int l = (k >= 9) + (k >= 18), r = k - 9 * l;
             const int j = (r >= 3) + (r >= 6), i = r - 3 * j;
             StarCell sc;
-            if (star_cell<NL>(c0x + i - 1, c0y + j - 1, c0z + l - 1,
-                              U3{sel3u(lx[0], lx[1], lx[2], i), sel3u(ly[0], ly[1], ly[2], j),
-                                 sel3u(lz[0], lz[1], lz[2], l)},
-                              d, fp, s_in2, s_out2, isaa, isab, isbb, sc) &&
+            if (star_cell<NL>(c0x + i - 1, c0y + j - 1, c0z + l - 1, hash_of(i, j, l), d, fp,
+                              s_in2, s_out2, isaa, isab, isbb, sc) &&
                 sc.e < 30.0f)
                 lit |= 1u << k;
         }
@@ -328,10 +401,8 @@ __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const 
             const int l = (k >= 9) + (k >= 18), r = k - 9 * l;
             const int j = (r >= 3) + (r >= 6), i = r - 3 * j;
             StarCell sc;
-            star_cell<NL>(c0x + i - 1, c0y + j - 1, c0z + l - 1,
-                          U3{sel3u(lx[0], lx[1], lx[2], i), sel3u(ly[0], ly[1], ly[2], j),
-                             sel3u(lz[0], lz[1], lz[2], l)},
-                          d, fp, s_in2, s_out2, isaa, isab, isbb, sc);
+            star_cell<NL>(c0x + i - 1, c0y + j - 1, c0z + l - 1, hash_of(i, j, l), d, fp, s_in2,
+                          s_out2, isaa, isab, isbb, sc);
             const float F = flux * (1.0f / (sc.uy > 2e-4f ? sc.uy : 2e-4f));
             const float T = exp_ieee(mixf(lt0, lt1, pow_ieee(sc.uz, 1.7f)));
             float bs[4];
@@ -339,37 +410,45 @@ __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const 
             const float amp = (F * norm) * exp_ieee(-0.5f * sc.e);
             for (int c = 0; c < 3; ++c) sum[c] += mixf(1.0f, bs[c], 0.7f) * amp;
         }
+        // every lane's reads of the table precede the next layer's writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     bool bad = false;                                                        // :103-104
     for (int k = 0; k < 3; ++k) bad = bad || __builtin_isnan(sum[k]) || __builtin_isinf(sum[k]);
     for (int k = 0; k < 3; ++k) {
-        const float s = bad ? 0.0f : (sum[k] < 64.0f ? sum[k] : 64.0f);
-        o[k] = mixf(s, bb[k] * ka, w);
+        const float v = bad ? 0.0f : (sum[k] < 64.0f ? sum[k] : 64.0f);
+        o[k] = mixf(v, bb[k] * ka, w);
     }
 }
 
 // sampleSky (sky.glsl:108-119): g from the observer's radius rcam, the backdrop and two star
-// layers. fp: makeFootprint(d, dx, dy, omega0).
+// layers. fp: makeFootprint(d, dx, dy, omega0). Called by all 64 lanes of a wave (the star layers
+// share a table in s_cells); o is meaningful where `on`.
 __device__ __forceinline__ void sample_sky(const SkyConst& k, const Cube& cube,
                                            const float* __restrict__ lut, float rcam,
                                            const V3<float>& d, const V3<float>& dx,
                                            const V3<float>& dy, const fp::Footprint& fp,
-                                           float omega0, float o[3]) {
+                                           float omega0, bool on, uint32_t* __restrict__ s_cells,
+                                           float o[3]) {
     float t = 1.0f - 2.0f / rcam;
     t = t > 1e-4f ? t : 1e-4f;
     const float g = mixf(1.0f, rsqrt_rn(t), k.doppler);
     const float g4 = ((g * g) * g) * g;
     float col[3] = {0.0f, 0.0f, 0.0f};
-    if (cube.texels) {
+    if (cube.texels && on) {
         cube_sample(cube, d, dx, dy, col);
         const float sb = k.sky_brightness * g4;
         for (int j = 0; j < 3; ++j) col[j] = col[j] * sb;
     }
     const float stb = k.star_brightness * g4;
     float s[3];
-    star_layer<200>(lut, d, fp, 0.40f * k.star_density, 0.0045f * stb, 0.0004f, omega0, 11u, g, s);
+    star_layer<200>(lut, d, fp, 0.40f * k.star_density, 0.0045f * stb, 0.0004f, omega0, 11u, g,
+                    on, s_cells, s);
     for (int j = 0; j < 3; ++j) col[j] += s[j];
-    star_layer<48>(lut, d, fp, 0.25f * k.star_density, 0.06f * stb, 0.0011f, omega0, 23u, g, s);
+    star_layer<48>(lut, d, fp, 0.25f * k.star_density, 0.06f * stb, 0.0011f, omega0, 23u, g, on,
+                   s_cells, s);
     for (int j = 0; j < 3; ++j) col[j] += s[j];
     for (int j = 0; j < 3; ++j) o[j] = col[j];
 }
