@@ -59,7 +59,7 @@ constexpr int kDppXor8 = 0x128;   // row_ror:8
 // One wave per 8x8-pixel tile of a camera (lane = 8 (row & 7) + (col & 7)); every lane rebuilds
 // its own quad lane from the planes (a position past the image is a helper pixel, as in the
 // footprint pass), fine derivatives take the row and column partners from lanes ^ 1 and ^ 8, and
-// all 64 lanes run the star layers together (their shared hash table in LDS); pixels that are not
+// all 64 lanes run the star layers together (their shared cell table in LDS); pixels that are not
 // escaped, lie past the image or have weight 0 keep their colour.
 template <bool COARSE, bool BATCH>
 __global__ __launch_bounds__(kSkyBlock) void k_sky(const MarchConst one,
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kSkyBlock) void k_sky(const MarchConst one,
                                                    const float* __restrict__ weight,
                                                    float* __restrict__ rgb) {
     __shared__ float lut[shade::kLutSize * 4];
-    __shared__ uint32_t s_cells[kSkyBlock / 64][3 * sky::kSkyBoxMax];
+    __shared__ uint32_t s_cells[kSkyBlock / 64][sky::kCellWords * sky::kSkyBoxMax];
     for (int i = threadIdx.x; i < shade::kLutSize * 4; i += kSkyBlock) lut[i] = lut_g[i];
     __syncthreads();
     const MarchConst c = BATCH ? table[blockIdx.y] : one;

@@ -227,6 +227,18 @@ __device__ __forceinline__ float div_lattice(float a) {
     return __builtin_fmaf(__builtin_fmaf(-q0, N, a), rN, q0);
 }
 
+// The screen-space exponent e of a star at unit direction sdir = q / |q| (sky.glsl:92-95) for the
+// pixel at d with footprint fp.
+__device__ __forceinline__ float star_exponent(const V3<float>& sdir, const V3<float>& d,
+                                               const fp::Footprint& fp, float isaa, float isab,
+                                               float isbb) {
+    const V3<float> delta{sdir.x - d.x, sdir.y - d.y, sdir.z - d.z};
+    const float d1 = dot3(fp.e1, delta), d2 = dot3(fp.e2, delta);
+    const float sp = fp.r11 * d1 + fp.r12 * d2;
+    const float sq = fp.r22 * d2;
+    return ((isaa * sp) * sp + ((2.0f * isab) * sp) * sq) + (isbb * sq) * sq;
+}
+
 // One cell of a star layer (sky.glsl:80-96) from its hash hh = pcg3d(c + salts): the star's
 // uniforms, position and screen-space exponent e; false when the shell test drops it (the density
 // test is the caller's).
@@ -248,15 +260,11 @@ __device__ __forceinline__ bool star_cell(int cx, int cy, int cz, const U3& hh, 
     const float ql2 = dot3(q, q);
     if (ql2 < s_in2 || ql2 > s_out2) return false;                     // :90
     const float iq = rsqrt_rn_core(ql2);   // ql2 within the shell: a positive normal
-    const V3<float> delta{q.x * iq - d.x, q.y * iq - d.y, q.z * iq - d.z};
-    const float d1 = dot3(fp.e1, delta), d2 = dot3(fp.e2, delta);
-    const float sp = fp.r11 * d1 + fp.r12 * d2;
-    const float sq = fp.r22 * d2;
-    o.e = ((isaa * sp) * sp + ((2.0f * isab) * sp) * sq) + (isbb * sq) * sq;
+    o.e = star_exponent(V3<float>{q.x * iq, q.y * iq, q.z * iq}, d, fp, isaa, isab, isbb);
     return true;
 }
 
-// Cells of a wave's shared hash table: the box of 3x3x3 neighbourhoods of its pixels (an 8x8 tile
+// Cells of a wave's shared table: the box of 3x3x3 neighbourhoods of its pixels (an 8x8 tile
 // spans 36-100 cells on the benchmark view).
 constexpr int kSkyBoxMax = 256;
 
@@ -283,16 +291,23 @@ __device__ __forceinline__ U3 cell_first(int cx, int cy, int cz, uint32_t salt) 
               ((uint32_t)cz + (salt * 13u + 2u)) * 1664525u + 1013904223u};
 }
 
+// Words of one cell in a wave's shared table: hash z (the star's brightness and colour
+// uniforms), the star's unit direction (3 floats), and 1 when the cell holds a star within the
+// shell (density and shell tests, which depend on the cell alone).
+constexpr int kCellWords = 5;
+
 // starLayer (sky.glsl:57-106) for escaped direction d with footprint fp; lut: blackbody table.
 // Called by all 64 lanes of a wave; `on`: the lane's pixel takes the sky. The 27 cells of a lane's
 // 3x3x3 neighbourhood are bit k = 9 (dz + 1) + 3 (dy + 1) + (dx + 1) of two masks, so that a lane
 // works only on its own candidate cells and the wave's lanes stay in step: the density test for
 // all 27 (the hash's x), then the shell and footprint tests over the cells that hold a star, then
 // the star's light over the few that pass, in ascending k, the shader's loop order, so the sum is
-// the same sequence of additions. The hashes come from a table the wave fills once in LDS
-// (s_cells, 3 kSkyBoxMax words) for the box spanning its pixels' neighbourhoods, each cell hashed
-// by one lane, when that box holds at most kSkyBoxMax cells; else each lane hashes its own cells,
-// sharing pcg3d's first round along each axis.
+// the same sequence of additions. When the box spanning the wave's pixels' neighbourhoods holds at
+// most kSkyBoxMax cells, the wave first evaluates everything that depends on a cell alone once per
+// cell (its hash, the density and shell tests, the star's direction) into LDS (s_cells,
+// kCellWords words per cell), and a lane's candidates are then its cells that hold a star within
+// the shell; else each lane hashes and tests its own cells, sharing pcg3d's first round along each
+// axis.
 template <int NL>
 __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const V3<float>& d,
                                            const fp::Footprint& fp, float prob, float flux,
@@ -342,23 +357,34 @@ __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const 
         uint32_t cand = 0;
         if (shared) {
             const int vol = nx * ny * nz;
+            const float inv16 = 1.0f / 65536.0f;
             for (int sl = lane; sl < vol; sl += 64) {
                 const int iz = sl / nxy, rem = sl - iz * nxy;
                 const int iy = rem / nx, ix = rem - iy * nx;
-                const U3 hh = pcg3d_rest(cell_first(b0x - 1 + ix, b0y - 1 + iy, b0z - 1 + iz, salt));
-                s_cells[3 * sl] = hh.x;
-                s_cells[3 * sl + 1] = hh.y;
-                s_cells[3 * sl + 2] = hh.z;
+                const int cx = b0x - 1 + ix, cy = b0y - 1 + iy, cz = b0z - 1 + iz;
+                const U3 hh = pcg3d_rest(cell_first(cx, cy, cz, salt));
+                const bool dense = !((float)(hh.x >> 16u) > pk);               // :85
+                const V3<float> q{div_lattice<NL>((float)cx + (float)(hh.x & 0xffffu) * inv16),
+                                  div_lattice<NL>((float)cy + (float)(hh.y >> 16u) * inv16),
+                                  div_lattice<NL>((float)cz + (float)(hh.y & 0xffffu) * inv16)};
+                const float ql2 = dot3(q, q);
+                const bool shell = !(ql2 < s_in2 || ql2 > s_out2);            // :90
+                const float iq = rsqrt_rn_core(shell ? ql2 : 1.0f);   // in the shell: normal
+                uint32_t* cw = s_cells + kCellWords * sl;
+                cw[0] = hh.z;
+                cw[1] = __builtin_bit_cast(uint32_t, q.x * iq);
+                cw[2] = __builtin_bit_cast(uint32_t, q.y * iq);
+                cw[3] = __builtin_bit_cast(uint32_t, q.z * iq);
+                cw[4] = dense && shell ? 1u : 0u;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (part) {
 #pragma unroll
-                for (int k = 0; k < 27; ++k) {                             // :85
-                    const uint32_t hx = s_cells[3 * (base + (k % 3) + nx * ((k / 3) % 3) + nxy * (k / 9))];
-                    cand |= (float)(hx >> 16u) > pk ? 0u : (1u << k);
-                }
+                for (int k = 0; k < 27; ++k)
+                    cand |= s_cells[kCellWords * (base + (k % 3) + nx * ((k / 3) % 3) +
+                                                  nxy * (k / 9)) + 4] << k;
             }
         } else if (part) {
 #pragma unroll
@@ -374,13 +400,24 @@ __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const 
                 cand |= (float)(hx >> 16u) > pk ? 0u : (1u << k);
             }
         }
-        auto hash_of = [&](int i, int j, int l) -> U3 {
+        // the star of candidate cell (i, j, l): uniforms and exponent; false when the shell drops it
+        auto star_of = [&](int i, int j, int l, StarCell& sc) -> bool {
             if (shared) {
-                const int sl = base + i + nx * j + nxy * l;
-                return U3{s_cells[3 * sl], s_cells[3 * sl + 1], s_cells[3 * sl + 2]};
+                const uint32_t* cw = s_cells + kCellWords * (base + i + nx * j + nxy * l);
+                const uint32_t hz = cw[0];
+                sc.uy = (float)(hz >> 16u) * (1.0f / 65536.0f);
+                sc.uz = (float)(hz & 0xffffu) * (1.0f / 65536.0f);
+                const V3<float> sdir{__builtin_bit_cast(float, cw[1]),
+                                     __builtin_bit_cast(float, cw[2]),
+                                     __builtin_bit_cast(float, cw[3])};
+                sc.e = star_exponent(sdir, d, fp, isaa, isab, isbb);
+                return true;   // the shared candidates passed the shell test
             }
-            return pcg3d_rest(U3{sel3u(lx[0], lx[1], lx[2], i), sel3u(ly[0], ly[1], ly[2], j),
-                                 sel3u(lz[0], lz[1], lz[2], l)});
+            const U3 hh = pcg3d_rest(U3{sel3u(lx[0], lx[1], lx[2], i),
+                                        sel3u(ly[0], ly[1], ly[2], j),
+                                        sel3u(lz[0], lz[1], lz[2], l)});
+            return star_cell<NL>(c0x + i - 1, c0y + j - 1, c0z + l - 1, hh, d, fp, s_in2, s_out2,
+                                 isaa, isab, isbb, sc);
         };
         // shell (:90) and footprint (:96)
         uint32_t lit = 0;
@@ -389,10 +426,7 @@ __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const 
             const int l = (k >= 9) + (k >= 18), r = k - 9 * l;
             const int j = (r >= 3) + (r >= 6), i = r - 3 * j;
             StarCell sc;
-            if (star_cell<NL>(c0x + i - 1, c0y + j - 1, c0z + l - 1, hash_of(i, j, l), d, fp,
-                              s_in2, s_out2, isaa, isab, isbb, sc) &&
-                sc.e < 30.0f)
-                lit |= 1u << k;
+            if (star_of(i, j, l, sc) && sc.e < 30.0f) lit |= 1u << k;
         }
         // the light of the stars that pass (:97-100), in ascending k
         const float lt0 = log_ieee(3000.0f), lt1 = log_ieee(22000.0f);
@@ -401,8 +435,7 @@ __device__ __forceinline__ void star_layer(const float* __restrict__ lut, const 
             const int l = (k >= 9) + (k >= 18), r = k - 9 * l;
             const int j = (r >= 3) + (r >= 6), i = r - 3 * j;
             StarCell sc;
-            star_cell<NL>(c0x + i - 1, c0y + j - 1, c0z + l - 1, hash_of(i, j, l), d, fp, s_in2,
-                          s_out2, isaa, isab, isbb, sc);
+            star_of(i, j, l, sc);
             const float F = flux * (1.0f / (sc.uy > 2e-4f ? sc.uy : 2e-4f));
             const float T = exp_ieee(mixf(lt0, lt1, pow_ieee(sc.uz, 1.7f)));
             float bs[4];
