@@ -59,6 +59,15 @@ __global__ __launch_bounds__(kBlock) void k_shade_planes(const MarchConst* __res
                                                          const uint16_t* __restrict__ vol,
                                                          const float* __restrict__ lut,
                                                          ShadeList list) {
+    // the cameras' first rays and lapses in LDS (ncam <= 64): a crossing ray finds its camera
+    // there instead of in the table in global memory
+    __shared__ int64_t s_off[kMaxBatchCams];
+    __shared__ float s_lapse[kMaxBatchCams];
+    for (uint32_t k = threadIdx.x; k < ncam; k += kBlock) {
+        s_off[k] = cams[k].ray_offset;
+        s_lapse[k] = cam_lapse(cams[k]);
+    }
+    __syncthreads();
     const int64_t n = rays.n;
     for (int64_t i0 = (int64_t)blockIdx.x * kBlock; i0 < n_rays; i0 += (int64_t)gridDim.x * kBlock) {
         const int64_t i = i0 + threadIdx.x;
@@ -68,8 +77,12 @@ __global__ __launch_bounds__(kBlock) void k_shade_planes(const MarchConst* __res
             const uint32_t nc = rays.n_cross[i];
             Composite cp{{0.0f, 0.0f, 0.0f}, 1.0f, false};
             if (nc > 0u) {
-                const MarchConst& c = cams[find_ray_cam(cams, ncam, i)];
-                const float lapse = cam_lapse(c);
+                uint32_t lo = 0, hi = ncam;                        // the ray's camera
+                while (hi - lo > 1u) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= i) lo = mid; else hi = mid;
+                }
+                const float lapse = s_lapse[lo];
                 const uint32_t m = nc < 2u ? nc : 2u;
                 for (uint32_t k = 0; k < m && !cp.opaque; ++k) {
                     float s[3], a;
