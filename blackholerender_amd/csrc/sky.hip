@@ -7,10 +7,12 @@
 // derivatives and footprint, and evaluates the backdrop lookup and the two star layers; other
 // pixels, and pixels whose transmittance is 0, leave the colour unchanged.
 //
-// Roofline: VALU. Per escaped pixel the two star layers hash 54 lattice cells (pcg3d) and evaluate
-// the stars near the pixel (a Gaussian, a blackbody lookup, exp/log/pow); the memory traffic (the
-// quad's flags and escape directions, the weight, the colour read and written: ~70 B per pixel)
-// is small next to it.
+// Roofline: VALU. Per tile and layer the wave evaluates the lattice cells of its pixels'
+// neighbourhoods (pcg3d, the density and shell tests, the star direction: 36-100 cells for the
+// tiles of the benchmark view); per pixel the stars of its 27 cells that pass both tests (~4 and ~2)
+// get the quadratic form of the footprint, and the few near the pixel a Gaussian, a blackbody lookup
+// and exp/log/pow. The memory traffic (flags, escape direction, weight, colour read and written:
+// 41 B per sky pixel) is small next to it.
 
 #include "sky.hpp"
 
