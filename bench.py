@@ -390,7 +390,7 @@ def main(argv=None) -> int:
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (every ray is "
-              "independent, so a frame shards by 8x8 tiles or whole cameras across GPUs, one "
+              "independent, so a frame shards by 16x4-pixel tiles or whole cameras across GPUs, one "
               "process each). Run with --gpus 1.", file=sys.stderr)
         return 2
 
@@ -966,12 +966,15 @@ def _c3_leg(tracer, steps, warmup):
     """The 64-camera batch (resolutions 64..1024 per axis, r_obs 8..1000 M, inclinations 0..90,
     FOV 20..90, 200-700 steps with their presets; seeded) captured in one HIP graph
     (bhs_graph_create) and replayed once per step; beside it the eager batch call with the plain
-    march kernel's roofline."""
+    march kernel's roofline. The launch shape is tuned on this batch (_tune_batch) and the
+    caller's shape restored afterwards."""
     import torch
     import blackholerender_amd as bhs
     from blackholerender_amd import workload as W
     cams, params = W.batch_c3()
     n = sum(c.width * c.height for c in cams)
+    prev = tracer.get_tuning()
+    tuned = _tune_batch(tracer, cams, params, prev)   # the launch shape, timed on this batch
     stream = torch.cuda.current_stream()
     out = bhs.RayBuffers(n, "cuda:0")
     eager = lambda: tracer.render_batch(cams, params, out, stream)
@@ -998,8 +1001,10 @@ def _c3_leg(tracer, steps, warmup):
     finally:
         g.close()
     del out
+    tracer.set_tuning(prev["mode"], prev["block"], prev["waves_per_simd"], prev["refill_min"])
     return {"workload": (f"64 cameras in one call, {n:,} rays (seeded sizes 64..1024 per axis, "
                          f"r_obs 8..1000 M, incl 0..90 deg, FOV 20..90 deg, 200/300/450/700 steps)"),
+            "launch": tuned,
             "graph_value": round(n * steps / wall_g / 1e6, 2), "unit": "Mrays/s",
             "graph_ms_per_step": round(wall_g / steps * 1e3, 4),
             "eager_value": round(n * steps / wall_e / 1e6, 2),

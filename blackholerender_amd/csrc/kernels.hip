@@ -70,13 +70,11 @@ __device__ __forceinline__ bool in_band(const MarchConst& c, float b) {
     return __builtin_fabsf(b - kBcritF) < c.band_abs;
 }
 
-// Work item q of one camera: ray index in 8x8-tile-major order over its image (tile t = q / 64
-// row-major over the tile grid, lane q % 64 row-major inside the tile). Items of the padding beyond
-// the image map to -1.
+// Work item q of one camera: ray index in work-item tile order over its image (item_col_row).
+// Items of the padding beyond the image map to -1.
 __device__ __forceinline__ int64_t item_pixel(const MarchConst& c, uint32_t q) {
-    const uint32_t tiles_x = ((uint32_t)c.width + 7u) >> 3;
-    const uint32_t t = q >> 6, l = q & 63u;
-    const uint32_t col = (t % tiles_x) * 8u + (l & 7u), row = (t / tiles_x) * 8u + (l >> 3);
+    uint32_t col, row;
+    item_col_row(item_tiles_x((uint32_t)c.width), q, col, row);
     return (col < (uint32_t)c.width && row < (uint32_t)c.height) ? (int64_t)row * c.width + col : -1;
 }
 
@@ -832,7 +830,8 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
 }
 
 uint32_t camera_items(int32_t width, int32_t height) {
-    return (uint32_t)((width + 7) / 8) * (uint32_t)((height + 7) / 8) * 64u;
+    return item_tiles_x((uint32_t)width) * (((uint32_t)height + kItemTileH - 1u) / kItemTileH) *
+           64u;
 }
 
 hipError_t launch_render(const MarchConst& c, const SoA& out, const Scratch& sc, const Tuning& t,

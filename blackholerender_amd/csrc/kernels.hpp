@@ -39,7 +39,41 @@ struct Scratch {
 // Cameras of one batched launch: the camera table and item prefix sums are staged in LDS.
 constexpr uint32_t kMaxBatchCams = 64;
 
-// Work items of one camera: its image padded to whole 8x8 tiles.
+// Work-item order of the integrator's kernels: a camera's items run over kItemTileW x kItemTileH
+// pixel tiles (row-major over the tile grid), 64 items per tile (row-major inside it), so the 64
+// consecutive items a wave claims are one tile of neighbouring rays.
+#ifndef BHS_ITEM_TILE_W
+#define BHS_ITEM_TILE_W 16
+#endif
+constexpr uint32_t kItemTileW = BHS_ITEM_TILE_W;
+constexpr uint32_t kItemTileH = 64u / kItemTileW;
+static_assert(kItemTileW * kItemTileH == 64u && (kItemTileW & (kItemTileW - 1u)) == 0u,
+              "a work-item tile is 64 pixels, power-of-two wide");
+
+__host__ __device__ __forceinline__ uint32_t item_tiles_x(uint32_t width) {
+    return (width + kItemTileW - 1u) / kItemTileW;
+}
+
+// Pixel (col, row) of lane l of tile t of a camera with tiles_x tiles per tile row (col may lie
+// beyond the width and row beyond the height in the padding of the last tile column / row).
+__device__ __forceinline__ void tile_col_row(uint32_t tiles_x, uint32_t t, uint32_t l,
+                                             uint32_t& col, uint32_t& row) {
+    col = (t % tiles_x) * kItemTileW + (l % kItemTileW);
+    row = (t / tiles_x) * kItemTileH + (l / kItemTileW);
+}
+// Pixel of work item q (tile q / 64, lane q % 64).
+__device__ __forceinline__ void item_col_row(uint32_t tiles_x, uint32_t q, uint32_t& col,
+                                             uint32_t& row) {
+    tile_col_row(tiles_x, q >> 6, q & 63u, col, row);
+}
+
+// Work item of pixel (col, row): the inverse of item_col_row.
+__device__ __forceinline__ uint32_t pixel_item(uint32_t tiles_x, uint32_t col, uint32_t row) {
+    return ((row / kItemTileH) * tiles_x + col / kItemTileW) * 64u +
+           (row % kItemTileH) * kItemTileW + col % kItemTileW;
+}
+
+// Work items of one camera: its image padded to whole work-item tiles.
 uint32_t camera_items(int32_t width, int32_t height);
 
 // Launch-shape choices of the FP32 march (see bhs_tuning in include/bhs_trace.h).

@@ -583,8 +583,8 @@ struct SurCamTable {
 };
 
 // Camera and pixel of the calling lane's item in the wave whose first item is q_wave: a wave's 64
-// items are one 8x8 tile of one camera, so the camera search, the camera's constants and the tile
-// position are wave-uniform (scalar) and only the pixel inside the tile is per lane.
+// items are one work-item tile of one camera, so the camera search, the camera's constants and the
+// tile position are wave-uniform (scalar) and only the pixel inside the tile is per lane.
 struct WaveItem {
     uint32_t ci;
     int64_t pix;     // -1: padding or past the end
@@ -598,10 +598,8 @@ __device__ __forceinline__ WaveItem wave_item(const Cams& cams, uint32_t q_wave,
     WaveItem w;
     w.ci = cams.find(qw < n_items ? qw : n_items - 1);
     const MarchConst& c = cams.cam(w.ci);
-    const uint32_t t = (qw - cams.base(w.ci)) >> 6;
-    const uint32_t tiles_x = ((uint32_t)c.width + 7u) >> 3;
-    w.col = (t % tiles_x) * 8u + (lane & 7u);
-    w.row = (t / tiles_x) * 8u + (lane >> 3);
+    tile_col_row(item_tiles_x((uint32_t)c.width), (qw - cams.base(w.ci)) >> 6, lane, w.col,
+                 w.row);
     w.pix = (qw + lane < n_items && w.col < (uint32_t)c.width && w.row < (uint32_t)c.height)
                 ? (int64_t)w.row * c.width + w.col
                 : -1;
@@ -621,7 +619,7 @@ struct ScanCursor {
 
 // As wave_item, with the camera's items taken in scanline order (item base + p is pixel p, the
 // items beyond width * height are padding): a wave's 64 rays are 64 consecutive pixels, so its
-// stores to the output planes are contiguous (an 8x8 tile writes eight 8-pixel row pieces).
+// stores to the output planes are contiguous (a work-item tile writes short row pieces).
 // `item` maps the pixel back to the integrator's tile order.
 template <typename Cams>
 __device__ __forceinline__ WaveItem wave_item_scan(const Cams& cams, uint32_t q_wave,
@@ -665,8 +663,7 @@ __device__ __forceinline__ WaveItem wave_item_scan(const Cams& cams, uint32_t q_
     w.col = col;
     w.row = row;
     w.pix = (qw + lane < n_items && row < (uint32_t)c.height) ? (int64_t)row * width + col : -1;
-    const uint32_t tiles_x = (width + 7u) >> 3;
-    w.item = base + ((row >> 3) * tiles_x + (col >> 3)) * 64u + ((row & 7u) << 3) + (col & 7u);
+    w.item = base + pixel_item(item_tiles_x(width), col, row);
     return w;
 }
 
