@@ -361,9 +361,12 @@ __device__ __forceinline__ Dec decode_plunge(const MarchConst& c, const RayGeom&
     return o;
 }
 
+// FULL: the caller guarantees every plane (the kernel is instantiated for it, so no per-plane test
+// reaches the compiled code).
+template <bool FULL = false>
 __device__ __forceinline__ void store_dec(const SoA& out, int64_t i, const Dec& o, float b) {
     const int64_t n = out.n;
-    if (out.cls && out.flags && out.n_cross && out.steps && out.esc && out.disk && out.b) {
+    if (FULL || (out.cls && out.flags && out.n_cross && out.steps && out.esc && out.disk && out.b)) {
         // every plane requested (the usual case): no per-plane tests
         out.cls[i] = (uint8_t)o.cls;
         out.flags[i] = (uint8_t)o.flags;
@@ -410,8 +413,6 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 // late list, the call's FP64 band list, the early FP32 list or the late band list.
 enum Dest : int { kToNone = -1, kToFb = 0, kToBand = 1, kToEarly = 2, kToLate = 3 };
 constexpr int kDests = 4;
-// LDS words of one push: per list, the four waves' counts and the block's base.
-constexpr int kPushWords = kDests * 4 + kDests;
 
 __device__ __forceinline__ uint32_t* list_count(const Scratch& sc, const SurLists& fl, int j) {
     return j == kToFb ? fl.fb_count
@@ -423,35 +424,27 @@ __device__ __forceinline__ uint32_t* list_ids(const Scratch& sc, const SurLists&
                       : (j == kToBand ? sc.band_ids : (j == kToEarly ? fl.early_ids : fl.lb_ids));
 }
 
-// Appends each lane's item to list `dest` (kToNone: none) with one atomic per list and block
-// (wave ballots, per-wave counts in LDS). Every thread of the block calls it, with a 256-thread
-// block. Successive calls alternate between the two halves of s_cnt (2 kPushWords words): a wave
-// writes a half again only after the barriers of the call in between, which every wave reaches
-// after its reads of that half, so two barriers per call suffice.
+// Appends each lane's item to list `dest` (kToNone: none): for each list that lanes of the wave
+// are bound for, one atomic by the wave's first lane reserves their slots, and a lane's slot is its
+// rank among them (ballot + mbcnt). Every lane of the wave calls it, in wave-uniform control flow;
+// no block barrier, so the waves of a block do not wait for each other's chunks. Most waves list
+// nothing (one ballot) and the rest few rays, so the atomics stay rare.
 __device__ __forceinline__ void push_lists(const Scratch& sc, const SurLists& fl, int dest,
-                                           uint32_t item, uint32_t* s_cnt) {
-    const int wave = threadIdx.x >> 6;
-    uint64_t mine = 0;
+                                           uint32_t item) {
+    if (__ballot(dest >= 0) == 0ull) return;
 #pragma unroll
     for (int j = 0; j < kDests; ++j) {
         const uint64_t m = __ballot(dest == j);
-        if ((threadIdx.x & 63) == 0) s_cnt[j * 4 + wave] = (uint32_t)__popcll(m);
-        if (dest == j) mine = m;
-    }
-    __syncthreads();
-    if (threadIdx.x < (unsigned)kDests) {
-        const int j = threadIdx.x;
-        const uint32_t tot = s_cnt[j * 4] + s_cnt[j * 4 + 1] + s_cnt[j * 4 + 2] + s_cnt[j * 4 + 3];
-        s_cnt[kDests * 4 + j] = tot ? atomicAdd(list_count(sc, fl, j), tot) : 0u;
-    }
-    __syncthreads();
-    if (dest >= 0) {
-        uint32_t base = s_cnt[kDests * 4 + dest];
-        for (int w = 0; w < wave; ++w) base += s_cnt[dest * 4 + w];
-        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-            (uint32_t)(mine >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
-        const uint32_t cap = dest == kToBand ? sc.band_cap : fl.fb_cap;
-        if (base + rank < cap) list_ids(sc, fl, dest)[base + rank] = item;
+        if (m == 0ull) continue;
+        uint32_t base = 0;
+        if ((threadIdx.x & 63u) == 0u) base = atomicAdd(list_count(sc, fl, j), (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, 0);
+        if (dest == j) {
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            const uint32_t cap = j == kToBand ? sc.band_cap : fl.fb_cap;
+            if (base + rank < cap) list_ids(sc, fl, j)[base + rank] = item;
+        }
     }
 }
 
@@ -859,15 +852,12 @@ template <typename Cams>
 __global__ __launch_bounds__(kBlock) void k_sur_pre(const Cams cams, const SurConst k,
                                                     uint32_t n_items, Scratch sc, SurLists fl,
                                                     uint32_t dests, const float* cam_pre) {
-    __shared__ uint32_t s_cnt[2 * kPushWords];
-    uint32_t half = 0;
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < n_items; q0 += gridDim.x * kBlock) {
         const WaveItem w = wave_item(cams, q0 + (threadIdx.x & ~63u), n_items);
         const MarchConst& c = cams.cam(w.ci);
         const SurCam pc = cam_pre ? load_sur_cam(cam_pre, w.ci) : sur_cam(c);
         const RayPrep r = prep_ray(c, pc, k, w);
-        push_lists(sc, fl, dest_of(r, false, dests), w.item, s_cnt + half);
-        half ^= kPushWords;
+        push_lists(sc, fl, dest_of(r, false, dests), w.item);
     }
 }
 
@@ -881,8 +871,6 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
                                                          uint32_t n_items, uint32_t dests) {
     __shared__ __attribute__((aligned(16))) uint8_t s_w[kPackedBytes];
     __shared__ float s_y[4][64 * kOut];
-    __shared__ uint32_t s_cnt[2 * kPushWords];
-    uint32_t half = 0;
     __shared__ uint32_t s_st[4][4];
     stage_weights(packed, s_w);
     __syncthreads();
@@ -910,8 +898,7 @@ __global__ __launch_bounds__(kBlock, 2) void k_surrogate(const Cams cams, const 
                 back = true;
             }
         }
-        push_lists(sc, fl, dest_of(r, back, dests), w.item, s_cnt + half);
-        half ^= kPushWords;
+        push_lists(sc, fl, dest_of(r, back, dests), w.item);
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     add_block_stats(sc.stats, n_served, n_cross, n_esc, n_fb, s_st);
@@ -1165,8 +1152,10 @@ constexpr uint32_t kTabChunksMin = 4, kTabChunksMax = 16;
 // Surrogate render in table mode: as k_surrogate with the network replaced by the table lookup.
 // Block b takes chunks [b cpb, (b + 1) cpb); phase 0 every chunk, phase 1 / 2 only the
 // near-critical / other ones (flags `near`). dests: the lists of the unserved rays; cam_pre: the
-// per-camera setup terms; part: the blocks' statistics partials.
-template <typename Cams>
+// per-camera setup terms; part: the blocks' statistics partials. FULL: every output plane is
+// present (the usual call): the stores carry no per-plane tests, whose hoisted masks would
+// otherwise crowd the scalar registers into lane spills.
+template <typename Cams, bool FULL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
 void k_surrogate_tab(const Cams cams, const SurConst k, const float* __restrict__ node,
                      const float* __restrict__ phi, const float* __restrict__ range,
@@ -1174,8 +1163,6 @@ void k_surrogate_tab(const Cams cams, const SurConst k, const float* __restrict_
                      Scratch sc, SurLists fl, uint32_t n_items, uint32_t dests,
                      uint32_t* __restrict__ part, uint32_t cpb, uint32_t phase,
                      const uint32_t* __restrict__ near, const float* __restrict__ cam_pre) {
-    __shared__ uint32_t s_cnt[2 * kPushWords];
-    uint32_t half = 0;
     __shared__ uint32_t s_st[4][4];
     uint32_t n_served = 0, n_cross = 0, n_esc = 0, n_fb = 0;
     const int64_t nodes = (int64_t)n2 * n3 * 4;
@@ -1201,7 +1188,7 @@ void k_surrogate_tab(const Cams cams, const SurConst k, const float* __restrict_
             const Dec o = decode_with(c, r.margin, r.b_axis, r.lapse_g, r.g, Phi,
                                       [&](int q) { return v[1 + q]; });
             if (o.serve) {
-                store_dec(out, c.ray_offset + pix, o, r.g.b);
+                store_dec<FULL>(out, c.ray_offset + pix, o, r.g.b);
                 n_served += 1;
                 n_cross += o.n_cross;
                 n_esc += o.escaping ? 1u : 0u;
@@ -1211,14 +1198,13 @@ void k_surrogate_tab(const Cams cams, const SurConst k, const float* __restrict_
         } else if (r.route == kPlunge) {
             const Dec o = decode_plunge(c, r.g);
             if (o.serve) {
-                store_dec(out, c.ray_offset + pix, o, r.g.b);
+                store_dec<FULL>(out, c.ray_offset + pix, o, r.g.b);
                 n_served += 1;
             } else {
                 back = true;
             }
         }
-        push_lists(sc, fl, dest_of(r, back, dests), w.item, s_cnt + half);
-        half ^= kPushWords;
+        push_lists(sc, fl, dest_of(r, back, dests), w.item);
         n_fb += (back || r.route == kEarly) ? 1u : 0u;
     }
     store_block_stats(part, n_served, n_cross, n_esc, n_fb, s_st);
@@ -1402,7 +1388,7 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     static const uint32_t pre_per_cu =
         blocks_per_cu(reinterpret_cast<const void*>(&k_sur_pre<Cams>));
     static const uint32_t tab_per_cu =
-        blocks_per_cu(reinterpret_cast<const void*>(&k_surrogate_tab<Cams>));
+        blocks_per_cu(reinterpret_cast<const void*>(&k_surrogate_tab<Cams, true>));
     auto grid_of = [&](uint32_t per_cu) {
         const uint32_t cap = per_cu * (uint32_t)(num_cus > 0 ? num_cus : 256);
         return dim3(gb0 < cap ? (gb0 ? gb0 : 1u) : cap);
@@ -1497,12 +1483,14 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
         const uint32_t tab_blocks = (gb0 + cpb - 1) / cpb;
         auto pass = [&](uint32_t dests, uint32_t phase) {
             uint32_t* part = fl.part + (phase == 2u ? 4u * tab_blocks : 0u);
-            hipLaunchKernelGGL(k_surrogate_tab<Cams>, dim3(tab_blocks ? tab_blocks : 1u),
-                               dim3(kBlock), 0, s, cams, k, (const float*)tab->node,
-                               (const float*)tab->phi, (const float*)tab->range,
-                               (const uint32_t*)tab->slot, tab->n2, tab->n3, out, sc, fl, n_items,
-                               dests, part, cpb, phase, (const uint32_t*)fl.near,
-                               (const float*)tab->cam);
+            const bool full = out.cls && out.flags && out.n_cross && out.steps && out.esc &&
+                              out.disk && out.b;
+            auto kern = full ? k_surrogate_tab<Cams, true> : k_surrogate_tab<Cams, false>;
+            hipLaunchKernelGGL(kern, dim3(tab_blocks ? tab_blocks : 1u), dim3(kBlock), 0, s, cams,
+                               k, (const float*)tab->node, (const float*)tab->phi,
+                               (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
+                               tab->n3, out, sc, fl, n_items, dests, part, cpb, phase,
+                               (const uint32_t*)fl.near, (const float*)tab->cam);
             const uint32_t red_blocks = (tab_blocks + 4 * kBlock - 1) / (4 * kBlock);
             hipLaunchKernelGGL(k_stats_reduce, dim3(red_blocks ? red_blocks : 1u), dim3(kBlock),
                                0, s, (const uint32_t*)part, tab_blocks, sc.stats);
