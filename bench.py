@@ -415,6 +415,7 @@ def main(argv=None) -> int:
                     "thin disk [6,20]M" % (args.frames, format(2073600 * args.frames, ",")))
     params = W.params_for(cfg)
     tuned_single = None
+    leg_tune = not (args.tuning or args.no_tune)   # the other batches tune their own shape too
     if args.tuning:
         tracer.set_tuning(*[int(v) for v in args.tuning.split(",")])
         tuned = tracer.get_tuning()
@@ -513,9 +514,9 @@ def main(argv=None) -> int:
         sur.close()
         if args.config == "c2" and not args.no_full_form:
             res["full_form"] = _full_form_leg(tracer, max(3, args.steps // 4), 2,
-                                              accuracy=not args.no_accuracy)
+                                              accuracy=not args.no_accuracy, tune=leg_tune)
     if args.config == "c2" and not args.no_c3:
-        res["batch64"] = _c3_leg(tracer, max(3, args.steps // 4), 2)
+        res["batch64"] = _c3_leg(tracer, max(3, args.steps // 4), 2, tune=leg_tune)
     if not args.no_cpu and args.config == "c2":
         res["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(res), flush=True)
@@ -922,11 +923,11 @@ def _surrogate_roofline(sur_rec, n, key):
     return rec
 
 
-def _full_form_leg(tracer, steps, warmup, accuracy=True):
+def _full_form_leg(tracer, steps, warmup, accuracy=True, tune=True):
     """The full form (3840x2160, 8 jittered spp, 4 poses: 32 cameras, 265,420,800 rays per call;
     700 steps, disk [6, 30]) through both legs with the full-form model (table mode 1024x128), in
-    the launch shape tuned for the 1080p batch; accuracy on camera 0 (8,294,400 rays) against the
-    FP64 march of the same rays."""
+    the launch shape tuned on this batch (tune; else the caller's) and the caller's shape restored
+    afterwards; accuracy on camera 0 (8,294,400 rays) against the FP64 march of the same rays."""
     import torch
     import blackholerender_amd as bhs
     from blackholerender_amd import workload as W
@@ -936,13 +937,16 @@ def _full_form_leg(tracer, steps, warmup, accuracy=True):
     sur = Surrogate.load(tracer, os.path.join(ROOT, "blackholerender_amd", "models",
                                               "lawS_700_disk6-30.bhsmlp"))
     sur.set_table(1024, 128)
+    prev = tracer.get_tuning()
+    tuned = _tune_batch(tracer, cams, params, prev) if tune else dict(prev, tuned_on="the caller")
     plain_rec, sur_rec, roof = _legs(tracer, sur, cams, params, steps, warmup)
+    tracer.set_tuning(prev["mode"], prev["block"], prev["waves_per_simd"], prev["refill_min"])
     n = sum(c.width * c.height for c in cams)
     rec = {"workload": ("full form: 3840x2160, 8 jittered spp, 4 poses (32 cameras, "
                         "265,420,800 rays per step), 700 steps, r_obs=15M, incl 80deg, FOV 60deg, "
                         "thin disk [6,30]M"),
            "model": "lawS_700_disk6-30.bhsmlp", "surrogate_mode": "table 1024x128",
-           "value": sur_rec.pop("value"), "unit": "Mrays/s",
+           "launch": tuned, "value": sur_rec.pop("value"), "unit": "Mrays/s",
            "ms_per_step": sur_rec.pop("ms_per_step"),
            "speedup_vs_plain": sur_rec.pop("speedup_vs_plain"), "steps": steps,
            "warmup": warmup, "plain": plain_rec, "surrogate": sur_rec, "roofline": roof,
@@ -962,19 +966,19 @@ def _full_form_leg(tracer, steps, warmup, accuracy=True):
     return rec
 
 
-def _c3_leg(tracer, steps, warmup):
+def _c3_leg(tracer, steps, warmup, tune=True):
     """The 64-camera batch (resolutions 64..1024 per axis, r_obs 8..1000 M, inclinations 0..90,
     FOV 20..90, 200-700 steps with their presets; seeded) captured in one HIP graph
     (bhs_graph_create) and replayed once per step; beside it the eager batch call with the plain
-    march kernel's roofline. The launch shape is tuned on this batch (_tune_batch) and the
-    caller's shape restored afterwards."""
+    march kernel's roofline. The launch shape is tuned on this batch (_tune_batch; tune false:
+    the caller's) and the caller's shape restored afterwards."""
     import torch
     import blackholerender_amd as bhs
     from blackholerender_amd import workload as W
     cams, params = W.batch_c3()
     n = sum(c.width * c.height for c in cams)
     prev = tracer.get_tuning()
-    tuned = _tune_batch(tracer, cams, params, prev)   # the launch shape, timed on this batch
+    tuned = _tune_batch(tracer, cams, params, prev) if tune else dict(prev, tuned_on="the caller")
     stream = torch.cuda.current_stream()
     out = bhs.RayBuffers(n, "cuda:0")
     eager = lambda: tracer.render_batch(cams, params, out, stream)

@@ -202,6 +202,8 @@ void layout_scratch(void* mem, int64_t n_items, bool roots, bhs::Scratch* sc) {
     sc->band_count = (uint32_t*)(base + sizeof(bhs::Stats));
     sc->work = (uint32_t*)(base + sizeof(bhs::Stats) + 4);
     sc->band_work = (uint32_t*)(base + sizeof(bhs::Stats) + 8);
+    sc->seg_done = (unsigned long long*)(base + sizeof(bhs::Stats) + 16);
+    sc->seg = (uint32_t*)(base + 256);
     sc->band_ids = (uint32_t*)(base + kStatsBytes);
     sc->band_cap = (uint32_t)n_items;
     sc->roots = roots ? (bhs::RootState<float>*)(base + kStatsBytes + band_bytes) : nullptr;

@@ -58,7 +58,9 @@ struct bhs_ctx {
 namespace bhs {
 namespace capi {
 
-constexpr size_t kStatsBytes = 256;   // Stats + band counter + work counters, padded
+// Launch counters, cleared per launch: Stats, band counter, work counters and the segmented
+// queue's mask in the first 256 bytes, then the segmented queue's counters.
+constexpr size_t kStatsBytes = 256 + (size_t)bhs::kSegs * bhs::kSegStride * 4;
 
 // Records `fmt` (one %s for `detail`) as the context's last error and returns `code`.
 int fail(bhs_ctx* ctx, int code, const char* fmt, const char* detail = nullptr);

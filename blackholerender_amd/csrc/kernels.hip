@@ -277,6 +277,47 @@ __device__ __forceinline__ bool claim_items(uint32_t* counter, uint32_t n, uint3
     return q < n;
 }
 
+// Whole-tile claims of the persistent march when a wave refills only with all its lanes idle
+// (refill threshold 64): a segmented queue. One counter on every claim serialises there when tiles
+// are short (rays of a few steps each: every wave waits on the atomic), so the launch's tiles are
+// split into nseg = min(kSegs, tiles) contiguous segments, each with its own counter (kSegStride
+// words apart: separate cache lines). A wave starts on segment wave * nseg / waves and takes one
+// tile per atomic; when its segment is used up it marks it in the shared mask seg_done and moves
+// on to the next segment not yet marked (cyclically), so the waves of finished segments help the
+// others and every tile is still handed out exactly once. `seg` and `seen` (the segments the wave
+// knows to be used up) are wave-uniform. Returns true for a lane that received a position below n
+// (in q); clears `more` once every segment is used up. Exit: each failed atomic marks a segment,
+// the mask only grows, so a wave stops after at most nseg failures.
+__device__ __forceinline__ uint32_t seg_first_tile(uint32_t s, uint32_t nseg, uint32_t tiles) {
+    return (uint32_t)(((uint64_t)s * tiles) / nseg);
+}
+
+__device__ __forceinline__ bool claim_seg(const Scratch& sc, uint32_t n, bool active, bool& more,
+                                          uint32_t& seg, uint64_t& seen, uint32_t& q) {
+    if (!more || __any(active)) return false;
+    const uint32_t tiles = (n + 63u) >> 6;
+    const uint32_t nseg = tiles < kSegs ? tiles : kSegs;
+    const uint64_t all = nseg == 64u ? ~0ull : ((1ull << nseg) - 1ull);
+    while (true) {
+        if ((seen & all) == all) {
+            more = false;
+            return false;
+        }
+        while ((seen >> seg) & 1ull) seg = seg + 1u == nseg ? 0u : seg + 1u;
+        uint32_t t = 0;
+        if ((threadIdx.x & 63u) == 0u) t = atomicAdd(sc.seg + seg * kSegStride, 1u);
+        t = (uint32_t)__shfl((int)t, 0);
+        const uint32_t tile = seg_first_tile(seg, nseg, tiles) + t;
+        if (tile < seg_first_tile(seg + 1u, nseg, tiles)) {
+            q = (tile << 6) + (threadIdx.x & 63u);
+            return q < n;
+        }
+        uint64_t g = 0;   // segment used up: publish it and learn the others'
+        if ((threadIdx.x & 63u) == 0u) g = atomicOr(sc.seg_done, 1ull << seg);
+        seen |= (1ull << seg) | (uint64_t)__shfl((long long)g, 0);
+    }
+}
+
 // Refill threshold of the band march: its rays run hundreds of FP64 iterations, so a refill (one
 // setup) is cheap next to an idle lane; 32 keeps the queue atomics to one per 32 rays.
 constexpr uint32_t kBandRefill = 32;
@@ -515,7 +556,9 @@ __global__ __launch_bounds__(kBlock) void k_render_f64(const MarchConst c, const
 // budgets (BATCH = true, camera table and item prefix sums staged in LDS; each lane looks up its
 // ray's camera there). Each wave keeps 64 rays in flight; when at least `refill_min` lanes have
 // finished, it takes that many new rays with one atomic (wave ballot + per-lane prefix count of the
-// idle lanes), so finished lanes are refilled instead of idling until the slowest lane ends.
+// idle lanes), so finished lanes are refilled instead of idling until the slowest lane ends. With
+// refill_min 64 (whole-wave refills) the wave takes whole tiles from a segmented queue
+// (claim_seg), so that short tiles do not serialise every wave on one counter.
 // Exit: a wave leaves when the queue is exhausted and all its lanes are finished; every ray ends
 // within its max_steps iterations and the counter only grows, so every wave reaches it.
 template <int BLOCK, bool ROOTS, int WPE, bool BATCH>
@@ -551,10 +594,21 @@ void k_persist(const MarchConst c1, const MarchConst* __restrict__ table,
     const MarchK<float>* kp = BATCH ? &s_k[0] : nullptr;
     bool active = false;
     bool more = true;   // wave-uniform: the queue may still hold items
+    // Whole-wave refills (threshold 64) of the launch's own items take tiles from the segmented
+    // queue (claim_seg); listed items and partial refills share the single counter.
+    const bool segq = refill_min >= 64u && list == nullptr;
+    uint32_t seg = 0;
+    uint64_t seen = 0;
+    if (segq) {
+        const uint32_t tiles = (n_items + 63u) >> 6, nseg = tiles < kSegs ? tiles : kSegs;
+        const uint32_t wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+        seg = (uint32_t)(((uint64_t)wave * nseg) / (gridDim.x * (BLOCK / 64)));
+    }
     uint32_t n_rays = 0, n_steps = 0, n_cross = 0, n_esc = 0;
     while (true) {
         uint32_t q = 0;
-        if (claim_items(sc.work, n_items, refill_min, active, more, q)) {
+        if (segq ? claim_seg(sc, n_items, active, more, seg, seen, q)
+                 : claim_items(sc.work, n_items, refill_min, active, more, q)) {
             const uint32_t item = list ? list[q] : q;
             uint32_t local = item, ci = 0;
             if (BATCH) {

@@ -34,7 +34,14 @@ struct Scratch {
     uint32_t* band_ids;          // work items (camera tile order) of the FP64 band rays
     uint32_t band_cap;           // capacity of band_ids (>= the launch's work items)
     RootState<float>* roots;     // 2 per ray (global ray index) when root refinement is on
+    uint32_t* seg;               // segmented work queue: kSegs counters, kSegStride words apart
+    unsigned long long* seg_done;   // its used-up segments (bit mask)
 };
+
+// Segmented work queue of the persistent march (claim_seg): counters per contiguous range of
+// tiles, each on its own 128-byte line; cleared with the launch counters.
+constexpr uint32_t kSegs = 64;
+constexpr uint32_t kSegStride = 32;
 
 // Cameras of one batched launch: the camera table and item prefix sums are staged in LDS.
 constexpr uint32_t kMaxBatchCams = 64;

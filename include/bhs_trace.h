@@ -184,7 +184,7 @@ int bhs_render(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* para
  * hipMalloc). Same arguments, kernels and results as bhs_render, but the render's scratch (stats
  * and counters, the FP64 band list, FP64 root states with root64) lives in a static device array
  * of the library, BHS_NOALLOC_SCRATCH_BYTES per GPU (held by each GPU the library runs on until
- * the process ends), instead of memory the context allocates. A render needs 256 bytes plus 4 per
+ * the process ends), instead of memory the context allocates. A render needs 8,448 bytes plus 4 per
  * work item (its image padded to whole 16x4-pixel tiles) plus, with root64, 56 per ray; larger renders
  * are rejected with BHS_E_INVALID (a 3840x2160 frame fits without root64). Every no-alloc call on
  * a GPU shares that GPU's array: calls that overlap in time on one GPU (other streams or other
