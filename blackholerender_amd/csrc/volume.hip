@@ -142,10 +142,44 @@ __global__ __launch_bounds__(kBlock) void k_vol_scan(VolCams cams, Scratch sc) {
     }
 }
 
+// Adds the block's rays, steps and escapes (band: also to the band count) to the launch counters:
+// wave sums in LDS, one atomic per counter and block from its first thread. Every thread of the
+// block calls it (the loop bounds are block-uniform); the barrier after the atomics keeps the LDS
+// slots of the next call from being overwritten before they are read.
+__device__ __forceinline__ void block_stats(Stats* st, uint32_t ray, uint32_t steps, uint32_t esc,
+                                            bool band, uint32_t (*s_st)[3]) {
+    ray = wsum(ray);
+    steps = wsum(steps);
+    esc = wsum(esc);
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0u) {
+        s_st[w][0] = ray;
+        s_st[w][1] = steps;
+        s_st[w][2] = esc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0, k = 0, e = 0;
+        for (uint32_t j = 0; j < kBlock / 64; ++j) {
+            r += s_st[j][0];
+            k += s_st[j][1];
+            e += s_st[j][2];
+        }
+        if (r) {
+            atomicAdd(&st->rays, (unsigned long long)r);
+            atomicAdd(&st->steps, (unsigned long long)k);
+            if (band) atomicAdd(&st->band64, (unsigned long long)r);
+            if (e) atomicAdd(&st->escapes, (unsigned long long)e);
+        }
+    }
+    __syncthreads();
+}
+
 // FP32 pass over every work item outside the FP64 band.
 __global__ __launch_bounds__(kBlock) void k_vol_f32(VolCams cams, shade::ShadeConst sh, SoA out,
                                                     ShadeOut so, const uint16_t* __restrict__ vol,
                                                     const float* __restrict__ lut, Scratch sc) {
+    __shared__ uint32_t s_st[kBlock / 64][3];
     for (uint32_t q0 = blockIdx.x * kBlock; q0 < cams.n_items; q0 += gridDim.x * kBlock) {
         const uint32_t q = q0 + threadIdx.x;
         uint32_t ray = 0, steps = 0, esc = 0;
@@ -158,14 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_vol_f32(VolCams cams, shade::ShadeCo
                 steps = vol_ray<float>(c, sh, c.ray_offset + pix, pix, out, so, vol, lut, 0u, esc);
             }
         }
-        ray = wsum(ray);
-        steps = wsum(steps);
-        esc = wsum(esc);
-        if ((threadIdx.x & 63u) == 0 && ray) {
-            atomicAdd(&sc.stats->rays, (unsigned long long)ray);
-            atomicAdd(&sc.stats->steps, (unsigned long long)steps);
-            if (esc) atomicAdd(&sc.stats->escapes, (unsigned long long)esc);
-        }
+        block_stats(sc.stats, ray, steps, esc, false, s_st);
     }
 }
 
@@ -174,6 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_vol_f64(VolCams cams, shade::ShadeCo
                                                     ShadeOut so, const uint16_t* __restrict__ vol,
                                                     const float* __restrict__ lut, Scratch sc,
                                                     bool all) {
+    __shared__ uint32_t s_st[kBlock / 64][3];
     const uint32_t n = all ? cams.n_items
                            : (*sc.band_count < sc.band_cap ? *sc.band_count : sc.band_cap);
     for (uint32_t i0 = blockIdx.x * kBlock; i0 < n; i0 += gridDim.x * kBlock) {
@@ -190,15 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_vol_f64(VolCams cams, shade::ShadeCo
                                         BHS_FLAG_BAND64, esc);
             }
         }
-        ray = wsum(ray);
-        steps = wsum(steps);
-        esc = wsum(esc);
-        if ((threadIdx.x & 63u) == 0 && ray) {
-            atomicAdd(&sc.stats->rays, (unsigned long long)ray);
-            atomicAdd(&sc.stats->steps, (unsigned long long)steps);
-            atomicAdd(&sc.stats->band64, (unsigned long long)ray);
-            if (esc) atomicAdd(&sc.stats->escapes, (unsigned long long)esc);
-        }
+        block_stats(sc.stats, ray, steps, esc, true, s_st);
     }
 }
 
