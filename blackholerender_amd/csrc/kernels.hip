@@ -979,13 +979,21 @@ hipError_t launch_batch(const MarchConst* table, const uint32_t* item_base, uint
                         hipEvent_t ev_begin, hipEvent_t ev_end) {
     const CamTable cams{table, item_base, ncam, n_items};
     hipError_t e;
-    // With a band: the band scan and then the FP64 band march on `s`, the FP32 march on `aux` from
-    // the point after the scan, and `s` waits for it. The band kernel, next on `s` when the scan
-    // ends, takes its SIMD slots before the march's persistent blocks fill them; with the band on
-    // the side stream the march's blocks came first and the band's long FP64 chains ran on after
-    // the march (the 64-camera batch: 2.33 ms per eager call that way, 2.01 ms this way).
+    // With a band of a batch of up to kBandQueueItems items (k_band64: one ray per lane, its
+    // longest chain sets its time): the band scan and then the FP64 band march on `s`, the FP32
+    // march on `aux` from the point after the scan, and `s` waits for it. The band kernel, next on
+    // `s` when the scan ends, takes its SIMD slots before the march's persistent blocks fill them;
+    // with the band on the side stream the march's blocks came first and the band's long FP64
+    // chains ran on after the march (the 64-camera batch: 2.33 ms per eager call that way, 2.01 ms
+    // this way). Larger batches keep the band on the side stream after the march's launch: their
+    // band is a queue of millions of FP64 rays whose blocks, dispatched first, would take the
+    // slots the march is tuned for (the full form: 122 against 118 ms).
+    const bool band_first = any_band && n_items <= kBandQueueItems;
+    if (any_band && !band_first &&
+        (e = fork_band(cams, out, sc, s, aux, ev_fork, ev_join)) != hipSuccess)
+        return e;
     hipStream_t m = s;
-    if (any_band) {
+    if (band_first) {
         hipLaunchKernelGGL(k_band_scan<CamTable>, item_grid(n_items), dim3(kBlock), 0, s, cams, sc);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = hipEventRecord(ev_fork, s)) != hipSuccess) return e;
@@ -1004,10 +1012,8 @@ hipError_t launch_batch(const MarchConst* table, const uint32_t* item_base, uint
         hipSuccess)
         return e;
     if (ev_end && (e = hipEventRecord(ev_end, m)) != hipSuccess) return e;
-    if (any_band) {
-        if ((e = hipEventRecord(ev_join, aux)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(s, ev_join, 0)) != hipSuccess) return e;
-    }
+    if (band_first && (e = hipEventRecord(ev_join, aux)) != hipSuccess) return e;
+    if (any_band && (e = hipStreamWaitEvent(s, ev_join, 0)) != hipSuccess) return e;
     if (any_root && sc.roots) {
         hipLaunchKernelGGL(k_root64<CamTable>, item_grid(n_items), dim3(kBlock), 0, s, cams, out, sc);
         if ((e = hipGetLastError()) != hipSuccess) return e;
