@@ -884,7 +884,7 @@ int bhs_autotune(bhs_ctx* ctx, const bhs_camera* cam, const bhs_march_params* pa
         {0, 256, 8, 64}, {0, 64, 8, 64},
         {1, 256, 8, 64}, {1, 256, 8, 32},
         {1, 256, 7, 64}, {1, 256, 7, 48}, {1, 256, 7, 32},
-        {1, 256, 6, 64}, {1, 256, 6, 32},
+        {1, 256, 6, 64}, {1, 256, 6, 32}, {1, 256, 5, 64},
         {1, 256, 4, 64}, {1, 256, 4, 48}, {1, 256, 4, 32},
         {1, 64, 7, 64}, {1, 64, 4, 64}, {1, 64, 4, 32},
     };

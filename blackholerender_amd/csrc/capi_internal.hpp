@@ -38,7 +38,7 @@ struct bhs_ctx {
     std::vector<TimingRec> timers;
     size_t timers_used = 0;
     int num_cus = 256;
-    bhs::Tuning tuning{1, 256, 7, 32};
+    bhs::Tuning tuning{1, 256, 6, 64};
     // Batch camera tables: device copy, pinned staging buffer and the event of its last upload.
     void* table_dev = nullptr;
     size_t table_dev_bytes = 0;

@@ -418,9 +418,11 @@ int bhs_valu_probe(bhs_ctx* ctx, int32_t kind, int32_t waves_per_simd, int32_t i
                    double* wave_instr);
 
 /* Launch shape of the FP32 march. mode 0: one lane per pixel on a static grid of 8x8-pixel wave
- * tiles; mode 1: resident waves pull rays from a global work queue and refill finished lanes
- * (wave ballot + prefix count, one atomic per refill). Chosen per context by bhs_autotune or set by
- * hand; bit-identical results in every shape. */
+ * tiles; mode 1: resident waves pull rays from a work queue and refill finished lanes: with
+ * refill_min 64, whole 16x4-pixel tiles from a queue split into 64 ranges with their own counters;
+ * below 64, the idle lanes with one atomic per refill on a single counter (wave ballot + prefix
+ * count). A context starts with {1, 256, 6, 64}; chosen per context by bhs_autotune or set by hand;
+ * bit-identical results in every shape. */
 typedef struct bhs_tuning {
     int32_t mode;             /* 0 static tiles, 1 persistent work queue */
     int32_t block;            /* threads per block: 64 or 256 */
