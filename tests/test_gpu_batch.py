@@ -57,6 +57,41 @@ def test_mixed_batch_bit_exact(tracer):
     assert st["rays"] == offs[-1] and st["steps"] == total
 
 
+@pytest.mark.parametrize("waves", [8, 4])
+def test_segmented_queue_migration_bit_exact(tracer, waves):
+    """Whole-tile claims from the segmented work queue (refill threshold 64) when the resident
+    waves far outnumber the tiles and the tile ranges differ widely in cost: a single-tile camera,
+    a near camera whose tiles orbit for hundreds of steps, a far one whose rays escape at once and
+    a 1x1 camera (padding items), so most waves find their first range used up and move on. Every
+    ray is handed out exactly once (the launch counters count each ray and step once) and equals
+    the oracle bit for bit."""
+    import torch
+    import blackholerender_amd as bhs
+    specs = [(30.0, 85.0, 60.0, 16, 4, 450), (8.0, 80.0, 75.0, 192, 128, 700),
+             (900.0, 40.0, 20.0, 96, 64, 200), (25.0, 30.0, 50.0, 1, 1, 450)]
+    cams = [bhs.orbit_camera(r, i, f, w, h, jitter_hashed=1, seed=13, camera_id=k)
+            for k, (r, i, f, w, h, _) in enumerate(specs)]
+    params = [bhs.march_params(max_steps=st) for (*_, st) in specs]
+    print(f"[batch] segmented queue, {waves} waves/SIMD", flush=True)
+    prev = tracer.get_tuning()
+    tracer.set_tuning(1, 256, waves, 64)
+    try:
+        out = tracer.render_batch(cams, params)
+        torch.cuda.synchronize()
+        st = tracer.last_stats()
+    finally:
+        tracer.set_tuning(prev["mode"], prev["block"], prev["waves_per_simd"], prev["refill_min"])
+    g = out.numpy()
+    offs = tracer.batch_offsets(cams)
+    total = 0
+    for k, (c, p) in enumerate(zip(cams, params)):
+        ref = _oracle_cam(c, p)
+        mm = bit_mismatches(_slice(g, offs[k], offs[k + 1]), ref)
+        assert all(v == 0 for v in mm.values()), (k, mm)
+        total += int(ref["steps"].sum())
+    assert st["rays"] == offs[-1] and st["steps"] == total
+
+
 def test_c3_batch_sampled_bit_exact(tracer):
     import torch
     from blackholerender_amd import workload as W
