@@ -290,7 +290,8 @@ def test_batch_equals_single_calls(tracer):
 def test_network_gain_over_prior_alone(tracer):
     """What the trained network adds to the exact-orbit prior: on the same validation rays, the
     share of in-domain rays whose (r_obs, b) cell meets the 1e-4 rad escape tolerance, with the
-    trained weights and with all weights zero (outputs 0: the prior alone)."""
+    trained weights and with all weights zero (outputs 0: the prior alone). The network must serve
+    over 85% and gain more than 12 points over the prior."""
     from blackholerender_amd.surrogate import Surrogate, MLP_PARAMS
     net = Surrogate.load(tracer, _model_path(MODEL_HIGH))
     prior = Surrogate.load(tracer, _model_path(MODEL_HIGH))
@@ -303,7 +304,8 @@ def test_network_gain_over_prior_alone(tracer):
     print(f"served share: network {share_n:.4f}, prior alone {share_p:.4f}")
     assert vn["mismatches"] == 0 and vp["mismatches"] == 0
     assert vn["phi_err_max"] <= 1e-4
-    assert share_n > share_p + 0.05, (share_n, share_p)
+    # measured on the committed model: 0.870 against 0.706 (seeded validation rays)
+    assert share_n > 0.85 and share_n > share_p + 0.12, (share_n, share_p)
     net.close()
     prior.close()
 
