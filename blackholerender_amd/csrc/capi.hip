@@ -359,6 +359,8 @@ bhs_ctx* bhs_create(int device, int* status) {
         hipStreamCreateWithFlags(&ctx->aux2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork2, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join2, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_tab0, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_tab1, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_upload, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
@@ -398,6 +400,8 @@ void bhs_destroy(bhs_ctx* ctx) {
         if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
         if (ctx->ev_fork2) (void)hipEventDestroy(ctx->ev_fork2);
         if (ctx->ev_join2) (void)hipEventDestroy(ctx->ev_join2);
+        if (ctx->ev_tab0) (void)hipEventDestroy(ctx->ev_tab0);
+        if (ctx->ev_tab1) (void)hipEventDestroy(ctx->ev_tab1);
         if (ctx->aux2) (void)hipStreamDestroy(ctx->aux2);
     }
     delete ctx;

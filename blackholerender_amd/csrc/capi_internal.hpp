@@ -17,6 +17,8 @@ struct bhs_ctx {
     // second side stream (surrogate render: early FP32 fallback march beside the FP64 band)
     hipStream_t aux2 = nullptr;
     hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
+    // surrogate render of one camera: the table built on aux2 beside the routing pre-pass
+    hipEvent_t ev_tab0 = nullptr, ev_tab1 = nullptr;
     // Stream of the context's most recent stream-ordered call (has_last: one was made). A call on
     // another stream first waits for it (ev_order): scratch, camera table and lists are shared.
     hipStream_t last_stream = nullptr;

@@ -695,8 +695,8 @@ int bhs_render_surrogate(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cam,
     if ((st = prepare_scratch(ctx, n, n_items, roots, &sc)) != BHS_OK) return st;
     M::SurLists fl;
     if ((st = lists_for(ctx, m, n_items, &fl)) != BHS_OK) return st;
-    const M::SurStreams ss{ctx->aux, ctx->aux2, ctx->ev_fork, ctx->ev_join, ctx->ev_fork2,
-                           ctx->ev_join2};
+    const M::SurStreams ss{ctx->aux,      ctx->aux2,    ctx->ev_fork, ctx->ev_join,
+                           ctx->ev_fork2, ctx->ev_join2, ctx->ev_tab0, ctx->ev_tab1};
     M::SurTable tab{};
     if ((st = table_for(ctx, m, 1, &tab)) != BHS_OK) return st;
     hipStream_t s = (hipStream_t)stream;
@@ -748,8 +748,8 @@ int bhs_render_surrogate_batch(bhs_ctx* ctx, bhs_mlp* m, const bhs_camera* cams,
     if ((st = lists_for(ctx, m, t.n_items, &fl)) != BHS_OK) return st;
     M::SurTable tab;
     if ((st = table_for(ctx, m, ncam, &tab)) != BHS_OK) return st;
-    const M::SurStreams ss{ctx->aux, ctx->aux2, ctx->ev_fork, ctx->ev_join, ctx->ev_fork2,
-                           ctx->ev_join2};
+    const M::SurStreams ss{ctx->aux,      ctx->aux2,    ctx->ev_fork, ctx->ev_join,
+                           ctx->ev_fork2, ctx->ev_join2, ctx->ev_tab0, ctx->ev_tab1};
     if ((st = pack(ctx, m, s)) != BHS_OK) return st;
     hipError_t e;
     bhs_ctx::TimingRec* tr = next_timer(ctx);

@@ -103,6 +103,7 @@ struct SurLists {
 struct SurStreams {
     hipStream_t aux, aux2;
     hipEvent_t ev_fork, ev_join, ev_fork2, ev_join2;
+    hipEvent_t ev_tab0, ev_tab1;   // the table build on aux2 (latency mode)
 };
 // Table mode: per camera, the network on an n2 x n3 grid over (x2, x3) at the camera's r_obs
 // (node: ncam n2 n3 x 4 floats; phi: ncam x kPhiNodes floats; range, slot: per camera).

@@ -1457,6 +1457,30 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     if (tab)   // x2 ranges, table slots and the per-camera setup terms
         hipLaunchKernelGGL(k_tab_range<Cams>, dim3((ncam + 63) / 64), dim3(64), 0, s, cams, ncam,
                            tab->range, tab->slot, tab->cam);
+    // the swept-angle table and the network's node table of the call's poses (after k_tab_range)
+    auto build_table = [&](hipStream_t st) -> hipError_t {
+        const int64_t nodes = (int64_t)tab->n2 * tab->n3;
+        hipLaunchKernelGGL(k_tab_phi<Cams>,
+                           dim3((uint32_t)(((int64_t)ncam * kPhiNodes + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, st, cams, ncam, (const float*)tab->range,
+                           (const uint32_t*)tab->slot, tab->phi);
+        hipLaunchKernelGGL(k_table<Cams>,
+                           dim3(persistent_blocks((nodes + kBlock - 1) / kBlock * ncam, num_cus)),
+                           dim3(kBlock), 0, st, cams, ncam, (const uint8_t*)packed,
+                           (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
+                           tab->n3, tab->node);
+        return hipGetLastError();
+    };
+    // Latency mode with a table: the table is built on aux2 beside the pre-pass and the band's
+    // launch (it depends on k_tab_range only), and `s` waits for it before the pass; the early
+    // list, later on aux2, follows it there.
+    const bool table_aside = routed && tab;
+    if (table_aside) {
+        if ((e = hipEventRecord(ss.ev_tab0, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(ss.aux2, ss.ev_tab0, 0)) != hipSuccess) return e;
+        if ((e = build_table(ss.aux2)) != hipSuccess) return e;
+        if ((e = hipEventRecord(ss.ev_tab1, ss.aux2)) != hipSuccess) return e;
+    }
     if (routed) {
         hipLaunchKernelGGL(k_sur_pre<Cams>, grid_of(pre_per_cu), dim3(kBlock), 0, s, cams, k,
                            n_items, sc, fl, pack_dests(kToBand, kToEarly, kNoList, kNoList),
@@ -1467,16 +1491,11 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
         if ((e = side_band()) != hipSuccess) return e;
     }
     if (tab) {
-        const int64_t nodes = (int64_t)tab->n2 * tab->n3;
-        hipLaunchKernelGGL(k_tab_phi<Cams>,
-                           dim3((uint32_t)(((int64_t)ncam * kPhiNodes + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, s, cams, ncam, (const float*)tab->range,
-                           (const uint32_t*)tab->slot, tab->phi);
-        hipLaunchKernelGGL(k_table<Cams>,
-                           dim3(persistent_blocks((nodes + kBlock - 1) / kBlock * ncam, num_cus)),
-                           dim3(kBlock), 0, s, cams, ncam, (const uint8_t*)packed,
-                           (const float*)tab->range, (const uint32_t*)tab->slot, tab->n2,
-                           tab->n3, tab->node);
+        if (table_aside) {
+            if ((e = hipStreamWaitEvent(s, ss.ev_tab1, 0)) != hipSuccess) return e;
+        } else if ((e = build_table(s)) != hipSuccess) {
+            return e;
+        }
         const uint32_t resident = tab_per_cu * (uint32_t)(num_cus > 0 ? num_cus : 256);
         uint32_t cpb = gb0 / (4u * resident);
         cpb = cpb < kTabChunksMin ? kTabChunksMin : (cpb > kTabChunksMax ? kTabChunksMax : cpb);
