@@ -1474,11 +1474,21 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
     // Latency mode with a table: the table is built on aux2 beside the pre-pass and the band's
     // launch (it depends on k_tab_range only), and `s` waits for it before the pass; the early
     // list, later on aux2, follows it there.
+    // Two-phase mode: the near-critical chunk flags (k_chunk_class, which needs the cameras only)
+    // are computed on aux2 beside the table build in the same way.
     const bool table_aside = routed && tab;
-    if (table_aside) {
+    const bool class_aside = phased;
+    if (table_aside || class_aside) {
         if ((e = hipEventRecord(ss.ev_tab0, s)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(ss.aux2, ss.ev_tab0, 0)) != hipSuccess) return e;
-        if ((e = build_table(ss.aux2)) != hipSuccess) return e;
+        if (table_aside) {
+            if ((e = build_table(ss.aux2)) != hipSuccess) return e;
+        } else {
+            const uint32_t segs = n_items >> 6;
+            hipLaunchKernelGGL(k_chunk_class<Cams>, dim3((segs + kBlock - 1) / kBlock),
+                               dim3(kBlock), 0, ss.aux2, cams, n_items, fl.near);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
         if ((e = hipEventRecord(ss.ev_tab1, ss.aux2)) != hipSuccess) return e;
     }
     if (routed) {
@@ -1517,9 +1527,7 @@ hipError_t launch_sur(const Cams& cams, uint32_t ncam, uint32_t n_items, const M
         };
         if (phased) {
             // near-critical chunks first, their band and fallback marches beside the rest
-            const uint32_t segs = n_items >> 6;
-            hipLaunchKernelGGL(k_chunk_class<Cams>, dim3((segs + kBlock - 1) / kBlock),
-                               dim3(kBlock), 0, s, cams, n_items, fl.near);
+            if ((e = hipStreamWaitEvent(s, ss.ev_tab1, 0)) != hipSuccess) return e;
             if ((e = pass(pack_dests(kToBand, kToEarly, kToBand, kToEarly), 1u)) != hipSuccess)
                 return e;
             if ((e = side()) != hipSuccess) return e;
